@@ -1,0 +1,672 @@
+// r2p_capi.cpp -- the C-ABI (include/r2p.h): device setup, the plain
+// per-image encoder, kernel-level entry points for the drop-in facade.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "r2p_common.h"
+
+namespace r2p {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char tl_err[512];
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(tl_err, sizeof(tl_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return R2P_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// constant tables (RFC 1951 section 3.2.5-3.2.6, PNG Annex D)
+// ---------------------------------------------------------------------------
+static uint32_t rev_bits(uint32_t v, int n)
+{
+    uint32_t r = 0;
+    for (int i = 0; i < n; i++) { r = (r << 1) | (v & 1); v >>= 1; }
+    return r;
+}
+
+// Fixed literal/length code of symbol s (RFC 1951 3.2.6), bit-reversed for
+// an LSB-first writer; *nb = code length.
+static uint32_t fixed_code(int s, int* nb)
+{
+    if (s < 144) { *nb = 8; return rev_bits(0x30 + s, 8); }
+    if (s < 256) { *nb = 9; return rev_bits(0x190 + (s - 144), 9); }
+    if (s < 280) { *nb = 7; return rev_bits(s - 256, 7); }
+    *nb = 8;
+    return rev_bits(0xC0 + (s - 280), 8);
+}
+
+static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev[32],
+                         uint32_t crc4[4][256], uint32_t x8pow[64])
+{
+    for (int b = 0; b < 256; b++) {
+        int nb;
+        const uint32_t c = fixed_code(b, &nb);
+        lit[b] = c | ((uint32_t)nb << 16);
+    }
+    // length codes 257..285: base lengths and extra bits
+    static const int base[29] = {3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31,
+                                 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+    static const int extra[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2,
+                                  3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+    len[0] = len[1] = len[2] = 0;
+    for (int L = 3; L <= 258; L++) {
+        int c = 28;
+        if (L < 258) { c = 0; while (c + 1 < 28 && base[c + 1] <= L) c++; }
+        int nb;
+        const uint32_t code = fixed_code(257 + c, &nb);
+        const uint32_t ev = (uint32_t)(L - base[c]);
+        len[L] = (code | (ev << nb)) | ((uint32_t)(nb + extra[c]) << 16);
+    }
+    for (int d = 0; d < 32; d++) dist_rev[d] = rev_bits((uint32_t)d, 5);
+    for (uint32_t n = 0; n < 256; n++) {
+        uint32_t c = n;
+        for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+        crc4[0][n] = c;
+    }
+    for (int t = 1; t < 4; t++)
+        for (int n = 0; n < 256; n++)
+            crc4[t][n] = (crc4[t - 1][n] >> 8) ^ crc4[0][crc4[t - 1][n] & 0xFF];
+    // x^(8*2^k) mod P in the reflected representation: start at x^8, square.
+    auto mulmod = [](uint32_t a, uint32_t b) {
+        uint32_t p = 0;
+        for (int k = 0; k < 32; k++) {
+            if (a & (0x80000000u >> k)) p ^= b;
+            b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+        }
+        return p;
+    };
+    uint32_t x = 0x00800000u;  // x^8
+    for (int k = 0; k < 64; k++) { x8pow[k] = x; x = mulmod(x, x); }
+}
+
+static std::mutex g_init_mu;
+static bool g_dev_ready[64];
+
+// ---------------------------------------------------------------------------
+// geometry / planning
+// ---------------------------------------------------------------------------
+int resolve(const r2p_image* img, Geometry* g)
+{
+    if (!img) { set_error("null image"); return R2P_ERR_ARG; }
+    if (img->width == 0 || img->height == 0 || img->channels < 1 || img->channels > 4 ||
+        (img->in_bits != 8 && img->in_bits != 16)) {
+        set_error("bad image geometry %ux%u ch=%d bits=%d", img->width, img->height, img->channels,
+                  img->in_bits);
+        return R2P_ERR_ARG;
+    }
+    int ob = img->in_bits;
+    switch (img->pix_mode) {
+    case R2P_PIX_READY: break;
+    case R2P_PIX_SWAP16:
+        if (img->in_bits != 16) { set_error("SWAP16 needs 16-bit input"); return R2P_ERR_ARG; }
+        break;
+    case R2P_PIX_DICOM:
+        if (img->dicom.bits_allocated != img->in_bits) {
+            set_error("DICOM bits_allocated %d != in_bits %d", img->dicom.bits_allocated, img->in_bits);
+            return R2P_ERR_ARG;
+        }
+        ob = img->dicom.bits_allocated;
+        break;
+    case R2P_PIX_DICOM8:
+        if (img->in_bits != 16) { set_error("DICOM8 needs 16-bit input"); return R2P_ERR_ARG; }
+        ob = 8;
+        break;
+    default: set_error("bad pix_mode %d", img->pix_mode); return R2P_ERR_ARG;
+    }
+    g->out_bits = ob;
+    g->bpp = img->channels * (ob / 8);
+    g->row_bytes = (size_t)img->width * (size_t)g->bpp;
+    g->filtered = (size_t)img->height * (g->row_bytes + 1);
+    g->in_row = (size_t)img->width * img->channels * (img->in_bits / 8);
+    g->pitch = img->src_pitch ? img->src_pitch : (int64_t)g->in_row;
+    if (g->pitch < (int64_t)g->in_row) { set_error("src_pitch < row bytes"); return R2P_ERR_ARG; }
+    // channels_to_color_type (pipeline.cpp:80-88)
+    g->color_type = img->channels == 1 ? 0 : img->channels == 3 ? 2 : img->channels == 4 ? 6 : 2;
+    if (img->channels == 2) g->color_type = 4;
+    return R2P_OK;
+}
+
+bool q15_coeffs(const r2p_dicom_params& p, int16_t* a_fp, int32_t* b_fp)
+{
+    const int shift = p.high_bit - p.bits_stored + 1;
+    if (shift != 0) return false;
+    float A_f, B_f;
+    if (p.apply_window && p.window_width > 0.f) {
+        const float lo = p.window_center - p.window_width * 0.5f;
+        const float slope = p.apply_rescale ? p.rescale_slope : 1.f;
+        const float intercept = p.apply_rescale ? p.rescale_intercept : 0.f;
+        A_f = slope * 255.f / p.window_width;
+        B_f = (intercept - lo) * 255.f / p.window_width;
+    } else if (!p.apply_window) {
+        const float slope = p.apply_rescale ? p.rescale_slope : 1.f;
+        const float intercept = p.apply_rescale ? p.rescale_intercept : 0.f;
+        const float range = (float)(1u << p.bits_stored);
+        if (p.pixel_rep == 1) {
+            const float half = range * 0.5f;
+            A_f = slope * 255.f / range;
+            B_f = (intercept + half) * 255.f / range;
+        } else {
+            A_f = slope * 255.f / (range - 1.f);
+            B_f = intercept * 255.f / (range - 1.f);
+        }
+    } else {
+        return false;
+    }
+    const float A_q15 = A_f * 32768.f;
+    if (A_q15 < -32767.f || A_q15 > 32767.f) return false;
+    *a_fp = (int16_t)(int)A_q15;
+    float bq;
+    if (A_f >= 0.f) bq = (A_q15 == (float)(int)A_q15) ? B_f * 32768.f : B_f * 32768.f + 0.5f;
+    else            bq = B_f * 32768.f - 0.5f;
+    *b_fp = (int32_t)bq;
+    return true;
+}
+
+static int check_dicom(const r2p_dicom_params& p, int need_alloc)
+{
+    if (p.bits_allocated != need_alloc) { set_error("bits_allocated %d unsupported here", p.bits_allocated); return R2P_ERR_ARG; }
+    if (p.bits_stored < 1 || p.bits_stored > p.bits_allocated) { set_error("bits_stored %d out of range", p.bits_stored); return R2P_ERR_ARG; }
+    if (p.high_bit < p.bits_stored - 1 || p.high_bit >= p.bits_allocated) {
+        set_error("high_bit %d inconsistent with bits_stored %d", p.high_bit, p.bits_stored);
+        return R2P_ERR_ARG;
+    }
+    if (p.pixel_rep != 0 && p.pixel_rep != 1) { set_error("pixel_rep %d", p.pixel_rep); return R2P_ERR_ARG; }
+    return R2P_OK;
+}
+
+int plan_window(const r2p_image* img, WindowPlan* wp)
+{
+    *wp = WindowPlan{};
+    const r2p_dicom_params& p = img->dicom;
+    switch (img->pix_mode) {
+    case R2P_PIX_READY: wp->wmode = WM_COPY; return R2P_OK;
+    case R2P_PIX_SWAP16: wp->wmode = WM_SWAP16; return R2P_OK;
+    case R2P_PIX_DICOM: {
+        int rc = check_dicom(p, img->in_bits);
+        if (rc) return rc;
+        if (p.bits_allocated == 8) {
+            wp->wmode = WM_LUT8_FROM8;
+            wp->lut_kind = 0; wp->lut_entries = 256; wp->lut_bytes = 256;
+            return R2P_OK;
+        }
+        const int sh = p.high_bit - p.bits_stored + 1;
+        wp->wmode = WM_LUT16;
+        wp->shift = sh > 0 ? sh : 0;
+        wp->mask = p.bits_stored < 16 ? (1u << p.bits_stored) - 1u : 0xFFFFu;
+        wp->lut_kind = 0;
+        wp->lut_entries = 1 << p.bits_stored;
+        wp->lut_bytes = 2 * wp->lut_entries;
+        if (wp->lut_bytes > 65536) {
+            set_error("16-bit stored DICOM window table exceeds LDS budget");
+            return R2P_ERR_UNSUPPORTED;
+        }
+        return R2P_OK;
+    }
+    case R2P_PIX_DICOM8: {
+        int rc = check_dicom(p, 16);
+        if (rc) return rc;
+        int16_t a; int32_t b;
+        if (q15_coeffs(p, &a, &b)) {
+            wp->wmode = WM_Q15; wp->q15_a = a; wp->q15_b = b; wp->pixel_rep = p.pixel_rep;
+            return R2P_OK;
+        }
+        wp->wmode = WM_LUT8_FROM16;
+        wp->shift = p.high_bit - p.bits_stored + 1;
+        wp->lut_kind = 1;
+        wp->lut_entries = 1 << p.bits_stored;
+        wp->mask = (uint32_t)(wp->lut_entries - 1);
+        wp->lut_bytes = wp->lut_entries;
+        return R2P_OK;
+    }
+    default: set_error("bad pix_mode"); return R2P_ERR_ARG;
+    }
+}
+
+LutCache::~LutCache()
+{
+    if (d_lut) (void)hipFree(d_lut);
+}
+
+int LutCache::ensure(const WindowPlan& wp, const r2p_dicom_params& pp, hipStream_t s)
+{
+    if (wp.lut_kind < 0) return R2P_OK;
+    if (valid && kind == wp.lut_kind && entries == wp.lut_entries && memcmp(&p, &pp, sizeof(pp)) == 0)
+        return R2P_OK;
+    const size_t need = ((size_t)wp.lut_bytes + 15) & ~(size_t)15;
+    if (need > cap) {
+        if (d_lut) R2P_HIP(hipFree(d_lut));
+        d_lut = nullptr;
+        R2P_HIP(hipMalloc(&d_lut, std::max<size_t>(need, 4096)));
+        cap = std::max<size_t>(need, 4096);
+    }
+    LutBuild lb;
+    lb.p = pp;
+    lb.entries = wp.lut_entries;
+    lb.kind = wp.lut_kind;
+    R2P_HIP(launch_build_lut(lb, d_lut, s));
+    valid = true; kind = wp.lut_kind; entries = wp.lut_entries; p = pp;
+    return R2P_OK;
+}
+
+size_t png_bound_bytes(size_t filtered, int segment_bytes, int chunk_bytes)
+{
+    const size_t nseg = (filtered + segment_bytes - 1) / segment_bytes;
+    const size_t z = filtered + filtered / 8 + 16 * nseg + 16;
+    const size_t nch = (z + chunk_bytes - 1) / chunk_bytes + 1;
+    return 8 + 25 + 12 * nch + z + 12 + 64;
+}
+
+int default_band_rows(int64_t total_rows)
+{
+    // aim for ~4 bands per CU slot on 256 CUs; never fewer than 1 or more than 16 rows
+    int64_t b = total_rows / 1024;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(16, b));
+}
+
+void fill_defaults(r2p_options* o)
+{
+    if (o->segment_bytes <= 0) o->segment_bytes = 65536;
+    if (o->idat_chunk_bytes <= 0) o->idat_chunk_bytes = 65536;
+    if (o->idat_chunk_bytes > 65536) o->idat_chunk_bytes = 65536;
+    if (o->lane_bytes != 16 && o->lane_bytes != 32) o->lane_bytes = 32;
+    if (o->lz77 < 0) o->lz77 = 1;
+}
+
+}  // namespace r2p
+
+using namespace r2p;
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* r2p_last_error(void) { return tl_err; }
+const char* r2p_version(void) { return "r2p-gfx950 0.1"; }
+
+int r2p_init(int device)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) { set_error("no HIP device"); return R2P_ERR_NODEV; }
+    if (device < 0 || device >= n || device >= 64) { set_error("device %d out of range", device); return R2P_ERR_ARG; }
+    R2P_HIP(hipSetDevice(device));
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_dev_ready[device]) return R2P_OK;
+    hipDeviceProp_t prop;
+    R2P_HIP(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx9", 4) != 0) {
+        set_error("device %d is %s, this build targets gfx950", device, prop.gcnArchName);
+        return R2P_ERR_NODEV;
+    }
+    static uint32_t lit[256], len[259], dist_rev[32], crc4[4][256], x8pow[64];
+    build_tables(lit, len, dist_rev, crc4, x8pow);
+    R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow));
+    g_dev_ready[device] = true;
+    return R2P_OK;
+}
+
+static int current_init()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    return r2p_init(dev);
+}
+
+int r2p_image_out_bits(const r2p_image* img)
+{
+    Geometry g;
+    return resolve(img, &g) == R2P_OK ? g.out_bits : -1;
+}
+size_t r2p_image_row_bytes(const r2p_image* img)
+{
+    Geometry g;
+    return resolve(img, &g) == R2P_OK ? g.row_bytes : 0;
+}
+size_t r2p_image_filtered_bytes(const r2p_image* img)
+{
+    Geometry g;
+    return resolve(img, &g) == R2P_OK ? g.filtered : 0;
+}
+size_t r2p_image_input_bytes(const r2p_image* img)
+{
+    Geometry g;
+    if (resolve(img, &g) != R2P_OK) return 0;
+    return (size_t)g.pitch * (img->height - 1) + g.in_row;
+}
+size_t r2p_png_bound(const r2p_image* img, int segment_bytes, int idat_chunk_bytes)
+{
+    Geometry g;
+    if (resolve(img, &g) != R2P_OK) return 0;
+    r2p_options o{};
+    o.segment_bytes = segment_bytes; o.idat_chunk_bytes = idat_chunk_bytes;
+    fill_defaults(&o);
+    return png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
+}
+
+void r2p_options_default(r2p_options* o)
+{
+    memset(o, 0, sizeof(*o));
+    o->segment_bytes = 65536;
+    o->lz77 = 1;
+    o->idat_chunk_bytes = 65536;
+    o->zlib_level = 3;       // PipelineConfig::deflate_level default (pipeline.h:17)
+    o->band_rows = 0;
+    o->lane_bytes = 32;
+}
+
+uint32_t r2p_adler32_combine(uint32_t a1, uint32_t a2, uint64_t len2)
+{
+    const uint64_t M = 65521;
+    const uint64_t A1 = a1 & 0xFFFF, B1 = a1 >> 16, A2 = a2 & 0xFFFF, B2 = a2 >> 16;
+    const uint64_t n = len2 % M;
+    const uint64_t A = (A1 + A2 + M - 1) % M;
+    const uint64_t B = (B1 + B2 + n * ((A1 + M - 1) % M)) % M;
+    return (uint32_t)((B << 16) | A);
+}
+
+uint32_t r2p_crc32_combine(uint32_t c1, uint32_t c2, uint64_t len2)
+{
+    auto mulmod = [](uint32_t a, uint32_t b) {
+        uint32_t p = 0;
+        for (int k = 0; k < 32; k++) {
+            if (a & (0x80000000u >> k)) p ^= b;
+            b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+        }
+        return p;
+    };
+    uint32_t r = 0x80000000u, sq = 0x00800000u;
+    for (uint64_t n = len2; n; n >>= 1) {
+        if (n & 1) r = mulmod(r, sq);
+        sq = mulmod(sq, sq);
+    }
+    return mulmod(r, c1) ^ c2;
+}
+
+// ---------------------------------------------------------------------------
+// filter rows (kernel-level)
+// ---------------------------------------------------------------------------
+static int launch_filter_job(const FilterJob& J, int band_rows, hipStream_t s)
+{
+    if (J.row_bytes > kMaxRowBytesLds) {
+        set_error("row of %d bytes exceeds the %d-byte LDS row buffer", J.row_bytes, kMaxRowBytesLds);
+        return R2P_ERR_UNSUPPORTED;
+    }
+    FilterLaunch L{};
+    L.jobs = nullptr;
+    L.band_start = nullptr;
+    L.njobs = 1;
+    L.band_rows = band_rows > 0 ? band_rows : default_band_rows(J.rows);
+    const uint32_t bands = (uint32_t)((J.rows + L.band_rows - 1) / L.band_rows);
+    L.total_bands = bands;
+    L.row_cap = (J.row_bytes + 15) & ~15;
+    L.lut_cap = (J.lut_bytes + 15) & ~15;
+    L.single = J;
+    const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 64;
+    const int grid = (int)std::min<uint32_t>(bands, 8192);
+    R2P_HIP(launch_filter(L, grid, lds, s));
+    return R2P_OK;
+}
+
+int r2p_filter_rows(const r2p_image* img, const void* d_src, int rows, const void* d_prior,
+                    void* d_filtered, void* d_last_row, int band_rows, void* stream)
+{
+    int rc = current_init();
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = resolve(img, &g))) return rc;
+    if (rows <= 0 || !d_src || !d_filtered) { set_error("bad filter arguments"); return R2P_ERR_ARG; }
+    WindowPlan wp;
+    if ((rc = plan_window(img, &wp))) return rc;
+    static thread_local LutCache cache;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = cache.ensure(wp, img->dicom, s))) return rc;
+    FilterJob J{};
+    J.src = (const uint8_t*)d_src; J.dst = (uint8_t*)d_filtered;
+    J.prior = (const uint8_t*)d_prior; J.last_row = (uint8_t*)d_last_row;
+    J.lut = cache.d_lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes; J.rows = rows;
+    J.bpp = g.bpp; J.wmode = wp.wmode; J.shift = wp.shift; J.mask = wp.mask;
+    J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
+    J.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
+    return launch_filter_job(J, band_rows, s);
+}
+
+// ---------------------------------------------------------------------------
+// checksums
+// ---------------------------------------------------------------------------
+size_t r2p_checksum_work_bytes(size_t len)
+{
+    return 4 * ((len + kChecksumBlock - 1) / kChecksumBlock + 1) + 256;
+}
+
+int r2p_adler32_device(const void* d_data, size_t len, void* d_work, uint32_t* d_result, void* stream)
+{
+    int rc = current_init();
+    if (rc) return rc;
+    if ((!d_data && len) || !d_work || !d_result) { set_error("bad checksum arguments"); return R2P_ERR_ARG; }
+    R2P_HIP(launch_adler((const uint8_t*)d_data, (int64_t)len, (uint32_t*)d_work, d_result, (hipStream_t)stream));
+    return R2P_OK;
+}
+
+int r2p_crc32_device(const void* d_data, size_t len, void* d_work, uint32_t* d_result, void* stream)
+{
+    int rc = current_init();
+    if (rc) return rc;
+    if ((!d_data && len) || !d_work || !d_result) { set_error("bad checksum arguments"); return R2P_ERR_ARG; }
+    R2P_HIP(launch_crc((const uint8_t*)d_data, (int64_t)len, (uint32_t*)d_work, d_result, (hipStream_t)stream));
+    return R2P_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Plain per-image encoder
+// ===========================================================================
+struct r2p_encoder {
+    r2p_options opt{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    size_t max_filtered = 0;
+    int max_seg = 0;
+    int seg_cap_dw = 0;
+    uint8_t* d_filt = nullptr;
+    uint32_t* d_seg_out = nullptr;
+    uint32_t* d_seg_len = nullptr;
+    uint32_t* d_seg_adler = nullptr;
+    uint64_t* d_seg_off = nullptr;
+    uint64_t* d_misc = nullptr;      // [0] png length, [1] adler
+    uint64_t* h_misc = nullptr;      // pinned mirror
+    uint8_t* d_src_stage = nullptr;  // encode_host staging
+    size_t src_stage_cap = 0;
+    uint8_t* d_png_stage = nullptr;
+    size_t png_stage_cap = 0;
+    LutCache lut;
+    uint32_t last_nseg = 0;
+    bool timed = false;
+};
+
+static int enc_grow(uint8_t** p, size_t* cap, size_t need)
+{
+    if (need <= *cap) return R2P_OK;
+    if (*p) R2P_HIP(hipFree(*p));
+    *p = nullptr;
+    R2P_HIP(hipMalloc((void**)p, need));
+    *cap = need;
+    return R2P_OK;
+}
+
+extern "C" {
+
+int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_encoder** out)
+{
+    int rc = current_init();
+    if (rc) return rc;
+    if (!out || max_filtered_bytes == 0) { set_error("bad encoder arguments"); return R2P_ERR_ARG; }
+    r2p_encoder* e = new r2p_encoder();
+    if (opt) e->opt = *opt; else r2p_options_default(&e->opt);
+    fill_defaults(&e->opt);
+    e->max_filtered = max_filtered_bytes;
+    e->max_seg = (int)((max_filtered_bytes + e->opt.segment_bytes - 1) / e->opt.segment_bytes);
+    e->seg_cap_dw = seg_cap_dwords(e->opt.segment_bytes);
+    hipError_t err;
+#define TRY(x) do { if ((err = (x)) != hipSuccess) { r2p_encoder_destroy(e); return hip_fail(err, #x); } } while (0)
+    TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto& v : e->ev) TRY(hipEventCreate(&v));
+    TRY(hipMalloc((void**)&e->d_filt, max_filtered_bytes));
+    TRY(hipMalloc((void**)&e->d_seg_out, (size_t)e->max_seg * e->seg_cap_dw * 4));
+    TRY(hipMalloc((void**)&e->d_seg_len, (size_t)e->max_seg * 4));
+    TRY(hipMalloc((void**)&e->d_seg_adler, (size_t)e->max_seg * 4));
+    TRY(hipMalloc((void**)&e->d_seg_off, ((size_t)e->max_seg + 1) * 8));
+    TRY(hipMalloc((void**)&e->d_misc, 64));
+    TRY(hipHostMalloc((void**)&e->h_misc, 64, hipHostMallocDefault));
+#undef TRY
+    *out = e;
+    return R2P_OK;
+}
+
+int r2p_encoder_destroy(r2p_encoder* e)
+{
+    if (!e) return R2P_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(e->d_filt); (void)hipFree(e->d_seg_out); (void)hipFree(e->d_seg_len);
+    (void)hipFree(e->d_seg_adler); (void)hipFree(e->d_seg_off); (void)hipFree(e->d_misc);
+    if (e->h_misc) (void)hipHostFree(e->h_misc);
+    (void)hipFree(e->d_src_stage); (void)hipFree(e->d_png_stage);
+    for (auto& v : e->ev) if (v) (void)hipEventDestroy(v);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return R2P_OK;
+}
+
+void* r2p_encoder_stream(r2p_encoder* e) { return e ? (void*)e->stream : nullptr; }
+
+int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, void* d_png,
+                      size_t png_cap, void* d_filtered_out)
+{
+    if (!e || !img || !d_src || !d_png) { set_error("bad encode arguments"); return R2P_ERR_ARG; }
+    Geometry g;
+    int rc = resolve(img, &g);
+    if (rc) return rc;
+    if (g.filtered > e->max_filtered) {
+        set_error("image needs %zu filtered bytes, encoder sized for %zu", g.filtered, e->max_filtered);
+        return R2P_ERR_SIZE;
+    }
+    const r2p_options& o = e->opt;
+    const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
+    if (png_cap < bound) { set_error("png capacity %zu < bound %zu", png_cap, bound); return R2P_ERR_SIZE; }
+    WindowPlan wp;
+    if ((rc = plan_window(img, &wp))) return rc;
+    hipStream_t s = e->stream;
+    if ((rc = e->lut.ensure(wp, img->dicom, s))) return rc;
+
+    uint8_t* filt = d_filtered_out ? (uint8_t*)d_filtered_out : e->d_filt;
+    R2P_HIP(hipEventRecord(e->ev[0], s));
+    FilterJob J{};
+    J.src = (const uint8_t*)d_src; J.dst = filt; J.prior = nullptr; J.last_row = nullptr;
+    J.lut = e->lut.d_lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes;
+    J.rows = (int32_t)img->height; J.bpp = g.bpp; J.wmode = wp.wmode; J.shift = wp.shift;
+    J.mask = wp.mask; J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
+    J.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
+    if ((rc = launch_filter_job(J, o.band_rows, s))) return rc;
+    R2P_HIP(hipEventRecord(e->ev[1], s));
+
+    const int nseg = (int)((g.filtered + o.segment_bytes - 1) / o.segment_bytes);
+    DeflateLaunch DL{};
+    DeflateJob& D = DL.single;
+    D.data = filt; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
+    D.seg_out = e->d_seg_out; D.seg_len = e->d_seg_len; D.seg_adler = e->d_seg_adler;
+    D.seg_stats = nullptr; D.seg_bytes = o.segment_bytes; D.seg_cap_dw = e->seg_cap_dw;
+    D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
+    D.lz77 = o.lz77; D.nseg = nseg;
+    DL.jobs = nullptr; DL.njobs = 1; DL.total_segs = (uint32_t)nseg;
+    R2P_HIP(launch_deflate(DL, o.lane_bytes, nseg, s));
+    R2P_HIP(hipEventRecord(e->ev[2], s));
+
+    AssembleLaunch AL{};
+    AssembleJob& A = AL.single;
+    A.seg_out = e->d_seg_out; A.seg_len = e->d_seg_len; A.seg_adler = e->d_seg_adler;
+    A.seg_off = e->d_seg_off; A.out = (uint8_t*)d_png; A.out_len = e->d_misc;
+    A.adler_out = (uint32_t*)(e->d_misc + 1); A.out_cap = (int64_t)png_cap;
+    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.seg_cap_dw = e->seg_cap_dw;
+    A.seg_bytes = o.segment_bytes; A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1;
+    A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
+    A.color_type = (uint8_t)g.color_type;
+    const int lvl = o.zlib_level;
+    A.zflg = lvl == 1 ? 0x01 : (lvl >= 2 && lvl <= 5) ? 0x5E : (lvl >= 6 && lvl <= 7) ? 0x9C : 0xDA;
+    const size_t zmax = g.filtered + g.filtered / 8 + 16 * (size_t)nseg + 16;
+    A.max_chunks = (int32_t)((zmax + o.idat_chunk_bytes - 1) / o.idat_chunk_bytes);
+    A.running = nullptr; A.base_slot = nullptr;
+    AL.jobs = nullptr; AL.njobs = 1; AL.total_chunks = (uint32_t)A.max_chunks;
+    R2P_HIP(launch_scan(AL, 1, s));
+    R2P_HIP(hipEventRecord(e->ev[3], s));
+    R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
+    R2P_HIP(hipEventRecord(e->ev[4], s));
+    R2P_HIP(hipMemcpyAsync(e->h_misc, e->d_misc, 16, hipMemcpyDeviceToHost, s));
+    e->last_nseg = (uint32_t)nseg;
+    e->timed = true;
+    return R2P_OK;
+}
+
+int r2p_encoder_sync(r2p_encoder* e, size_t* png_len, uint32_t* adler, uint32_t* nseg)
+{
+    if (!e) { set_error("null encoder"); return R2P_ERR_ARG; }
+    R2P_HIP(hipStreamSynchronize(e->stream));
+    if (png_len) *png_len = (size_t)e->h_misc[0];
+    if (adler) *adler = (uint32_t)e->h_misc[1];
+    if (nseg) *nseg = e->last_nseg;
+    return R2P_OK;
+}
+
+int r2p_encode_host(r2p_encoder* e, const r2p_image* img, const void* h_src, void* h_png,
+                    size_t png_cap, size_t* png_len)
+{
+    if (!e || !img || !h_src || !h_png) { set_error("bad encode arguments"); return R2P_ERR_ARG; }
+    Geometry g;
+    int rc = resolve(img, &g);
+    if (rc) return rc;
+    const size_t in_bytes = (size_t)g.pitch * (img->height - 1) + g.in_row;
+    const size_t bound = png_bound_bytes(g.filtered, e->opt.segment_bytes, e->opt.idat_chunk_bytes);
+    if ((rc = enc_grow(&e->d_src_stage, &e->src_stage_cap, in_bytes))) return rc;
+    if ((rc = enc_grow(&e->d_png_stage, &e->png_stage_cap, bound))) return rc;
+    R2P_HIP(hipMemcpyAsync(e->d_src_stage, h_src, in_bytes, hipMemcpyHostToDevice, e->stream));
+    if ((rc = r2p_encode_device(e, img, e->d_src_stage, e->d_png_stage, e->png_stage_cap, nullptr))) return rc;
+    size_t n = 0;
+    if ((rc = r2p_encoder_sync(e, &n, nullptr, nullptr))) return rc;
+    if (n > png_cap) { set_error("png of %zu bytes exceeds host capacity %zu", n, png_cap); return R2P_ERR_SIZE; }
+    R2P_HIP(hipMemcpy(h_png, e->d_png_stage, n, hipMemcpyDeviceToHost));
+    if (png_len) *png_len = n;
+    return R2P_OK;
+}
+
+int r2p_encoder_timings(r2p_encoder* e, r2p_timings* t)
+{
+    if (!e || !t || !e->timed) { set_error("no timings"); return R2P_ERR_STATE; }
+    R2P_HIP(hipEventSynchronize(e->ev[4]));
+    R2P_HIP(hipEventElapsedTime(&t->filter_ms, e->ev[0], e->ev[1]));
+    R2P_HIP(hipEventElapsedTime(&t->deflate_ms, e->ev[1], e->ev[2]));
+    R2P_HIP(hipEventElapsedTime(&t->scan_ms, e->ev[2], e->ev[3]));
+    R2P_HIP(hipEventElapsedTime(&t->assemble_ms, e->ev[3], e->ev[4]));
+    R2P_HIP(hipEventElapsedTime(&t->total_ms, e->ev[0], e->ev[4]));
+    return R2P_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// r2p_internal.h -- descriptors shared by the HIP kernels and their host-side
+// launchers.  Not part of the public C-ABI.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "r2p.h"
+
+namespace r2p {
+
+// ---------------------------------------------------------------------------
+// Window stage modes (how one input row becomes a PNG-ready row).
+// ---------------------------------------------------------------------------
+enum WinMode : int32_t {
+    WM_COPY        = 0,  // bytes already PNG-ready
+    WM_SWAP16      = 1,  // LE 16-bit -> BE 16-bit
+    WM_LUT16       = 2,  // (raw16 >> shift) & mask -> u16 table -> BE 16-bit
+    WM_LUT8_FROM8  = 3,  // raw8 -> u8 table                 (8-bit DICOM window)
+    WM_Q15         = 4,  // (raw16 * A + B) >> 15, clamp     (Q15 16 -> 8)
+    WM_LUT8_FROM16 = 5,  // (raw16 >> shift) & mask -> u8 table (LUT 16 -> 8)
+};
+
+// One image (or strip) for the window+filter kernel.
+struct FilterJob {
+    const uint8_t* src;        // raw input rows
+    uint8_t*       dst;        // filtered rows: rows * (row_bytes + 1)
+    const uint8_t* prior;      // PNG-ready row above row 0, or nullptr (= zeros)
+    uint8_t*       last_row;   // receives the PNG-ready last row, or nullptr
+    const void*    lut;        // device table for the LUT modes
+    int64_t        src_pitch;  // bytes between input rows
+    int32_t        row_bytes;  // PNG-ready bytes per row
+    int32_t        rows;
+    int32_t        bpp;        // PNG filter distance (bytes per complete pixel, >= 1)
+    int32_t        wmode;
+    int32_t        shift;      // right shift of the raw 16-bit sample
+    uint32_t       mask;       // mask after the shift (LUT index range - 1)
+    int32_t        q15_a, q15_b, pixel_rep;
+    int32_t        lut_bytes;  // size of the table in bytes (0 = none)
+};
+
+// Parameters of the DICOM table builder (one table per parameter set).
+struct LutBuild {
+    r2p_dicom_params p;
+    int32_t entries;
+    int32_t kind;   // 0: a2 float window -> u16 (bits_allocated 16) or u8 (8)
+                    // 1: a4 8-bit LUT -> u8
+};
+
+// One stream of filtered bytes to compress into independent segments.
+struct DeflateJob {
+    const uint8_t* data;       // base of the addressable bytes
+    int64_t        first;      // offset of the first byte to compress
+    int64_t        len;        // bytes to compress
+    int64_t        dict_lo;    // lowest offset a match may reference
+    uint32_t*      seg_out;    // nseg * seg_cap_dw dwords of scratch
+    uint32_t*      seg_len;    // compressed bytes per segment
+    uint32_t*      seg_adler;  // adler32 of each segment's input
+    uint32_t*      seg_stats;  // optional: 4 u32 per segment (matches, matched bytes, dist sum lo, max len)
+    int32_t        seg_bytes;
+    int32_t        seg_cap_dw;
+    int32_t        final_block; // the job's last segment sets BFINAL (else sync flush)
+    int32_t        bpp;
+    int32_t        row_stride; // filtered row size (match candidate one row up), 0 = none
+    int32_t        lz77;
+    int32_t        nseg;
+    int32_t        pad_;
+};
+
+// Where a job's segments go once their sizes are known.
+struct AssembleJob {
+    const uint32_t* seg_out;
+    const uint32_t* seg_len;
+    const uint32_t* seg_adler;
+    uint64_t*       seg_off;     // scan output: nseg + 1 entries (byte offsets in the zlib payload)
+    uint8_t*        out;         // PNG (or raw) destination
+    uint64_t*       out_len;     // total bytes written (device)
+    uint32_t*       adler_out;   // final adler32 (device), optional
+    int64_t         out_cap;
+    int64_t         total_in;    // uncompressed bytes (for the adler combine)
+    int32_t         nseg;
+    int32_t         seg_cap_dw;
+    int32_t         seg_bytes;
+    int32_t         chunk_bytes; // IDAT payload bytes per chunk
+    int32_t         mode;        // 0 = raw deflate concatenation, 1 = complete PNG
+    uint32_t        width, height;
+    uint8_t         bit_depth, color_type, zflg, pad_;
+    int32_t         max_chunks;  // chunks this job may use (grid sizing bound)
+    uint64_t*       running;     // optional: running output offset (raw mode, strip appends)
+    uint64_t*       base_slot;   // optional: where k_scan records this job's base offset
+};
+
+// Kernel launch descriptors: a job table for the batched form, or one job
+// passed by value (jobs == nullptr) for the plain form.
+struct FilterLaunch {
+    const FilterJob*  jobs;
+    const uint32_t*   band_start;  // prefix of band counts per job (njobs + 1)
+    int32_t           njobs;
+    int32_t           band_rows;
+    uint32_t          total_bands;
+    int32_t           row_cap;     // LDS row capacity (bytes, multiple of 16)
+    int32_t           lut_cap;     // LDS table capacity (bytes, multiple of 16)
+    FilterJob         single;
+};
+
+struct DeflateLaunch {
+    const DeflateJob* jobs;
+    const uint32_t*   seg_start;   // prefix of segment counts per job (njobs + 1)
+    int32_t           njobs;
+    uint32_t          total_segs;
+    DeflateJob        single;
+};
+
+struct AssembleLaunch {
+    const AssembleJob* jobs;
+    const uint32_t*    chunk_start; // prefix of max_chunks per job (njobs + 1)
+    int32_t            njobs;
+    uint32_t           total_chunks;
+    AssembleJob        single;
+};
+
+constexpr int kFilterThreads = 256;
+constexpr int kDeflateThreads = 256;
+constexpr int kAssembleThreads = 256;
+constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
+constexpr int kFilterMaxK = kMaxRowBytesLds / (16 * kFilterThreads);
+constexpr int kWindow = 32768;           // DEFLATE window
+constexpr int kChecksumBlock = 65536;    // bytes per checksum workgroup
+
+// Worst-case compressed dwords of one segment of `seg_bytes` input bytes:
+// 9 bits per literal, 3-bit header, 7-bit EOB, <=7 pad, 3+32 bits of sync flush.
+inline int seg_cap_dwords(int seg_bytes) { return (int)(((int64_t)seg_bytes * 9 + 64) / 32) + 4; }
+
+}  // namespace r2p

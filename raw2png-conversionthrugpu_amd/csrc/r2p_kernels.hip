@@ -1,0 +1,1214 @@
+// r2p_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the PNG encoder.
+//
+//   k_build_lut     DICOM pixel tables (a2 float window, a4 8-bit LUT)
+//   k_filter        fused window + adaptive PNG filter, rows staged in LDS
+//   k_deflate       per-segment LZ77 + fixed-Huffman DEFLATE, 32 KiB primed
+//                   window, sync-flush segment ends, per-segment Adler-32
+//   k_scan          segment byte offsets, zlib payload size, Adler combine
+//   k_assemble      concatenation / IDAT chunking with on-device CRC-32
+//   k_adler_*, k_crc_*   standalone checksums (partials + combine)
+//
+// Everything is integer byte work bound by HBM bandwidth; nothing is a dense
+// contraction, so no MFMA.  Wave64 throughout: reductions and scans use
+// cross-lane shuffles, bit packing uses per-lane bit offsets from a block scan.
+// Float code (the DICOM window) is compiled with -ffp-contract=off and IEEE
+// division so it reproduces the oracle bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "r2p_internal.h"
+
+namespace r2p {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef u32x4 u32x4_ua __attribute__((aligned(1)));
+typedef uint32_t u32_ua __attribute__((aligned(1)));
+
+// ---------------------------------------------------------------------------
+// Constant tables (filled by r2p_upload_tables)
+// ---------------------------------------------------------------------------
+__constant__ uint32_t c_lit[256];      // reversed fixed-Huffman literal code | nbits << 16
+__constant__ uint32_t c_len[259];      // length L: (revcode | extra << cbits) | total bits << 16
+__constant__ uint32_t c_dist_rev[32];  // 5-bit reversed distance codes
+__constant__ uint32_t c_crc[4][256];   // slice-by-4 CRC-32 tables
+__constant__ uint32_t c_x8pow[64];     // x^(8 * 2^k) mod P, reflected
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ld32u(const uint8_t* p) { return *(const u32_ua*)p; }
+__device__ __forceinline__ int64_t min64(int64_t a, int64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ int64_t max64(int64_t a, int64_t b) { return a > b ? a : b; }
+
+// Byte-wise a - b (mod 256 in every byte), no borrow between bytes.
+__device__ __forceinline__ uint32_t sub8(uint32_t x, uint32_t y)
+{
+    return ((x | 0x80808080u) - (y & 0x7F7F7F7Fu)) ^ ((x ^ ~y) & 0x80808080u);
+}
+// Byte-wise floor((a + b) / 2).
+__device__ __forceinline__ uint32_t avg8(uint32_t a, uint32_t b)
+{
+    return (a & b) + (((a ^ b) >> 1) & 0x7F7F7F7Fu);
+}
+// Sum over the 4 bytes of |int8(v)| = min(v, 256 - v), added to acc.
+__device__ __forceinline__ uint32_t cost8(uint32_t v, uint32_t acc)
+{
+    return __builtin_amdgcn_sad_u8(v ^ 0x80808080u, 0x80808080u, acc);
+}
+// Paeth predictor for 4 packed bytes.
+__device__ __forceinline__ uint32_t paeth4(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int ak = (a >> (8 * k)) & 0xFF, bk = (b >> (8 * k)) & 0xFF, ck = (c >> (8 * k)) & 0xFF;
+        const int pa = abs(bk - ck), pb = abs(ak - ck), pc = abs(ak + bk - 2 * ck);
+        const int p = (pa <= pb && pa <= pc) ? ak : (pb <= pc ? bk : ck);
+        r |= (uint32_t)p << (8 * k);
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// float -> int32 conversion with x86 cvttss2si semantics (out of range or
+// NaN gives INT32_MIN), as the reference's host-side LUT builder executes it.
+__device__ __forceinline__ int32_t x86_f2i(float f)
+{
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : (int32_t)0x80000000;
+}
+
+// ---------------------------------------------------------------------------
+// DICOM pixel arithmetic (a2: gpu_filter.cu:71-162; a4: batch_processor.cpp:100-147)
+// ---------------------------------------------------------------------------
+// Value of a post-shift/mask stored code (bits_allocated 16) or raw byte (8).
+__device__ float dicom_code_value(uint32_t code, const r2p_dicom_params& p)
+{
+    float val;
+    if (p.bits_allocated == 16) {
+        if (p.pixel_rep == 1) {
+            int32_t sv;
+            if (p.bits_stored == 16) {
+                sv = (int32_t)(int16_t)(uint16_t)code;
+            } else {
+                const int half = 1 << (p.bits_stored - 1);
+                sv = (int32_t)code;
+                if (sv >= half) sv -= (1 << p.bits_stored);
+            }
+            if (p.apply_rescale) {
+                const float m = (float)sv * p.rescale_slope;
+                val = m + p.rescale_intercept;
+            } else {
+                val = (float)sv;
+            }
+        } else {
+            val = (float)code;
+            if (p.apply_rescale) {
+                const float m = val * p.rescale_slope;
+                val = m + p.rescale_intercept;
+            }
+        }
+    } else {
+        val = p.pixel_rep == 1 ? (float)(int8_t)(uint8_t)code : (float)(uint8_t)code;
+        if (p.apply_rescale) {
+            const float m = val * p.rescale_slope;
+            val = m + p.rescale_intercept;
+        }
+    }
+    return val;
+}
+
+__device__ uint32_t dicom_window(float val, const r2p_dicom_params& p)
+{
+    const int out_max = (p.bits_allocated == 16) ? 65535 : 255;
+    if (p.apply_window) {
+        const float wm1 = p.window_width - 1.0f;
+        const float cen = p.window_center;
+        if (wm1 <= 0.0f) {
+            val = (val >= cen) ? (float)out_max : 0.0f;
+        } else {
+            const float half = wm1 * 0.5f;
+            const float lo = (cen - half) - 0.5f;
+            const float hi = (cen + half) - 0.5f;
+            if (val <= lo) {
+                val = 0.0f;
+            } else if (val > hi) {
+                val = (float)out_max;
+            } else {
+                float t = val - cen;
+                t = t + 0.5f;
+                t = t / wm1;
+                t = t + 0.5f;
+                val = t * (float)out_max;
+            }
+        }
+    }
+    val = fmaxf(0.f, fminf((float)out_max, val));
+    return (uint32_t)val;
+}
+
+__device__ uint32_t lut8_entry(int i, const r2p_dicom_params& p, int lut_size)
+{
+    const int mask = lut_size - 1;
+    float norm_scale = 1.f, norm_offset = 0.f;
+    if (!p.apply_window) {
+        if (p.pixel_rep == 1) {
+            const float half = (float)(lut_size >> 1);
+            const float range = (float)lut_size;
+            norm_scale = 1.f / range;
+            norm_offset = half / range;
+        } else {
+            norm_scale = 1.f / (float)(lut_size - 1);
+        }
+    }
+    float lo = 0.f, inv_range = 1.f;
+    if (p.apply_window) {
+        lo = p.window_center - p.window_width * 0.5f;
+        const float hi = p.window_center + p.window_width * 0.5f;
+        inv_range = (hi > lo) ? 1.f / (hi - lo) : 0.f;
+    }
+    float val;
+    if (p.pixel_rep == 1 && p.bits_stored < 16) {
+        const int sign_bit = lut_size >> 1;
+        int32_t s = i;
+        if (s & sign_bit) s |= ~mask;
+        val = (float)s;
+    } else if (p.pixel_rep == 1) {
+        val = (float)(int16_t)(uint16_t)(unsigned)i;
+    } else {
+        val = (float)(unsigned)i;
+    }
+    if (p.apply_rescale) {
+        const float m = val * p.rescale_slope;
+        val = m + p.rescale_intercept;
+    }
+    float norm;
+    if (p.apply_window) {
+        norm = (val - lo) * inv_range;
+    } else {
+        const float m = val * norm_scale;
+        norm = m + norm_offset;
+    }
+    const float s255 = norm * 255.f;
+    const int v = x86_f2i(s255 + 0.5f);
+    return (uint32_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+}
+
+__global__ void k_build_lut(LutBuild lb, void* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lb.entries) return;
+    if (lb.kind == 0) {
+        const uint32_t v = dicom_window(dicom_code_value((uint32_t)i, lb.p), lb.p);
+        if (lb.p.bits_allocated == 16) ((uint16_t*)out)[i] = (uint16_t)v;
+        else                           ((uint8_t*)out)[i] = (uint8_t)v;
+    } else {
+        ((uint8_t*)out)[i] = (uint8_t)lut8_entry(i, lb.p, lb.entries);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_filter: fused window + adaptive filter
+//
+// A workgroup owns a band of consecutive rows.  For each row it windows the
+// raw input into a PNG-ready row in LDS (the previous row stays resident in
+// the other LDS buffer), scores the five PNG filters on 16-byte items with
+// SWAR byte arithmetic (v_sad_u8 for the |int8| cost), reduces the scores
+// across the workgroup, and writes [filter byte | chosen row].  Ties go to
+// the lowest filter index, as filter_select_kernel (gpu_filter.cu:267-273).
+// ---------------------------------------------------------------------------
+
+
+// Window one raw row into LDS as PNG-ready bytes at row8[0..row_bytes).
+__device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* row8,
+                           const uint8_t* lut_lds)
+{
+    const int rb = J.row_bytes;
+    const int nfull = rb >> 4;
+    uint32_t* row32 = (uint32_t*)row8;
+    for (int i = threadIdx.x; i < nfull; i += blockDim.x) {
+        u32x4 o;
+        switch (J.wmode) {
+        case WM_COPY: {
+            o = *(const u32x4_ua*)(src_row + 16 * i);
+            break;
+        }
+        case WM_SWAP16: {
+            const u32x4 v = *(const u32x4_ua*)(src_row + 16 * i);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = ((v[k] & 0x00FF00FFu) << 8) | ((v[k] >> 8) & 0x00FF00FFu);
+            break;
+        }
+        case WM_LUT16: {
+            const u32x4 v = *(const u32x4_ua*)(src_row + 16 * i);
+            const uint16_t* t16 = (const uint16_t*)lut_lds;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t s0 = t16[((v[k] & 0xFFFFu) >> J.shift) & J.mask];
+                const uint32_t s1 = t16[((v[k] >> 16) >> J.shift) & J.mask];
+                const uint32_t w = s0 | (s1 << 16);
+                o[k] = ((w & 0x00FF00FFu) << 8) | ((w >> 8) & 0x00FF00FFu);
+            }
+            break;
+        }
+        case WM_LUT8_FROM8: {
+            const u32x4 v = *(const u32x4_ua*)(src_row + 16 * i);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                o[k] = (uint32_t)lut_lds[v[k] & 0xFF] | ((uint32_t)lut_lds[(v[k] >> 8) & 0xFF] << 8) |
+                       ((uint32_t)lut_lds[(v[k] >> 16) & 0xFF] << 16) | ((uint32_t)lut_lds[v[k] >> 24] << 24);
+            }
+            break;
+        }
+        case WM_Q15: {
+            const u32x4 v0 = *(const u32x4_ua*)(src_row + 32 * i);
+            const u32x4 v1 = *(const u32x4_ua*)(src_row + 32 * i + 16);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
+                const uint32_t w1 = k < 2 ? v0[2 * k + 1] : v1[2 * k - 3];
+                uint32_t r = 0;
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
+                    const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
+                    const int32_t q = (int32_t)((uint32_t)x * (uint32_t)J.q15_a + (uint32_t)J.q15_b) >> 15;
+                    r |= (uint32_t)min(max(q, 0), 255) << (8 * h);
+                }
+                o[k] = r;
+            }
+            break;
+        }
+        default: {  // WM_LUT8_FROM16
+            const u32x4 v0 = *(const u32x4_ua*)(src_row + 32 * i);
+            const u32x4 v1 = *(const u32x4_ua*)(src_row + 32 * i + 16);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
+                const uint32_t w1 = k < 2 ? v0[2 * k + 1] : v1[2 * k - 3];
+                uint32_t r = 0;
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
+                    r |= (uint32_t)lut_lds[(raw >> J.shift) & J.mask] << (8 * h);
+                }
+                o[k] = r;
+            }
+            break;
+        }
+        }
+        *(u32x4*)(row32 + 4 * i) = o;
+    }
+    // Tail bytes (row_bytes not a multiple of 16): one thread per output byte.
+    const int tail0 = nfull << 4;
+    for (int j = tail0 + (int)threadIdx.x; j < rb; j += blockDim.x) {
+        uint32_t v;
+        switch (J.wmode) {
+        case WM_COPY: v = src_row[j]; break;
+        case WM_SWAP16: v = src_row[j ^ 1]; break;
+        case WM_LUT16: {
+            const int s = j >> 1;
+            const uint32_t raw = (uint32_t)src_row[2 * s] | ((uint32_t)src_row[2 * s + 1] << 8);
+            const uint32_t w = ((const uint16_t*)lut_lds)[(raw >> J.shift) & J.mask];
+            v = (j & 1) ? (w & 0xFF) : (w >> 8);
+            break;
+        }
+        case WM_LUT8_FROM8: v = lut_lds[src_row[j]]; break;
+        case WM_Q15: {
+            const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
+            const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
+            const int32_t q = (int32_t)((uint32_t)x * (uint32_t)J.q15_a + (uint32_t)J.q15_b) >> 15;
+            v = (uint32_t)min(max(q, 0), 255);
+            break;
+        }
+        default: {
+            const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
+            v = lut_lds[(raw >> J.shift) & J.mask];
+            break;
+        }
+        }
+        row8[j] = (uint8_t)v;
+    }
+}
+
+// The 6 dwords around item i: d[0..1] = 8 bytes before, d[2..5] = the item.
+__device__ __forceinline__ void load_item(const uint32_t* row32, int i, uint32_t d[6])
+{
+    const u32x2 lo = *(const u32x2*)(row32 + 4 * i - 2);
+    const u32x4 hi = *(const u32x4*)(row32 + 4 * i);
+    d[0] = lo[0]; d[1] = lo[1]; d[2] = hi[0]; d[3] = hi[1]; d[4] = hi[2]; d[5] = hi[3];
+}
+
+// Left neighbours (distance bpp) of the 4 dwords of an item.
+__device__ __forceinline__ void left4(const uint32_t d[6], int near, uint32_t sh, uint32_t a[4])
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        a[k] = near ? __builtin_amdgcn_alignbyte(d[2 + k], d[1 + k], sh)
+                    : __builtin_amdgcn_alignbyte(d[1 + k], d[k], sh);
+}
+
+__device__ __forceinline__ uint32_t byte_mask(int nvalid_in_dword)
+{
+    return nvalid_in_dword >= 4 ? 0xFFFFFFFFu : nvalid_in_dword <= 0 ? 0u : ((1u << (8 * nvalid_in_dword)) - 1u);
+}
+
+__global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
+{
+    extern __shared__ u32x4 smem_filter[];
+    uint8_t* lut_lds = (uint8_t*)smem_filter;
+    uint8_t* bufs = lut_lds + L.lut_cap;
+    const int rbuf = L.row_cap + 32;                      // 16 B front pad + row + 16 B slack
+    uint32_t* red = (uint32_t*)(bufs + 2 * rbuf);         // 5 scores x 4 waves
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+    int cur_job = -1;
+    for (uint32_t g = blockIdx.x; g < L.total_bands; g += gridDim.x) {
+        int j = 0;
+        if (L.jobs) {
+            int lo = 0, hi = L.njobs - 1;
+            while (lo < hi) {                              // last job with band_start <= g
+                const int mid = (lo + hi + 1) >> 1;
+                if (L.band_start[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+        }
+        const FilterJob& J = L.jobs ? L.jobs[j] : L.single;
+        const int band = (int)(g - (L.jobs ? L.band_start[j] : 0u));
+        const int r0 = band * L.band_rows;
+        const int r1 = min(J.rows, r0 + L.band_rows);
+        const int rb = J.row_bytes;
+
+        __syncthreads();
+        if (j != cur_job) {
+            if (J.lut_bytes > 0) {
+                const int n16 = (J.lut_bytes + 15) >> 4;
+                for (int i = tid; i < n16; i += blockDim.x)
+                    ((u32x4*)lut_lds)[i] = ((const u32x4*)J.lut)[i];
+            }
+            cur_job = j;
+        }
+        // zero the front pads of both row buffers
+        if (tid < 8) ((uint32_t*)(bufs + (tid >> 2) * rbuf))[tid & 3] = 0;
+        __syncthreads();
+
+        // previous row of r0 -> buffer (r0 + 1) & 1
+        {
+            uint8_t* prev = bufs + ((r0 + 1) & 1) * rbuf + 16;
+            if (r0 > 0) {
+                window_row(J, J.src + (int64_t)(r0 - 1) * J.src_pitch, prev, lut_lds);
+            } else if (J.prior) {
+                for (int i = tid; i < rb; i += blockDim.x) prev[i] = J.prior[i];
+            } else {
+                for (int i = tid; i < ((rb + 3) >> 2); i += blockDim.x) ((uint32_t*)prev)[i] = 0;
+            }
+        }
+
+        const int nitems = (rb + 15) >> 4;
+        const int near = J.bpp <= 4;                      // neighbour within 1 dword back
+        const uint32_t sh = (uint32_t)((4 - (J.bpp & 3)) & 3);
+        const int kmax = (nitems + (int)blockDim.x - 1) / (int)blockDim.x;
+
+        for (int r = r0; r < r1; r++) {
+            uint8_t* cur8 = bufs + (r & 1) * rbuf + 16;
+            const uint32_t* cur = (const uint32_t*)cur8;
+            const uint32_t* prv = (const uint32_t*)(bufs + ((r + 1) & 1) * rbuf + 16);
+            window_row(J, J.src + (int64_t)r * J.src_pitch, cur8, lut_lds);
+            __syncthreads();
+
+            uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+            uint32_t f4c[kFilterMaxK][4];
+#pragma unroll
+            for (int k = 0; k < kFilterMaxK; k++) {
+                const int i = tid + k * (int)blockDim.x;
+                if (k < kmax && i < nitems) {
+                    uint32_t dc[6], dp[6], a[4], c[4];
+                    load_item(cur, i, dc);
+                    load_item(prv, i, dp);
+                    left4(dc, near, sh, a);
+                    left4(dp, near, sh, c);
+                    const int nv = min(16, rb - 16 * i);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t m = byte_mask(nv - 4 * q);
+                        const uint32_t x = dc[2 + q], b = dp[2 + q];
+                        const uint32_t f4 = sub8(x, paeth4(a[q], b, c[q]));
+                        f4c[k][q] = f4;
+                        s0 = cost8(x & m, s0);
+                        s1 = cost8(sub8(x, a[q]) & m, s1);
+                        s2 = cost8(sub8(x, b) & m, s2);
+                        s3 = cost8(sub8(x, avg8(a[q], b)) & m, s3);
+                        s4 = cost8(f4 & m, s4);
+                    }
+                }
+            }
+            s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3); s4 = wave_sum(s4);
+            if (lane == 0) {
+                red[wave * 5 + 0] = s0; red[wave * 5 + 1] = s1; red[wave * 5 + 2] = s2;
+                red[wave * 5 + 3] = s3; red[wave * 5 + 4] = s4;
+            }
+            __syncthreads();
+            uint32_t sc[5] = {0, 0, 0, 0, 0};
+            const int nw = (int)blockDim.x >> 6;
+            for (int w = 0; w < nw; w++)
+#pragma unroll
+                for (int f = 0; f < 5; f++) sc[f] += red[w * 5 + f];
+            int best = 0; uint32_t bsc = sc[0];
+#pragma unroll
+            for (int f = 1; f < 5; f++) if (sc[f] < bsc) { bsc = sc[f]; best = f; }
+
+            uint8_t* dst = J.dst + (int64_t)r * (rb + 1);
+            if (tid == 0) dst[0] = (uint8_t)best;
+#pragma unroll
+            for (int k = 0; k < kFilterMaxK; k++) {
+                const int i = tid + k * (int)blockDim.x;
+                if (k < kmax && i < nitems) {
+                    uint32_t o[4];
+                    if (best == 4) {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) o[q] = f4c[k][q];
+                    } else {
+                        uint32_t dc[6], dp[6], a[4];
+                        load_item(cur, i, dc);
+                        load_item(prv, i, dp);
+                        left4(dc, near, sh, a);
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const uint32_t x = dc[2 + q], b = dp[2 + q];
+                            o[q] = best == 0 ? x : best == 1 ? sub8(x, a[q])
+                                 : best == 2 ? sub8(x, b) : sub8(x, avg8(a[q], b));
+                        }
+                    }
+                    uint8_t* p = dst + 1 + 16 * i;
+                    const int nv = min(16, rb - 16 * i);
+                    if (nv == 16) {
+                        *(u32x4_ua*)p = (u32x4){o[0], o[1], o[2], o[3]};
+                    } else {
+                        for (int t = 0; t < nv; t++) p[t] = (uint8_t)(o[t >> 2] >> (8 * (t & 3)));
+                    }
+                }
+            }
+            if (J.last_row && r == J.rows - 1)
+                for (int i = tid; i < rb; i += blockDim.x) J.last_row[i] = cur8[i];
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adler-32 of a byte range, computed by a whole workgroup.
+// Per 16-byte item at offset o: S += sum(x), T += o*sum(x) + sum(j*x_j);
+// then A = 1 + S, B = n + n*S - T  (mod 65521), B = n + sum((n-i) x_i).
+// Returns the adler value on every thread.  `red` needs 4*nwaves u32.
+// ---------------------------------------------------------------------------
+__device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    uint32_t S = 0;
+    uint64_t T = 0;
+    const int64_t nfull = n >> 4;
+    for (int64_t i = tid; i < nfull; i += blockDim.x) {
+        const u32x4 v = *(const u32x4_ua*)(p + 16 * i);
+        uint32_t s16 = 0, w16 = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t sk = __builtin_amdgcn_udot4(v[k], 0x01010101u, 0u, false);
+            w16 = __builtin_amdgcn_udot4(v[k], 0x03020100u, w16, false) + (uint32_t)(4 * k) * sk;
+            s16 += sk;
+        }
+        S += s16;
+        T += (uint64_t)(16 * i) * s16 + w16;
+    }
+    for (int64_t j = (nfull << 4) + tid; j < n; j += blockDim.x) {
+        S += p[j];
+        T += (uint64_t)j * p[j];
+    }
+    S = wave_sum(S);
+    T = wave_sum64(T);
+    __syncthreads();
+    if (lane == 0) { red[wave * 3] = S; red[wave * 3 + 1] = (uint32_t)T; red[wave * 3 + 2] = (uint32_t)(T >> 32); }
+    __syncthreads();
+    uint64_t St = 0, Tt = 0;
+    for (int w = 0; w < nw; w++) { St += red[w * 3]; Tt += (uint64_t)red[w * 3 + 1] | ((uint64_t)red[w * 3 + 2] << 32); }
+    const uint64_t M = 65521;
+    const uint64_t A = (1 + St) % M;
+    const uint64_t nn = (uint64_t)n;
+    const uint64_t W = ((nn % M) * (St % M) + M * M - (Tt % M)) % M;   // (n*S - T) mod M
+    const uint64_t B = (nn % M + W) % M;
+    return (uint32_t)((B << 16) | A);
+}
+
+__device__ __forceinline__ uint32_t adler_combine_dev(uint32_t a1, uint32_t a2, uint64_t len2)
+{
+    const uint64_t M = 65521;
+    const uint64_t A1 = a1 & 0xFFFF, B1 = a1 >> 16, A2 = a2 & 0xFFFF, B2 = a2 >> 16;
+    const uint64_t n = len2 % M;
+    const uint64_t A = (A1 + A2 + M - 1) % M;
+    const uint64_t B = (B1 + B2 + n * ((A1 + M - 1) % M)) % M;
+    return (uint32_t)((B << 16) | A);
+}
+
+// ---------------------------------------------------------------------------
+// CRC-32 helpers (reflected polynomial 0xEDB88320)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t gf2_mulmod(uint32_t a, uint32_t b)
+{
+    uint32_t prod = 0;
+    for (int k = 0; k < 32; k++) {
+        if (a & (0x80000000u >> k)) prod ^= b;
+        b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+    }
+    return prod;
+}
+__device__ __forceinline__ uint32_t gf2_x8n(uint64_t n)
+{
+    uint32_t r = 0x80000000u;
+    for (int k = 0; n; k++, n >>= 1)
+        if (n & 1) r = gf2_mulmod(r, c_x8pow[k]);
+    return r;
+}
+__device__ __forceinline__ uint32_t crc_combine_dev(uint32_t c1, uint32_t c2, uint64_t len2)
+{
+    return len2 ? gf2_mulmod(gf2_x8n(len2), c1) ^ c2 : c1;
+}
+// crc32 update over n bytes from generic memory, slice-by-4 tables in LDS.
+__device__ uint32_t crc_bytes(uint32_t crc, const uint8_t* p, int64_t n, const uint32_t (*T)[256])
+{
+    uint32_t c = ~crc;
+    int64_t i = 0;
+    for (; i + 4 <= n; i += 4) {
+        c ^= ld32u(p + i);
+        c = T[3][c & 0xFF] ^ T[2][(c >> 8) & 0xFF] ^ T[1][(c >> 16) & 0xFF] ^ T[0][c >> 24];
+    }
+    for (; i < n; i++) c = T[0][(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return ~c;
+}
+
+// CRC of [p, p+n) by a whole workgroup: contiguous per-thread ranges, then a
+// tree combine in LDS.  `tab` = slice-by-4 tables in LDS; `red` needs
+// 2*blockDim u32.  Result on every thread.
+__device__ uint32_t block_crc(const uint8_t* p, int64_t n, const uint32_t (*tab)[256], uint32_t* red)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int64_t per = (((n + nt - 1) / nt) + 3) & ~(int64_t)3;   // dword-aligned ranges
+    const int64_t b = min64(n, per * tid), e = min64(n, b + per);
+    uint32_t c = crc_bytes(0, p + b, e - b, tab);
+    uint32_t len = (uint32_t)(e - b);
+    __syncthreads();
+    red[tid] = c; red[nt + tid] = len;
+    __syncthreads();
+    for (int s = 1; s < nt; s <<= 1) {
+        if ((tid & (2 * s - 1)) == 0 && tid + s < nt) {
+            const uint32_t l2 = red[nt + tid + s];
+            red[tid] = crc_combine_dev(red[tid], red[tid + s], l2);
+            red[nt + tid] += l2;
+        }
+        __syncthreads();
+    }
+    const uint32_t r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ void load_crc_tables(uint32_t (*tab)[256])
+{
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) (&tab[0][0])[i] = (&c_crc[0][0])[i];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Standalone checksums: partial per 64 KiB block, then one combine block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adler_partial(const uint8_t* p, int64_t n, uint32_t* part)
+{
+    __shared__ uint32_t red[16];
+    const int64_t b = (int64_t)blockIdx.x * kChecksumBlock;
+    const int64_t len = min64(kChecksumBlock, n - b);
+    const uint32_t a = block_adler(p + b, len, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n, uint32_t* part)
+{
+    __shared__ uint32_t tab[4][256];
+    __shared__ uint32_t red[512];
+    load_crc_tables(tab);
+    const int64_t b = (int64_t)blockIdx.x * kChecksumBlock;
+    const int64_t len = min64(kChecksumBlock, n - b);
+    const uint32_t c = block_crc(p + b, len, tab, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = c;
+}
+
+// Combine nb partials in order (each block kChecksumBlock bytes but the last).
+// kind 0 = adler, 1 = crc.  One workgroup: per-thread runs, then a tree.
+__global__ __launch_bounds__(256) void k_checksum_final(const uint32_t* part, int nb, int64_t n, int kind,
+                                                         uint32_t* result)
+{
+    __shared__ uint32_t val[256];
+    __shared__ uint64_t len[256];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int per = (nb + nt - 1) / nt;
+    const int b = min(nb, per * tid), e = min(nb, b + per);
+    uint32_t v = kind ? 0u : 1u;
+    uint64_t l = 0;
+    for (int i = b; i < e; i++) {
+        const uint64_t li = (uint64_t)min64(kChecksumBlock, n - (int64_t)i * kChecksumBlock);
+        if (l == 0) v = part[i];
+        else v = kind ? crc_combine_dev(v, part[i], li) : adler_combine_dev(v, part[i], li);
+        l += li;
+    }
+    val[tid] = v; len[tid] = l;
+    __syncthreads();
+    for (int s = 1; s < nt; s <<= 1) {
+        if ((tid & (2 * s - 1)) == 0 && tid + s < nt) {
+            const uint64_t l2 = len[tid + s];
+            if (l2) {
+                if (len[tid] == 0) val[tid] = val[tid + s];
+                else val[tid] = kind ? crc_combine_dev(val[tid], val[tid + s], l2)
+                                     : adler_combine_dev(val[tid], val[tid + s], l2);
+                len[tid] += l2;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *result = (n == 0) ? (kind ? 0u : 1u) : val[0];
+}
+
+// ---------------------------------------------------------------------------
+// k_deflate: one workgroup per 64 KiB segment.
+//
+// Segment layout on the wire: [BFINAL|BTYPE=01 fixed-Huffman block][EOB]
+// then, unless it is the stream's final segment, an empty stored block
+// (sync flush: 3 header bits, pad to a byte, 00 00 FF FF).  Every segment is
+// therefore a byte-aligned, independently decodable piece whose history
+// (the 32 KiB before it) is already in memory, so all segments of all
+// images run in parallel and are joined by a prefix sum of their lengths.
+//
+// Inside a segment, sub-blocks of 256*LANE positions:
+//   1. match phase (rounds of 1024 positions): every position looks up a
+//      4-byte hash table in LDS (most recent earlier position, atomicMax
+//      inserts) plus the run / pixel / row-above distances, and keeps the
+//      longest match capped at its parse lane's end;
+//   2. parse: lane t walks its LANE positions greedily (token bitmask);
+//   3. block exclusive scan of per-lane bit counts;
+//   4. emission: each lane packs its tokens into 32-bit words at its bit
+//      offset (interior words plain stores, the two shared edge words with
+//      LDS atomicOr), and full words stream out to global memory.
+// ---------------------------------------------------------------------------
+
+
+constexpr int kHashBits = 13;
+constexpr int kRound = 1024;
+
+__device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
+
+__device__ __forceinline__ int match_len(const uint8_t* d, int64_t P, int64_t C, int cap)
+{
+    int n = 0;
+    while (n + 4 <= cap) {
+        const uint32_t x = ld32u(d + P + n) ^ ld32u(d + C + n);
+        if (x) return n + ((__ffs(x) - 1) >> 3);
+        n += 4;
+    }
+    while (n < cap && d[P + n] == d[C + n]) n++;
+    return n;
+}
+
+__device__ __forceinline__ void dist_code(uint32_t dist, uint32_t& bits, uint32_t& nbits)
+{
+    const uint32_t dm = dist - 1;
+    if (dm < 4) { bits = c_dist_rev[dm]; nbits = 5; return; }
+    const uint32_t n = 31 - __clz(dm);
+    const uint32_t eb = n - 1;
+    const uint32_t code = 2 * n + ((dm >> eb) & 1);
+    bits = c_dist_rev[code] | ((dm & ((1u << eb) - 1)) << 5);
+    nbits = 5 + eb;
+}
+
+__device__ __forceinline__ uint32_t match_bits(uint32_t len, uint32_t dist)
+{
+    uint32_t db, dn;
+    dist_code(dist, db, dn);
+    return (c_len[len] >> 16) + dn;
+}
+
+template <int LANE>
+__global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
+{
+    constexpr int NT = kDeflateThreads;
+    constexpr int SUB = NT * LANE;
+    constexpr int OBUF = (SUB * 9 + 31) / 32 + 8;
+    __shared__ uint32_t htab[1 << kHashBits];
+    __shared__ uint32_t M[SUB];
+    __shared__ uint32_t obuf[OBUF];
+    __shared__ uint32_t red[64];
+    __shared__ uint32_t st[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    for (uint32_t g = blockIdx.x; g < DL.total_segs; g += gridDim.x) {
+        int j = 0;
+        if (DL.jobs) {
+            int lo = 0, hi = DL.njobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (DL.seg_start[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+        }
+        const DeflateJob& J = DL.jobs ? DL.jobs[j] : DL.single;
+        const int seg = (int)(g - (DL.jobs ? DL.seg_start[j] : 0u));
+        const uint8_t* d = J.data;
+        const int64_t s0 = J.first + (int64_t)seg * J.seg_bytes;
+        const int64_t s1 = min64(s0 + J.seg_bytes, J.first + J.len);
+        const int64_t dend = J.first + J.len;
+        const bool last = (seg == J.nseg - 1) && J.final_block;
+        const int64_t wlo = max64(J.dict_lo, s0 - (int64_t)kWindow);
+        uint32_t* out = J.seg_out + (int64_t)seg * J.seg_cap_dw;
+
+        __syncthreads();
+        for (int i = tid; i < (1 << kHashBits); i += NT) htab[i] = 0;
+        if (tid < 4) st[tid] = 0;
+        __syncthreads();
+        if (J.lz77)
+            for (int64_t P = wlo + tid; P < s0; P += NT)
+                if (P + 4 <= dend) atomicMax(&htab[hash4(ld32u(d + P))], (uint32_t)(P - wlo + 1));
+
+        uint32_t carry = last ? 3u : 2u;   // BFINAL | BTYPE=01 << 1
+        uint32_t carry_bits = 3;
+        int64_t out_dw = 0;
+
+        for (int64_t q0 = s0; q0 < s1; q0 += SUB) {
+            const int n = (int)min64(SUB, s1 - q0);
+            __syncthreads();                       // previous flush done with obuf
+            for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;
+            // ---- 1. match phase -------------------------------------------------
+            for (int r0 = 0; r0 < n; r0 += kRound) {
+                __syncthreads();                   // inserts of the previous round visible
+                const int r1 = min(n, r0 + kRound);
+                for (int pr = r0 + tid; pr < r1; pr += NT) {
+                    const int64_t P = q0 + pr;
+                    const int lane_end = min(n, (pr / LANE + 1) * LANE);
+                    const int cap = min(258, lane_end - pr);
+                    uint32_t best_len = 0, best_dist = 0;
+                    if (J.lz77 && cap >= 3 && P + 4 <= dend) {
+                        const uint32_t w = ld32u(d + P);
+                        const uint32_t hv = htab[hash4(w)];
+                        int64_t cands[4];
+                        cands[0] = hv ? (int64_t)hv - 1 + wlo : -1;
+                        cands[1] = P - 1;
+                        cands[2] = J.bpp > 1 ? P - J.bpp : -1;
+                        cands[3] = J.row_stride > 0 ? P - J.row_stride : -1;
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const int64_t C = cands[c];
+                            if (C < wlo || C >= P || P - C > kWindow) continue;
+                            const int l = match_len(d, P, C, cap);
+                            const uint32_t dist = (uint32_t)(P - C);
+                            if ((uint32_t)l > best_len || ((uint32_t)l == best_len && dist < best_dist)) {
+                                best_len = (uint32_t)l; best_dist = dist;
+                            }
+                        }
+                        if (best_len < 3 || match_bits(best_len, best_dist) > 8 * best_len) best_len = 0;
+                    }
+                    M[pr] = (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | (uint32_t)d[P];
+                }
+                __syncthreads();                   // lookups of this round done
+                if (J.lz77)
+                    for (int pr = r0 + tid; pr < r1; pr += NT) {
+                        const int64_t P = q0 + pr;
+                        if (P + 4 <= dend) atomicMax(&htab[hash4(ld32u(d + P))], (uint32_t)(P - wlo + 1));
+                    }
+            }
+            __syncthreads();
+            if (tid == 0) obuf[0] = carry;
+            // ---- 2. parse ----------------------------------------------------------
+            const int lb = tid * LANE;
+            const int le = min(n, lb + LANE);
+            uint32_t tok = 0, nbits = 0;
+            uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
+            for (int p = lb; p < le;) {
+                const uint32_t m = M[p];
+                const uint32_t len = m >> 23;
+                tok |= 1u << (p - lb);
+                if (len >= 3) {
+                    const uint32_t dist = ((m >> 8) & 0x7FFFu) + 1;
+                    nbits += match_bits(len, dist);
+                    m_cnt++; m_bytes += len; m_dist += dist; m_max = max(m_max, len);
+                    p += (int)len;
+                } else {
+                    nbits += c_lit[m & 0xFF] >> 16;
+                    p += 1;
+                }
+            }
+            // ---- 3. scan of bit counts ------------------------------------------------
+            const uint32_t incl = wave_incl_scan(nbits);
+            if (lane == 63) red[wave] = incl;
+            if (J.seg_stats) {
+                atomicAdd(&st[0], m_cnt); atomicAdd(&st[1], m_bytes);
+                atomicAdd(&st[2], m_dist); atomicMax(&st[3], m_max);
+            }
+            __syncthreads();
+            uint32_t wbase = 0, total = 0;
+            for (int w = 0; w < NT / 64; w++) { if (w < wave) wbase += red[w]; total += red[w]; }
+            const uint32_t off = carry_bits + wbase + incl - nbits;
+            // ---- 4. emission -----------------------------------------------------------
+            {
+                uint32_t w = off >> 5;
+                uint32_t nacc = off & 31;
+                uint64_t acc = 0;
+                bool first = true;
+                uint32_t tk = tok;
+                while (tk) {
+                    const int i = __ffs(tk) - 1;
+                    tk &= tk - 1;
+                    const uint32_t m = M[lb + i];
+                    const uint32_t len = m >> 23;
+                    uint32_t v, nb;
+                    if (len >= 3) {
+                        const uint32_t dist = ((m >> 8) & 0x7FFFu) + 1;
+                        const uint32_t lc = c_len[len];
+                        uint32_t db, dn;
+                        dist_code(dist, db, dn);
+                        const uint32_t ln = lc >> 16;
+                        v = (lc & 0xFFFFu) | (db << ln);
+                        nb = ln + dn;
+                    } else {
+                        const uint32_t lc = c_lit[m & 0xFF];
+                        v = lc & 0xFFFFu;
+                        nb = lc >> 16;
+                    }
+                    acc |= (uint64_t)v << nacc;
+                    nacc += nb;
+                    if (nacc >= 32) {
+                        if (first) atomicOr(&obuf[w], (uint32_t)acc); else obuf[w] = (uint32_t)acc;
+                        first = false;
+                        w++;
+                        acc >>= 32;
+                        nacc -= 32;
+                    }
+                }
+                if (nacc > 0) atomicOr(&obuf[w], (uint32_t)acc);
+            }
+            __syncthreads();
+            const uint32_t tb = carry_bits + total;
+            const uint32_t full = tb >> 5;
+            for (uint32_t i = tid; i < full; i += NT) out[out_dw + i] = obuf[i];
+            carry = (tb & 31) ? obuf[full] : 0u;
+            carry_bits = tb & 31;
+            out_dw += full;
+        }
+
+        // ---- segment end: EOB, then sync flush or final pad ------------------------
+        __syncthreads();
+        if (tid == 0) {
+            uint64_t acc = carry;
+            uint32_t nb = carry_bits + 7;                 // EOB = 7 zero bits
+            int64_t o = out_dw;
+            if (!last) nb += 3;                           // stored block header, BFINAL=0
+            nb = (nb + 7) & ~7u;                          // byte align
+            if (nb >= 32) { out[o++] = (uint32_t)acc; acc >>= 32; nb -= 32; }
+            if (!last) {
+                acc |= (uint64_t)0xFFFF0000u << nb;       // LEN = 0000, NLEN = FFFF
+                nb += 32;
+                if (nb >= 32) { out[o++] = (uint32_t)acc; acc >>= 32; nb -= 32; }
+            }
+            if (nb > 0) out[o] = (uint32_t)acc;
+            J.seg_len[seg] = (uint32_t)(o * 4 + nb / 8);
+            if (J.seg_stats) {
+                uint32_t* ss = J.seg_stats + 4 * (int64_t)seg;
+                ss[0] = st[0]; ss[1] = st[1]; ss[2] = st[2]; ss[3] = st[3];
+            }
+        }
+        const uint32_t ad = block_adler(d + s0, s1 - s0, red);
+        if (tid == 0) J.seg_adler[seg] = ad;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scan: per job, exclusive scan of segment lengths, payload size and the
+// zlib trailer's Adler-32 (combine of the per-segment values).
+// ---------------------------------------------------------------------------
+
+
+__device__ __forceinline__ int64_t png_payload(const AssembleJob& A, uint64_t segbytes)
+{
+    return A.mode == 1 ? (int64_t)segbytes + 6 : (int64_t)segbytes;
+}
+
+__global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
+{
+    __shared__ uint64_t wsum[4];
+    __shared__ uint32_t aval[256];
+    __shared__ uint64_t alen[256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const AssembleJob& A = AL.jobs ? AL.jobs[blockIdx.x] : AL.single;
+    uint64_t base = 0;
+    for (int c0 = 0; c0 < A.nseg; c0 += 256) {
+        const int i = c0 + tid;
+        const uint32_t v = i < A.nseg ? A.seg_len[i] : 0u;
+        const uint32_t incl = wave_incl_scan(v);
+        __syncthreads();
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint64_t wb = 0, tot = 0;
+        for (int w = 0; w < 4; w++) { if (w < wave) wb += wsum[w]; tot += wsum[w]; }
+        if (i < A.nseg) A.seg_off[i] = base + wb + incl - v;
+        base += tot;
+    }
+    // adler combine: contiguous runs per thread, then a tree
+    const int per = (A.nseg + 255) / 256;
+    const int b = min(A.nseg, per * tid), e = min(A.nseg, b + per);
+    uint32_t v = 1; uint64_t l = 0;
+    for (int i = b; i < e; i++) {
+        const uint64_t li = (uint64_t)min64(A.seg_bytes, A.total_in - (int64_t)i * A.seg_bytes);
+        v = l ? adler_combine_dev(v, A.seg_adler[i], li) : A.seg_adler[i];
+        l += li;
+    }
+    aval[tid] = v; alen[tid] = l;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {
+        if ((tid & (2 * s - 1)) == 0 && tid + s < 256) {
+            const uint64_t l2 = alen[tid + s];
+            if (l2) {
+                aval[tid] = alen[tid] ? adler_combine_dev(aval[tid], aval[tid + s], l2) : aval[tid + s];
+                alen[tid] += l2;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        A.seg_off[A.nseg] = base;
+        const uint32_t adler = A.total_in ? aval[0] : 1u;
+        if (A.adler_out) *A.adler_out = adler;
+        const int64_t z = png_payload(A, base);
+        if (A.running) {
+            const uint64_t b0 = *A.running;
+            *A.base_slot = b0;
+            *A.running = b0 + (uint64_t)z;
+        }
+        int64_t total = z;
+        if (A.mode == 1) {
+            const int64_t nch = (z + A.chunk_bytes - 1) / A.chunk_bytes;
+            total = 8 + 25 + nch * 12 + z + 12;
+        }
+        *A.out_len = (uint64_t)total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_assemble: one workgroup per output chunk.
+// mode 1 (PNG): chunk c holds zlib payload bytes [c*C, (c+1)*C) of
+//   [78 FLG][segments...][adler BE] as [len BE]["IDAT"][data][crc BE];
+//   chunk 0 also writes the signature and IHDR, the last chunk IEND.
+// mode 0 (raw): chunk c copies payload bytes [c*C, (c+1)*C) to out.
+// ---------------------------------------------------------------------------
+constexpr int kMaxChunk = 65536;
+
+__device__ __forceinline__ void put_be32(uint8_t* p, uint32_t v)
+{
+    p[0] = (uint8_t)(v >> 24); p[1] = (uint8_t)(v >> 16); p[2] = (uint8_t)(v >> 8); p[3] = (uint8_t)v;
+}
+
+__global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL)
+{
+    __shared__ uint8_t buf[kMaxChunk + 8];
+    __shared__ uint32_t tab[4][256];
+    __shared__ uint32_t red[2 * kAssembleThreads];
+    __shared__ int s_seg;
+    const int tid = threadIdx.x;
+    load_crc_tables(tab);
+
+    for (uint32_t g = blockIdx.x; g < AL.total_chunks; g += gridDim.x) {
+        int j = 0;
+        if (AL.jobs) {
+            int lo = 0, hi = AL.njobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (AL.chunk_start[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+        }
+        const AssembleJob& A = AL.jobs ? AL.jobs[j] : AL.single;
+        const int c = (int)(g - (AL.jobs ? AL.chunk_start[j] : 0u));
+        const uint64_t segbytes = A.seg_off[A.nseg];
+        const int64_t z = png_payload(A, segbytes);
+        const int64_t C = A.chunk_bytes;
+        const int64_t nch = (z + C - 1) / C;
+        if (c >= nch) continue;
+        const int64_t z0 = c * C, z1 = min64(z, z0 + C);
+        const int n = (int)(z1 - z0);
+        const int64_t hdr = A.mode == 1 ? 2 : 0;
+        __syncthreads();
+        // header / trailer bytes
+        if (A.mode == 1) {
+            if (tid < 2) {
+                const int64_t zz = tid;
+                if (zz >= z0 && zz < z1) buf[zz - z0] = tid == 0 ? (uint8_t)0x78 : A.zflg;
+            }
+            if (tid < 4) {
+                const int64_t zz = hdr + (int64_t)segbytes + tid;
+                if (zz >= z0 && zz < z1) {
+                    const uint32_t ad = *A.adler_out;
+                    buf[zz - z0] = (uint8_t)(ad >> (24 - 8 * tid));
+                }
+            }
+        }
+        // segment bytes overlapping [z0, z1)
+        const int64_t a0 = max64(z0 - hdr, 0), a1 = min64(z1 - hdr, (int64_t)segbytes);
+        if (a1 > a0) {
+            if (tid == 0) {
+                int lo = 0, hi = A.nseg - 1;           // last segment with seg_off <= a0
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((int64_t)A.seg_off[mid] <= a0) lo = mid; else hi = mid - 1;
+                }
+                s_seg = lo;
+            }
+            __syncthreads();
+            for (int s = s_seg; s < A.nseg && (int64_t)A.seg_off[s] < a1; s++) {
+                const int64_t so = (int64_t)A.seg_off[s], se = (int64_t)A.seg_off[s + 1];
+                const int64_t b = max64(a0, so), e = min64(a1, se);
+                const uint8_t* src = (const uint8_t*)(A.seg_out + (int64_t)s * A.seg_cap_dw) + (b - so);
+                uint8_t* dstb = buf + (b + hdr - z0);
+                for (int64_t i = tid; i < e - b; i += blockDim.x) dstb[i] = src[i];
+            }
+        }
+        __syncthreads();
+        if (A.mode == 1) {
+            uint8_t* o = A.out + 33 + c * (C + 12);
+            if (c == 0 && tid == 0) {
+                const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+                for (int i = 0; i < 8; i++) A.out[i] = sig[i];
+                uint8_t ih[17] = {'I', 'H', 'D', 'R'};
+                put_be32(ih + 4, A.width); put_be32(ih + 8, A.height);
+                ih[12] = A.bit_depth; ih[13] = A.color_type; ih[14] = 0; ih[15] = 0; ih[16] = 0;
+                put_be32(A.out + 8, 13);
+                for (int i = 0; i < 17; i++) A.out[12 + i] = ih[i];
+                put_be32(A.out + 29, crc_bytes(0, ih, 17, tab));
+            }
+            const uint32_t crc_d = block_crc(buf, n, tab, red);
+            if (tid == 0) {
+                const uint8_t idat[4] = {'I', 'D', 'A', 'T'};
+                const uint32_t crc_t = crc_bytes(0, idat, 4, tab);
+                put_be32(o, (uint32_t)n);
+                o[4] = 'I'; o[5] = 'D'; o[6] = 'A'; o[7] = 'T';
+                put_be32(o + 8 + n, crc_combine_dev(crc_t, crc_d, (uint64_t)n));
+                if (c == nch - 1) {
+                    uint8_t* ie = o + 12 + n;
+                    put_be32(ie, 0);
+                    ie[4] = 'I'; ie[5] = 'E'; ie[6] = 'N'; ie[7] = 'D';
+                    put_be32(ie + 8, 0xAE426082u);
+                }
+            }
+            for (int i = tid; i < n; i += blockDim.x) o[8 + i] = buf[i];
+        } else {
+            uint8_t* o = A.out + z0 + (A.base_slot ? *A.base_slot : 0ull);
+            for (int i = tid; i < n; i += blockDim.x) o[i] = buf[i];
+        }
+    }
+}
+
+// Writes up to 16 literal bytes, then (optionally) a big-endian CRC read
+// from device memory right after them.  Used for IDAT chunk framing.
+struct PutBytes { uint8_t b[16]; };
+__global__ void k_put_bytes(uint8_t* dst, PutBytes pb, int n, const uint32_t* crc_src)
+{
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; i++) dst[i] = pb.b[i];
+        if (crc_src) put_be32(dst + n, *crc_src);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch wrappers (called from r2p_capi.cpp)
+// ---------------------------------------------------------------------------
+hipError_t launch_build_lut(const LutBuild& lb, void* out, hipStream_t s)
+{
+    k_build_lut<<<(lb.entries + 255) / 256, 256, 0, s>>>(lb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
+{
+    k_filter<<<grid, kFilterThreads, lds, s>>>(L);
+    return hipGetLastError();
+}
+
+hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hipStream_t s)
+{
+    if (lane_bytes == 16) k_deflate<16><<<grid, kDeflateThreads, 0, s>>>(DL);
+    else                  k_deflate<32><<<grid, kDeflateThreads, 0, s>>>(DL);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
+{
+    k_scan<<<njobs, 256, 0, s>>>(AL);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(const AssembleLaunch& AL, int grid, hipStream_t s)
+{
+    k_assemble<<<grid, kAssembleThreads, 0, s>>>(AL);
+    return hipGetLastError();
+}
+
+hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s)
+{
+    const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
+    if (nb > 0) k_adler_partial<<<nb, 256, 0, s>>>(p, n, part);
+    k_checksum_final<<<1, 256, 0, s>>>(part, nb, n, 0, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s)
+{
+    const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
+    if (nb > 0) k_crc_partial<<<nb, 256, 0, s>>>(p, n, part);
+    k_checksum_final<<<1, 256, 0, s>>>(part, nb, n, 1, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_put_bytes(uint8_t* dst, const uint8_t* bytes16, int n, const uint32_t* crc_src,
+                            hipStream_t s)
+{
+    PutBytes pb;
+    for (int i = 0; i < 16; i++) pb.b[i] = i < n ? bytes16[i] : 0;
+    k_put_bytes<<<1, 64, 0, s>>>(dst, pb, n, crc_src);
+    return hipGetLastError();
+}
+
+hipError_t upload_tables(const uint32_t* lit, const uint32_t* len, const uint32_t* dist_rev,
+                         const uint32_t* crc4, const uint32_t* x8pow)
+{
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_lit), lit, sizeof(uint32_t) * 256)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_len), len, sizeof(uint32_t) * 259)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_dist_rev), dist_rev, sizeof(uint32_t) * 32)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc), crc4, sizeof(uint32_t) * 1024)) != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_x8pow), x8pow, sizeof(uint32_t) * 64);
+}
+
+}  // namespace r2p
