@@ -1,0 +1,187 @@
+"""Benchmark: encoded MP/s of the MI355X PNG encoder on one GPU.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload radiograph]
+                  [--mode plain|batch] [--no-cpu]
+
+A step is one pass of the hot path (window -> filter -> LZ77 + fixed-Huffman
+DEFLATE -> checksums -> IDAT assembly) over one batch of synthetic input that
+is already resident in HBM; the PNG stays in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import statistics
+import sys
+import time
+import zlib
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(REPO, "raw2png-conversionthrugpu_amd")
+METRIC = "encoded MP/s + p50/p99 latency per image, batched vs plain, 1 MI355X, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    name = "raw2png_conversionthrugpu_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
+    """Times the oracle's CPU path (window + filter + chunked zlib) on the same image."""
+    img = s.img
+    mode = {0: oracle.MODE_READY, 1: oracle.MODE_SWAP16, 2: oracle.MODE_DICOM, 3: oracle.MODE_DICOM8}[img.pix_mode]
+    p = oracle.DicomParams(*[getattr(img.dicom, f) for f, _ in img.dicom._fields_])
+    times, png_len = [], 0
+    t_end = time.perf_counter() + budget_s
+    it = 0
+    while it < 3 or (time.perf_counter() < t_end and it < 18):
+        t0 = time.perf_counter()
+        e = oracle.encode_png(s.pixels, img.width, img.height, img.channels, img.in_bits, mode, p,
+                              strip_height=128, level=level, threads=threads, want_filtered=False)
+        dt = time.perf_counter() - t0
+        png_len = len(e.png)
+        if it >= 3:           # 3 warm-ups, then timed iterations (bench_deflate's 3 + 15)
+            times.append(dt)
+        it += 1
+    med = statistics.median(times)
+    return {"value": round(s.megapixels / med, 2), "unit": "MP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(times)} timed encodes (after 3 warm-ups) of the same {img.width}x{img.height} "
+                      f"image, zlib {oracle.zlib_version()} level {level}, strip height 128, "
+                      f"{threads} threads; median",
+            "png_bytes": png_len}
+
+
+def run_plain(args, r2p, synth, torch):
+    s = synth.workload(args.workload)
+    img = s.img
+    opt = r2p.options(lz77=args.lz77)
+    fb = r2p.filtered_bytes(img)
+    enc = r2p.Encoder(fb, opt)
+    d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
+    cap = r2p.png_bound(img, opt)
+    d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    d_filt = torch.empty(fb, dtype=torch.uint8, device="cuda")
+    for _ in range(args.warmup):
+        enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap, d_filt.data_ptr())
+        enc.sync()
+    torch.cuda.synchronize()
+    lat, ft, dt, st, at = [], [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap, d_filt.data_ptr())
+        n, adler, nseg = enc.sync()
+        lat.append(time.perf_counter() - ts)
+        t = enc.timings()
+        ft.append(t.filter_ms); dt.append(t.deflate_ms); st.append(t.scan_ms); at.append(t.assemble_ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # correctness of the timed output: inflate and compare with the filtered scanlines
+    png = d_png[:n].cpu().numpy().tobytes()
+    filt = d_filt.cpu().numpy().tobytes()
+    pos, z = 8, b""
+    while pos < len(png):
+        ln = int.from_bytes(png[pos:pos + 4], "big")
+        if png[pos + 4:pos + 8] == b"IDAT":
+            z += png[pos + 8:pos + 8 + ln]
+        pos += 12 + ln
+    ok = zlib.decompress(z) == filt
+    mp = s.megapixels
+    raw_in = r2p.input_bytes(img)
+    kern = {"filter": statistics.mean(ft), "deflate": statistics.mean(dt),
+            "scan": statistics.mean(st), "assemble": statistics.mean(at)}
+    algo = {"filter": raw_in + fb, "deflate": fb + len(z), "scan": 0, "assemble": 2 * len(z)}
+    dom = max(kern, key=kern.get)
+    res = {
+        "workload": args.workload, "mp": mp, "steps": args.steps, "wall": wall, "lat": lat,
+        "png_bytes": n, "filtered_bytes": fb, "raw_in": raw_in, "zlen": len(z), "ok": ok,
+        "kern_ms": kern, "algo_bytes": algo, "dominant": dom, "nseg": nseg, "s": s,
+    }
+    enc.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="radiograph", choices=["ct_slice", "radiograph", "camera_rgb"])
+    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--lz77", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-level", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this encoder is measured on one GPU only; --gpus must be 1 "
+              "(the path shards by image with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if REPO not in sys.path:
+        sys.path.insert(0, REPO)
+    r2p = load_pkg()
+    import importlib
+    synth = importlib.import_module("raw2png_conversionthrugpu_amd.synth")
+    torch.cuda.set_device(0)
+    log(f"bench: workload={args.workload} steps={args.steps} warmup={args.warmup}")
+    r = run_plain(args, r2p, synth, torch)
+    kern = r["kern_ms"]
+    dom = r["dominant"]
+    achieved = r["algo_bytes"][dom] / (kern[dom] * 1e-3) / 1e9
+    value = r["mp"] * r["steps"] / r["wall"]
+    lat_ms = [x * 1e3 for x in r["lat"]]
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "MP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r["wall"] / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"{args.workload} {r['s'].img.width}x{r['s'].img.height}",
+                   "form": "plain per-image", "segment_bytes": 65536, "lz77": args.lz77,
+                   "bit_exact": r["ok"]},
+        "latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},
+        "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
+        "ratio": {"gpu_png_over_filtered": round(r["png_bytes"] / r["filtered_bytes"], 4)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+    }
+    fk = "filter"
+    fach = r["algo_bytes"][fk] / (kern[fk] * 1e-3) / 1e9
+    out["filter_kernel"] = {"achieved": round(fach, 1), "frac": round(fach / HBM_PEAK_GBS, 4)}
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        cb = cpu_baseline(O, r["s"], args.cpu_level, threads)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["ratio"]["zlib_level%d_png_over_filtered" % args.cpu_level] = round(cb["png_bytes"] / r["filtered_bytes"], 4)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -302,3 +302,67 @@ def idat_chunk(d_slice: int, n: int, is_first: bool, is_last: bool, adler: int, 
     check(lib().r2p_idat_chunk(d_slice or None, n, int(is_first), int(is_last), adler, level, d_chunk,
                                cap, d_work, C.byref(out), stream or None))
     return out.value
+
+
+class Batch:
+    """Batched form (r2p_batch_*): many images per launch sequence / HIP graph."""
+
+    def __init__(self, max_images: int, max_input: int, max_filtered: int, opt: Options | None = None):
+        self.opt = opt or options()
+        h = C.c_void_p()
+        check(lib().r2p_batch_create(max_images, max_input, max_filtered, C.byref(self.opt), C.byref(h)))
+        self.h = h
+        self.n = 0
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            lib().r2p_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, imgs, d_srcs):
+        n = len(imgs)
+        arr = (Image * n)(*imgs)
+        ptrs = (C.c_void_p * n)(*[C.c_void_p(p) for p in d_srcs])
+        self._keep = (arr, ptrs)
+        check(lib().r2p_batch_set(self.h, n, arr, ptrs))
+        self.n = n
+
+    def set_options(self, opt: Options):
+        check(lib().r2p_batch_options(self.h, C.byref(opt)))
+        self.opt = opt
+
+    def run(self, graph: bool = True):
+        check(lib().r2p_batch_run(self.h, int(graph)))
+
+    def sync(self):
+        check(lib().r2p_batch_sync(self.h))
+
+    def result(self, i: int):
+        p, n = C.c_void_p(), C.c_size_t()
+        check(lib().r2p_batch_result(self.h, i, C.byref(p), C.byref(n)))
+        return p.value, n.value
+
+    def copy_result(self, i: int) -> bytes:
+        _, n = self.result(i)
+        out = np.empty(max(n, 1), np.uint8)
+        m = C.c_size_t()
+        check(lib().r2p_batch_copy_result(self.h, i, out.ctypes.data, out.size, C.byref(m)))
+        return out[: m.value].tobytes()
+
+    def timings(self) -> Timings:
+        t = Timings()
+        check(lib().r2p_batch_timings(self.h, C.byref(t)))
+        return t
+
+    def autotune(self):
+        o, ms = Options(), C.c_double()
+        check(lib().r2p_batch_autotune(self.h, C.byref(o), C.byref(ms)))
+        self.opt = o
+        return o, ms.value

@@ -25,6 +25,7 @@ SOURCES = [
     "r2p_strip.cpp",
     "r2p_batch.cpp",
     "facade.cpp",
+    "pipeline.cpp",
 ]
 
 COMMON = [
@@ -57,7 +58,23 @@ def build(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_facade_check()
     return LIB
+
+
+def _build_facade_check() -> None:
+    """tests/cpp/facade_check: C++ driver of the drop-in API (needs oracle/liboracle.so)."""
+    src = os.path.join(REPO, "tests", "cpp", "facade_check.cpp")
+    oracle_dir = os.path.join(REPO, "oracle")
+    if not os.path.exists(src) or not os.path.exists(os.path.join(oracle_dir, "liboracle.so")):
+        return
+    out = os.path.join(REPO, "tests", "cpp", "facade_check")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"), "-I" + oracle_dir, src,
+           "-L" + HERE, "-lr2p", "-L" + oracle_dir, "-loracle", "-lz",
+           "-Wl,-rpath," + HERE, "-Wl,-rpath," + oracle_dir, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade_check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -1,23 +1,823 @@
-// r2p_batch.cpp -- batched form (placeholder until the batched launch lands).
+// r2p_batch.cpp -- batched form: many images of uneven size and bit depth
+// encoded by one fixed launch sequence (window+filter -> deflate -> scan ->
+// assemble) over device job tables, captured once in a HIP graph.
+//
+// The kernels are persistent: fixed grids loop over the work items (row
+// bands, 64 KiB segments, IDAT chunks) of all images, and read the item
+// counts from device memory, so the captured graph replays unchanged for any
+// batch that fits the capacities fixed at create time.  Per batch the host
+// only rewrites the pinned job tables, which the graph's first node uploads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
 #include "r2p_common.h"
 
 using namespace r2p;
 
-extern "C" {
-int r2p_batch_create(int, size_t, size_t, const r2p_options*, r2p_batch**) { set_error("batch not built"); return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_destroy(r2p_batch*) { return R2P_OK; }
-int r2p_batch_set(r2p_batch*, int, const r2p_image*, const void* const*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_run(r2p_batch*, int) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_sync(r2p_batch*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_result(r2p_batch*, int, const void**, size_t*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_copy_result(r2p_batch*, int, void*, size_t, size_t*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_timings(r2p_batch*, r2p_timings*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_autotune(r2p_batch*, r2p_options*, double*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_batch_options(r2p_batch*, const r2p_options*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_server_create(const r2p_server_config*, r2p_server**) { return R2P_ERR_UNSUPPORTED; }
-int r2p_server_destroy(r2p_server*) { return R2P_OK; }
-int r2p_server_submit(r2p_server*, const r2p_image*, const void*, int64_t*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_server_wait(r2p_server*, int64_t, int, void*, size_t, size_t*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_server_stats_get(r2p_server*, r2p_server_stats*) { return R2P_ERR_UNSUPPORTED; }
-int r2p_server_loadtest(r2p_server*, int, const double*, const int*, const r2p_image*, const void* const*, double*, uint64_t*, double*) { return R2P_ERR_UNSUPPORTED; }
+namespace {
+
+struct LutKey {
+    r2p_dicom_params p;
+    int kind, entries;
+    bool operator<(const LutKey& o) const
+    {
+        if (kind != o.kind) return kind < o.kind;
+        if (entries != o.entries) return entries < o.entries;
+        return memcmp(&p, &o.p, sizeof(p)) < 0;
+    }
+};
+
+constexpr int kLutSlot = 65536;      // bytes per table slot
+constexpr int kMaxLuts = 64;
+
+}  // namespace
+
+struct r2p_batch {
+    r2p_options opt{};
+    int max_images = 0;
+    size_t max_input = 0, max_filtered = 0;
+    size_t max_segs = 0, max_out = 0;
+    int seg_bytes_cap = 0;            // smallest segment size the arenas were sized for
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    // device arenas
+    uint8_t* d_filt = nullptr;
+    uint32_t* d_seg_out = nullptr;
+    uint32_t* d_seg_len = nullptr;
+    uint32_t* d_seg_adler = nullptr;
+    uint64_t* d_seg_off = nullptr;
+    uint8_t* d_out = nullptr;
+    uint8_t* d_lut = nullptr;
+    // device tables (one allocation) and their pinned host mirror
+    void* d_tab = nullptr;
+    void* h_tab = nullptr;
+    size_t tab_bytes = 0;
+    FilterJob* d_fj = nullptr; DeflateJob* d_dj = nullptr; AssembleJob* d_aj = nullptr;
+    uint32_t* d_band_start = nullptr; uint32_t* d_seg_start = nullptr; uint32_t* d_chunk_start = nullptr;
+    uint32_t* d_counts = nullptr;      // [0..1] filter, [2..3] deflate, [4..5] assemble
+    uint64_t* d_res = nullptr;         // per image: png length, adler
+    uint64_t* h_res = nullptr;         // pinned
+    size_t off_fj, off_dj, off_aj, off_bs, off_ss, off_cs, off_cnt;
+    // current batch
+    int n = 0;
+    std::vector<size_t> out_off;
+    std::vector<std::pair<LutKey, int>> luts;
+    int band_rows = 1, row_cap = 16, lut_cap = 0;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1;
+    bool ran = false;
+    std::vector<r2p_image> shadow_img;
+    std::vector<const void*> shadow_src;
+};
+
+static int batch_enqueue(r2p_batch* b, hipStream_t s)
+{
+    // 1. job tables
+    R2P_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, s));
+    // 2. window + filter over all row bands
+    FilterLaunch L{};
+    L.jobs = b->d_fj; L.band_start = b->d_band_start; L.njobs = 0; L.band_rows = b->band_rows;
+    L.total_bands = 0; L.row_cap = b->row_cap; L.lut_cap = b->lut_cap; L.counts = b->d_counts;
+    const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 128;
+    R2P_HIP(hipEventRecord(b->ev[0], s));
+    R2P_HIP(launch_filter(L, 4096, lds, s));
+    R2P_HIP(hipEventRecord(b->ev[1], s));
+    // 3. deflate over all segments
+    DeflateLaunch DL{};
+    DL.jobs = b->d_dj; DL.seg_start = b->d_seg_start; DL.counts = b->d_counts + 2;
+    R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, 1024, s));
+    R2P_HIP(hipEventRecord(b->ev[2], s));
+    // 4. per-image scan, 5. assemble all chunks
+    AssembleLaunch AL{};
+    AL.jobs = b->d_aj; AL.chunk_start = b->d_chunk_start; AL.counts = b->d_counts + 4;
+    R2P_HIP(launch_scan(AL, b->max_images, s));
+    R2P_HIP(hipEventRecord(b->ev[3], s));
+    R2P_HIP(launch_assemble(AL, 2048, s));
+    R2P_HIP(hipEventRecord(b->ev[4], s));
+    // 6. lengths and adlers back to pinned host memory
+    R2P_HIP(hipMemcpyAsync(b->h_res, b->d_res, (size_t)b->max_images * 16, hipMemcpyDeviceToHost, s));
+    return R2P_OK;
 }
+
+extern "C" {
+
+int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered_bytes,
+                     const r2p_options* opt, r2p_batch** out)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int rc = r2p_init(dev);
+    if (rc) return rc;
+    if (!out || max_images <= 0 || max_filtered_bytes == 0) { set_error("bad batch arguments"); return R2P_ERR_ARG; }
+    r2p_batch* b = new r2p_batch();
+    if (opt) b->opt = *opt; else r2p_options_default(&b->opt);
+    fill_defaults(&b->opt);
+    b->max_images = max_images;
+    b->max_input = max_input_bytes;
+    b->max_filtered = max_filtered_bytes;
+    // arenas sized for segments as small as 32 KiB (the autotuner's smallest)
+    b->seg_bytes_cap = std::min(32768, b->opt.segment_bytes);
+    b->max_segs = max_filtered_bytes / b->seg_bytes_cap + (size_t)max_images + 1;
+    const size_t cap_dw = (size_t)seg_cap_dwords(b->seg_bytes_cap);
+    // largest per-segment capacity for any segment size <= 4x the smallest
+    const size_t seg_arena_dw = b->max_segs * cap_dw;
+    b->max_out = max_filtered_bytes + max_filtered_bytes / 8 + 32 * b->max_segs + 4096 * (size_t)max_images;
+    b->off_fj = 0;
+    b->off_dj = b->off_fj + sizeof(FilterJob) * (size_t)max_images;
+    b->off_aj = b->off_dj + sizeof(DeflateJob) * (size_t)max_images;
+    b->off_bs = b->off_aj + sizeof(AssembleJob) * (size_t)max_images;
+    b->off_ss = b->off_bs + 4 * ((size_t)max_images + 1);
+    b->off_cs = b->off_ss + 4 * ((size_t)max_images + 1);
+    b->off_cnt = b->off_cs + 4 * ((size_t)max_images + 1);
+    b->tab_bytes = b->off_cnt + 64;
+    hipError_t e;
+#define TRY(x) do { if ((e = (x)) != hipSuccess) { r2p_batch_destroy(b); return hip_fail(e, #x); } } while (0)
+    TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto& v : b->ev) TRY(hipEventCreate(&v));
+    TRY(hipMalloc((void**)&b->d_filt, max_filtered_bytes + 256 * (size_t)max_images));
+    TRY(hipMalloc((void**)&b->d_seg_out, seg_arena_dw * 4));
+    TRY(hipMalloc((void**)&b->d_seg_len, b->max_segs * 4));
+    TRY(hipMalloc((void**)&b->d_seg_adler, b->max_segs * 4));
+    TRY(hipMalloc((void**)&b->d_seg_off, (b->max_segs + (size_t)max_images + 1) * 8));
+    TRY(hipMalloc((void**)&b->d_out, b->max_out));
+    TRY(hipMalloc((void**)&b->d_lut, (size_t)kLutSlot * kMaxLuts));
+    TRY(hipMalloc(&b->d_tab, b->tab_bytes));
+    TRY(hipHostMalloc(&b->h_tab, b->tab_bytes, hipHostMallocDefault));
+    TRY(hipMalloc((void**)&b->d_res, (size_t)max_images * 16));
+    TRY(hipHostMalloc((void**)&b->h_res, (size_t)max_images * 16, hipHostMallocDefault));
+#undef TRY
+    memset(b->h_tab, 0, b->tab_bytes);
+    uint8_t* dt = (uint8_t*)b->d_tab;
+    b->d_fj = (FilterJob*)(dt + b->off_fj);
+    b->d_dj = (DeflateJob*)(dt + b->off_dj);
+    b->d_aj = (AssembleJob*)(dt + b->off_aj);
+    b->d_band_start = (uint32_t*)(dt + b->off_bs);
+    b->d_seg_start = (uint32_t*)(dt + b->off_ss);
+    b->d_chunk_start = (uint32_t*)(dt + b->off_cs);
+    b->d_counts = (uint32_t*)(dt + b->off_cnt);
+    *out = b;
+    return R2P_OK;
+}
+
+int r2p_batch_destroy(r2p_batch* b)
+{
+    if (!b) return R2P_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    (void)hipFree(b->d_filt); (void)hipFree(b->d_seg_out); (void)hipFree(b->d_seg_len);
+    (void)hipFree(b->d_seg_adler); (void)hipFree(b->d_seg_off); (void)hipFree(b->d_out);
+    (void)hipFree(b->d_lut); (void)hipFree(b->d_tab); (void)hipFree(b->d_res);
+    if (b->h_tab) (void)hipHostFree(b->h_tab);
+    if (b->h_res) (void)hipHostFree(b->h_res);
+    for (auto& v : b->ev) if (v) (void)hipEventDestroy(v);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return R2P_OK;
+}
+
+int r2p_batch_options(r2p_batch* b, const r2p_options* opt)
+{
+    if (!b || !opt) { set_error("bad options"); return R2P_ERR_ARG; }
+    r2p_options o = *opt;
+    fill_defaults(&o);
+    if (o.segment_bytes < b->seg_bytes_cap) {
+        set_error("segment_bytes %d below the arena's %d", o.segment_bytes, b->seg_bytes_cap);
+        return R2P_ERR_ARG;
+    }
+    if (o.idat_chunk_bytes > 65536) o.idat_chunk_bytes = 65536;
+    b->opt = o;
+    return R2P_OK;
+}
+
+int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const* d_src)
+{
+    if (!b || n <= 0 || !imgs || !d_src) { set_error("bad batch set arguments"); return R2P_ERR_ARG; }
+    if (n > b->max_images) { set_error("%d images exceed batch capacity %d", n, b->max_images); return R2P_ERR_SIZE; }
+    R2P_HIP(hipStreamSynchronize(b->stream));   // previous run no longer reads the tables
+    const r2p_options& o = b->opt;
+    uint8_t* ht = (uint8_t*)b->h_tab;
+    FilterJob* fj = (FilterJob*)(ht + b->off_fj);
+    DeflateJob* dj = (DeflateJob*)(ht + b->off_dj);
+    AssembleJob* aj = (AssembleJob*)(ht + b->off_aj);
+    uint32_t* bs = (uint32_t*)(ht + b->off_bs);
+    uint32_t* ss = (uint32_t*)(ht + b->off_ss);
+    uint32_t* cs = (uint32_t*)(ht + b->off_cs);
+    uint32_t* cnt = (uint32_t*)(ht + b->off_cnt);
+    const int cap_dw = seg_cap_dwords(o.segment_bytes);
+    const size_t seg_arena_dw = b->max_segs * (size_t)seg_cap_dwords(b->seg_bytes_cap);
+
+    // band height: auto from the batch's total rows unless fixed by the options
+    int64_t total_rows = 0;
+    size_t total_f = 0;
+    std::vector<Geometry> geo((size_t)n);
+    std::vector<WindowPlan> plan((size_t)n);
+    for (int i = 0; i < n; i++) {
+        int rc = resolve(&imgs[i], &geo[i]);
+        if (rc) { set_error("image %d: %s", i, r2p_last_error()); return rc; }
+        if ((rc = plan_window(&imgs[i], &plan[i]))) return rc;
+        if (geo[i].row_bytes > (size_t)kMaxRowBytesLds) { set_error("image %d row too wide", i); return R2P_ERR_UNSUPPORTED; }
+        if (plan[i].lut_bytes > kLutSlot) { set_error("image %d table too large", i); return R2P_ERR_UNSUPPORTED; }
+        total_rows += imgs[i].height;
+        total_f += geo[i].filtered;
+        if (!d_src[i]) { set_error("image %d has no source", i); return R2P_ERR_ARG; }
+    }
+    if (total_f > b->max_filtered) { set_error("batch needs %zu filtered bytes > %zu", total_f, b->max_filtered); return R2P_ERR_SIZE; }
+    b->band_rows = o.band_rows > 0 ? o.band_rows : (int)std::max<int64_t>(1, std::min<int64_t>(16, total_rows / 8192));
+
+    // device tables, deduplicated by parameter set
+    std::map<LutKey, int> lut_slot;
+    b->luts.clear();
+    size_t foff = 0, segs = 0, ooff = 0;
+    uint32_t bands = 0, chunks = 0;
+    int rowmax = 16, lutmax = 0;
+    b->out_off.assign((size_t)n, 0);
+    for (int i = 0; i < n; i++) {
+        const Geometry& g = geo[i];
+        const WindowPlan& wp = plan[i];
+        const uint8_t* lut = nullptr;
+        if (wp.lut_kind >= 0) {
+            LutKey k{imgs[i].dicom, wp.lut_kind, wp.lut_entries};
+            auto it = lut_slot.find(k);
+            int slot;
+            if (it == lut_slot.end()) {
+                slot = (int)lut_slot.size();
+                if (slot >= kMaxLuts) { set_error("more than %d distinct DICOM parameter sets", kMaxLuts); return R2P_ERR_UNSUPPORTED; }
+                lut_slot[k] = slot;
+                b->luts.push_back({k, slot});
+            } else {
+                slot = it->second;
+            }
+            lut = b->d_lut + (size_t)slot * kLutSlot;
+            lutmax = std::max(lutmax, (wp.lut_bytes + 15) & ~15);
+        }
+        rowmax = std::max(rowmax, (int)((g.row_bytes + 15) & ~(size_t)15));
+        if (foff + g.filtered > b->max_filtered + 256 * (size_t)i) { set_error("filtered arena exhausted"); return R2P_ERR_SIZE; }
+        FilterJob& F = fj[i];
+        F = FilterJob{};
+        F.src = (const uint8_t*)d_src[i]; F.dst = b->d_filt + foff; F.prior = nullptr; F.last_row = nullptr;
+        F.lut = lut; F.src_pitch = g.pitch; F.row_bytes = (int32_t)g.row_bytes; F.rows = (int32_t)imgs[i].height;
+        F.bpp = g.bpp; F.wmode = wp.wmode; F.shift = wp.shift; F.mask = wp.mask; F.q15_a = wp.q15_a;
+        F.q15_b = wp.q15_b; F.pixel_rep = wp.pixel_rep; F.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
+        bs[i] = bands;
+        bands += (uint32_t)((imgs[i].height + b->band_rows - 1) / b->band_rows);
+
+        const int nseg = (int)((g.filtered + o.segment_bytes - 1) / o.segment_bytes);
+        if ((segs + nseg) * (size_t)cap_dw > seg_arena_dw || segs + nseg > b->max_segs) {
+            set_error("segment arena exhausted"); return R2P_ERR_SIZE;
+        }
+        DeflateJob& D = dj[i];
+        D = DeflateJob{};
+        D.data = F.dst; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
+        D.seg_out = b->d_seg_out + segs * (size_t)cap_dw; D.seg_len = b->d_seg_len + segs;
+        D.seg_adler = b->d_seg_adler + segs; D.seg_stats = nullptr; D.seg_bytes = o.segment_bytes;
+        D.seg_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
+        D.lz77 = o.lz77; D.nseg = nseg;
+        ss[i] = (uint32_t)segs;
+
+        const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
+        if (ooff + bound > b->max_out) { set_error("output arena exhausted"); return R2P_ERR_SIZE; }
+        AssembleJob& A = aj[i];
+        A = AssembleJob{};
+        A.seg_out = D.seg_out; A.seg_len = D.seg_len; A.seg_adler = D.seg_adler;
+        A.seg_off = b->d_seg_off + segs + (size_t)i; A.out = b->d_out + ooff;
+        A.out_len = b->d_res + 2 * (size_t)i; A.adler_out = (uint32_t*)(b->d_res + 2 * (size_t)i + 1);
+        A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.seg_cap_dw = cap_dw;
+        A.seg_bytes = o.segment_bytes; A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1;
+        A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
+        A.color_type = (uint8_t)g.color_type;
+        const int lvl = o.zlib_level;
+        A.zflg = lvl == 1 ? 0x01 : (lvl >= 2 && lvl <= 5) ? 0x5E : (lvl >= 6 && lvl <= 7) ? 0x9C : 0xDA;
+        const size_t zmax = g.filtered + g.filtered / 8 + 16 * (size_t)nseg + 16;
+        A.max_chunks = (int32_t)((zmax + o.idat_chunk_bytes - 1) / o.idat_chunk_bytes);
+        cs[i] = chunks;
+        chunks += (uint32_t)A.max_chunks;
+        b->out_off[i] = ooff;
+        ooff += (bound + 255) & ~(size_t)255;
+        foff += (g.filtered + 255) & ~(size_t)255;
+        segs += (size_t)nseg;
+    }
+    bs[n] = bands; ss[n] = (uint32_t)segs; cs[n] = chunks;
+    cnt[0] = (uint32_t)n; cnt[1] = bands;
+    cnt[2] = (uint32_t)n; cnt[3] = (uint32_t)segs;
+    cnt[4] = (uint32_t)n; cnt[5] = chunks;
+    b->row_cap = rowmax;
+    b->lut_cap = lutmax;
+    b->n = n;
+    // build the distinct tables now (outside the graph)
+    for (auto& kv : b->luts) {
+        LutBuild lb;
+        lb.p = kv.first.p; lb.entries = kv.first.entries; lb.kind = kv.first.kind;
+        R2P_HIP(launch_build_lut(lb, b->d_lut + (size_t)kv.second * kLutSlot, b->stream));
+    }
+    b->ran = false;
+    if (b->shadow_img.data() != imgs) {
+        b->shadow_img.assign(imgs, imgs + n);
+        b->shadow_src.assign(d_src, d_src + n);
+    }
+    return R2P_OK;
+}
+
+int r2p_batch_run(r2p_batch* b, int use_graph)
+{
+    if (!b || b->n <= 0) { set_error("batch not set"); return R2P_ERR_STATE; }
+    hipStream_t s = b->stream;
+    if (!use_graph) {
+        int rc = batch_enqueue(b, s);
+        if (rc) return rc;
+        b->ran = true;
+        return R2P_OK;
+    }
+    // graph keyed by the launch parameters that are baked into its nodes
+    const int lane = b->opt.lane_bytes;
+    if (!b->gexec || b->g_row_cap != b->row_cap || b->g_lut_cap != b->lut_cap ||
+        b->g_band != b->band_rows || b->g_lane != lane) {
+        if (b->gexec) { R2P_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+        if (b->graph) { R2P_HIP(hipGraphDestroy(b->graph)); b->graph = nullptr; }
+        R2P_HIP(hipStreamSynchronize(s));
+        R2P_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = batch_enqueue(b, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        b->graph = g;
+        R2P_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+        b->g_row_cap = b->row_cap; b->g_lut_cap = b->lut_cap; b->g_band = b->band_rows; b->g_lane = lane;
+    }
+    R2P_HIP(hipGraphLaunch(b->gexec, s));
+    b->ran = true;
+    return R2P_OK;
+}
+
+int r2p_batch_sync(r2p_batch* b)
+{
+    if (!b) { set_error("null batch"); return R2P_ERR_ARG; }
+    R2P_HIP(hipStreamSynchronize(b->stream));
+    return R2P_OK;
+}
+
+int r2p_batch_result(r2p_batch* b, int i, const void** d_png, size_t* len)
+{
+    if (!b || i < 0 || i >= b->n || !b->ran) { set_error("bad batch result index"); return R2P_ERR_ARG; }
+    if (d_png) *d_png = b->d_out + b->out_off[(size_t)i];
+    if (len) *len = (size_t)b->h_res[2 * (size_t)i];
+    return R2P_OK;
+}
+
+int r2p_batch_copy_result(r2p_batch* b, int i, void* h_dst, size_t cap, size_t* len)
+{
+    const void* d = nullptr;
+    size_t n = 0;
+    int rc = r2p_batch_result(b, i, &d, &n);
+    if (rc) return rc;
+    if (n > cap) { set_error("png of %zu bytes exceeds %zu", n, cap); return R2P_ERR_SIZE; }
+    R2P_HIP(hipMemcpy(h_dst, d, n, hipMemcpyDeviceToHost));
+    if (len) *len = n;
+    return R2P_OK;
+}
+
+int r2p_batch_timings(r2p_batch* b, r2p_timings* t)
+{
+    if (!b || !t || !b->ran) { set_error("no batch timings"); return R2P_ERR_STATE; }
+    R2P_HIP(hipEventSynchronize(b->ev[4]));
+    R2P_HIP(hipEventElapsedTime(&t->filter_ms, b->ev[0], b->ev[1]));
+    R2P_HIP(hipEventElapsedTime(&t->deflate_ms, b->ev[1], b->ev[2]));
+    R2P_HIP(hipEventElapsedTime(&t->scan_ms, b->ev[2], b->ev[3]));
+    R2P_HIP(hipEventElapsedTime(&t->assemble_ms, b->ev[3], b->ev[4]));
+    R2P_HIP(hipEventElapsedTime(&t->total_ms, b->ev[0], b->ev[4]));
+    return R2P_OK;
+}
+
+// Autotuner: times each candidate (segment_bytes x lane_bytes x band_rows)
+// on the declared batch -- one warm-up graph replay, then the best of three
+// timed replays -- and keeps the fastest, re-declaring the batch with it.
+int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
+{
+    if (!b || b->n <= 0) { set_error("autotune needs a declared batch"); return R2P_ERR_STATE; }
+    const std::vector<r2p_image> imgs = b->shadow_img;
+    const std::vector<const void*> srcs = b->shadow_src;
+    const int n = b->n;
+    const r2p_options base = b->opt;
+    r2p_options best = base;
+    double best_t = 1e30;
+    static const int segs[] = {32768, 65536};
+    static const int lanes[] = {16, 32};
+    static const int bands[] = {4, 8, 16};
+    for (int sg : segs) {
+        if (sg < b->seg_bytes_cap) continue;
+        for (int ln : lanes)
+            for (int bd : bands) {
+                r2p_options o = base;
+                o.segment_bytes = sg; o.lane_bytes = ln; o.band_rows = bd;
+                int rc = r2p_batch_options(b, &o);
+                if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
+                if (rc) continue;
+                double t_min = 1e30;
+                for (int rep = 0; rep < 4 && !rc; rep++) {
+                    rc = r2p_batch_run(b, 1);
+                    if (!rc) rc = r2p_batch_sync(b);
+                    r2p_timings t{};
+                    if (!rc && rep > 0 && r2p_batch_timings(b, &t) == R2P_OK)
+                        t_min = std::min(t_min, (double)t.total_ms);
+                }
+                if (rc) return rc;
+                if (t_min < best_t) { best_t = t_min; best = o; }
+            }
+    }
+    int rc = r2p_batch_options(b, &best);
+    if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
+    if (rc) return rc;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+    return R2P_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Served form: continuous batching under an HBM budget.
+//
+// A scheduler thread owns `inflight_batches` lanes.  Each lane is a batch
+// (its own stream, arenas and captured graph) plus a device input arena and
+// a pinned output staging buffer.  Whenever a lane is free and requests are
+// waiting, the scheduler admits waiting requests FIFO while their device
+// footprint fits both the lane and the global HBM budget, uploads their
+// pixels, replays the lane's graph and goes on; lanes run concurrently on
+// the device, so requests that arrive while others are on the GPU start as
+// soon as a lane frees up.  A finished lane copies its PNGs back, completes
+// its requests and returns their footprint to the budget.
+// ===========================================================================
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using SClock = std::chrono::steady_clock;
+
+struct Request {
+    int64_t id = 0;
+    r2p_image img{};
+    const void* h_src = nullptr;
+    size_t in_bytes = 0, filtered = 0, footprint = 0;
+    SClock::time_point arrival{}, done_at{};
+    std::vector<uint8_t> png;
+    int status = R2P_OK;
+    bool done = false;
+};
+
+struct Lane {
+    r2p_batch* b = nullptr;
+    uint8_t* d_in = nullptr;
+    size_t in_cap = 0;
+    uint8_t* h_stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t ev_run = nullptr, ev_copy = nullptr;
+    int state = 0;  // 0 free, 1 running, 2 copying
+    std::vector<Request*> reqs;
+    std::vector<size_t> stage_off;
+    size_t footprint = 0;
+};
+
+}  // namespace
+
+struct r2p_server {
+    r2p_server_config cfg{};
+    std::vector<Lane> lanes;
+    size_t lane_filtered = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> waiting;
+    std::unordered_map<int64_t, Request*> all;
+    int64_t next_id = 0;
+    size_t admitted = 0;
+    bool stop = false;
+    std::thread th;
+    r2p_server_stats st{};
+    uint64_t batch_images = 0;
+    int dev = 0;
+};
+
+static size_t req_footprint(size_t in_b, size_t filt)
+{
+    // input + filtered + segment scratch (<= 9/8) + PNG (<= 9/8) + slack
+    return in_b + filt + 2 * (filt + filt / 8) + 8192;
+}
+
+static void lane_launch(r2p_server* s, Lane& L)
+{
+    const int n = (int)L.reqs.size();
+    std::vector<r2p_image> imgs((size_t)n);
+    std::vector<const void*> srcs((size_t)n);
+    size_t off = 0;
+    hipStream_t st = L.b->stream;
+    for (int i = 0; i < n; i++) {
+        Request* r = L.reqs[(size_t)i];
+        imgs[(size_t)i] = r->img;
+        srcs[(size_t)i] = L.d_in + off;
+        off += (r->in_bytes + 255) & ~(size_t)255;
+    }
+    int rc = r2p_batch_set(L.b, n, imgs.data(), srcs.data());
+    off = 0;
+    for (int i = 0; i < n && !rc; i++) {
+        Request* r = L.reqs[(size_t)i];
+        if (hipMemcpyAsync(L.d_in + off, r->h_src, r->in_bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = R2P_ERR_HIP;
+        off += (r->in_bytes + 255) & ~(size_t)255;
+    }
+    if (!rc) rc = r2p_batch_run(L.b, 1);
+    if (!rc && hipEventRecord(L.ev_run, st) != hipSuccess) rc = R2P_ERR_HIP;
+    if (rc) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (Request* r : L.reqs) { r->status = rc; r->done = true; r->done_at = SClock::now(); }
+        s->admitted -= L.footprint;
+        L.reqs.clear(); L.footprint = 0; L.state = 0;
+        s->cv_done.notify_all();
+        return;
+    }
+    L.state = 1;
+}
+
+static void lane_copy(Lane& L)
+{
+    hipStream_t st = L.b->stream;
+    L.stage_off.assign(L.reqs.size(), 0);
+    size_t off = 0;
+    for (size_t i = 0; i < L.reqs.size(); i++) {
+        const void* d = nullptr;
+        size_t len = 0;
+        r2p_batch_result(L.b, (int)i, &d, &len);
+        L.stage_off[i] = off;
+        if (len && off + len <= L.stage_cap)
+            (void)hipMemcpyAsync(L.h_stage + off, d, len, hipMemcpyDeviceToHost, st);
+        L.reqs[i]->png.resize(len);
+        off += (len + 255) & ~(size_t)255;
+    }
+    (void)hipEventRecord(L.ev_copy, st);
+    L.state = 2;
+}
+
+static void lane_finish(r2p_server* s, Lane& L)
+{
+    for (size_t i = 0; i < L.reqs.size(); i++) {
+        Request* r = L.reqs[i];
+        if (!r->png.empty()) memcpy(r->png.data(), L.h_stage + L.stage_off[i], r->png.size());
+    }
+    const auto now = SClock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Request* r : L.reqs) { r->done_at = now; r->done = true; }
+    s->admitted -= L.footprint;
+    s->st.completed += L.reqs.size();
+    L.reqs.clear();
+    L.footprint = 0;
+    L.state = 0;
+    s->cv_done.notify_all();
+}
+
+static void server_loop(r2p_server* s)
+{
+    (void)hipSetDevice(s->dev);
+    for (;;) {
+        bool progressed = false;
+        for (Lane& L : s->lanes) {
+            if (L.state == 1 && hipEventQuery(L.ev_run) == hipSuccess) { lane_copy(L); progressed = true; }
+            if (L.state == 2 && hipEventQuery(L.ev_copy) == hipSuccess) { lane_finish(s, L); progressed = true; }
+        }
+        // admission into free lanes
+        for (Lane& L : s->lanes) {
+            if (L.state != 0) continue;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                size_t in_off = 0, filt = 0;
+                while (!s->waiting.empty() && (int)L.reqs.size() < s->cfg.max_batch_images) {
+                    Request* r = s->waiting.front();
+                    const size_t in_need = in_off + ((r->in_bytes + 255) & ~(size_t)255);
+                    if (in_need > L.in_cap || filt + r->filtered > s->lane_filtered) break;
+                    if (s->admitted + r->footprint > s->cfg.hbm_budget_bytes && !L.reqs.empty()) break;
+                    if (s->admitted + r->footprint > s->cfg.hbm_budget_bytes && s->admitted > 0) break;
+                    s->waiting.pop_front();
+                    L.reqs.push_back(r);
+                    L.footprint += r->footprint;
+                    s->admitted += r->footprint;
+                    in_off = in_need;
+                    filt += r->filtered;
+                }
+                if (L.reqs.empty()) continue;
+                s->st.batches++;
+                s->batch_images += L.reqs.size();
+                s->st.peak_admitted_bytes = std::max<uint64_t>(s->st.peak_admitted_bytes, s->admitted);
+            }
+            lane_launch(s, L);
+            progressed = true;
+        }
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop) {
+                bool busy = false;
+                for (Lane& L : s->lanes) busy |= L.state != 0;
+                if (!busy && s->waiting.empty()) return;
+            }
+            if (!progressed) {
+                bool busy = false;
+                for (Lane& L : s->lanes) busy |= L.state != 0;
+                if (busy) {
+                    lk.unlock();
+                    std::this_thread::sleep_for(std::chrono::microseconds(50));
+                } else {
+                    s->cv_work.wait_for(lk, std::chrono::milliseconds(2),
+                                        [s] { return s->stop || !s->waiting.empty(); });
+                }
+            }
+        }
+    }
+}
+
+extern "C" {
+
+int r2p_server_create(const r2p_server_config* cfg, r2p_server** out)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int rc = r2p_init(dev);
+    if (rc) return rc;
+    if (!cfg || !out || cfg->hbm_budget_bytes < ((size_t)64 << 20) || cfg->max_batch_images <= 0 ||
+        cfg->inflight_batches <= 0 || cfg->inflight_batches > 16) {
+        set_error("bad server configuration");
+        return R2P_ERR_ARG;
+    }
+    r2p_server* s = new r2p_server();
+    s->cfg = *cfg;
+    s->dev = dev;
+    fill_defaults(&s->cfg.opt);
+    const int nl = cfg->inflight_batches;
+    const size_t lane_budget = cfg->hbm_budget_bytes / (size_t)nl;
+    // per lane: input arena 1/4, filtered 1/4 of what remains after the input
+    const size_t in_cap = lane_budget / 4;
+    s->lane_filtered = std::max<size_t>((lane_budget - in_cap) / 4, (size_t)16 << 20);
+    s->lanes.resize((size_t)nl);
+    for (Lane& L : s->lanes) {
+        r2p_options o = s->cfg.opt;
+        rc = r2p_batch_create(cfg->max_batch_images, in_cap, s->lane_filtered, &o, &L.b);
+        if (rc) { r2p_server_destroy(s); return rc; }
+        L.in_cap = in_cap;
+        L.stage_cap = L.b->max_out;
+        if (hipMalloc((void**)&L.d_in, in_cap) != hipSuccess ||
+            hipHostMalloc((void**)&L.h_stage, L.stage_cap, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&L.ev_run, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&L.ev_copy, hipEventDisableTiming) != hipSuccess) {
+            set_error("server lane allocation failed");
+            r2p_server_destroy(s);
+            return R2P_ERR_HIP;
+        }
+    }
+    s->th = std::thread(server_loop, s);
+    *out = s;
+    return R2P_OK;
+}
+
+int r2p_server_destroy(r2p_server* s)
+{
+    if (!s) return R2P_OK;
+    if (s->th.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->th.join();
+    }
+    for (Lane& L : s->lanes) {
+        if (L.b) r2p_batch_destroy(L.b);
+        if (L.d_in) (void)hipFree(L.d_in);
+        if (L.h_stage) (void)hipHostFree(L.h_stage);
+        if (L.ev_run) (void)hipEventDestroy(L.ev_run);
+        if (L.ev_copy) (void)hipEventDestroy(L.ev_copy);
+    }
+    for (auto& kv : s->all) delete kv.second;
+    delete s;
+    return R2P_OK;
+}
+
+static int submit_at(r2p_server* s, const r2p_image* img, const void* h_src, SClock::time_point arrival,
+                     int64_t* id)
+{
+    Geometry g;
+    int rc = resolve(img, &g);
+    if (rc) return rc;
+    WindowPlan wp;
+    if ((rc = plan_window(img, &wp))) return rc;
+    if (!h_src) { set_error("null source"); return R2P_ERR_ARG; }
+    auto* r = new Request();
+    r->img = *img;
+    r->h_src = h_src;
+    r->in_bytes = (size_t)g.pitch * (img->height - 1) + g.in_row;
+    r->filtered = g.filtered;
+    r->footprint = req_footprint(r->in_bytes, r->filtered);
+    r->arrival = arrival;
+    if (r->filtered > s->lane_filtered || r->in_bytes > s->lanes[0].in_cap || r->footprint > s->cfg.hbm_budget_bytes) {
+        delete r;
+        set_error("request larger than a serving lane");
+        return R2P_ERR_SIZE;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        s->all[r->id] = r;
+        s->waiting.push_back(r);
+        s->st.submitted++;
+    }
+    s->cv_work.notify_one();
+    if (id) *id = r->id;
+    return R2P_OK;
+}
+
+int r2p_server_submit(r2p_server* s, const r2p_image* img, const void* h_src, int64_t* id)
+{
+    if (!s) { set_error("null server"); return R2P_ERR_ARG; }
+    return submit_at(s, img, h_src, SClock::now(), id);
+}
+
+int r2p_server_wait(r2p_server* s, int64_t id, int timeout_ms, void* h_png, size_t cap, size_t* len)
+{
+    if (!s) { set_error("null server"); return R2P_ERR_ARG; }
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) { set_error("unknown request %lld", (long long)id); return R2P_ERR_ARG; }
+    Request* r = it->second;
+    auto pred = [r] { return r->done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, pred);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) {
+        set_error("request %lld not done", (long long)id);
+        return R2P_ERR_STATE;
+    }
+    if (r->status) { set_error("request failed with %d", r->status); return r->status; }
+    if (len) *len = r->png.size();
+    if (h_png) {
+        if (r->png.size() > cap) { set_error("png of %zu bytes exceeds %zu", r->png.size(), cap); return R2P_ERR_SIZE; }
+        memcpy(h_png, r->png.data(), r->png.size());
+    }
+    s->all.erase(it);
+    delete r;
+    return R2P_OK;
+}
+
+int r2p_server_stats_get(r2p_server* s, r2p_server_stats* st)
+{
+    if (!s || !st) { set_error("bad stats arguments"); return R2P_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->st;
+    st->mean_batch_images = s->st.batches ? (double)s->batch_images / (double)s->st.batches : 0.0;
+    return R2P_OK;
+}
+
+int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int* kind, const r2p_image* imgs,
+                        const void* const* h_src, double* latency_s, uint64_t* png_len, double* wall_s)
+{
+    if (!s || n <= 0 || !arrival_s || !kind || !imgs || !h_src || !latency_s) { set_error("bad loadtest arguments"); return R2P_ERR_ARG; }
+    std::vector<int64_t> ids((size_t)n, -1);
+    const auto t0 = SClock::now() + std::chrono::milliseconds(5);
+    int rc = R2P_OK;
+    std::thread client([&] {
+        for (int i = 0; i < n; i++) {
+            const auto at = t0 + std::chrono::duration_cast<SClock::duration>(std::chrono::duration<double>(arrival_s[i]));
+            std::this_thread::sleep_until(at);
+            int64_t id = -1;
+            const int k = kind[i];
+            int e = submit_at(s, &imgs[k], h_src[k], at, &id);
+            if (e) { rc = e; break; }
+            ids[(size_t)i] = id;
+        }
+    });
+    client.join();
+    if (rc) return rc;
+    SClock::time_point last = t0;
+    for (int i = 0; i < n; i++) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        Request* r = s->all[ids[(size_t)i]];
+        s->cv_done.wait(lk, [r] { return r->done; });
+        latency_s[i] = std::chrono::duration<double>(r->done_at - r->arrival).count();
+        if (png_len) png_len[i] = r->png.size();
+        if (r->done_at > last) last = r->done_at;
+        if (r->status) rc = r->status;
+        s->all.erase(ids[(size_t)i]);
+        delete r;
+    }
+    if (wall_s) *wall_s = std::chrono::duration<double>(last - t0).count();
+    return rc;
+}
+
+}  // extern "C"

@@ -421,7 +421,7 @@ static int launch_filter_job(const FilterJob& J, int band_rows, hipStream_t s)
     L.row_cap = (J.row_bytes + 15) & ~15;
     L.lut_cap = (J.lut_bytes + 15) & ~15;
     L.single = J;
-    const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 64;
+    const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 128;
     const int grid = (int)std::min<uint32_t>(bands, 8192);
     R2P_HIP(launch_filter(L, grid, lds, s));
     return R2P_OK;

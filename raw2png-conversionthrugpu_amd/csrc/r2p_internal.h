@@ -100,6 +100,7 @@ struct FilterLaunch {
     uint32_t          total_bands;
     int32_t           row_cap;     // LDS row capacity (bytes, multiple of 16)
     int32_t           lut_cap;     // LDS table capacity (bytes, multiple of 16)
+    const uint32_t*   counts;      // optional device [njobs, total]: overrides the two fields
     FilterJob         single;
 };
 
@@ -108,6 +109,7 @@ struct DeflateLaunch {
     const uint32_t*   seg_start;   // prefix of segment counts per job (njobs + 1)
     int32_t           njobs;
     uint32_t          total_segs;
+    const uint32_t*   counts;      // optional device [njobs, total]
     DeflateJob        single;
 };
 
@@ -116,6 +118,7 @@ struct AssembleLaunch {
     const uint32_t*    chunk_start; // prefix of max_chunks per job (njobs + 1)
     int32_t            njobs;
     uint32_t           total_chunks;
+    const uint32_t*    counts;      // optional device [njobs, total]
     AssembleJob        single;
 };
 

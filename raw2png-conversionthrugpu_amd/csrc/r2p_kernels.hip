@@ -385,17 +385,19 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
     int cur_job = -1;
-    for (uint32_t g = blockIdx.x; g < L.total_bands; g += gridDim.x) {
+    const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
+    const int njobs = L.counts ? (int)L.counts[0] : L.njobs;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         int j = 0;
         if (L.jobs) {
-            int lo = 0, hi = L.njobs - 1;
+            int lo = 0, hi = njobs - 1;
             while (lo < hi) {                              // last job with band_start <= g
                 const int mid = (lo + hi + 1) >> 1;
                 if (L.band_start[mid] <= g) lo = mid; else hi = mid - 1;
             }
             j = lo;
         }
-        const FilterJob& J = L.jobs ? L.jobs[j] : L.single;
+        const FilterJob J = L.jobs ? L.jobs[j] : L.single;
         const int band = (int)(g - (L.jobs ? L.band_start[j] : 0u));
         const int r0 = band * L.band_rows;
         const int r1 = min(J.rows, r0 + L.band_rows);
@@ -767,17 +769,19 @@ __global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
     __shared__ uint32_t st[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    for (uint32_t g = blockIdx.x; g < DL.total_segs; g += gridDim.x) {
+    const uint32_t total = DL.counts ? DL.counts[1] : DL.total_segs;
+    const int njobs = DL.counts ? (int)DL.counts[0] : DL.njobs;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         int j = 0;
         if (DL.jobs) {
-            int lo = 0, hi = DL.njobs - 1;
+            int lo = 0, hi = njobs - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (DL.seg_start[mid] <= g) lo = mid; else hi = mid - 1;
             }
             j = lo;
         }
-        const DeflateJob& J = DL.jobs ? DL.jobs[j] : DL.single;
+        const DeflateJob J = DL.jobs ? DL.jobs[j] : DL.single;
         const int seg = (int)(g - (DL.jobs ? DL.seg_start[j] : 0u));
         const uint8_t* d = J.data;
         const int64_t s0 = J.first + (int64_t)seg * J.seg_bytes;
@@ -963,7 +967,8 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
     __shared__ uint32_t aval[256];
     __shared__ uint64_t alen[256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const AssembleJob& A = AL.jobs ? AL.jobs[blockIdx.x] : AL.single;
+    if (AL.counts && blockIdx.x >= AL.counts[0]) return;
+    const AssembleJob A = AL.jobs ? AL.jobs[blockIdx.x] : AL.single;
     uint64_t base = 0;
     for (int c0 = 0; c0 < A.nseg; c0 += 256) {
         const int i = c0 + tid;
@@ -1040,17 +1045,19 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
     const int tid = threadIdx.x;
     load_crc_tables(tab);
 
-    for (uint32_t g = blockIdx.x; g < AL.total_chunks; g += gridDim.x) {
+    const uint32_t total = AL.counts ? AL.counts[1] : AL.total_chunks;
+    const int njobs = AL.counts ? (int)AL.counts[0] : AL.njobs;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         int j = 0;
         if (AL.jobs) {
-            int lo = 0, hi = AL.njobs - 1;
+            int lo = 0, hi = njobs - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (AL.chunk_start[mid] <= g) lo = mid; else hi = mid - 1;
             }
             j = lo;
         }
-        const AssembleJob& A = AL.jobs ? AL.jobs[j] : AL.single;
+        const AssembleJob A = AL.jobs ? AL.jobs[j] : AL.single;
         const int c = (int)(g - (AL.jobs ? AL.chunk_start[j] : 0u));
         const uint64_t segbytes = A.seg_off[A.nseg];
         const int64_t z = png_payload(A, segbytes);
@@ -1152,6 +1159,11 @@ hipError_t launch_build_lut(const LutBuild& lb, void* out, hipStream_t s)
 
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
 {
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_filter,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     k_filter<<<grid, kFilterThreads, lds, s>>>(L);
     return hipGetLastError();
 }

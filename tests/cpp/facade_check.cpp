@@ -1,0 +1,267 @@
+// facade_check.cpp -- exercises the drop-in C++ API (the reference's header
+// signatures) on the GPU and checks every result against zlib and the CPU
+// oracle.  Built by raw2png-conversionthrugpu_amd/build.py; run by
+// tests/test_gpu_facade.py.  Prints one line per check, exits non-zero on the
+// first failure.
+#include <zlib.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "compression_backend.h"
+#include "gpu_adler32.h"
+#include "gpu_crc32.h"
+#include "gpu_deflate_backend.h"
+#include "gpu_filter.h"
+#include "gpu_png_assemble.h"
+#include "image_source.h"
+#include "parallel_deflate.h"
+#include "pipeline.h"
+#include "r2p_oracle.h"
+
+#include <hip/hip_runtime.h>
+
+#define CHECK(cond, ...)                                  \
+    do {                                                  \
+        if (!(cond)) {                                    \
+            printf("FAIL %s:%d ", __FILE__, __LINE__);    \
+            printf(__VA_ARGS__);                          \
+            printf("\n");                                 \
+            fflush(stdout);                               \
+            exit(1);                                      \
+        }                                                 \
+    } while (0)
+
+static std::vector<uint8_t> inflate_all(const uint8_t* z, size_t n, int wbits)
+{
+    std::vector<uint8_t> out(1 << 20);
+    z_stream s{};
+    inflateInit2(&s, wbits);
+    s.next_in = const_cast<Bytef*>(z);
+    s.avail_in = (uInt)n;
+    size_t have = 0;
+    int rc;
+    do {
+        if (have == out.size()) out.resize(out.size() * 2);
+        s.next_out = out.data() + have;
+        s.avail_out = (uInt)(out.size() - have);
+        rc = inflate(&s, Z_NO_FLUSH);
+        have = out.size() - s.avail_out;
+    } while (rc == Z_OK);
+    inflateEnd(&s);
+    CHECK(rc == Z_STREAM_END, "inflate rc=%d", rc);
+    out.resize(have);
+    return out;
+}
+
+static std::vector<uint8_t> read_file(const std::string& p)
+{
+    FILE* f = fopen(p.c_str(), "rb");
+    CHECK(f, "open %s", p.c_str());
+    std::vector<uint8_t> v;
+    uint8_t buf[65536];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) v.insert(v.end(), buf, buf + n);
+    fclose(f);
+    return v;
+}
+
+// zlib payload of a PNG file (IDAT concatenation), checking chunk CRCs.
+static std::vector<uint8_t> png_idat(const std::vector<uint8_t>& png)
+{
+    CHECK(png.size() > 8 && png[0] == 0x89 && png[1] == 'P', "signature");
+    std::vector<uint8_t> z;
+    size_t pos = 8;
+    while (pos + 12 <= png.size()) {
+        const uint32_t n = ((uint32_t)png[pos] << 24) | ((uint32_t)png[pos + 1] << 16) | ((uint32_t)png[pos + 2] << 8) | png[pos + 3];
+        const uint32_t crc = ((uint32_t)png[pos + 8 + n] << 24) | ((uint32_t)png[pos + 9 + n] << 16) |
+                             ((uint32_t)png[pos + 10 + n] << 8) | png[pos + 11 + n];
+        CHECK(crc32(0, &png[pos + 4], n + 4) == crc, "chunk crc");
+        if (!memcmp(&png[pos + 4], "IDAT", 4)) z.insert(z.end(), png.begin() + pos + 8, png.begin() + pos + 8 + n);
+        pos += 12 + n;
+    }
+    CHECK(pos == png.size(), "trailing bytes");
+    return z;
+}
+
+struct VecSource : ImageSource {
+    ImageInfo inf;
+    std::vector<uint8_t> px;
+    DicomPixelParams dp;
+    bool dicom = false;
+    int next = 0;
+    const ImageInfo& info() const override { return inf; }
+    int read_strip(uint8_t* out, int max_rows) override
+    {
+        const int left = (int)inf.height - next;
+        if (left <= 0) return 0;
+        const int n = left < max_rows ? left : max_rows;
+        const size_t rb = (size_t)inf.width * inf.bpp;
+        memcpy(out, px.data() + (size_t)next * rb, (size_t)n * rb);
+        next += n;
+        return n;
+    }
+    const DicomPixelParams* dicom_params() const override { return dicom ? &dp : nullptr; }
+};
+
+int main(int argc, char** argv)
+{
+    const std::string tmp = argc > 1 ? argv[1] : "/tmp";
+    std::mt19937 rng(1234);
+    // ---- 1. strip-wise filter with an explicitly threaded prior row == whole image
+    {
+        const int W = 333, H = 97, bpp = 2, SH = 16;
+        std::vector<uint8_t> raw((size_t)W * H * bpp);
+        for (size_t i = 0; i < raw.size(); i += 2) { uint16_t v = (uint16_t)((i / 2 * 7 + rng() % 50) & 0xFFF); raw[i] = v & 0xFF; raw[i + 1] = v >> 8; }
+        DicomPixelParams dp;
+        dp.bits_allocated = 16; dp.bits_stored = 12; dp.high_bit = 11; dp.pixel_rep = 0;
+        dp.apply_rescale = 1; dp.rescale_slope = 1.f; dp.rescale_intercept = -1024.f;
+        dp.apply_window = 1; dp.window_center = 40.f; dp.window_width = 400.f;
+        // oracle: window + filter of the whole image
+        r2po_dicom_params op;
+        memcpy(&op, &dp, sizeof(op));
+        std::vector<uint8_t> ready(raw.size()), exp((size_t)H * (W * bpp + 1));
+        r2po_dicom_window(raw.data(), ready.data(), (size_t)W * H, &op);
+        r2po_filter_rows(ready.data(), nullptr, exp.data(), H, W * bpp, bpp);
+        // two contexts round-robin, prior row threaded through the host
+        GpuFilterContext* ctx[2] = {gpu_filter_create(W, bpp, SH), gpu_filter_create(W, bpp, SH)};
+        gpu_filter_reset(ctx[0]); gpu_filter_reset(ctx[1]);
+        std::vector<uint8_t> prior((size_t)W * bpp, 0), got;
+        for (int s = 0, r = 0; r < H; s++, r += SH) {
+            const int rows = H - r < SH ? H - r : SH;
+            GpuTimings t;
+            const uint8_t* f = gpu_filter_process_from_host(ctx[s & 1], raw.data() + (size_t)r * W * bpp, prior.data(),
+                                                            rows, &t, &dp);
+            got.insert(got.end(), f, f + gpu_filter_output_size(ctx[s & 1], rows));
+            gpu_filter_copy_prior_row_to_host(ctx[s & 1], prior.data());
+        }
+        CHECK(got == exp, "strip-wise filter differs from the oracle");
+        CHECK(!memcmp(prior.data(), ready.data() + (size_t)(H - 1) * W * bpp, (size_t)W * bpp), "carried row");
+        gpu_filter_destroy(ctx[0]); gpu_filter_destroy(ctx[1]);
+        printf("ok filter strips+prior carry (%dx%d, strip %d)\n", W, H, SH);
+    }
+    // ---- 2. checksums
+    {
+        std::vector<uint8_t> d(3000007);
+        for (auto& b : d) b = (uint8_t)rng();
+        uint8_t* dd;
+        hipMalloc((void**)&dd, d.size());
+        hipMemcpy(dd, d.data(), d.size(), hipMemcpyHostToDevice);
+        GpuAdler32Context* a = gpu_adler32_create(d.size());
+        GpuCrc32Context* c = gpu_crc32_create(d.size());
+        CHECK(gpu_adler32_compute(a, dd, d.size()) == adler32(1, d.data(), (uInt)d.size()), "adler");
+        gpu_adler32_launch(a, dd, 12345);
+        CHECK(gpu_adler32_collect(a) == adler32(1, d.data(), 12345), "adler async");
+        CHECK(gpu_crc32_compute(c, dd, d.size()) == crc32(0, d.data(), (uInt)d.size()), "crc");
+        const uint32_t c1 = crc32(0, d.data(), 1000), c2 = crc32(0, d.data() + 1000, 5000);
+        CHECK(gpu_crc32_combine(c1, c2, 5000) == crc32(0, d.data(), 6000), "crc combine");
+        gpu_adler32_destroy(a); gpu_crc32_destroy(c);
+        // ---- 3. deflate strips + incremental IDAT assembly, as run_one_modern does
+        const int rows = 64, rb = 4097;
+        GpuDeflateContext* g = gpu_deflate_create(rows, rb, (size_t)d.size() * 9 + 1000, true);
+        GpuPngAssembleContext* pa = gpu_png_assemble_create(d.size() * 2);
+        std::vector<uint8_t> file, zstream;
+        size_t flushed = 0, in_off = 0;
+        ParallelDeflateState st;
+        const size_t strip_b = (size_t)rows * rb;
+        std::vector<uint8_t> chunk;
+        bool first = true;
+        while (in_off < d.size()) {
+            const size_t n = d.size() - in_off < strip_b ? d.size() - in_off : strip_b;
+            const int r = (int)(n / rb);
+            const size_t nb = (size_t)r * rb;
+            const bool last = in_off + nb >= d.size() || r == 0;
+            const size_t use = r ? nb : n;
+            gpu_deflate_compress_strip(g, dd + in_off, r ? r : 1, r ? rb : (int)n, last);
+            gpu_deflate_stream_sync(g);
+            CHECK(gpu_deflate_flushable_byte_length(g) == gpu_deflate_output_byte_length(g), "byte aligned");
+            DeflateResult dr;
+            dr.strip_adler = adler32(1, d.data() + in_off, (uInt)use);
+            dr.input_size = use;
+            accum_adler(st, dr);
+            const size_t avail = gpu_deflate_flushable_byte_length(g);
+            const size_t len = gpu_png_assemble_idat_chunk(pa, gpu_deflate_output(g) + flushed, avail - flushed, first,
+                                                           last, (uint32_t)st.running_adler, 3);
+            chunk.resize(len);
+            gpu_png_assemble_copy_to_host(pa, chunk.data(), len);
+            file.insert(file.end(), chunk.begin(), chunk.end());
+            flushed = avail;
+            first = false;
+            in_off += use;
+            if (last) break;
+        }
+        // parse the chunks: CRCs and the zlib stream
+        size_t pos = 0;
+        while (pos < file.size()) {
+            const uint32_t n = ((uint32_t)file[pos] << 24) | ((uint32_t)file[pos + 1] << 16) | ((uint32_t)file[pos + 2] << 8) | file[pos + 3];
+            CHECK(!memcmp(&file[pos + 4], "IDAT", 4), "idat type");
+            const uint32_t crc = ((uint32_t)file[pos + 8 + n] << 24) | ((uint32_t)file[pos + 9 + n] << 16) |
+                                 ((uint32_t)file[pos + 10 + n] << 8) | file[pos + 11 + n];
+            CHECK(crc32(0, &file[pos + 4], n + 4) == crc, "idat crc");
+            zstream.insert(zstream.end(), file.begin() + pos + 8, file.begin() + pos + 8 + n);
+            pos += 12 + n;
+        }
+        std::vector<uint8_t> back = inflate_all(zstream.data(), zstream.size(), 15);
+        CHECK(back == std::vector<uint8_t>(d.begin(), d.begin() + in_off), "deflate strips roundtrip (%zu vs %zu)", back.size(), in_off);
+        LzStats ls = gpu_deflate_lz_stats(g);
+        printf("ok checksums, deflate strips, IDAT assembly (%zu bytes, matches %llu)\n", in_off,
+               (unsigned long long)ls.matches_found);
+        gpu_deflate_destroy(g);
+        gpu_png_assemble_destroy(pa);
+        hipFree(dd);
+    }
+    // ---- 4. compression backends: GPU and CPU outputs inflate to the same bytes
+    {
+        std::vector<uint8_t> d(900000);
+        for (size_t i = 0; i < d.size(); i++) d[i] = (uint8_t)((i / 300) * 3 + (rng() % 4));
+        ParallelDeflateConfig cfg;
+        auto be = make_compression_backend(cfg, 4);
+        CPUZlibBackend cpu(cfg, 4);
+        for (CompressionBackend* b : {be.get(), (CompressionBackend*)&cpu}) {
+            std::vector<uint8_t> raw;
+            for (size_t off = 0; off < d.size(); off += 300000) {
+                DeflateResult r = b->compress_strip(d.data() + off, 300000, off + 300000 >= d.size());
+                CHECK(r.strip_adler == adler32(1, d.data() + off, 300000), "strip adler");
+                raw.insert(raw.end(), r.data.begin(), r.data.end());
+            }
+            CHECK(inflate_all(raw.data(), raw.size(), -15) == d, "%s roundtrip", b->name());
+            printf("ok backend %s (%zu -> %zu bytes)\n", b->name(), d.size(), raw.size());
+        }
+    }
+    // ---- 5. encode_source_to_png, both forms; the CPU-deflate form is
+    //         byte-identical to the oracle's CPU path with the same settings
+    {
+        const int W = 640, H = 300;
+        VecSource src;
+        src.inf.width = W; src.inf.height = H; src.inf.channels = 1; src.inf.bits_per_sample = 16; src.inf.bpp = 2;
+        src.px.resize((size_t)W * H * 2);
+        for (size_t i = 0; i < (size_t)W * H; i++) { uint16_t v = (uint16_t)(1000 + (i % W) + rng() % 30); src.px[2 * i] = v & 0xFF; src.px[2 * i + 1] = v >> 8; }
+        src.dicom = true;
+        src.dp.bits_allocated = 16; src.dp.bits_stored = 12; src.dp.high_bit = 11;
+        src.dp.apply_window = 1; src.dp.window_center = 1300.f; src.dp.window_width = 700.f;
+        r2po_dicom_params op;
+        memcpy(&op, &src.dp, sizeof(op));
+        std::vector<uint8_t> oracle_png(4u << 20), ofilt((size_t)H * (W * 2 + 1));
+        const size_t on = r2po_encode_png(src.px.data(), W, H, 1, 16, 2, &op, 128, 3, 6, oracle_png.data(),
+                                          oracle_png.size(), ofilt.data());
+        CHECK(on > 0, "oracle encode");
+        oracle_png.resize(on);
+        PipelineConfig cfg;
+        cfg.strip_height = 128; cfg.deflate_threads = 6; cfg.deflate_level = 3;
+        const std::string p1 = tmp + "/facade_cpu.png", p2 = tmp + "/facade_gpu.png";
+        CHECK(encode_source_to_png(src, p1.c_str(), cfg), "encode cpu-deflate form");
+        CHECK(read_file(p1) == oracle_png, "CPU-deflate form differs from the oracle's CPU path");
+        src.next = 0;
+        cfg.use_gpu_deflate = true; cfg.use_gpu_lz77 = true;
+        CHECK(encode_source_to_png(src, p2.c_str(), cfg), "encode gpu form");
+        std::vector<uint8_t> z = png_idat(read_file(p2));
+        CHECK(inflate_all(z.data(), z.size(), 15) == ofilt, "GPU form inflates to other bytes");
+        printf("ok encode_source_to_png: CPU-deflate form byte-identical to oracle, GPU form bit-exact after inflate\n");
+    }
+    printf("ALL OK\n");
+    return 0;
+}
