@@ -126,13 +126,90 @@ def run_plain(args, r2p, synth, torch):
     return res
 
 
+def run_batch(args, r2p, synth, torch):
+    """Mixed batch: the batched form (graph replays) against the plain form on the same images."""
+    t0 = time.perf_counter()
+    items = synth.mixed_batch_device(torch, args.batch_n, seed=3, max_side=args.batch_max_side)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    imgs = [im for im, _ in items]
+    srcs = [t.data_ptr() for _, t in items]
+    mp = sum(im.width * im.height for im in imgs) / 1e6
+    fsum = sum(r2p.filtered_bytes(im) for im in imgs)
+    isum = sum(r2p.input_bytes(im) for im in imgs)
+    log(f"bench: batch of {len(imgs)} images, {mp:.1f} MP, {fsum / 1e9:.2f} GB filtered (gen {gen_s:.1f}s)")
+    opt = r2p.options()
+    b = r2p.Batch(len(imgs), isum, fsum, opt)
+    b.set(imgs, srcs)
+    tuned, tuned_ms = b.autotune() if args.autotune else (b.opt, None)
+    for _ in range(2):
+        b.run(True); b.sync()
+    steps = max(3, args.batch_steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ktimes = []
+    for _ in range(steps):
+        b.run(True); b.sync()
+        t = b.timings()
+        ktimes.append((t.filter_ms, t.deflate_ms, t.scan_ms, t.assemble_ms, t.total_ms))
+    torch.cuda.synchronize()
+    bwall = (time.perf_counter() - t1) / steps
+    # plain form, one image at a time, same options
+    maxf = max(r2p.filtered_bytes(im) for im in imgs)
+    enc = r2p.Encoder(maxf, tuned)
+    cap = max(r2p.png_bound(im, tuned) for im in imgs)
+    d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    for im, sp in list(zip(imgs, srcs))[:8]:
+        enc.encode_device(im, sp, d_png.data_ptr(), cap); enc.sync()
+    lat = []
+    same = True
+    rng = np.random.default_rng(0)
+    check = set(rng.choice(len(imgs), size=min(16, len(imgs)), replace=False).tolist())
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for i, (im, sp) in enumerate(zip(imgs, srcs)):
+        ts = time.perf_counter()
+        enc.encode_device(im, sp, d_png.data_ptr(), cap)
+        n, _, _ = enc.sync()
+        lat.append(time.perf_counter() - ts)
+        if i in check:
+            same &= d_png[:n].cpu().numpy().tobytes() == b.copy_result(i)
+    pwall = sum(lat)          # the check copies between encodes are not timed
+    _ = t2
+    enc.close()
+    png_total = sum(b.result(i)[1] for i in range(len(imgs)))
+    b.close()
+    kt = np.array(ktimes).mean(axis=0)
+    lat_ms = [x * 1e3 for x in lat]
+    return {
+        "images": len(imgs), "megapixels": round(mp, 1), "filtered_gb": round(fsum / 1e9, 3),
+        "batched_mp_per_s": round(mp / bwall, 1), "batched_ms": round(bwall * 1e3, 3),
+        "plain_mp_per_s": round(mp / pwall, 1), "plain_ms": round(pwall * 1e3, 3),
+        "speedup_batched_over_plain": round(pwall / bwall, 3),
+        "plain_latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},
+        "batched_latency_ms": {"p50": round(bwall * 1e3, 3), "p99": round(bwall * 1e3, 3)},
+        "kernels_ms": {"filter": round(kt[0], 3), "deflate": round(kt[1], 3), "scan": round(kt[2], 3),
+                       "assemble": round(kt[3], 3)},
+        "filter_gbs": round((isum + fsum) / (kt[0] * 1e-3) / 1e9, 1),
+        "deflate_gbs": round((fsum + png_total) / (kt[1] * 1e-3) / 1e9, 1),
+        "autotuned": {"segment_bytes": tuned.segment_bytes, "lane_bytes": tuned.lane_bytes,
+                      "band_rows": tuned.band_rows, "best_ms": tuned_ms},
+        "png_over_filtered": round(png_total / fsum, 4),
+        "byte_identical_to_plain": bool(same), "steps": steps,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="radiograph", choices=["ct_slice", "radiograph", "camera_rgb"])
-    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--mode", default="all", choices=["plain", "batch", "all"])
+    ap.add_argument("--batch-n", type=int, default=1024)
+    ap.add_argument("--batch-max-side", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--lz77", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
@@ -180,6 +257,8 @@ def main():
         cb = cpu_baseline(O, r["s"], args.cpu_level, threads)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["ratio"]["zlib_level%d_png_over_filtered" % args.cpu_level] = round(cb["png_bytes"] / r["filtered_bytes"], 4)
+    if args.mode in ("batch", "all"):
+        out["batched"] = run_batch(args, r2p, synth, torch)
     print(json.dumps(out), flush=True)
 
 

@@ -166,3 +166,190 @@ def workload(name: str) -> SynthImage:
     if name == "camera_rgb":
         return camera_rgb(2)
     raise KeyError(name)
+
+
+# ---------------------------------------------------------------------------
+# Device-side generator for the mixed batch (same recipes, drawn with torch's
+# CUDA generator so 1024 images of up to 4096^2 appear in seconds).  The
+# window of each radiograph uses percentiles of a 1-in-16 pixel subsample.
+# ---------------------------------------------------------------------------
+def _dev_ct(torch, g, side):
+    dev = "cuda"
+    yy = torch.arange(side, device=dev, dtype=torch.float32).view(-1, 1)
+    xx = torch.arange(side, device=dev, dtype=torch.float32).view(1, -1)
+    base = torch.zeros((side, side), device=dev, dtype=torch.float32)
+    r = torch.rand(64, generator=g, device=dev).tolist()
+    k = 3 + int(r[0] * 8)
+    for e in range(k):
+        q = r[1 + 6 * e: 7 + 6 * e]
+        cy, cx = q[0] * side, q[1] * side
+        ry, rx = 20 + 180 * q[2], 20 + 180 * q[3]
+        th = q[4] * math.pi
+        lvl = 800 + 800 * q[5]
+        c, s = math.cos(th), math.sin(th)
+        dx, dy = xx - cx, yy - cy
+        u = (dx * c + dy * s) / rx
+        v = (-dx * s + dy * c) / ry
+        base = torch.where(u * u + v * v <= 1.0, torch.full_like(base, lvl), base)
+    n = torch.randn((side, side), generator=g, device=dev)
+    return torch.clamp(torch.round(base + 15.0 * n), 0, 4095).to(torch.int32)
+
+
+def _dev_rad(torch, g, side):
+    dev = "cuda"
+    yy = torch.arange(side, device=dev, dtype=torch.float32).view(-1, 1)
+    xx = torch.arange(side, device=dev, dtype=torch.float32).view(1, -1)
+    r = torch.rand(24, generator=g, device=dev).tolist()
+    acc = torch.zeros((side, side), device=dev, dtype=torch.float32)
+    for k in range(8):
+        f, ang, ph = 0.01 * r[3 * k], 2 * math.pi * r[3 * k + 1], 2 * math.pi * r[3 * k + 2]
+        acc += torch.cos(2 * math.pi * (f * math.cos(ang) * xx + f * math.sin(ang) * yy) + ph)
+    v = 2048.0 + (1500.0 / 8.0) * acc + 600.0 * yy / max(1, side - 1)
+    v = v + torch.sqrt(torch.clamp(v, min=0)) / 4.0 * torch.randn((side, side), generator=g, device=dev)
+    v = torch.clamp(torch.round(v), 0, 4095)
+    b = int(round(0.10 * side))
+    if b:
+        v[:b, :] = 4095; v[-b:, :] = 4095; v[:, :b] = 4095; v[:, -b:] = 4095
+    sub = v[::4, ::4].reshape(-1)
+    p1, p99 = torch.quantile(sub, torch.tensor([0.01, 0.99], device=dev)).tolist()
+    return v.to(torch.int32), float("%.2f" % ((p1 + p99) / 2.0)), float("%.2f" % max(p99 - p1, 1.0))
+
+
+def _dev_rgb(torch, g, side):
+    dev = "cuda"
+    yy = torch.arange(side, device=dev, dtype=torch.float32).view(-1, 1, 1)
+    xx = torch.arange(side, device=dev, dtype=torch.float32).view(1, -1, 1)
+    r = torch.rand(9 + 50 * 7 + 4, generator=g, device=dev).tolist()
+    a = torch.tensor([-120 + 240 * r[i] for i in range(3)], device=dev).view(1, 1, 3)
+    bb = torch.tensor([-120 + 240 * r[3 + i] for i in range(3)], device=dev).view(1, 1, 3)
+    c0 = torch.tensor([60 + 130 * r[6 + i] for i in range(3)], device=dev).view(1, 1, 3)
+    img = c0 + a * xx / max(1, side - 1) + bb * yy / max(1, side - 1)
+    for e in range(50):
+        q = r[9 + 7 * e: 16 + 7 * e]
+        rh, rw = int(50 + 750 * q[0]), int(50 + 750 * q[1])
+        y0, x0 = int(q[2] * max(1, side - rh)), int(q[3] * max(1, side - rw))
+        img[y0:y0 + rh, x0:x0 + rw, :] = torch.tensor([255 * q[4], 255 * q[5], 255 * q[6]], device=dev)
+    tile = 255 * torch.rand((32, 32, 3), generator=g, device=dev)
+    th = min(1024, side)
+    ty = int(r[-2] * (side - th + 1)); tx = int(r[-1] * (side - th + 1))
+    img[ty:ty + th, tx:tx + th, :] = tile.repeat(th // 32 + 1, th // 32 + 1, 1)[:th, :th]
+    img = img + 3.0 * torch.randn((side, side, 3), generator=g, device=dev)
+    return torch.clamp(torch.round(img), 0, 255).to(torch.uint8)
+
+
+def mixed_batch_device(torch, n: int = 1024, seed: int = 3, max_side: int = 4096, min_side: int = 256):
+    """The mixed batch generated in device memory: list of (Image, uint8 tensor)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = torch.Generator(device="cuda")
+    out = []
+    for i in range(n):
+        side = int(math.floor(math.exp(rng.uniform(math.log(min_side), math.log(max_side)))))
+        side = max(min_side, min(max_side, side))
+        u = rng.uniform()
+        g.manual_seed(int(rng.integers(0, 2**62)))
+        if u < 0.5:
+            px = _dev_ct(torch, g, side).to(torch.int16).view(torch.uint8).reshape(-1)
+            img = Image.make(side, side, 1, 16, PIX_DICOM8, ct_params())
+        elif u < 0.8:
+            v, wc, ww = _dev_rad(torch, g, side)
+            px = v.to(torch.int16).view(torch.uint8).reshape(-1)
+            img = Image.make(side, side, 1, 16, PIX_DICOM, DicomParams.make(16, 12, 11, 0, 0, 1.0, 0.0, 1, wc, ww))
+        else:
+            px = _dev_rgb(torch, g, side).reshape(-1)
+            img = Image.make(side, side, 3, 8, PIX_READY)
+        out.append((img, px.contiguous()))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Container writers for the workloads' specified formats: uncompressed
+# Explicit VR Little Endian DICOM Part 10 and uncompressed little-endian TIFF.
+# ---------------------------------------------------------------------------
+def _ds(v) -> bytes:
+    s = ("%.6g" % v).encode()
+    return s + (b" " if len(s) % 2 else b"")
+
+
+def write_dicom(path: str, pixels: np.ndarray, p: DicomParams, frames: int = 1,
+                slope_str=None, intercept_str=None, wc_str=None, ww_str=None) -> None:
+    """Explicit VR LE Part 10 file holding `pixels` (uint16 or uint8, (F,)H,W[,C])."""
+    import struct
+
+    def el(g, e, vr, val: bytes):
+        if len(val) % 2:
+            val += b"\0" if vr in (b"UI", b"OB") else b" "
+        if vr in (b"OB", b"OW", b"SQ", b"UN", b"UT"):
+            return struct.pack("<HH2sHI", g, e, vr, 0, len(val)) + val
+        return struct.pack("<HH2sH", g, e, vr, len(val)) + val
+
+    ts = b"1.2.840.10008.1.2.1"
+    meta_body = el(0x0002, 0x0001, b"OB", b"\0\1") + el(0x0002, 0x0010, b"UI", ts)
+    meta = el(0x0002, 0x0000, b"UL", struct.pack("<I", len(meta_body))) + meta_body
+    px = np.ascontiguousarray(pixels)
+    h, w = px.shape[-2:] if px.ndim == 2 or (px.ndim == 3 and frames > 1) else px.shape[-3:-1]
+    spp = 1 if (px.ndim == 2 or (px.ndim == 3 and frames > 1)) else px.shape[-1]
+    us = lambda v: struct.pack("<H", v)
+    ds = b""
+    ds += el(0x0028, 0x0002, b"US", us(spp))
+    if frames > 1:
+        ds += el(0x0028, 0x0008, b"IS", str(frames).encode())
+    ds += el(0x0028, 0x0010, b"US", us(h)) + el(0x0028, 0x0011, b"US", us(w))
+    ds += el(0x0028, 0x0100, b"US", us(p.bits_allocated)) + el(0x0028, 0x0101, b"US", us(p.bits_stored))
+    ds += el(0x0028, 0x0102, b"US", us(p.high_bit)) + el(0x0028, 0x0103, b"US", us(p.pixel_rep))
+    if p.apply_window:
+        ds += el(0x0028, 0x1050, b"DS", wc_str or _ds(p.window_center))
+        ds += el(0x0028, 0x1051, b"DS", ww_str or _ds(p.window_width))
+    if p.apply_rescale:
+        ds += el(0x0028, 0x1052, b"DS", intercept_str or _ds(p.rescale_intercept))
+        ds += el(0x0028, 0x1053, b"DS", slope_str or _ds(p.rescale_slope))
+    data = px.astype("<u2" if p.bits_allocated == 16 else np.uint8).tobytes()
+    ds += el(0x7FE0, 0x0010, b"OW" if p.bits_allocated == 16 else b"OB", data)
+    with open(path, "wb") as f:
+        f.write(b"\0" * 128 + b"DICM" + meta + ds)
+
+
+def write_tiff(path: str, pixels: np.ndarray, rows_per_strip: int = 64) -> None:
+    """Uncompressed little-endian TIFF (chunky), uint8 or uint16, gray or RGB."""
+    import struct
+    px = np.ascontiguousarray(pixels)
+    h, w = px.shape[:2]
+    spp = 1 if px.ndim == 2 else px.shape[2]
+    bps = 16 if px.dtype.itemsize == 2 else 8
+    data = px.astype("<u2" if bps == 16 else np.uint8).tobytes()
+    rb = w * spp * bps // 8
+    strips = [(r, min(rows_per_strip, h - r)) for r in range(0, h, rows_per_strip)]
+    hdr = 8
+    data_off = hdr
+    offs = [data_off + r * rb for r, _ in strips]
+    cnts = [n * rb for _, n in strips]
+    extra_off = data_off + len(data)
+    extra = b""
+
+    def arr(vals, typ):
+        nonlocal extra
+        o = extra_off + len(extra)
+        extra += b"".join(struct.pack("<H" if typ == 3 else "<I", v) for v in vals)
+        if len(extra) % 2:
+            extra += b"\0"
+        return o
+
+    entries = []
+
+    def tag(t, typ, vals):
+        sz = 2 if typ == 3 else 4
+        if len(vals) * sz <= 4:
+            raw = b"".join(struct.pack("<H" if typ == 3 else "<I", v) for v in vals).ljust(4, b"\0")
+            entries.append(struct.pack("<HHI", t, typ, len(vals)) + raw)
+        else:
+            entries.append(struct.pack("<HHII", t, typ, len(vals), arr(vals, typ)))
+
+    tag(256, 4, [w]); tag(257, 4, [h]); tag(258, 3, [bps] * spp); tag(259, 3, [1])
+    tag(262, 3, [2 if spp >= 3 else 1]); tag(273, 4, offs); tag(277, 3, [spp])
+    tag(278, 4, [rows_per_strip]); tag(279, 4, cnts); tag(284, 3, [1])
+    ifd_off = extra_off + len(extra)
+    if ifd_off % 2:
+        extra += b"\0"
+        ifd_off += 1
+    ifd = struct.pack("<H", len(entries)) + b"".join(entries) + struct.pack("<I", 0)
+    with open(path, "wb") as f:
+        f.write(b"II" + struct.pack("<HI", 42, ifd_off) + data + extra + ifd)

@@ -110,6 +110,20 @@ struct VecSource : ImageSource {
 
 int main(int argc, char** argv)
 {
+    // facade_check encode <tiff|dicom|dicom-all> <in> <out> <gpu_deflate 0|1> [strip_height]
+    if (argc >= 6 && std::string(argv[1]) == "encode") {
+        PipelineConfig cfg;
+        cfg.use_gpu_deflate = atoi(argv[5]) != 0;
+        cfg.use_gpu_lz77 = cfg.use_gpu_deflate;
+        cfg.strip_height = argc > 6 ? atoi(argv[6]) : 128;
+        const std::string kind = argv[2];
+        bool ok = false;
+        if (kind == "tiff") ok = encode_tiff_to_png(argv[3], argv[4], cfg);
+        else if (kind == "dicom") ok = encode_dicom_to_png(argv[3], argv[4], cfg);
+        else if (kind == "dicom-all") ok = encode_dicom_all_frames_to_png(argv[3], argv[4], cfg);
+        printf("%s\n", ok ? "encode ok" : "encode failed");
+        return ok ? 0 : 1;
+    }
     const std::string tmp = argc > 1 ? argv[1] : "/tmp";
     std::mt19937 rng(1234);
     // ---- 1. strip-wise filter with an explicitly threaded prior row == whole image

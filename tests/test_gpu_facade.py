@@ -1,0 +1,68 @@
+"""The drop-in C++ API (reference header signatures) on the GPU: runs the
+tests/cpp/facade_check driver built by build()."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+
+
+def test_facade_check(torch_dev, tmp_path):
+    exe = os.path.join(REPO, "tests", "cpp", "facade_check")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/facade_check was not built (run __graft_entry__.build())")
+    print("case facade_check", flush=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=110)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    assert r.returncode == 0 and "ALL OK" in r.stdout
+
+
+@pytest.mark.parametrize("gpu_deflate", [0, 1])
+def test_file_readers_through_pipeline(torch_dev, r2p, synth, oracle, tmp_path, gpu_deflate):
+    """TIFF and DICOM Part 10 files -> encode_*_to_png -> PIL / oracle."""
+    import io
+    import numpy as np
+    from PIL import Image as PILImage
+    exe = os.path.join(REPO, "tests", "cpp", "facade_check")
+    print(f"case readers gpu_deflate={gpu_deflate}", flush=True)
+    # RGB8 TIFF
+    cam = synth.camera_rgb(2, 300, 200)
+    tif = str(tmp_path / "cam.tif")
+    synth.write_tiff(tif, cam.pixels, rows_per_strip=37)
+    out = str(tmp_path / "cam.png")
+    r = subprocess.run([exe, "encode", "tiff", tif, out, str(gpu_deflate)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert np.array_equal(np.array(PILImage.open(out)), cam.pixels)
+    # 16-bit grayscale TIFF (loader byte-swaps to big-endian)
+    g16 = np.random.default_rng(1).integers(0, 65536, (50, 61)).astype(np.uint16)
+    synth.write_tiff(str(tmp_path / "g.tif"), g16, rows_per_strip=7)
+    r = subprocess.run([exe, "encode", "tiff", str(tmp_path / "g.tif"), str(tmp_path / "g.png"), str(gpu_deflate)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert np.array_equal(np.array(PILImage.open(str(tmp_path / "g.png"))), g16)
+    # CT slice DICOM: the single-file path keeps 16 bits (float window, gpu_filter.cu:71-162)
+    ct = synth.ct_slice(0, 128)
+    dcm = str(tmp_path / "ct.dcm")
+    synth.write_dicom(dcm, ct.pixels, ct.img.dicom)
+    r = subprocess.run([exe, "encode", "dicom", dcm, str(tmp_path / "ct.png"), str(gpu_deflate)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p = oracle.DicomParams(*[getattr(ct.img.dicom, f) for f, _ in ct.img.dicom._fields_])
+    exp = oracle.dicom_window(ct.pixels.reshape(-1), p).view(">u2").reshape(128, 128)
+    assert np.array_equal(np.array(PILImage.open(str(tmp_path / "ct.png"))).astype(np.int64), exp.astype(np.int64))
+    # multi-frame
+    frames = np.stack([synth.ct_slice(k, 64).pixels for k in range(3)])
+    synth.write_dicom(str(tmp_path / "mf.dcm"), frames, ct.img.dicom, frames=3)
+    os.makedirs(str(tmp_path / "mf"), exist_ok=True)
+    r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "mf"), str(gpu_deflate)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for k in range(3):
+        got = np.array(PILImage.open(str(tmp_path / "mf" / f"slice_{k:04d}.png"))).astype(np.int64)
+        exp = oracle.dicom_window(frames[k].reshape(-1), p).view(">u2").reshape(64, 64)
+        assert np.array_equal(got, exp.astype(np.int64))
