@@ -48,7 +48,6 @@ private:
     r2p_deflate* ctx_ = nullptr;
     uint8_t* d_in_ = nullptr;
     size_t cap_ = 0;
-    size_t emitted_ = 0;
 };
 
 // Returns the GPU backend when a gfx9 device is usable, else the CPU one.

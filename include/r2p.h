@@ -172,6 +172,13 @@ const void* r2p_deflate_output(const r2p_deflate* d);
 size_t r2p_deflate_output_bytes(const r2p_deflate* d);
 uint32_t r2p_deflate_adler(const r2p_deflate* d);     /* adler32 of all input so far */
 int r2p_deflate_copy_to_host(const r2p_deflate* d, void* h_dst, size_t n);
+/* Drops the output delivered so far: the next strip's bytes start at offset 0
+ * of the output buffer; the 32 KiB history and the Adler-32 state are kept.
+ * Lets a caller that copies each strip's bytes out (a CompressionBackend)
+ * size the output buffer for one strip instead of a whole image.  No
+ * reference counterpart (the reference keeps the whole image on the device,
+ * gpu_deflate_backend.h:40-62). */
+int r2p_deflate_rewind(r2p_deflate* d);
 void* r2p_deflate_stream(r2p_deflate* d);
 typedef struct r2p_lz_stats {
     uint64_t matches, matched_bytes, positions;

@@ -125,6 +125,7 @@ def lib():
             "r2p_deflate_output_bytes": ([vp], sz), "r2p_deflate_adler": ([vp], u32),
             "r2p_deflate_copy_to_host": ([vp, vp, sz], ip), "r2p_deflate_stream": ([vp], vp),
             "r2p_deflate_lz_stats": ([vp, C.POINTER(LzStats)], ip),
+            "r2p_deflate_rewind": ([vp], ip),
             "r2p_idat_chunk": ([vp, sz, ip, ip, u32, ip, vp, sz, vp, C.POINTER(sz), vp], ip),
             "r2p_batch_create": ([ip, sz, sz, optp, C.POINTER(vp)], ip),
             "r2p_batch_destroy": ([vp], ip),

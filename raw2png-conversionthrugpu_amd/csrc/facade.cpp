@@ -588,7 +588,7 @@ GpuDeflateBackend::GpuDeflateBackend(size_t max_strip_bytes, bool lz77)
 {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return;
-    if (r2p_deflate_create(max_strip_bytes, max_strip_bytes * 4096, lz77 ? 1 : 0, 65536, &ctx_) != R2P_OK) {
+    if (r2p_deflate_create(max_strip_bytes, max_strip_bytes, lz77 ? 1 : 0, 65536, &ctx_) != R2P_OK) {
         ctx_ = nullptr;
         return;
     }
@@ -619,16 +619,11 @@ DeflateResult GpuDeflateBackend::compress_strip(const uint8_t* input, size_t n, 
     chk(r2p_deflate_strip(ctx_, d_in_, n, is_last ? 1 : 0, 1, 0), "GpuDeflateBackend");
     chk(r2p_deflate_sync(ctx_), "GpuDeflateBackend");
     const size_t total = r2p_deflate_output_bytes(ctx_);
-    r.data.resize(total - emitted_);
-    if (total > emitted_)
-        hchk(hipMemcpy(r.data.data(), (const uint8_t*)r2p_deflate_output(ctx_) + emitted_, total - emitted_,
-                       hipMemcpyDeviceToHost), "GpuDeflateBackend");
-    emitted_ = total;
+    r.data.resize(total);
+    if (total)
+        hchk(hipMemcpy(r.data.data(), r2p_deflate_output(ctx_), total, hipMemcpyDeviceToHost), "GpuDeflateBackend");
+    chk(is_last ? r2p_deflate_reset(ctx_) : r2p_deflate_rewind(ctx_), "GpuDeflateBackend");
     r.compress_us = us_since(t0);
-    if (is_last) {
-        chk(r2p_deflate_reset(ctx_), "GpuDeflateBackend");
-        emitted_ = 0;
-    }
     return r;
 }
 

@@ -93,6 +93,17 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+// clamp((x*A + B) >> 15, 0, 255) with int32 wrap-around (batch_processor.cpp:250-263).
+// The clamp is applied to the Q15 accumulator before the shift (identical
+// result: acc < 0 -> 0, acc >= 256<<15 -> 255), which keeps the compiler from
+// fusing shift+saturate+pack into v_ashr_pk_u8_i32 -- that fused form gave
+// results different from the scalar definition on gfx950.
+__device__ __forceinline__ uint32_t q15_clamp(int32_t x, int32_t a, int32_t b)
+{
+    const int32_t acc = (int32_t)((uint32_t)x * (uint32_t)a + (uint32_t)b);
+    return (uint32_t)min(max(acc, 0), (256 << 15) - 1) >> 15;
+}
+
 // float -> int32 conversion with x86 cvttss2si semantics (out of range or
 // NaN gives INT32_MIN), as the reference's host-side LUT builder executes it.
 __device__ __forceinline__ int32_t x86_f2i(float f)
@@ -294,8 +305,7 @@ __device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* 
                 for (int h = 0; h < 4; h++) {
                     const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
                     const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
-                    const int32_t q = (int32_t)((uint32_t)x * (uint32_t)J.q15_a + (uint32_t)J.q15_b) >> 15;
-                    r |= (uint32_t)min(max(q, 0), 255) << (8 * h);
+                    r |= q15_clamp(x, J.q15_a, J.q15_b) << (8 * h);
                 }
                 o[k] = r;
             }
@@ -339,8 +349,7 @@ __device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* 
         case WM_Q15: {
             const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
             const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
-            const int32_t q = (int32_t)((uint32_t)x * (uint32_t)J.q15_a + (uint32_t)J.q15_b) >> 15;
-            v = (uint32_t)min(max(q, 0), 255);
+            v = q15_clamp(x, J.q15_a, J.q15_b);
             break;
         }
         default: {
@@ -1153,12 +1162,14 @@ __global__ void k_put_bytes(uint8_t* dst, PutBytes pb, int n, const uint32_t* cr
 // ---------------------------------------------------------------------------
 hipError_t launch_build_lut(const LutBuild& lb, void* out, hipStream_t s)
 {
+    (void)hipGetLastError();
     k_build_lut<<<(lb.entries + 255) / 256, 256, 0, s>>>(lb, out);
     return hipGetLastError();
 }
 
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
 {
+    (void)hipGetLastError();
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)k_filter,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1170,6 +1181,7 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
 
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hipStream_t s)
 {
+    (void)hipGetLastError();
     if (lane_bytes == 16) k_deflate<16><<<grid, kDeflateThreads, 0, s>>>(DL);
     else                  k_deflate<32><<<grid, kDeflateThreads, 0, s>>>(DL);
     return hipGetLastError();
@@ -1177,18 +1189,21 @@ hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hip
 
 hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
 {
+    (void)hipGetLastError();
     k_scan<<<njobs, 256, 0, s>>>(AL);
     return hipGetLastError();
 }
 
 hipError_t launch_assemble(const AssembleLaunch& AL, int grid, hipStream_t s)
 {
+    (void)hipGetLastError();
     k_assemble<<<grid, kAssembleThreads, 0, s>>>(AL);
     return hipGetLastError();
 }
 
 hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s)
 {
+    (void)hipGetLastError();
     const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
     if (nb > 0) k_adler_partial<<<nb, 256, 0, s>>>(p, n, part);
     k_checksum_final<<<1, 256, 0, s>>>(part, nb, n, 0, result);
@@ -1197,6 +1212,7 @@ hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* r
 
 hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s)
 {
+    (void)hipGetLastError();
     const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
     if (nb > 0) k_crc_partial<<<nb, 256, 0, s>>>(p, n, part);
     k_checksum_final<<<1, 256, 0, s>>>(part, nb, n, 1, result);
@@ -1206,6 +1222,7 @@ hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* res
 hipError_t launch_put_bytes(uint8_t* dst, const uint8_t* bytes16, int n, const uint32_t* crc_src,
                             hipStream_t s)
 {
+    (void)hipGetLastError();
     PutBytes pb;
     for (int i = 0; i < 16; i++) pb.b[i] = i < n ? bytes16[i] : 0;
     k_put_bytes<<<1, 64, 0, s>>>(dst, pb, n, crc_src);

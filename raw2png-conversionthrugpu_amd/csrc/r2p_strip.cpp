@@ -206,6 +206,15 @@ int r2p_deflate_copy_to_host(const r2p_deflate* d, void* h_dst, size_t n)
     return R2P_OK;
 }
 
+int r2p_deflate_rewind(r2p_deflate* d)
+{
+    if (!d) { set_error("null deflate context"); return R2P_ERR_ARG; }
+    if (d->pending) { int rc = r2p_deflate_sync(d); if (rc) return rc; }
+    R2P_HIP(hipMemsetAsync(d->d_state, 0, 8, d->stream));
+    d->out_bytes = 0;
+    return R2P_OK;
+}
+
 int r2p_deflate_lz_stats(const r2p_deflate* d, r2p_lz_stats* s)
 {
     if (!d || !s) { set_error("bad stats arguments"); return R2P_ERR_ARG; }

@@ -210,8 +210,21 @@ def test_encode_png_workloads(torch_dev, r2p, synth, oracle, kind, lz77):
     png, filt, adler, nseg, t = encode_device(torch_dev, r2p, s.pixels, s.img, opt)
     z = _check_png(oracle, r2p, png, s.pixels, s.img, filt)
     assert adler == zlib.adler32(filt.tobytes())
-    if lz77 and kind == "camera_small":
-        assert len(z) < 0.9 * filt.size, "LZ77 found no matches on the textured image"
+
+
+def test_lz77_finds_repeats(torch_dev, r2p, oracle):
+    """Noise-free tiled texture, flat areas and gradients: LZ77 must shrink them."""
+    log("case lz77 repeats")
+    rng = np.random.default_rng(6)
+    w, h = 1000, 300
+    tile = rng.integers(0, 256, (32, 32, 3), dtype=np.uint8)
+    px = np.tile(tile, (h // 32 + 1, w // 32 + 1, 1))[:h, :w].copy()
+    px[200:, :, :] = 77
+    img = r2p.Image.make(w, h, 3, 8, 0)
+    png0, filt, _, _, _ = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=0))
+    png1, filt1, _, _, _ = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=1))
+    z1 = _check_png(oracle, r2p, png1, px, img, filt1)
+    assert len(png1) < 0.35 * len(png0), (len(png1), len(png0))
 
 
 @pytest.mark.parametrize("seg,chunk", [(4096, 1000), (65536, 65536), (1 << 20, 8192)])
