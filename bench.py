@@ -248,6 +248,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},
     }
+    log("bench: plain " + json.dumps({k: out[k] for k in ("value", "ms_per_step", "latency_ms", "kernels_ms", "roofline")}))
     fk = "filter"
     fach = r["algo_bytes"][fk] / (kern[fk] * 1e-3) / 1e9
     out["filter_kernel"] = {"achieved": round(fach, 1), "frac": round(fach / HBM_PEAK_GBS, 4)}

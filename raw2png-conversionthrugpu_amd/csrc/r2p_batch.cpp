@@ -37,8 +37,7 @@ struct LutKey {
     }
 };
 
-constexpr int kLutSlot = 65536;      // bytes per table slot
-constexpr int kMaxLuts = 64;
+constexpr int kLutSlot = 65536;      // bytes per table slot (the largest table)
 
 }  // namespace
 
@@ -152,7 +151,7 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     TRY(hipMalloc((void**)&b->d_seg_adler, b->max_segs * 4));
     TRY(hipMalloc((void**)&b->d_seg_off, (b->max_segs + (size_t)max_images + 1) * 8));
     TRY(hipMalloc((void**)&b->d_out, b->max_out));
-    TRY(hipMalloc((void**)&b->d_lut, (size_t)kLutSlot * kMaxLuts));
+    TRY(hipMalloc((void**)&b->d_lut, (size_t)kLutSlot * (size_t)max_images));
     TRY(hipMalloc(&b->d_tab, b->tab_bytes));
     TRY(hipHostMalloc(&b->h_tab, b->tab_bytes, hipHostMallocDefault));
     TRY(hipMalloc((void**)&b->d_res, (size_t)max_images * 16));
@@ -254,7 +253,6 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
             int slot;
             if (it == lut_slot.end()) {
                 slot = (int)lut_slot.size();
-                if (slot >= kMaxLuts) { set_error("more than %d distinct DICOM parameter sets", kMaxLuts); return R2P_ERR_UNSUPPORTED; }
                 lut_slot[k] = slot;
                 b->luts.push_back({k, slot});
             } else {

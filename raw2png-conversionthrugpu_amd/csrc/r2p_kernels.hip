@@ -55,18 +55,32 @@ __device__ __forceinline__ uint32_t cost8(uint32_t v, uint32_t acc)
 {
     return __builtin_amdgcn_sad_u8(v ^ 0x80808080u, 0x80808080u, acc);
 }
-// Paeth predictor for 4 packed bytes.
+// Paeth predictor (PNG spec 9.4) for 4 packed bytes.  The bytes are split
+// into even/odd 16-bit lanes and processed with packed 16-bit min/max/sub:
+//   pa = |b-c|, pb = |a-c|, pc = |a+b-2c|   (all <= 510, so 16 bits suffice)
+//   a if pa <= min(pb, pc), else b if pb <= pc, else c.
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef short ss2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t paeth_lanes(uint32_t a32, uint32_t b32, uint32_t c32)
+{
+    const us2 a = __builtin_bit_cast(us2, a32), b = __builtin_bit_cast(us2, b32), c = __builtin_bit_cast(us2, c32);
+    const us2 pa = __builtin_elementwise_max(b, c) - __builtin_elementwise_min(b, c);
+    const us2 pb = __builtin_elementwise_max(a, c) - __builtin_elementwise_min(a, c);
+    const us2 s = a + b, t = c + c;
+    const us2 pc = __builtin_elementwise_max(s, t) - __builtin_elementwise_min(s, t);
+    const us2 m = __builtin_elementwise_min(pb, pc);
+    const uint32_t not_a = __builtin_bit_cast(uint32_t, (ss2)(__builtin_bit_cast(ss2, m - pa) >> 15));
+    const uint32_t take_c = __builtin_bit_cast(uint32_t, (ss2)(__builtin_bit_cast(ss2, pc - pb) >> 15));
+    const uint32_t bc = (c32 & take_c) | (b32 & ~take_c);
+    return (bc & not_a) | (a32 & ~not_a);
+}
+
 __device__ __forceinline__ uint32_t paeth4(uint32_t a, uint32_t b, uint32_t c)
 {
-    uint32_t r = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int ak = (a >> (8 * k)) & 0xFF, bk = (b >> (8 * k)) & 0xFF, ck = (c >> (8 * k)) & 0xFF;
-        const int pa = abs(bk - ck), pb = abs(ak - ck), pc = abs(ak + bk - 2 * ck);
-        const int p = (pa <= pb && pa <= pc) ? ak : (pb <= pc ? bk : ck);
-        r |= (uint32_t)p << (8 * k);
-    }
-    return r;
+    const uint32_t lo = paeth_lanes(a & 0x00FF00FFu, b & 0x00FF00FFu, c & 0x00FF00FFu);
+    const uint32_t hi = paeth_lanes((a >> 8) & 0x00FF00FFu, (b >> 8) & 0x00FF00FFu, (c >> 8) & 0x00FF00FFu);
+    return lo | (hi << 8);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
@@ -384,6 +398,7 @@ __device__ __forceinline__ uint32_t byte_mask(int nvalid_in_dword)
     return nvalid_in_dword >= 4 ? 0xFFFFFFFFu : nvalid_in_dword <= 0 ? 0u : ((1u << (8 * nvalid_in_dword)) - 1u);
 }
 
+template <int MAXK>
 __global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
 {
     extern __shared__ u32x4 smem_filter[];
@@ -450,9 +465,9 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
             __syncthreads();
 
             uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-            uint32_t f4c[kFilterMaxK][4];
+            uint32_t f4c[MAXK][4];
 #pragma unroll
-            for (int k = 0; k < kFilterMaxK; k++) {
+            for (int k = 0; k < MAXK; k++) {
                 const int i = tid + k * (int)blockDim.x;
                 if (k < kmax && i < nitems) {
                     uint32_t dc[6], dp[6], a[4], c[4];
@@ -493,7 +508,7 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
             uint8_t* dst = J.dst + (int64_t)r * (rb + 1);
             if (tid == 0) dst[0] = (uint8_t)best;
 #pragma unroll
-            for (int k = 0; k < kFilterMaxK; k++) {
+            for (int k = 0; k < MAXK; k++) {
                 const int i = tid + k * (int)blockDim.x;
                 if (k < kmax && i < nitems) {
                     uint32_t o[4];
@@ -1167,16 +1182,34 @@ hipError_t launch_build_lut(const LutBuild& lb, void* out, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int K>
+static hipError_t launch_filter_k(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
+{
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_filter<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    k_filter<K><<<grid, kFilterThreads, lds, s>>>(L);
+    return hipGetLastError();
+}
+
+// Instantiation by the widest row: K 16-byte items per thread cover rows of
+// up to K * 4096 bytes, and K sets how many Paeth results stay in registers.
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
 {
     (void)hipGetLastError();
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_filter,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
+    const int items = (L.row_cap + 15) / 16;
+    const int k = (items + kFilterThreads - 1) / kFilterThreads;
+    switch (k) {
+    case 1: return launch_filter_k<1>(L, grid, lds, s);
+    case 2: return launch_filter_k<2>(L, grid, lds, s);
+    case 3: return launch_filter_k<3>(L, grid, lds, s);
+    case 4: return launch_filter_k<4>(L, grid, lds, s);
+    case 5: return launch_filter_k<5>(L, grid, lds, s);
+    case 6: return launch_filter_k<6>(L, grid, lds, s);
+    default: return launch_filter_k<8>(L, grid, lds, s);
     }
-    k_filter<<<grid, kFilterThreads, lds, s>>>(L);
-    return hipGetLastError();
 }
 
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hipStream_t s)
