@@ -79,7 +79,7 @@ def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
 def run_plain(args, r2p, synth, torch):
     s = synth.workload(args.workload)
     img = s.img
-    opt = r2p.options(lz77=args.lz77)
+    opt = r2p.options(lz77=args.lz77, lane_bytes=args.lane)
     fb = r2p.filtered_bytes(img)
     enc = r2p.Encoder(fb, opt)
     d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
@@ -211,6 +211,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--lz77", type=int, default=1)
+    ap.add_argument("--lane", type=int, default=32, choices=[16, 32])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
     args = ap.parse_args()

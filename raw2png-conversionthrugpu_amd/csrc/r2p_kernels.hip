@@ -750,7 +750,7 @@ constexpr int kRound = 1024;
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
 
-__device__ __forceinline__ int match_len(const uint8_t* d, int64_t P, int64_t C, int cap)
+__device__ __forceinline__ int match_len(const uint8_t* d, int P, int C, int cap)
 {
     int n = 0;
     while (n + 4 <= cap) {
@@ -780,16 +780,21 @@ __device__ __forceinline__ uint32_t match_bits(uint32_t len, uint32_t dist)
     return (c_len[len] >> 16) + dn;
 }
 
-template <int LANE>
-__global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
+// U consecutive rounds of the match phase are handled by each thread for U
+// positions at once, so the global loads of those positions (the 4 bytes at
+// P and the first 4 bytes at every candidate) are all in flight together.
+template <int NT, int LANE, int U>
+__global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 {
-    constexpr int NT = kDeflateThreads;
-    constexpr int SUB = NT * LANE;
+    constexpr int SUB = NT * LANE;               // positions per sub-block
+    // U: positions per thread per round
+    constexpr int ROUND = NT * U;
     constexpr int OBUF = (SUB * 9 + 31) / 32 + 8;
+    constexpr int NW = NT / 64;
     __shared__ uint32_t htab[1 << kHashBits];
     __shared__ uint32_t M[SUB];
     __shared__ uint32_t obuf[OBUF];
-    __shared__ uint32_t red[64];
+    __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -819,9 +824,22 @@ __global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
         for (int i = tid; i < (1 << kHashBits); i += NT) htab[i] = 0;
         if (tid < 4) st[tid] = 0;
         __syncthreads();
-        if (J.lz77)
-            for (int64_t P = wlo + tid; P < s0; P += NT)
-                if (P + 4 <= dend) atomicMax(&htab[hash4(ld32u(d + P))], (uint32_t)(P - wlo + 1));
+        if (J.lz77) {
+            // prime with the (up to) 32 KiB before the segment
+            for (int64_t P0 = wlo + tid; P0 < s0; P0 += (int64_t)NT * U) {
+                uint32_t w[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int64_t P = P0 + (int64_t)u * NT;
+                    w[u] = (P < s0 && P + 4 <= dend) ? ld32u(d + P) : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int64_t P = P0 + (int64_t)u * NT;
+                    if (P < s0 && P + 4 <= dend) atomicMax(&htab[hash4(w[u])], (uint32_t)(P - wlo + 1));
+                }
+            }
+        }
 
         uint32_t carry = last ? 3u : 2u;   // BFINAL | BTYPE=01 << 1
         uint32_t carry_bits = 3;
@@ -832,42 +850,74 @@ __global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
             __syncthreads();                       // previous flush done with obuf
             for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;
             // ---- 1. match phase -------------------------------------------------
-            for (int r0 = 0; r0 < n; r0 += kRound) {
+            // positions are handled relative to wb = d + wlo (window base, < 2^31)
+            const uint8_t* wb = d + wlo;
+            const int qrel = (int)(q0 - wlo);
+            const int end_rel = (int)min64(dend - wlo, 0x7FFFFFF0);
+            for (int r0 = 0; r0 < n; r0 += ROUND) {
                 __syncthreads();                   // inserts of the previous round visible
-                const int r1 = min(n, r0 + kRound);
-                for (int pr = r0 + tid; pr < r1; pr += NT) {
-                    const int64_t P = q0 + pr;
+                uint32_t w[U];
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int pr = r0 + u * NT + tid;
+                    ok[u] = pr < n && qrel + pr + 4 <= end_rel;
+                    w[u] = ok[u] ? ld32u(wb + qrel + pr) : 0u;
+                }
+                // candidates (window-relative; -1 = none) and their first dwords,
+                // all loads issued before any comparison
+                int cand[U][4];
+                uint32_t cw[U][4];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int Pr = qrel + r0 + u * NT + tid;
+                    const uint32_t hv = (J.lz77 && ok[u]) ? htab[hash4(w[u])] : 0u;
+                    cand[u][0] = (int)hv - 1;
+                    cand[u][1] = Pr - 1;
+                    cand[u][2] = J.bpp > 1 ? Pr - J.bpp : -1;
+                    cand[u][3] = J.row_stride > 0 ? Pr - J.row_stride : -1;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int C = cand[u][c];
+                        const bool good = J.lz77 && ok[u] && C >= 0 && C < Pr && Pr - C <= kWindow;
+                        if (!good) cand[u][c] = -1;
+                        cw[u][c] = good ? ld32u(wb + C) : ~w[u];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int pr = r0 + u * NT + tid;
+                    if (pr >= n) continue;
+                    const int Pr = qrel + pr;
                     const int lane_end = min(n, (pr / LANE + 1) * LANE);
                     const int cap = min(258, lane_end - pr);
                     uint32_t best_len = 0, best_dist = 0;
-                    if (J.lz77 && cap >= 3 && P + 4 <= dend) {
-                        const uint32_t w = ld32u(d + P);
-                        const uint32_t hv = htab[hash4(w)];
-                        int64_t cands[4];
-                        cands[0] = hv ? (int64_t)hv - 1 + wlo : -1;
-                        cands[1] = P - 1;
-                        cands[2] = J.bpp > 1 ? P - J.bpp : -1;
-                        cands[3] = J.row_stride > 0 ? P - J.row_stride : -1;
+                    if (cap >= 3) {
 #pragma unroll
                         for (int c = 0; c < 4; c++) {
-                            const int64_t C = cands[c];
-                            if (C < wlo || C >= P || P - C > kWindow) continue;
-                            const int l = match_len(d, P, C, cap);
-                            const uint32_t dist = (uint32_t)(P - C);
+                            const int C = cand[u][c];
+                            if (C < 0) continue;
+                            const uint32_t x = w[u] ^ cw[u][c];
+                            int l;
+                            if (x) l = (__ffs(x) - 1) >> 3;
+                            else l = 4 + match_len(wb, Pr + 4, C + 4, cap - 4);
+                            l = min(l, cap);
+                            const uint32_t dist = (uint32_t)(Pr - C);
                             if ((uint32_t)l > best_len || ((uint32_t)l == best_len && dist < best_dist)) {
                                 best_len = (uint32_t)l; best_dist = dist;
                             }
                         }
                         if (best_len < 3 || match_bits(best_len, best_dist) > 8 * best_len) best_len = 0;
                     }
-                    M[pr] = (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | (uint32_t)d[P];
+                    const uint32_t byte = ok[u] ? (w[u] & 0xFFu) : (uint32_t)wb[Pr];
+                    M[pr] = (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | byte;
                 }
                 __syncthreads();                   // lookups of this round done
-                if (J.lz77)
-                    for (int pr = r0 + tid; pr < r1; pr += NT) {
-                        const int64_t P = q0 + pr;
-                        if (P + 4 <= dend) atomicMax(&htab[hash4(ld32u(d + P))], (uint32_t)(P - wlo + 1));
-                    }
+                if (J.lz77) {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (ok[u]) atomicMax(&htab[hash4(w[u])], (uint32_t)(qrel + r0 + u * NT + tid + 1));
+                }
             }
             __syncthreads();
             if (tid == 0) obuf[0] = carry;
@@ -898,8 +948,8 @@ __global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
                 atomicAdd(&st[2], m_dist); atomicMax(&st[3], m_max);
             }
             __syncthreads();
-            uint32_t wbase = 0, total = 0;
-            for (int w = 0; w < NT / 64; w++) { if (w < wave) wbase += red[w]; total += red[w]; }
+            uint32_t wbase = 0, tot = 0;
+            for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
             // ---- 4. emission -----------------------------------------------------------
             {
@@ -940,7 +990,7 @@ __global__ __launch_bounds__(kDeflateThreads) void k_deflate(DeflateLaunch DL)
                 if (nacc > 0) atomicOr(&obuf[w], (uint32_t)acc);
             }
             __syncthreads();
-            const uint32_t tb = carry_bits + total;
+            const uint32_t tb = carry_bits + tot;
             const uint32_t full = tb >> 5;
             for (uint32_t i = tid; i < full; i += NT) out[out_dw + i] = obuf[i];
             carry = (tb & 31) ? obuf[full] : 0u;
@@ -1212,11 +1262,13 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
     }
 }
 
+// lane_bytes selects the workgroup shape: 16 -> 1024 threads x 16-byte parse
+// spans, 32 -> 512 x 32 (both 16 K positions per sub-block).
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hipStream_t s)
 {
     (void)hipGetLastError();
-    if (lane_bytes == 16) k_deflate<16><<<grid, kDeflateThreads, 0, s>>>(DL);
-    else                  k_deflate<32><<<grid, kDeflateThreads, 0, s>>>(DL);
+    if (lane_bytes == 16) k_deflate<1024, 16, 2><<<grid, 1024, 0, s>>>(DL);
+    else                  k_deflate<512, 32, 4><<<grid, 512, 0, s>>>(DL);
     return hipGetLastError();
 }
 
