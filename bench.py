@@ -147,13 +147,18 @@ def run_batch(args, r2p, synth, torch):
     steps = max(3, args.batch_steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ktimes = []
+    gtimes = []
     for _ in range(steps):
         b.run(True); b.sync()
-        t = b.timings()
-        ktimes.append((t.filter_ms, t.deflate_ms, t.scan_ms, t.assemble_ms, t.total_ms))
+        gtimes.append(b.timings().total_ms)
     torch.cuda.synchronize()
     bwall = (time.perf_counter() - t1) / steps
+    # per-kernel split from eager (non-graph) runs of the same batch
+    ktimes = []
+    for _ in range(2):
+        b.run(False); b.sync()
+        t = b.timings()
+        ktimes.append((t.filter_ms, t.deflate_ms, t.scan_ms, t.assemble_ms, t.total_ms))
     # plain form, one image at a time, same options
     maxf = max(r2p.filtered_bytes(im) for im in imgs)
     enc = r2p.Encoder(maxf, tuned)
@@ -184,6 +189,7 @@ def run_batch(args, r2p, synth, torch):
     return {
         "images": len(imgs), "megapixels": round(mp, 1), "filtered_gb": round(fsum / 1e9, 3),
         "batched_mp_per_s": round(mp / bwall, 1), "batched_ms": round(bwall * 1e3, 3),
+        "graph_device_ms": round(float(np.median(gtimes)), 3),
         "plain_mp_per_s": round(mp / pwall, 1), "plain_ms": round(pwall * 1e3, 3),
         "speedup_batched_over_plain": round(pwall / bwall, 3),
         "plain_latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},

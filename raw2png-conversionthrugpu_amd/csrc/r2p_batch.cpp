@@ -77,11 +77,14 @@ struct r2p_batch {
     hipGraphExec_t gexec = nullptr;
     int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1;
     bool ran = false;
+    bool timed_kernels = false;
     std::vector<r2p_image> shadow_img;
     std::vector<const void*> shadow_src;
 };
 
-static int batch_enqueue(r2p_batch* b, hipStream_t s)
+// Enqueues one batch run.  Per-kernel events are recorded only for eager
+// runs (timed); a captured graph is timed as a whole around its launch.
+static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
 {
     // 1. job tables
     R2P_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, s));
@@ -90,21 +93,21 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s)
     L.jobs = b->d_fj; L.band_start = b->d_band_start; L.njobs = 0; L.band_rows = b->band_rows;
     L.total_bands = 0; L.row_cap = b->row_cap; L.lut_cap = b->lut_cap; L.counts = b->d_counts;
     const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 128;
-    R2P_HIP(hipEventRecord(b->ev[0], s));
+    if (timed) R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(launch_filter(L, 4096, lds, s));
-    R2P_HIP(hipEventRecord(b->ev[1], s));
+    if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
     // 3. deflate over all segments
     DeflateLaunch DL{};
     DL.jobs = b->d_dj; DL.seg_start = b->d_seg_start; DL.counts = b->d_counts + 2;
     R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, 1024, s));
-    R2P_HIP(hipEventRecord(b->ev[2], s));
+    if (timed) R2P_HIP(hipEventRecord(b->ev[2], s));
     // 4. per-image scan, 5. assemble all chunks
     AssembleLaunch AL{};
     AL.jobs = b->d_aj; AL.chunk_start = b->d_chunk_start; AL.counts = b->d_counts + 4;
     R2P_HIP(launch_scan(AL, b->max_images, s));
-    R2P_HIP(hipEventRecord(b->ev[3], s));
+    if (timed) R2P_HIP(hipEventRecord(b->ev[3], s));
     R2P_HIP(launch_assemble(AL, 2048, s));
-    R2P_HIP(hipEventRecord(b->ev[4], s));
+    if (timed) R2P_HIP(hipEventRecord(b->ev[4], s));
     // 6. lengths and adlers back to pinned host memory
     R2P_HIP(hipMemcpyAsync(b->h_res, b->d_res, (size_t)b->max_images * 16, hipMemcpyDeviceToHost, s));
     return R2P_OK;
@@ -333,9 +336,10 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
     if (!b || b->n <= 0) { set_error("batch not set"); return R2P_ERR_STATE; }
     hipStream_t s = b->stream;
     if (!use_graph) {
-        int rc = batch_enqueue(b, s);
+        int rc = batch_enqueue(b, s, true);
         if (rc) return rc;
         b->ran = true;
+        b->timed_kernels = true;
         return R2P_OK;
     }
     // graph keyed by the launch parameters that are baked into its nodes
@@ -346,7 +350,7 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
         if (b->graph) { R2P_HIP(hipGraphDestroy(b->graph)); b->graph = nullptr; }
         R2P_HIP(hipStreamSynchronize(s));
         R2P_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = batch_enqueue(b, s);
+        int rc = batch_enqueue(b, s, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -355,8 +359,11 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
         R2P_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
         b->g_row_cap = b->row_cap; b->g_lut_cap = b->lut_cap; b->g_band = b->band_rows; b->g_lane = lane;
     }
+    R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(hipGraphLaunch(b->gexec, s));
+    R2P_HIP(hipEventRecord(b->ev[4], s));
     b->ran = true;
+    b->timed_kernels = false;
     return R2P_OK;
 }
 
@@ -391,10 +398,13 @@ int r2p_batch_timings(r2p_batch* b, r2p_timings* t)
 {
     if (!b || !t || !b->ran) { set_error("no batch timings"); return R2P_ERR_STATE; }
     R2P_HIP(hipEventSynchronize(b->ev[4]));
-    R2P_HIP(hipEventElapsedTime(&t->filter_ms, b->ev[0], b->ev[1]));
-    R2P_HIP(hipEventElapsedTime(&t->deflate_ms, b->ev[1], b->ev[2]));
-    R2P_HIP(hipEventElapsedTime(&t->scan_ms, b->ev[2], b->ev[3]));
-    R2P_HIP(hipEventElapsedTime(&t->assemble_ms, b->ev[3], b->ev[4]));
+    *t = r2p_timings{};
+    if (b->timed_kernels) {
+        R2P_HIP(hipEventElapsedTime(&t->filter_ms, b->ev[0], b->ev[1]));
+        R2P_HIP(hipEventElapsedTime(&t->deflate_ms, b->ev[1], b->ev[2]));
+        R2P_HIP(hipEventElapsedTime(&t->scan_ms, b->ev[2], b->ev[3]));
+        R2P_HIP(hipEventElapsedTime(&t->assemble_ms, b->ev[3], b->ev[4]));
+    }
     R2P_HIP(hipEventElapsedTime(&t->total_ms, b->ev[0], b->ev[4]));
     return R2P_OK;
 }

@@ -37,6 +37,19 @@ __constant__ uint32_t c_x8pow[64];     // x^(8 * 2^k) mod P, reflected
 // Small helpers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t ld32u(const uint8_t* p) { return *(const u32_ua*)p; }
+
+// Loads from pointers known to address global memory.  The job descriptors
+// carry generic pointers; casting to the global address space lets the
+// compiler emit global_load (vmcnt only) instead of flat_load, whose
+// completion is also tracked by lgkmcnt and would serialise it against the
+// LDS traffic it is interleaved with.
+#define R2P_GLOBAL __attribute__((address_space(1)))
+typedef R2P_GLOBAL const u32_ua gu32_ua;
+typedef R2P_GLOBAL const u32x4_ua gu32x4_ua;
+typedef R2P_GLOBAL const uint8_t gu8;
+__device__ __forceinline__ uint32_t g32u(const uint8_t* p) { return *(gu32_ua*)p; }
+__device__ __forceinline__ u32x4 g128u(const uint8_t* p) { return *(gu32x4_ua*)p; }
+__device__ __forceinline__ uint32_t g8(const uint8_t* p) { return *(gu8*)p; }
 __device__ __forceinline__ int64_t min64(int64_t a, int64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ int64_t max64(int64_t a, int64_t b) { return a > b ? a : b; }
 
@@ -277,17 +290,17 @@ __device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* 
         u32x4 o;
         switch (J.wmode) {
         case WM_COPY: {
-            o = *(const u32x4_ua*)(src_row + 16 * i);
+            o = g128u(src_row + 16 * i);
             break;
         }
         case WM_SWAP16: {
-            const u32x4 v = *(const u32x4_ua*)(src_row + 16 * i);
+            const u32x4 v = g128u(src_row + 16 * i);
 #pragma unroll
             for (int k = 0; k < 4; k++) o[k] = ((v[k] & 0x00FF00FFu) << 8) | ((v[k] >> 8) & 0x00FF00FFu);
             break;
         }
         case WM_LUT16: {
-            const u32x4 v = *(const u32x4_ua*)(src_row + 16 * i);
+            const u32x4 v = g128u(src_row + 16 * i);
             const uint16_t* t16 = (const uint16_t*)lut_lds;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -299,7 +312,7 @@ __device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* 
             break;
         }
         case WM_LUT8_FROM8: {
-            const u32x4 v = *(const u32x4_ua*)(src_row + 16 * i);
+            const u32x4 v = g128u(src_row + 16 * i);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 o[k] = (uint32_t)lut_lds[v[k] & 0xFF] | ((uint32_t)lut_lds[(v[k] >> 8) & 0xFF] << 8) |
@@ -308,8 +321,8 @@ __device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* 
             break;
         }
         case WM_Q15: {
-            const u32x4 v0 = *(const u32x4_ua*)(src_row + 32 * i);
-            const u32x4 v1 = *(const u32x4_ua*)(src_row + 32 * i + 16);
+            const u32x4 v0 = g128u(src_row + 32 * i);
+            const u32x4 v1 = g128u(src_row + 32 * i + 16);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
@@ -326,8 +339,8 @@ __device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* 
             break;
         }
         default: {  // WM_LUT8_FROM16
-            const u32x4 v0 = *(const u32x4_ua*)(src_row + 32 * i);
-            const u32x4 v1 = *(const u32x4_ua*)(src_row + 32 * i + 16);
+            const u32x4 v0 = g128u(src_row + 32 * i);
+            const u32x4 v1 = g128u(src_row + 32 * i + 16);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
@@ -556,7 +569,7 @@ __device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
     uint64_t T = 0;
     const int64_t nfull = n >> 4;
     for (int64_t i = tid; i < nfull; i += blockDim.x) {
-        const u32x4 v = *(const u32x4_ua*)(p + 16 * i);
+        const u32x4 v = g128u(p + 16 * i);
         uint32_t s16 = 0, w16 = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -568,8 +581,9 @@ __device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
         T += (uint64_t)(16 * i) * s16 + w16;
     }
     for (int64_t j = (nfull << 4) + tid; j < n; j += blockDim.x) {
-        S += p[j];
-        T += (uint64_t)j * p[j];
+        const uint32_t b = g8(p + j);
+        S += b;
+        T += (uint64_t)j * b;
     }
     S = wave_sum(S);
     T = wave_sum64(T);
@@ -754,11 +768,11 @@ __device__ __forceinline__ int match_len(const uint8_t* d, int P, int C, int cap
 {
     int n = 0;
     while (n + 4 <= cap) {
-        const uint32_t x = ld32u(d + P + n) ^ ld32u(d + C + n);
+        const uint32_t x = g32u(d + P + n) ^ g32u(d + C + n);
         if (x) return n + ((__ffs(x) - 1) >> 3);
         n += 4;
     }
-    while (n < cap && d[P + n] == d[C + n]) n++;
+    while (n < cap && g8(d + P + n) == g8(d + C + n)) n++;
     return n;
 }
 
@@ -792,7 +806,11 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
     constexpr int OBUF = (SUB * 9 + 31) / 32 + 8;
     constexpr int NW = NT / 64;
     __shared__ uint32_t htab[1 << kHashBits];
-    __shared__ uint32_t M[SUB];
+    // Per-position match records, stored lane-major with one word of padding
+    // per row (index (p % LANE) * (NT + 1) + p / LANE) so that the parse,
+    // where lane t walks positions t*LANE.., reads consecutive words, and the
+    // match phase's consecutive positions land in distinct banks.
+    __shared__ uint32_t M[LANE * (NT + 1)];
     __shared__ uint32_t obuf[OBUF];
     __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
@@ -831,7 +849,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const int64_t P = P0 + (int64_t)u * NT;
-                    w[u] = (P < s0 && P + 4 <= dend) ? ld32u(d + P) : 0u;
+                    w[u] = (P < s0 && P + 4 <= dend) ? g32u(d + P) : 0u;
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
@@ -862,7 +880,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                 for (int u = 0; u < U; u++) {
                     const int pr = r0 + u * NT + tid;
                     ok[u] = pr < n && qrel + pr + 4 <= end_rel;
-                    w[u] = ok[u] ? ld32u(wb + qrel + pr) : 0u;
+                    w[u] = ok[u] ? g32u(wb + qrel + pr) : 0u;
                 }
                 // candidates (window-relative; -1 = none) and their first dwords,
                 // all loads issued before any comparison
@@ -881,7 +899,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                         const int C = cand[u][c];
                         const bool good = J.lz77 && ok[u] && C >= 0 && C < Pr && Pr - C <= kWindow;
                         if (!good) cand[u][c] = -1;
-                        cw[u][c] = good ? ld32u(wb + C) : ~w[u];
+                        cw[u][c] = good ? g32u(wb + C) : ~w[u];
                     }
                 }
 #pragma unroll
@@ -909,8 +927,8 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                         }
                         if (best_len < 3 || match_bits(best_len, best_dist) > 8 * best_len) best_len = 0;
                     }
-                    const uint32_t byte = ok[u] ? (w[u] & 0xFFu) : (uint32_t)wb[Pr];
-                    M[pr] = (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | byte;
+                    const uint32_t byte = ok[u] ? (w[u] & 0xFFu) : g8(wb + Pr);
+                    M[(pr % LANE) * (NT + 1) + pr / LANE] = (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | byte;
                 }
                 __syncthreads();                   // lookups of this round done
                 if (J.lz77) {
@@ -927,7 +945,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             uint32_t tok = 0, nbits = 0;
             uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
             for (int p = lb; p < le;) {
-                const uint32_t m = M[p];
+                const uint32_t m = M[(p - lb) * (NT + 1) + tid];
                 const uint32_t len = m >> 23;
                 tok |= 1u << (p - lb);
                 if (len >= 3) {
@@ -961,7 +979,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                 while (tk) {
                     const int i = __ffs(tk) - 1;
                     tk &= tk - 1;
-                    const uint32_t m = M[lb + i];
+                    const uint32_t m = M[i * (NT + 1) + tid];
                     const uint32_t len = m >> 23;
                     uint32_t v, nb;
                     if (len >= 3) {
