@@ -79,7 +79,7 @@ def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
 def run_plain(args, r2p, synth, torch):
     s = synth.workload(args.workload)
     img = s.img
-    opt = r2p.options(lz77=args.lz77, lane_bytes=args.lane)
+    opt = r2p.options(lz77=args.lz77, lane_bytes=args.lane, wg_threads=args.wg)
     fb = r2p.filtered_bytes(img)
     enc = r2p.Encoder(fb, opt)
     d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
@@ -199,6 +199,7 @@ def run_batch(args, r2p, synth, torch):
         "filter_gbs": round((isum + fsum) / (kt[0] * 1e-3) / 1e9, 1),
         "deflate_gbs": round((fsum + png_total) / (kt[1] * 1e-3) / 1e9, 1),
         "autotuned": {"segment_bytes": tuned.segment_bytes, "lane_bytes": tuned.lane_bytes,
+                      "wg_threads": tuned.wg_threads,
                       "band_rows": tuned.band_rows, "best_ms": tuned_ms},
         "png_over_filtered": round(png_total / fsum, 4),
         "byte_identical_to_plain": bool(same), "steps": steps,
@@ -217,7 +218,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--lz77", type=int, default=1)
-    ap.add_argument("--lane", type=int, default=32, choices=[16, 32])
+    ap.add_argument("--lane", type=int, default=16, choices=[16, 32])
+    ap.add_argument("--wg", type=int, default=512, choices=[512, 1024])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
     args = ap.parse_args()

@@ -75,7 +75,7 @@ struct r2p_batch {
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1;
+    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1, g_wg = -1;
     bool ran = false;
     bool timed_kernels = false;
     std::vector<r2p_image> shadow_img;
@@ -99,7 +99,7 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     // 3. deflate over all segments
     DeflateLaunch DL{};
     DL.jobs = b->d_dj; DL.seg_start = b->d_seg_start; DL.counts = b->d_counts + 2;
-    R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, 1024, s));
+    R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, b->opt.wg_threads, 1024, s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[2], s));
     // 4. per-image scan, 5. assemble all chunks
     AssembleLaunch AL{};
@@ -345,7 +345,7 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
     // graph keyed by the launch parameters that are baked into its nodes
     const int lane = b->opt.lane_bytes;
     if (!b->gexec || b->g_row_cap != b->row_cap || b->g_lut_cap != b->lut_cap ||
-        b->g_band != b->band_rows || b->g_lane != lane) {
+        b->g_band != b->band_rows || b->g_lane != lane || b->g_wg != b->opt.wg_threads) {
         if (b->gexec) { R2P_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
         if (b->graph) { R2P_HIP(hipGraphDestroy(b->graph)); b->graph = nullptr; }
         R2P_HIP(hipStreamSynchronize(s));
@@ -357,7 +357,7 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
         b->graph = g;
         R2P_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-        b->g_row_cap = b->row_cap; b->g_lut_cap = b->lut_cap; b->g_band = b->band_rows; b->g_lane = lane;
+        b->g_row_cap = b->row_cap; b->g_lut_cap = b->lut_cap; b->g_band = b->band_rows; b->g_lane = lane; b->g_wg = b->opt.wg_threads;
     }
     R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(hipGraphLaunch(b->gexec, s));
@@ -422,14 +422,14 @@ int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
     r2p_options best = base;
     double best_t = 1e30;
     static const int segs[] = {32768, 65536};
-    static const int lanes[] = {16, 32};
+    static const int shapes[][2] = {{16, 512}, {16, 1024}, {32, 512}};   // lane_bytes, wg_threads
     static const int bands[] = {4, 8, 16};
     for (int sg : segs) {
         if (sg < b->seg_bytes_cap) continue;
-        for (int ln : lanes)
+        for (const auto& sh : shapes)
             for (int bd : bands) {
                 r2p_options o = base;
-                o.segment_bytes = sg; o.lane_bytes = ln; o.band_rows = bd;
+                o.segment_bytes = sg; o.lane_bytes = sh[0]; o.wg_threads = sh[1]; o.band_rows = bd;
                 int rc = r2p_batch_options(b, &o);
                 if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
                 if (rc) continue;

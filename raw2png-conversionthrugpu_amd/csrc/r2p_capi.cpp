@@ -287,7 +287,9 @@ void fill_defaults(r2p_options* o)
     if (o->segment_bytes <= 0) o->segment_bytes = 65536;
     if (o->idat_chunk_bytes <= 0) o->idat_chunk_bytes = 65536;
     if (o->idat_chunk_bytes > 65536) o->idat_chunk_bytes = 65536;
-    if (o->lane_bytes != 16 && o->lane_bytes != 32) o->lane_bytes = 32;
+    if (o->lane_bytes != 16 && o->lane_bytes != 32) o->lane_bytes = 16;
+    if (o->wg_threads != 512 && o->wg_threads != 1024) o->wg_threads = 512;
+    if (o->lane_bytes == 32) o->wg_threads = 512;
     if (o->lz77 < 0) o->lz77 = 1;
 }
 
@@ -371,7 +373,8 @@ void r2p_options_default(r2p_options* o)
     o->idat_chunk_bytes = 65536;
     o->zlib_level = 3;       // PipelineConfig::deflate_level default (pipeline.h:17)
     o->band_rows = 0;
-    o->lane_bytes = 32;
+    o->lane_bytes = 16;
+    o->wg_threads = 512;
 }
 
 uint32_t r2p_adler32_combine(uint32_t a1, uint32_t a2, uint64_t len2)
@@ -598,7 +601,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
     D.lz77 = o.lz77; D.nseg = nseg;
     DL.jobs = nullptr; DL.njobs = 1; DL.total_segs = (uint32_t)nseg;
-    R2P_HIP(launch_deflate(DL, o.lane_bytes, nseg, s));
+    R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, nseg, s));
     R2P_HIP(hipEventRecord(e->ev[2], s));
 
     AssembleLaunch AL{};

@@ -28,7 +28,7 @@ struct DeflateLaunch;
 struct AssembleLaunch;
 hipError_t launch_build_lut(const LutBuild& lb, void* out, hipStream_t s);
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s);
-hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hipStream_t s);
+hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s);
 hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s);
 hipError_t launch_assemble(const AssembleLaunch& AL, int grid, hipStream_t s);
 hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s);

@@ -50,6 +50,9 @@ typedef R2P_GLOBAL const uint8_t gu8;
 __device__ __forceinline__ uint32_t g32u(const uint8_t* p) { return *(gu32_ua*)p; }
 __device__ __forceinline__ u32x4 g128u(const uint8_t* p) { return *(gu32x4_ua*)p; }
 __device__ __forceinline__ uint32_t g8(const uint8_t* p) { return *(gu8*)p; }
+typedef uint64_t u64_ua __attribute__((aligned(1)));
+typedef R2P_GLOBAL const u64_ua gu64_ua;
+__device__ __forceinline__ uint64_t g64u(const uint8_t* p) { return *(gu64_ua*)p; }
 __device__ __forceinline__ int64_t min64(int64_t a, int64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ int64_t max64(int64_t a, int64_t b) { return a > b ? a : b; }
 
@@ -868,28 +871,29 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             __syncthreads();                       // previous flush done with obuf
             for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;
             // ---- 1. match phase -------------------------------------------------
-            // positions are handled relative to wb = d + wlo (window base, < 2^31)
+            // Lookups see the hash table as it stood at the sub-block start
+            // (every position before q0); no barrier inside the phase, so the
+            // waves run their loads independently.  Runs, pixel repeats and the
+            // row above are explicit candidates and need no table.
+            // Positions are relative to wb = d + wlo (window base, < 2^31).
             const uint8_t* wb = d + wlo;
             const int qrel = (int)(q0 - wlo);
             const int end_rel = (int)min64(dend - wlo, 0x7FFFFFF0);
             for (int r0 = 0; r0 < n; r0 += ROUND) {
-                __syncthreads();                   // inserts of the previous round visible
-                uint32_t w[U];
+                uint64_t p8[U];
                 bool ok[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const int pr = r0 + u * NT + tid;
-                    ok[u] = pr < n && qrel + pr + 4 <= end_rel;
-                    w[u] = ok[u] ? g32u(wb + qrel + pr) : 0u;
+                    ok[u] = pr < n && qrel + pr + 8 <= end_rel;
+                    p8[u] = ok[u] ? g64u(wb + qrel + pr) : 0ull;
                 }
-                // candidates (window-relative; -1 = none) and their first dwords,
-                // all loads issued before any comparison
                 int cand[U][4];
-                uint32_t cw[U][4];
+                uint64_t c8[U][4];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const int Pr = qrel + r0 + u * NT + tid;
-                    const uint32_t hv = (J.lz77 && ok[u]) ? htab[hash4(w[u])] : 0u;
+                    const uint32_t hv = (J.lz77 && ok[u]) ? htab[hash4((uint32_t)p8[u])] : 0u;
                     cand[u][0] = (int)hv - 1;
                     cand[u][1] = Pr - 1;
                     cand[u][2] = J.bpp > 1 ? Pr - J.bpp : -1;
@@ -899,7 +903,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                         const int C = cand[u][c];
                         const bool good = J.lz77 && ok[u] && C >= 0 && C < Pr && Pr - C <= kWindow;
                         if (!good) cand[u][c] = -1;
-                        cw[u][c] = good ? g32u(wb + C) : ~w[u];
+                        c8[u][c] = good ? g64u(wb + C) : ~p8[u];
                     }
                 }
 #pragma unroll
@@ -915,10 +919,10 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                         for (int c = 0; c < 4; c++) {
                             const int C = cand[u][c];
                             if (C < 0) continue;
-                            const uint32_t x = w[u] ^ cw[u][c];
+                            const uint64_t x = p8[u] ^ c8[u][c];
                             int l;
-                            if (x) l = (__ffs(x) - 1) >> 3;
-                            else l = 4 + match_len(wb, Pr + 4, C + 4, cap - 4);
+                            if (x) l = (int)(__builtin_ctzll(x) >> 3);
+                            else l = cap > 8 ? 8 + match_len(wb, Pr + 8, C + 8, cap - 8) : 8;
                             l = min(l, cap);
                             const uint32_t dist = (uint32_t)(Pr - C);
                             if ((uint32_t)l > best_len || ((uint32_t)l == best_len && dist < best_dist)) {
@@ -927,17 +931,16 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                         }
                         if (best_len < 3 || match_bits(best_len, best_dist) > 8 * best_len) best_len = 0;
                     }
-                    const uint32_t byte = ok[u] ? (w[u] & 0xFFu) : g8(wb + Pr);
-                    M[(pr % LANE) * (NT + 1) + pr / LANE] = (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | byte;
-                }
-                __syncthreads();                   // lookups of this round done
-                if (J.lz77) {
-#pragma unroll
-                    for (int u = 0; u < U; u++)
-                        if (ok[u]) atomicMax(&htab[hash4(w[u])], (uint32_t)(qrel + r0 + u * NT + tid + 1));
+                    const uint32_t byte = ok[u] ? (uint32_t)(p8[u] & 0xFFu) : g8(wb + Pr);
+                    M[(pr % LANE) * (NT + 1) + pr / LANE] =
+                        (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | byte;
                 }
             }
-            __syncthreads();
+            __syncthreads();                       // all lookups and records of the sub-block done
+            // this sub-block's positions enter the table for the next sub-blocks
+            if (J.lz77)
+                for (int pr = tid; pr < n; pr += NT)
+                    if (qrel + pr + 4 <= end_rel) atomicMax(&htab[hash4(g32u(wb + qrel + pr))], (uint32_t)(qrel + pr + 1));
             if (tid == 0) obuf[0] = carry;
             // ---- 2. parse ----------------------------------------------------------
             const int lb = tid * LANE;
@@ -1280,13 +1283,15 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
     }
 }
 
-// lane_bytes selects the workgroup shape: 16 -> 1024 threads x 16-byte parse
-// spans, 32 -> 512 x 32 (both 16 K positions per sub-block).
-hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int grid, hipStream_t s)
+// Workgroup shapes (threads x parse span): 512 x 16 (8 K positions per
+// sub-block, two workgroups per CU), 1024 x 16 and 512 x 32 (16 K positions,
+// one workgroup per CU).
+hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s)
 {
     (void)hipGetLastError();
-    if (lane_bytes == 16) k_deflate<1024, 16, 2><<<grid, 1024, 0, s>>>(DL);
-    else                  k_deflate<512, 32, 4><<<grid, 512, 0, s>>>(DL);
+    if (lane_bytes == 32)         k_deflate<512, 32, 4><<<grid, 512, 0, s>>>(DL);
+    else if (wg_threads == 1024)  k_deflate<1024, 16, 2><<<grid, 1024, 0, s>>>(DL);
+    else                          k_deflate<512, 16, 4><<<grid, 512, 0, s>>>(DL);
     return hipGetLastError();
 }
 

@@ -130,7 +130,7 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     D.final_block = is_last ? 1 : 0; D.bpp = bpp > 0 ? bpp : 1; D.row_stride = row_stride;
     D.lz77 = d->lz77; D.nseg = nseg;
     DL.jobs = nullptr; DL.njobs = 1; DL.total_segs = (uint32_t)nseg;
-    R2P_HIP(launch_deflate(DL, 32, nseg, s));
+    R2P_HIP(launch_deflate(DL, 16, 512, nseg, s));
 
     AssembleLaunch AL{};
     AssembleJob& A = AL.single;
