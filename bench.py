@@ -79,7 +79,7 @@ def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
 def run_plain(args, r2p, synth, torch):
     s = synth.workload(args.workload)
     img = s.img
-    opt = r2p.options(lz77=args.lz77, lane_bytes=args.lane, wg_threads=args.wg)
+    opt = r2p.options(lz77=args.lz77, wg_threads=args.wg, segment_bytes=args.seg)
     fb = r2p.filtered_bytes(img)
     enc = r2p.Encoder(fb, opt)
     d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
@@ -87,22 +87,26 @@ def run_plain(args, r2p, synth, torch):
     d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
     d_filt = torch.empty(fb, dtype=torch.uint8, device="cuda")
     for _ in range(args.warmup):
-        enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap, d_filt.data_ptr())
+        enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap)
         enc.sync()
     torch.cuda.synchronize()
     lat, ft, dt, st, at = [], [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap, d_filt.data_ptr())
+        enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap)
         n, adler, nseg = enc.sync()
         lat.append(time.perf_counter() - ts)
         t = enc.timings()
         ft.append(t.filter_ms); dt.append(t.deflate_ms); st.append(t.scan_ms); at.append(t.assemble_ms)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # correctness of the timed output: inflate and compare with the filtered scanlines
+    # correctness of the timed output: inflate and compare with the filtered
+    # scanlines, which one more (untimed) encode also copies out
     png = d_png[:n].cpu().numpy().tobytes()
+    enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap, d_filt.data_ptr())
+    enc.sync()
+    assert d_png[:n].cpu().numpy().tobytes() == png, "encode is not deterministic"
     filt = d_filt.cpu().numpy().tobytes()
     pos, z = 8, b""
     while pos < len(png):
@@ -218,8 +222,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--lz77", type=int, default=1)
-    ap.add_argument("--lane", type=int, default=16, choices=[16, 32])
-    ap.add_argument("--wg", type=int, default=512, choices=[512, 1024])
+    ap.add_argument("--wg", type=int, default=512, choices=[256, 512, 1024])
+    ap.add_argument("--seg", type=int, default=65536, choices=[16384, 32768, 65536])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
     args = ap.parse_args()
@@ -248,7 +252,7 @@ def main():
         "ms_per_step": round(r["wall"] / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"{args.workload} {r['s'].img.width}x{r['s'].img.height}",
-                   "form": "plain per-image", "segment_bytes": 65536, "lz77": args.lz77,
+                   "form": "plain per-image", "segment_bytes": args.seg, "wg_threads": args.wg, "lz77": args.lz77,
                    "bit_exact": r["ok"]},
         "latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},

@@ -97,8 +97,8 @@ typedef struct r2p_options {
     int idat_chunk_bytes;  /* IDAT chunk payload size; default 65536 */
     int zlib_level;        /* only sets the zlib header FLEVEL (parallel_deflate.cpp:214-228) */
     int band_rows;         /* rows per filter workgroup; 0 = auto */
-    int lane_bytes;        /* LZ77 parse span per lane (16 or 32); 0 = auto */
-    int wg_threads;        /* DEFLATE workgroup size (512 or 1024); 0 = auto */
+    int lane_bytes;        /* LZ77 parse span per lane; 16 is the one supported value */
+    int wg_threads;        /* DEFLATE workgroup size (256, 512 or 1024); 0 = auto */
 } r2p_options;
 
 void r2p_options_default(r2p_options* o);
@@ -214,7 +214,7 @@ int r2p_batch_sync(r2p_batch* b);
 int r2p_batch_result(r2p_batch* b, int i, const void** d_png, size_t* len);
 int r2p_batch_copy_result(r2p_batch* b, int i, void* h_dst, size_t cap, size_t* len);
 int r2p_batch_timings(r2p_batch* b, r2p_timings* t);
-/* Autotuner: times candidate (band_rows, lane_bytes, segment_bytes)
+/* Autotuner: times candidate (band_rows, wg_threads, segment_bytes)
  * settings on the declared batch and keeps the fastest.  Writes the choice
  * to *chosen. */
 int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms);

@@ -409,7 +409,7 @@ int r2p_batch_timings(r2p_batch* b, r2p_timings* t)
     return R2P_OK;
 }
 
-// Autotuner: times each candidate (segment_bytes x lane_bytes x band_rows)
+// Autotuner: times each candidate (segment_bytes x wg_threads x band_rows)
 // on the declared batch -- one warm-up graph replay, then the best of three
 // timed replays -- and keeps the fastest, re-declaring the batch with it.
 int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
@@ -422,14 +422,14 @@ int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
     r2p_options best = base;
     double best_t = 1e30;
     static const int segs[] = {32768, 65536};
-    static const int shapes[][2] = {{16, 512}, {16, 1024}, {32, 512}};   // lane_bytes, wg_threads
+    static const int shapes[] = {256, 512, 1024};   // DEFLATE wg_threads
     static const int bands[] = {4, 8, 16};
     for (int sg : segs) {
         if (sg < b->seg_bytes_cap) continue;
-        for (const auto& sh : shapes)
+        for (int sh : shapes)
             for (int bd : bands) {
                 r2p_options o = base;
-                o.segment_bytes = sg; o.lane_bytes = sh[0]; o.wg_threads = sh[1]; o.band_rows = bd;
+                o.segment_bytes = sg; o.wg_threads = sh; o.band_rows = bd;
                 int rc = r2p_batch_options(b, &o);
                 if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
                 if (rc) continue;

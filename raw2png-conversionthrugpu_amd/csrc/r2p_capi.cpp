@@ -287,9 +287,8 @@ void fill_defaults(r2p_options* o)
     if (o->segment_bytes <= 0) o->segment_bytes = 65536;
     if (o->idat_chunk_bytes <= 0) o->idat_chunk_bytes = 65536;
     if (o->idat_chunk_bytes > 65536) o->idat_chunk_bytes = 65536;
-    if (o->lane_bytes != 16 && o->lane_bytes != 32) o->lane_bytes = 16;
-    if (o->wg_threads != 512 && o->wg_threads != 1024) o->wg_threads = 512;
-    if (o->lane_bytes == 32) o->wg_threads = 512;
+    o->lane_bytes = 16;
+    if (o->wg_threads != 256 && o->wg_threads != 512 && o->wg_threads != 1024) o->wg_threads = 512;
     if (o->lz77 < 0) o->lz77 = 1;
 }
 
@@ -534,7 +533,7 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
 #define TRY(x) do { if ((err = (x)) != hipSuccess) { r2p_encoder_destroy(e); return hip_fail(err, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& v : e->ev) TRY(hipEventCreate(&v));
-    TRY(hipMalloc((void**)&e->d_filt, max_filtered_bytes));
+    TRY(hipMalloc((void**)&e->d_filt, max_filtered_bytes + kTailSlack));
     TRY(hipMalloc((void**)&e->d_seg_out, (size_t)e->max_seg * e->seg_cap_dw * 4));
     TRY(hipMalloc((void**)&e->d_seg_len, (size_t)e->max_seg * 4));
     TRY(hipMalloc((void**)&e->d_seg_adler, (size_t)e->max_seg * 4));
@@ -581,7 +580,9 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     hipStream_t s = e->stream;
     if ((rc = e->lut.ensure(wp, img->dicom, s))) return rc;
 
-    uint8_t* filt = d_filtered_out ? (uint8_t*)d_filtered_out : e->d_filt;
+    // The deflate stage reads past the data end (kTailSlack), so it always
+    // works on the encoder's own buffer; a caller's copy is made afterwards.
+    uint8_t* filt = e->d_filt;
     R2P_HIP(hipEventRecord(e->ev[0], s));
     FilterJob J{};
     J.src = (const uint8_t*)d_src; J.dst = filt; J.prior = nullptr; J.last_row = nullptr;
@@ -591,6 +592,8 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     J.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
     if ((rc = launch_filter_job(J, o.band_rows, s))) return rc;
     R2P_HIP(hipEventRecord(e->ev[1], s));
+    if (d_filtered_out)
+        R2P_HIP(hipMemcpyAsync(d_filtered_out, filt, g.filtered, hipMemcpyDeviceToDevice, s));
 
     const int nseg = (int)((g.filtered + o.segment_bytes - 1) / o.segment_bytes);
     DeflateLaunch DL{};

@@ -128,6 +128,9 @@ constexpr int kAssembleThreads = 256;
 constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
 constexpr int kFilterMaxK = kMaxRowBytesLds / (16 * kFilterThreads);
 constexpr int kWindow = 32768;           // DEFLATE window
+// Every buffer k_deflate reads holds at least this many readable bytes after
+// the last data byte, so its 16-byte loads need no tail guard.
+constexpr int kTailSlack = 64;
 constexpr int kChecksumBlock = 65536;    // bytes per checksum workgroup
 
 // Worst-case compressed dwords of one segment of `seg_bytes` input bytes:

@@ -740,7 +740,7 @@ __global__ __launch_bounds__(256) void k_checksum_final(const uint32_t* part, in
 }
 
 // ---------------------------------------------------------------------------
-// k_deflate: one workgroup per 64 KiB segment.
+// k_deflate: one workgroup per segment (64 KiB by default).
 //
 // Segment layout on the wire: [BFINAL|BTYPE=01 fixed-Huffman block][EOB]
 // then, unless it is the stream's final segment, an empty stored block
@@ -749,71 +749,207 @@ __global__ __launch_bounds__(256) void k_checksum_final(const uint32_t* part, in
 // (the 32 KiB before it) is already in memory, so all segments of all
 // images run in parallel and are joined by a prefix sum of their lengths.
 //
-// Inside a segment, sub-blocks of 256*LANE positions:
-//   1. match phase (rounds of 1024 positions): every position looks up a
-//      4-byte hash table in LDS (most recent earlier position, atomicMax
-//      inserts) plus the run / pixel / row-above distances, and keeps the
-//      longest match capped at its parse lane's end;
-//   2. parse: lane t walks its LANE positions greedily (token bitmask);
-//   3. block exclusive scan of per-lane bit counts;
-//   4. emission: each lane packs its tokens into 32-bit words at its bit
-//      offset (interior words plain stores, the two shared edge words with
-//      LDS atomicOr), and full words stream out to global memory.
+// Inside a segment, sub-blocks of NT * 16 positions; thread t owns the 16
+// consecutive positions of its parse lane and keeps them in registers:
+//   A. every position's 4-byte hash enters T2 (first position of the
+//      sub-block with that hash, atomicMin);
+//   B. candidates per position: distance 1, distance bpp and one row up
+//      (equality bitmasks from the lane's own bytes, no extra loads), and
+//      one hashed earlier position -- T2's entry when it precedes the
+//      position, else T1's (most recent position before the sub-block).
+//      The longest match wins, ties to the shorter distance; matches stop
+//      at the lane end, so each lane parses greedily on its own;
+//   C. per-lane bit counts, block exclusive scan, then each lane packs its
+//      tokens into 32-bit words at its bit offset (the two words shared
+//      with neighbouring lanes by LDS atomicOr) and full words stream out;
+//      the sub-block's positions then enter T1 (atomicMax) and leave T2.
+// The result depends only on the input bytes: every table update is an
+// order-independent min/max or a plain reset, fenced by barriers.
 // ---------------------------------------------------------------------------
 
 
-constexpr int kHashBits = 13;
-constexpr int kRound = 1024;
+#ifndef R2P_MATCH_GROUP
+#define R2P_MATCH_GROUP 4
+#endif
+constexpr int kHashBits = 13;      // T1: 8 Ki entries
+constexpr int kHash2Bits = 12;     // T2: 4 Ki entries
+constexpr int kLane = 16;          // positions per parse lane
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
 
-__device__ __forceinline__ int match_len(const uint8_t* d, int P, int C, int cap)
+// Bits 0..3 flag the zero bytes 0..3 of v.
+__device__ __forceinline__ uint32_t zero_nibble(uint32_t v)
 {
-    int n = 0;
-    while (n + 4 <= cap) {
-        const uint32_t x = g32u(d + P + n) ^ g32u(d + C + n);
-        if (x) return n + ((__ffs(x) - 1) >> 3);
-        n += 4;
-    }
-    while (n < cap && g8(d + P + n) == g8(d + C + n)) n++;
-    return n;
+    const uint32_t nz = ((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v;   // bit 7 of a byte: byte != 0
+    const uint32_t z = (~nz & 0x80808080u) >> 7;                  // bits 0, 8, 16, 24
+    return ((z * 0x00204081u) >> 21) & 0xFu;                      // gathered into bits 0..3
 }
 
-__device__ __forceinline__ void dist_code(uint32_t dist, uint32_t& bits, uint32_t& nbits)
+// The 4 bytes starting at byte i (< 16) of the dword array o; i is a
+// constant after unrolling, so o stays in registers.
+__device__ __forceinline__ uint32_t bytes_at(const uint32_t* o, int i)
+{
+    return __builtin_amdgcn_alignbyte(o[(i >> 2) + 1], o[i >> 2], i & 3);
+}
+
+// Positions j < 16 where byte j equals byte j - d (1 <= d <= 8).  X[0..1] are
+// the 8 bytes before the lane, X[2..] the lane's own bytes.
+__device__ __forceinline__ uint32_t eq_mask(const uint32_t* X, int d)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kLane / 4; k++) {
+        const uint32_t prev = d <= 4 ? __builtin_amdgcn_alignbyte(X[2 + k], X[1 + k], 4 - d)
+                                     : __builtin_amdgcn_alignbyte(X[1 + k], X[k], 8 - d);
+        m |= zero_nibble(X[2 + k] ^ prev) << (4 * k);
+    }
+    return m;
+}
+
+// Fixed-Huffman codes (RFC 1951 3.2.6), bit-reversed for the LSB-first
+// stream.  Literal b: 8-bit codes 0x30+b (b < 144), 9-bit 0x190+b-144.
+__device__ __forceinline__ void lit_code(uint32_t b, uint32_t& v, uint32_t& nb)
+{
+    const bool hi = b >= 144;
+    nb = hi ? 9u : 8u;
+    v = __builtin_bitreverse32(hi ? b + 0x100u : b + 0x30u) >> (32 - nb);
+}
+
+// Extra bits of a distance code: 0 for distances 1..4, else floor(log2(d-1)) - 1.
+__device__ __forceinline__ uint32_t dist_extra_bits(uint32_t dist)
 {
     const uint32_t dm = dist - 1;
-    if (dm < 4) { bits = c_dist_rev[dm]; nbits = 5; return; }
-    const uint32_t n = 31 - __clz(dm);
-    const uint32_t eb = n - 1;
-    const uint32_t code = 2 * n + ((dm >> eb) & 1);
-    bits = c_dist_rev[code] | ((dm & ((1u << eb) - 1)) << 5);
-    nbits = 5 + eb;
+    return dm < 4 ? 0u : 30u - __clz(dm);
 }
 
+// Length 3..18 (symbols 257..268, 7-bit codes) and distance 1..32768.
+__device__ __forceinline__ void match_code(uint32_t len, uint32_t dist, uint32_t& v, uint32_t& nb)
+{
+    const uint32_t m = len - 3;
+    const uint32_t leb = m >= 8 ? 1u : 0u;
+    const uint32_t sym = m < 8 ? m + 1 : 9 + ((m >> 1) & 3);      // symbol - 256
+    const uint32_t lex = m & leb;
+    const uint32_t dm = dist - 1;
+    const uint32_t deb = dist_extra_bits(dist);
+    const uint32_t dc = dm < 4 ? dm : 2 * (deb + 1) + ((dm >> deb) & 1);
+    const uint32_t dex = dm & ((1u << deb) - 1);
+    v = (__builtin_bitreverse32(sym) >> 25) | (lex << 7) | ((__builtin_bitreverse32(dc) >> 27) << (7 + leb)) |
+        (dex << (12 + leb));
+    nb = 12 + leb + deb;
+}
+
+// Bits of a match token, for the "cheaper than literals" test.
 __device__ __forceinline__ uint32_t match_bits(uint32_t len, uint32_t dist)
 {
-    uint32_t db, dn;
-    dist_code(dist, db, dn);
-    return (c_len[len] >> 16) + dn;
+    return 12u + (len >= 11 ? 1u : 0u) + dist_extra_bits(dist);
 }
 
-// U consecutive rounds of the match phase are handled by each thread for U
-// positions at once, so the global loads of those positions (the 4 bytes at
-// P and the first 4 bytes at every candidate) are all in flight together.
-template <int NT, int LANE, int U>
+__device__ __forceinline__ void take(uint32_t l, uint32_t d, uint32_t& bl, uint32_t& bd)
+{
+    const bool b = (l > bl) | ((l == bl) & (d < bd));
+    bl = b ? l : bl;
+    bd = b ? d : bd;
+}
+
+// Compares the lane's bytes from position i with the KW dwords c: the number
+// of leading equal bytes (up to 4 * KW).  i and KW are constants after
+// unrolling.
+__device__ __forceinline__ uint32_t cmp_run(const uint32_t* o, const uint32_t* c, int i, int KW)
+{
+    uint32_t l = 4 * KW;
+#pragma unroll
+    for (int k = 3; k >= 0; k--) {
+        if (k >= KW) continue;
+        const uint32_t x = bytes_at(o, i + 4 * k) ^ c[k];
+        l = x ? 4 * k + (__builtin_ctz(x) >> 3) : l;
+    }
+    return l;
+}
+
+__device__ __forceinline__ void load_words(const uint8_t* p, uint32_t* c, int KW)
+{
+    typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+    typedef u32x3 u32x3_ua __attribute__((aligned(1)));
+    if (KW == 4) { const u32x4 v = g128u(p); c[0] = v[0]; c[1] = v[1]; c[2] = v[2]; c[3] = v[3]; }
+    else if (KW == 3) { const u32x3 v = *(R2P_GLOBAL const u32x3_ua*)p; c[0] = v[0]; c[1] = v[1]; c[2] = v[2]; }
+    else if (KW == 2) { const uint64_t v = g64u(p); c[0] = (uint32_t)v; c[1] = (uint32_t)(v >> 32); }
+    else c[0] = g32u(p);
+}
+
+// Best match of every position of the lane, as len << 16 | (dist - 1), 0 for
+// a literal.  Positions go in groups of G; per group three passes, so that
+// the group's table reads, then its candidate loads, are in flight together.
+// Branch-free: an unusable candidate is marked -1 and its load reads the
+// window base.  A scheduling barrier between groups bounds the registers.
+template <int G>
+__device__ __forceinline__ void lane_matches(uint32_t* pk, const uint8_t* wb, const uint32_t* o, int lb, int le,
+                                             int end_rel, uint32_t m1, uint32_t mb, uint32_t mr, int bpp, int dr,
+                                             const uint32_t* T1, const uint32_t* T2)
+{
+#pragma unroll
+    for (int g0 = 0; g0 < kLane; g0 += G) {
+        __builtin_amdgcn_sched_barrier(0);
+        int C[G];
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            const int i = g0 + u;
+            const uint32_t cap = (uint32_t)max(le - i, 0);
+            uint32_t bl = min((uint32_t)__builtin_ctz(~(m1 >> i)), cap), bd = 1;
+            take(min((uint32_t)__builtin_ctz(~(mb >> i)), cap), (uint32_t)bpp, bl, bd);
+            take(min((uint32_t)__builtin_ctz(~(mr >> i)), cap), (uint32_t)dr, bl, bd);
+            pk[i] = (bl << 16) | bd;               // dist <= 32768 fits 16 bits
+            const int P = lb + i;
+            const uint32_t h = hash4(bytes_at(o, i));
+            const uint32_t f = T2[h >> (kHashBits - kHash2Bits)];
+            const uint32_t t = T1[h];
+            const bool fv = f < (uint32_t)P;
+            const bool tv = (t != 0) & (P - (int)(t - 1) <= kWindow);
+            const bool hv = (fv | tv) & (bl < cap) & (P + 4 <= end_rel);
+            C[u] = hv ? (fv ? (int)f : (int)t - 1) : -1;
+        }
+        uint32_t c[G][4];
+#pragma unroll
+        for (int u = 0; u < G; u++) load_words(wb + max(C[u], 0), c[u], (kLane - (g0 + u) + 3) / 4);
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            const int i = g0 + u;
+            const uint32_t cap = (uint32_t)max(le - i, 0);
+            uint32_t bl = pk[i] >> 16, bd = pk[i] & 0xFFFFu;
+            const uint32_t lh = C[u] >= 0 ? min(cmp_run(o, c[u], i, (kLane - i + 3) / 4), cap) : 0u;
+            take(lh, (uint32_t)(lb + i - C[u]), bl, bd);
+            const bool ok = (bl >= 3) & (match_bits(bl, bd) <= 8 * bl);
+            pk[i] = ok ? (bl << 16) | (bd - 1) : 0u;
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Hash of every position I < kLane of the lane, applied to f(I, hash).
+template <int I, typename F>
+__device__ __forceinline__ void each_hash(const uint32_t* o, F&& f)
+{
+    f(I, hash4(bytes_at(o, I)));
+    if constexpr (I + 1 < kLane) each_hash<I + 1>(o, f);
+}
+
+// The lane's bytes [p, p + 20) (the 4 after it feed the hashes of its last
+// positions); in bounds by the buffers' tail slack.
+__device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
+{
+    const u32x4 v = g128u(p);
+    o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    o[4] = g32u(p + 16);
+}
+
+template <int NT>
 __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 {
+    constexpr int LANE = kLane;
     constexpr int SUB = NT * LANE;               // positions per sub-block
-    // U: positions per thread per round
-    constexpr int ROUND = NT * U;
     constexpr int OBUF = (SUB * 9 + 31) / 32 + 8;
     constexpr int NW = NT / 64;
-    __shared__ uint32_t htab[1 << kHashBits];
-    // Per-position match records, stored lane-major with one word of padding
-    // per row (index (p % LANE) * (NT + 1) + p / LANE) so that the parse,
-    // where lane t walks positions t*LANE.., reads consecutive words, and the
-    // match phase's consecutive positions land in distinct banks.
-    __shared__ uint32_t M[LANE * (NT + 1)];
+    __shared__ uint32_t T1[1 << kHashBits];      // most recent position before the sub-block, + 1 (0 = none)
+    __shared__ uint32_t T2[1 << kHash2Bits];     // first position inside the sub-block (~0 = none)
     __shared__ uint32_t obuf[OBUF];
     __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
@@ -840,25 +976,29 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
         const bool last = (seg == J.nseg - 1) && J.final_block;
         const int64_t wlo = max64(J.dict_lo, s0 - (int64_t)kWindow);
         uint32_t* out = J.seg_out + (int64_t)seg * J.seg_cap_dw;
+        // Positions below are relative to wb = d + wlo (window base).
+        const uint8_t* wb = d + wlo;
+        const int s0_rel = (int)(s0 - wlo);
+        const int s1_rel = (int)(s1 - wlo);
+        const int end_rel = (int)min64(dend - wlo, 0x7FFFFFF0);
+        const bool lz = J.lz77 != 0;
+        const int bpp = J.bpp;
+        const int dr = J.row_stride;
 
         __syncthreads();
-        for (int i = tid; i < (1 << kHashBits); i += NT) htab[i] = 0;
+        for (int i = tid; i < (1 << kHashBits); i += NT) T1[i] = 0;
+        for (int i = tid; i < (1 << kHash2Bits); i += NT) T2[i] = ~0u;
         if (tid < 4) st[tid] = 0;
         __syncthreads();
-        if (J.lz77) {
-            // prime with the (up to) 32 KiB before the segment
-            for (int64_t P0 = wlo + tid; P0 < s0; P0 += (int64_t)NT * U) {
-                uint32_t w[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int64_t P = P0 + (int64_t)u * NT;
-                    w[u] = (P < s0 && P + 4 <= dend) ? g32u(d + P) : 0u;
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int64_t P = P0 + (int64_t)u * NT;
-                    if (P < s0 && P + 4 <= dend) atomicMax(&htab[hash4(w[u])], (uint32_t)(P - wlo + 1));
-                }
+        if (lz) {
+            // prime T1 with the (up to) 32 KiB before the segment
+            for (int b = tid * LANE; b < s0_rel; b += SUB) {
+                uint32_t o[LANE / 4 + 1];
+                load_own(wb + b, o);
+                each_hash<0>(o, [&](int i, uint32_t h) {
+                    const int P = b + i;
+                    atomicMax(&T1[h], (P < s0_rel && P + 4 <= end_rel) ? (uint32_t)(P + 1) : 0u);
+                });
             }
         }
 
@@ -866,149 +1006,106 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
         uint32_t carry_bits = 3;
         int64_t out_dw = 0;
 
-        for (int64_t q0 = s0; q0 < s1; q0 += SUB) {
-            const int n = (int)min64(SUB, s1 - q0);
+        for (int q = s0_rel; q < s1_rel; q += SUB) {
+            const int n = min(SUB, s1_rel - q);
             __syncthreads();                       // previous flush done with obuf
             for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;
-            // ---- 1. match phase -------------------------------------------------
-            // Lookups see the hash table as it stood at the sub-block start
-            // (every position before q0); no barrier inside the phase, so the
-            // waves run their loads independently.  Runs, pixel repeats and the
-            // row above are explicit candidates and need no table.
-            // Positions are relative to wb = d + wlo (window base, < 2^31).
-            const uint8_t* wb = d + wlo;
-            const int qrel = (int)(q0 - wlo);
-            const int end_rel = (int)min64(dend - wlo, 0x7FFFFFF0);
-            for (int r0 = 0; r0 < n; r0 += ROUND) {
-                uint64_t p8[U];
-                bool ok[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int pr = r0 + u * NT + tid;
-                    ok[u] = pr < n && qrel + pr + 8 <= end_rel;
-                    p8[u] = ok[u] ? g64u(wb + qrel + pr) : 0ull;
-                }
-                int cand[U][4];
-                uint64_t c8[U][4];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int Pr = qrel + r0 + u * NT + tid;
-                    const uint32_t hv = (J.lz77 && ok[u]) ? htab[hash4((uint32_t)p8[u])] : 0u;
-                    cand[u][0] = (int)hv - 1;
-                    cand[u][1] = Pr - 1;
-                    cand[u][2] = J.bpp > 1 ? Pr - J.bpp : -1;
-                    cand[u][3] = J.row_stride > 0 ? Pr - J.row_stride : -1;
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int C = cand[u][c];
-                        const bool good = J.lz77 && ok[u] && C >= 0 && C < Pr && Pr - C <= kWindow;
-                        if (!good) cand[u][c] = -1;
-                        c8[u][c] = good ? g64u(wb + C) : ~p8[u];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int pr = r0 + u * NT + tid;
-                    if (pr >= n) continue;
-                    const int Pr = qrel + pr;
-                    const int lane_end = min(n, (pr / LANE + 1) * LANE);
-                    const int cap = min(258, lane_end - pr);
-                    uint32_t best_len = 0, best_dist = 0;
-                    if (cap >= 3) {
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            const int C = cand[u][c];
-                            if (C < 0) continue;
-                            const uint64_t x = p8[u] ^ c8[u][c];
-                            int l;
-                            if (x) l = (int)(__builtin_ctzll(x) >> 3);
-                            else l = cap > 8 ? 8 + match_len(wb, Pr + 8, C + 8, cap - 8) : 8;
-                            l = min(l, cap);
-                            const uint32_t dist = (uint32_t)(Pr - C);
-                            if ((uint32_t)l > best_len || ((uint32_t)l == best_len && dist < best_dist)) {
-                                best_len = (uint32_t)l; best_dist = dist;
-                            }
-                        }
-                        if (best_len < 3 || match_bits(best_len, best_dist) > 8 * best_len) best_len = 0;
-                    }
-                    const uint32_t byte = ok[u] ? (uint32_t)(p8[u] & 0xFFu) : g8(wb + Pr);
-                    M[(pr % LANE) * (NT + 1) + pr / LANE] =
-                        (best_len << 23) | (best_len ? ((best_dist - 1) << 8) : 0u) | byte;
-                }
-            }
-            __syncthreads();                       // all lookups and records of the sub-block done
-            // this sub-block's positions enter the table for the next sub-blocks
-            if (J.lz77)
-                for (int pr = tid; pr < n; pr += NT)
-                    if (qrel + pr + 4 <= end_rel) atomicMax(&htab[hash4(g32u(wb + qrel + pr))], (uint32_t)(qrel + pr + 1));
+            const int lb = q + tid * LANE;         // this lane's first position
+            const int le = max(0, min(LANE, n - tid * LANE));
+            uint32_t o[LANE / 4 + 1] = {0, 0, 0, 0, 0};
+            if (le > 0) load_own(wb + lb, o);
+            // ---- A. the sub-block's first positions per hash --------------------
+            if (lz && le > 0)
+                each_hash<0>(o, [&](int i, uint32_t h) {
+                    atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i) : ~0u);
+                });
+            __syncthreads();
+            // ---- B. best match per position, greedy parse of the lane -----------
             if (tid == 0) obuf[0] = carry;
-            // ---- 2. parse ----------------------------------------------------------
-            const int lb = tid * LANE;
-            const int le = min(n, lb + LANE);
-            uint32_t tok = 0, nbits = 0;
-            uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
-            for (int p = lb; p < le;) {
-                const uint32_t m = M[(p - lb) * (NT + 1) + tid];
-                const uint32_t len = m >> 23;
-                tok |= 1u << (p - lb);
-                if (len >= 3) {
-                    const uint32_t dist = ((m >> 8) & 0x7FFFu) + 1;
-                    nbits += match_bits(len, dist);
-                    m_cnt++; m_bytes += len; m_dist += dist; m_max = max(m_max, len);
-                    p += (int)len;
-                } else {
-                    nbits += c_lit[m & 0xFF] >> 16;
-                    p += 1;
+            uint32_t pk[LANE];
+#pragma unroll
+            for (int i = 0; i < LANE; i++) pk[i] = 0;
+            if (lz && le > 0) {
+                uint32_t X[LANE / 4 + 2];
+                uint64_t pr = 0;
+                if (lb >= 8) pr = g64u(wb + lb - 8);
+                else
+                    for (int k = 8 - lb; k < 8; k++) pr |= (uint64_t)g8(wb + lb - 8 + k) << (8 * k);
+                X[0] = (uint32_t)pr; X[1] = (uint32_t)(pr >> 32);
+#pragma unroll
+                for (int k = 0; k < LANE / 4; k++) X[2 + k] = o[k];
+                // a distance reaching before the window base is no candidate
+                const uint32_t m1 = eq_mask(X, 1) & (lb >= 1 ? ~0u : ~1u);
+                uint32_t mb = 0, mr = 0;
+                if (bpp > 1) mb = eq_mask(X, bpp) & (lb >= bpp ? ~0u : (~0u << (bpp - lb)));
+                const bool row_ok = dr > 1 && dr != bpp && dr <= kWindow && lb >= dr;
+                if (row_ok) {
+                    const u32x4 r = g128u(wb + lb - dr);
+#pragma unroll
+                    for (int k = 0; k < LANE / 4; k++) mr |= zero_nibble(o[k] ^ r[k]) << (4 * k);
                 }
+                lane_matches<R2P_MATCH_GROUP>(pk, wb, o, lb, le, end_rel, m1, mb, mr, bpp > 1 ? bpp : 1, row_ok ? dr : 1, T1, T2);
             }
-            // ---- 3. scan of bit counts ------------------------------------------------
+            // greedy parse: token starts and the lane's bit count
+            uint32_t nextp = 0, nbits = 0, tokm = 0;
+            uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
+#pragma unroll
+            for (int i = 0; i < LANE; i++) {
+                const uint32_t L = pk[i] >> 16;
+                const uint32_t D = (pk[i] & 0xFFFFu) + 1;
+                const uint32_t byte = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                const bool tok = ((uint32_t)i == nextp) & (i < le);
+                const uint32_t nb = L ? match_bits(L, D) : (byte >= 144 ? 9u : 8u);
+                nbits += tok ? nb : 0u;
+                tokm |= tok ? 1u << i : 0u;
+                nextp = tok ? (uint32_t)i + max(L, 1u) : nextp;
+                const uint32_t Lt = tok ? L : 0u;
+                m_cnt += Lt ? 1u : 0u; m_bytes += Lt; m_dist += Lt ? D : 0u; m_max = max(m_max, Lt);
+            }
+            // ---- block scan of bit counts ----------------------------------------------
             const uint32_t incl = wave_incl_scan(nbits);
             if (lane == 63) red[wave] = incl;
             if (J.seg_stats) {
                 atomicAdd(&st[0], m_cnt); atomicAdd(&st[1], m_bytes);
                 atomicAdd(&st[2], m_dist); atomicMax(&st[3], m_max);
             }
-            __syncthreads();
+            __syncthreads();                       // every lookup of T1/T2 done, red complete
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
-            // ---- 4. emission -----------------------------------------------------------
+            // the sub-block's positions enter T1 and leave T2
+            if (lz && le > 0)
+                // (T2 holds entries of this sub-block only, so resetting the slot
+                // of any hash here, valid position or not, leaves it all ~0)
+                each_hash<0>(o, [&](int i, uint32_t h) {
+                    atomicMax(&T1[h], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i + 1) : 0u);
+                    T2[h >> (kHashBits - kHash2Bits)] = ~0u;
+                });
+            // ---- emission ------------------------------------------------------------------
+            // Every word is ORed in (obuf starts zeroed and each bit has one
+            // writer), so the words shared with the neighbouring lanes need no
+            // special case and a position without a full word ORs in zero.
             {
-                uint32_t w = off >> 5;
-                uint32_t nacc = off & 31;
+                uint32_t wi = off >> 5, nacc = off & 31;
                 uint64_t acc = 0;
-                bool first = true;
-                uint32_t tk = tok;
-                while (tk) {
-                    const int i = __ffs(tk) - 1;
-                    tk &= tk - 1;
-                    const uint32_t m = M[i * (NT + 1) + tid];
-                    const uint32_t len = m >> 23;
-                    uint32_t v, nb;
-                    if (len >= 3) {
-                        const uint32_t dist = ((m >> 8) & 0x7FFFu) + 1;
-                        const uint32_t lc = c_len[len];
-                        uint32_t db, dn;
-                        dist_code(dist, db, dn);
-                        const uint32_t ln = lc >> 16;
-                        v = (lc & 0xFFFFu) | (db << ln);
-                        nb = ln + dn;
-                    } else {
-                        const uint32_t lc = c_lit[m & 0xFF];
-                        v = lc & 0xFFFFu;
-                        nb = lc >> 16;
-                    }
-                    acc |= (uint64_t)v << nacc;
-                    nacc += nb;
-                    if (nacc >= 32) {
-                        if (first) atomicOr(&obuf[w], (uint32_t)acc); else obuf[w] = (uint32_t)acc;
-                        first = false;
-                        w++;
-                        acc >>= 32;
-                        nacc -= 32;
-                    }
+#pragma unroll
+                for (int i = 0; i < LANE; i++) {
+                    if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t L = pk[i] >> 16;
+                    const uint32_t D = (pk[i] & 0xFFFFu) + 1;
+                    uint32_t mv, mn, lv, ln;
+                    match_code(max(L, 3u), D, mv, mn);
+                    lit_code((o[i >> 2] >> (8 * (i & 3))) & 0xFFu, lv, ln);
+                    const bool tok = (tokm >> i) & 1u;
+                    acc |= (uint64_t)(tok ? (L ? mv : lv) : 0u) << nacc;
+                    nacc += tok ? (L ? mn : ln) : 0u;
+                    const bool fl = nacc >= 32;
+                    atomicOr(&obuf[wi], fl ? (uint32_t)acc : 0u);
+                    wi += fl ? 1u : 0u;
+                    acc = fl ? acc >> 32 : acc;
+                    nacc -= fl ? 32u : 0u;
                 }
-                if (nacc > 0) atomicOr(&obuf[w], (uint32_t)acc);
+                if (nacc > 0) atomicOr(&obuf[wi], (uint32_t)acc);
             }
             __syncthreads();
             const uint32_t tb = carry_bits + tot;
@@ -1283,15 +1380,15 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
     }
 }
 
-// Workgroup shapes (threads x parse span): 512 x 16 (8 K positions per
-// sub-block, two workgroups per CU), 1024 x 16 and 512 x 32 (16 K positions,
-// one workgroup per CU).
+// Workgroup sizes: 256 (4 K positions per sub-block, three workgroups per
+// CU), 512 (8 K, two per CU) and 1024 (16 K, two per CU).
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s)
 {
+    (void)lane_bytes;
     (void)hipGetLastError();
-    if (lane_bytes == 32)         k_deflate<512, 32, 4><<<grid, 512, 0, s>>>(DL);
-    else if (wg_threads == 1024)  k_deflate<1024, 16, 2><<<grid, 1024, 0, s>>>(DL);
-    else                          k_deflate<512, 16, 4><<<grid, 512, 0, s>>>(DL);
+    if (wg_threads == 1024)      k_deflate<1024><<<grid, 1024, 0, s>>>(DL);
+    else if (wg_threads == 256)  k_deflate<256><<<grid, 256, 0, s>>>(DL);
+    else                         k_deflate<512><<<grid, 512, 0, s>>>(DL);
     return hipGetLastError();
 }
 

@@ -63,7 +63,7 @@ int r2p_deflate_create(size_t max_strip_bytes, size_t max_total_bytes, int lz77,
     hipError_t e;
 #define TRY(x) do { if ((e = (x)) != hipSuccess) { r2p_deflate_destroy(d); return hip_fail(e, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
-    for (int i = 0; i < 2; i++) TRY(hipMalloc((void**)&d->d_win[i], kWindow + max_strip_bytes + 16));
+    for (int i = 0; i < 2; i++) TRY(hipMalloc((void**)&d->d_win[i], kWindow + max_strip_bytes + kTailSlack));
     TRY(hipMalloc((void**)&d->d_seg_out, (size_t)d->max_seg * d->seg_cap_dw * 4));
     TRY(hipMalloc((void**)&d->d_seg_len, (size_t)d->max_seg * 4));
     TRY(hipMalloc((void**)&d->d_seg_adler, (size_t)d->max_seg * 4));
