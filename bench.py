@@ -222,9 +222,11 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--lz77", type=int, default=1)
-    ap.add_argument("--wg", type=int, default=512, choices=[256, 512, 1024])
+    ap.add_argument("--wg", type=int, default=256, choices=[256, 512])
     ap.add_argument("--seg", type=int, default=65536, choices=[16384, 32768, 65536])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device three times (profiler calibration)")
     ap.add_argument("--cpu-level", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -273,6 +275,16 @@ def main():
         out["ratio"]["zlib_level%d_png_over_filtered" % args.cpu_level] = round(cb["png_bytes"] / r["filtered_bytes"], 4)
     if args.mode in ("batch", "all"):
         out["batched"] = run_batch(args, r2p, synth, torch)
+    if args.calib_copy_mib:
+        # a plain device copy of known size, for scaling the HBM byte counters
+        # of a profiler run (FETCH_SIZE / WRITE_SIZE) to bytes
+        n = args.calib_copy_mib << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {n} bytes x3 done")
     print(json.dumps(out), flush=True)
 
 

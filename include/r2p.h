@@ -98,7 +98,7 @@ typedef struct r2p_options {
     int zlib_level;        /* only sets the zlib header FLEVEL (parallel_deflate.cpp:214-228) */
     int band_rows;         /* rows per filter workgroup; 0 = auto */
     int lane_bytes;        /* LZ77 parse span per lane; 16 is the one supported value */
-    int wg_threads;        /* DEFLATE workgroup size (256, 512 or 1024); 0 = auto */
+    int wg_threads;        /* DEFLATE workgroup size (256 or 512); 0 = auto (256) */
 } r2p_options;
 
 void r2p_options_default(r2p_options* o);

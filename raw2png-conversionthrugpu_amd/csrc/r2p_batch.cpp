@@ -422,7 +422,7 @@ int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
     r2p_options best = base;
     double best_t = 1e30;
     static const int segs[] = {32768, 65536};
-    static const int shapes[] = {256, 512, 1024};   // DEFLATE wg_threads
+    static const int shapes[] = {256, 512};   // DEFLATE wg_threads
     static const int bands[] = {4, 8, 16};
     for (int sg : segs) {
         if (sg < b->seg_bytes_cap) continue;

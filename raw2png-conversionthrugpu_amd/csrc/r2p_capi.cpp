@@ -55,7 +55,7 @@ static uint32_t fixed_code(int s, int* nb)
 }
 
 static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev[32],
-                         uint32_t crc4[4][256], uint32_t x8pow[64])
+                         uint32_t crc4[4][256], uint32_t x8pow[64], uint32_t shift[kCrcShiftLevels][4][256])
 {
     for (int b = 0; b < 256; b++) {
         int nb;
@@ -96,6 +96,11 @@ static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev
     };
     uint32_t x = 0x00800000u;  // x^8
     for (int k = 0; k < 64; k++) { x8pow[k] = x; x = mulmod(x, x); }
+    // shift[k]: the register after 64 * 2^k zero bytes (c -> c * x^(8*64*2^k)),
+    // split by input byte so that it is four lookups.
+    for (int k = 0; k < kCrcShiftLevels; k++)
+        for (int b = 0; b < 4; b++)
+            for (uint32_t v = 0; v < 256; v++) shift[k][b][v] = mulmod(x8pow[6 + k], v << (8 * b));
 }
 
 static std::mutex g_init_mu;
@@ -288,7 +293,7 @@ void fill_defaults(r2p_options* o)
     if (o->idat_chunk_bytes <= 0) o->idat_chunk_bytes = 65536;
     if (o->idat_chunk_bytes > 65536) o->idat_chunk_bytes = 65536;
     o->lane_bytes = 16;
-    if (o->wg_threads != 256 && o->wg_threads != 512 && o->wg_threads != 1024) o->wg_threads = 512;
+    if (o->wg_threads != 256 && o->wg_threads != 512) o->wg_threads = 256;
     if (o->lz77 < 0) o->lz77 = 1;
 }
 
@@ -319,9 +324,9 @@ int r2p_init(int device)
         set_error("device %d is %s, this build targets gfx950", device, prop.gcnArchName);
         return R2P_ERR_NODEV;
     }
-    static uint32_t lit[256], len[259], dist_rev[32], crc4[4][256], x8pow[64];
-    build_tables(lit, len, dist_rev, crc4, x8pow);
-    R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow));
+    static uint32_t lit[256], len[259], dist_rev[32], crc4[4][256], x8pow[64], shift[kCrcShiftLevels][4][256];
+    build_tables(lit, len, dist_rev, crc4, x8pow, shift);
+    R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow, &shift[0][0][0]));
     g_dev_ready[device] = true;
     return R2P_OK;
 }
@@ -373,7 +378,7 @@ void r2p_options_default(r2p_options* o)
     o->zlib_level = 3;       // PipelineConfig::deflate_level default (pipeline.h:17)
     o->band_rows = 0;
     o->lane_bytes = 16;
-    o->wg_threads = 512;
+    o->wg_threads = 256;
 }
 
 uint32_t r2p_adler32_combine(uint32_t a1, uint32_t a2, uint64_t len2)

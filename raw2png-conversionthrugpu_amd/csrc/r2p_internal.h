@@ -124,7 +124,8 @@ struct AssembleLaunch {
 
 constexpr int kFilterThreads = 256;
 constexpr int kDeflateThreads = 256;
-constexpr int kAssembleThreads = 256;
+constexpr int kAssembleThreads = 1024;
+constexpr int kCrcShiftLevels = 10;       // CRC shift tables for 64 * 2^k zero bytes, k < 10
 constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
 constexpr int kFilterMaxK = kMaxRowBytesLds / (16 * kFilterThreads);
 constexpr int kWindow = 32768;           // DEFLATE window

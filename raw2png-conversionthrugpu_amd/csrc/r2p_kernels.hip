@@ -32,6 +32,7 @@ __constant__ uint32_t c_len[259];      // length L: (revcode | extra << cbits) |
 __constant__ uint32_t c_dist_rev[32];  // 5-bit reversed distance codes
 __constant__ uint32_t c_crc[4][256];   // slice-by-4 CRC-32 tables
 __constant__ uint32_t c_x8pow[64];     // x^(8 * 2^k) mod P, reflected
+__constant__ uint32_t c_crcshift[kCrcShiftLevels][4][256];   // c -> c * x^(8*64*2^k) mod P, by input byte
 
 // ---------------------------------------------------------------------------
 // Small helpers
@@ -924,6 +925,15 @@ __device__ __forceinline__ void lane_matches(uint32_t* pk, const uint8_t* wb, co
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// Hides the values from the optimiser, so that expressions computed from
+// them before a barrier are recomputed after it instead of being held in
+// registers across it (which costs occupancy).
+__device__ __forceinline__ void launder(uint32_t* v, int n)
+{
+#pragma unroll
+    for (int i = 0; i < n; i++) asm volatile("" : "+v"(v[i]));
+}
+
 // Hash of every position I < kLane of the lane, applied to f(I, hash).
 template <int I, typename F>
 __device__ __forceinline__ void each_hash(const uint32_t* o, F&& f)
@@ -1020,6 +1030,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                     atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i) : ~0u);
                 });
             __syncthreads();
+            launder(o, LANE / 4 + 1);
             // ---- B. best match per position, greedy parse of the lane -----------
             if (tid == 0) obuf[0] = carry;
             uint32_t pk[LANE];
@@ -1046,6 +1057,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                 }
                 lane_matches<R2P_MATCH_GROUP>(pk, wb, o, lb, le, end_rel, m1, mb, mr, bpp > 1 ? bpp : 1, row_ok ? dr : 1, T1, T2);
             }
+            launder(pk, LANE);
             // greedy parse: token starts and the lane's bit count
             uint32_t nextp = 0, nbits = 0, tokm = 0;
             uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
@@ -1073,14 +1085,16 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
-            // the sub-block's positions enter T1 and leave T2
-            if (lz && le > 0)
-                // (T2 holds entries of this sub-block only, so resetting the slot
-                // of any hash here, valid position or not, leaves it all ~0)
-                each_hash<0>(o, [&](int i, uint32_t h) {
-                    atomicMax(&T1[h], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i + 1) : 0u);
-                    T2[h >> (kHashBits - kHash2Bits)] = ~0u;
-                });
+            launder(pk, LANE);
+            launder(o, LANE / 4 + 1);
+            // the sub-block's positions enter T1; T2 is emptied for the next one
+            if (lz) {
+                if (le > 0)
+                    each_hash<0>(o, [&](int i, uint32_t h) {
+                        atomicMax(&T1[h], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i + 1) : 0u);
+                    });
+                for (int i = tid; i < (1 << kHash2Bits); i += NT) T2[i] = ~0u;
+            }
             // ---- emission ------------------------------------------------------------------
             // Every word is ORed in (obuf starts zeroed and each bit has one
             // writer), so the words shared with the neighbouring lanes need no
@@ -1220,21 +1234,59 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
 //   [78 FLG][segments...][adler BE] as [len BE]["IDAT"][data][crc BE];
 //   chunk 0 also writes the signature and IHDR, the last chunk IEND.
 // mode 0 (raw): chunk c copies payload bytes [c*C, (c+1)*C) to out.
+// The chunk's data is cut into 1024 pieces of 64 bytes counted back from its
+// end (the first pieces of a short chunk are partly or wholly empty).
+// Thread t moves piece t straight from the segment buffers to the output in
+// 16-byte pieces (unaligned loads and stores) and folds it into R(piece),
+// the CRC register run from 0 with no final inversion.  R is linear and
+// leading zeros do not change it, so every piece acts as exactly 64 bytes
+// and the pieces combine in a tree R(A|B) = R(A) * x^(8|B|) + R(B) whose
+// shift at level k is one fixed multiplication by x^(8*64*2^k), done as four
+// table lookups.  The chunk CRC is crc32("IDAT"|D) = ~(g * x^(8|D|) + R(D))
+// with g the register after "IDAT".
 // ---------------------------------------------------------------------------
-constexpr int kMaxChunk = 65536;
+constexpr int kAsmBytes = 64;                  // payload bytes per thread
+static_assert(kAssembleThreads * kAsmBytes == 65536, "one 64 KiB chunk per workgroup");
 
 __device__ __forceinline__ void put_be32(uint8_t* p, uint32_t v)
 {
     p[0] = (uint8_t)(v >> 24); p[1] = (uint8_t)(v >> 16); p[2] = (uint8_t)(v >> 8); p[3] = (uint8_t)v;
 }
 
+typedef R2P_GLOBAL u32x4_ua gu32x4_ua_w;
+__device__ __forceinline__ void st128u(uint8_t* p, u32x4 v) { *(gu32x4_ua_w*)p = v; }
+
+// Payload byte zz (mode 1: [78 FLG][segments][adler BE]; mode 0: the
+// segments).  s is a segment index at or before the one holding it and is
+// advanced in place.
+__device__ __forceinline__ uint32_t payload_byte(const AssembleJob& A, int64_t hdr, int64_t segbytes, int64_t zz,
+                                                 int& s)
+{
+    if (zz < hdr) return zz == 0 ? 0x78u : (uint32_t)A.zflg;
+    const int64_t a = zz - hdr;
+    if (a >= segbytes) return (*A.adler_out >> (24 - 8 * (int)(a - segbytes))) & 0xFFu;
+    while ((int64_t)A.seg_off[s + 1] <= a) s++;
+    return g8((const uint8_t*)(A.seg_out + (int64_t)s * A.seg_cap_dw) + (a - (int64_t)A.seg_off[s]));
+}
+
+__device__ __forceinline__ uint32_t crc_step4(uint32_t c, uint32_t w, const uint32_t (*T)[256])
+{
+    c ^= w;
+    return T[3][c & 0xFF] ^ T[2][(c >> 8) & 0xFF] ^ T[1][(c >> 16) & 0xFF] ^ T[0][c >> 24];
+}
+
+__device__ __forceinline__ uint32_t crc_shift(uint32_t c, const uint32_t (*S)[256])
+{
+    return S[0][c & 0xFF] ^ S[1][(c >> 8) & 0xFF] ^ S[2][(c >> 16) & 0xFF] ^ S[3][c >> 24];
+}
+
 __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL)
 {
-    __shared__ uint8_t buf[kMaxChunk + 8];
     __shared__ uint32_t tab[4][256];
-    __shared__ uint32_t red[2 * kAssembleThreads];
-    __shared__ int s_seg;
-    const int tid = threadIdx.x;
+    __shared__ uint32_t shf[kCrcShiftLevels][4][256];
+    __shared__ uint32_t red[kAssembleThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < kCrcShiftLevels * 1024; i += kAssembleThreads) (&shf[0][0][0])[i] = (&c_crcshift[0][0][0])[i];
     load_crc_tables(tab);
 
     const uint32_t total = AL.counts ? AL.counts[1] : AL.total_chunks;
@@ -1251,53 +1303,77 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         }
         const AssembleJob A = AL.jobs ? AL.jobs[j] : AL.single;
         const int c = (int)(g - (AL.jobs ? AL.chunk_start[j] : 0u));
-        const uint64_t segbytes = A.seg_off[A.nseg];
-        const int64_t z = png_payload(A, segbytes);
+        const int64_t segbytes = (int64_t)A.seg_off[A.nseg];
+        const int64_t z = png_payload(A, (uint64_t)segbytes);
         const int64_t C = A.chunk_bytes;
         const int64_t nch = (z + C - 1) / C;
-        if (c >= nch) continue;
+        if (c >= nch) continue;                    // uniform per workgroup
         const int64_t z0 = c * C, z1 = min64(z, z0 + C);
         const int n = (int)(z1 - z0);
         const int64_t hdr = A.mode == 1 ? 2 : 0;
-        __syncthreads();
-        // header / trailer bytes
-        if (A.mode == 1) {
-            if (tid < 2) {
-                const int64_t zz = tid;
-                if (zz >= z0 && zz < z1) buf[zz - z0] = tid == 0 ? (uint8_t)0x78 : A.zflg;
-            }
-            if (tid < 4) {
-                const int64_t zz = hdr + (int64_t)segbytes + tid;
-                if (zz >= z0 && zz < z1) {
-                    const uint32_t ad = *A.adler_out;
-                    buf[zz - z0] = (uint8_t)(ad >> (24 - 8 * tid));
-                }
-            }
-        }
-        // segment bytes overlapping [z0, z1)
-        const int64_t a0 = max64(z0 - hdr, 0), a1 = min64(z1 - hdr, (int64_t)segbytes);
-        if (a1 > a0) {
-            if (tid == 0) {
-                int lo = 0, hi = A.nseg - 1;           // last segment with seg_off <= a0
+        uint8_t* o = A.mode == 1 ? A.out + 33 + c * (C + 12) : nullptr;
+        // output address of payload byte zz is dst + zz
+        uint8_t* dst = A.mode == 1 ? o + 8 - z0 : A.out + (A.base_slot ? *A.base_slot : 0ull);
+
+        // piece tid: [e - 64, e) with e counted back from the chunk end
+        const int64_t e = z1 - (int64_t)(kAssembleThreads - 1 - tid) * kAsmBytes;
+        const int64_t b = max64(z0, e - kAsmBytes);
+        uint32_t r = 0;
+        if (b < e) {
+            int s = 0;
+            const int64_t a0 = b - hdr;
+            if (a0 > 0 && a0 < segbytes) {             // last segment starting at or before a0
+                int lo = 0, hi = A.nseg - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if ((int64_t)A.seg_off[mid] <= a0) lo = mid; else hi = mid - 1;
                 }
-                s_seg = lo;
+                s = lo;
             }
-            __syncthreads();
-            for (int s = s_seg; s < A.nseg && (int64_t)A.seg_off[s] < a1; s++) {
-                const int64_t so = (int64_t)A.seg_off[s], se = (int64_t)A.seg_off[s + 1];
-                const int64_t b = max64(a0, so), e = min64(a1, se);
-                const uint8_t* src = (const uint8_t*)(A.seg_out + (int64_t)s * A.seg_cap_dw) + (b - so);
-                uint8_t* dstb = buf + (b + hdr - z0);
-                for (int64_t i = tid; i < e - b; i += blockDim.x) dstb[i] = src[i];
+            for (int64_t q = b; q < e; q += 16) {
+                const int64_t a = q - hdr;
+                bool fast = q + 16 <= e && a >= 0 && a + 16 <= segbytes;
+                if (fast) {
+                    while ((int64_t)A.seg_off[s + 1] <= a) s++;
+                    fast = a + 16 <= (int64_t)A.seg_off[s + 1];
+                }
+                if (fast) {
+                    const u32x4 v = g128u((const uint8_t*)(A.seg_out + (int64_t)s * A.seg_cap_dw) +
+                                          (a - (int64_t)A.seg_off[s]));
+                    st128u(dst + q, v);
+                    r = crc_step4(r, v[0], tab);
+                    r = crc_step4(r, v[1], tab);
+                    r = crc_step4(r, v[2], tab);
+                    r = crc_step4(r, v[3], tab);
+                } else {
+                    const int64_t qe = min64(q + 16, e);
+                    for (int64_t x = q; x < qe; x++) {
+                        const uint32_t by = payload_byte(A, hdr, segbytes, x, s);
+                        dst[x] = (uint8_t)by;
+                        r = tab[0][(r ^ by) & 0xFF] ^ (r >> 8);
+                    }
+                }
             }
         }
+        // tree: levels 0..5 inside the wave, 6..9 across the 16 waves
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const uint32_t right = __shfl_down(r, 1 << k, 64);
+            if ((lane & ((2 << k) - 1)) == 0) r = crc_shift(r, shf[k]) ^ right;
+        }
+        __syncthreads();                           // previous chunk done with red
+        if (lane == 0) red[wave] = r;
         __syncthreads();
-        if (A.mode == 1) {
-            uint8_t* o = A.out + 33 + c * (C + 12);
-            if (c == 0 && tid == 0) {
+        if (wave == 0) {
+            r = lane < kAssembleThreads / 64 ? red[lane] : 0u;
+#pragma unroll
+            for (int k = 6; k < kCrcShiftLevels; k++) {
+                const uint32_t right = __shfl_down(r, 1 << (k - 6), 64);
+                if ((lane & ((2 << (k - 6)) - 1)) == 0) r = crc_shift(r, shf[k]) ^ right;
+            }
+        }
+        if (A.mode == 1 && tid == 0) {
+            if (c == 0) {
                 const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
                 for (int i = 0; i < 8; i++) A.out[i] = sig[i];
                 uint8_t ih[17] = {'I', 'H', 'D', 'R'};
@@ -1307,24 +1383,22 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 for (int i = 0; i < 17; i++) A.out[12 + i] = ih[i];
                 put_be32(A.out + 29, crc_bytes(0, ih, 17, tab));
             }
-            const uint32_t crc_d = block_crc(buf, n, tab, red);
-            if (tid == 0) {
-                const uint8_t idat[4] = {'I', 'D', 'A', 'T'};
-                const uint32_t crc_t = crc_bytes(0, idat, 4, tab);
-                put_be32(o, (uint32_t)n);
-                o[4] = 'I'; o[5] = 'D'; o[6] = 'A'; o[7] = 'T';
-                put_be32(o + 8 + n, crc_combine_dev(crc_t, crc_d, (uint64_t)n));
-                if (c == nch - 1) {
-                    uint8_t* ie = o + 12 + n;
-                    put_be32(ie, 0);
-                    ie[4] = 'I'; ie[5] = 'E'; ie[6] = 'N'; ie[7] = 'D';
-                    put_be32(ie + 8, 0xAE426082u);
-                }
+            // g * x^(8n): n = 64q + rem, as rem zero-byte steps and the set bits of q
+            uint32_t gsh = ~crc_bytes(0, (const uint8_t*)"IDAT", 4, tab);
+            int q = n / kAsmBytes, rem = n % kAsmBytes;
+            if (q >= (1 << kCrcShiftLevels)) { q -= 1; rem += kAsmBytes; }
+            for (int i = 0; i < rem; i++) gsh = tab[0][gsh & 0xFF] ^ (gsh >> 8);
+            for (int k = 0; k < kCrcShiftLevels; k++)
+                if (q & (1 << k)) gsh = crc_shift(gsh, shf[k]);
+            put_be32(o, (uint32_t)n);
+            o[4] = 'I'; o[5] = 'D'; o[6] = 'A'; o[7] = 'T';
+            put_be32(o + 8 + n, ~(gsh ^ r));
+            if (c == nch - 1) {
+                uint8_t* ie = o + 12 + n;
+                put_be32(ie, 0);
+                ie[4] = 'I'; ie[5] = 'E'; ie[6] = 'N'; ie[7] = 'D';
+                put_be32(ie + 8, 0xAE426082u);
             }
-            for (int i = tid; i < n; i += blockDim.x) o[8 + i] = buf[i];
-        } else {
-            uint8_t* o = A.out + z0 + (A.base_slot ? *A.base_slot : 0ull);
-            for (int i = tid; i < n; i += blockDim.x) o[i] = buf[i];
         }
     }
 }
@@ -1381,14 +1455,14 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
 }
 
 // Workgroup sizes: 256 (4 K positions per sub-block, three workgroups per
-// CU), 512 (8 K, two per CU) and 1024 (16 K, two per CU).
+// CU) and 512 (8 K positions, one or two per CU).  Both need ~165 VGPRs, so
+// 1024-thread groups (at most 128) would spill.
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s)
 {
     (void)lane_bytes;
     (void)hipGetLastError();
-    if (wg_threads == 1024)      k_deflate<1024><<<grid, 1024, 0, s>>>(DL);
-    else if (wg_threads == 256)  k_deflate<256><<<grid, 256, 0, s>>>(DL);
-    else                         k_deflate<512><<<grid, 512, 0, s>>>(DL);
+    if (wg_threads == 256) k_deflate<256><<<grid, 256, 0, s>>>(DL);
+    else                   k_deflate<512><<<grid, 512, 0, s>>>(DL);
     return hipGetLastError();
 }
 
@@ -1435,13 +1509,15 @@ hipError_t launch_put_bytes(uint8_t* dst, const uint8_t* bytes16, int n, const u
 }
 
 hipError_t upload_tables(const uint32_t* lit, const uint32_t* len, const uint32_t* dist_rev,
-                         const uint32_t* crc4, const uint32_t* x8pow)
+                         const uint32_t* crc4, const uint32_t* x8pow, const uint32_t* crc_shift)
 {
     hipError_t e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_lit), lit, sizeof(uint32_t) * 256)) != hipSuccess) return e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_len), len, sizeof(uint32_t) * 259)) != hipSuccess) return e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_dist_rev), dist_rev, sizeof(uint32_t) * 32)) != hipSuccess) return e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc), crc4, sizeof(uint32_t) * 1024)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_crcshift), crc_shift, sizeof(uint32_t) * kCrcShiftLevels * 1024)) != hipSuccess)
+        return e;
     return hipMemcpyToSymbol(HIP_SYMBOL(c_x8pow), x8pow, sizeof(uint32_t) * 64);
 }
 
