@@ -79,7 +79,7 @@ def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
 def run_plain(args, r2p, synth, torch):
     s = synth.workload(args.workload)
     img = s.img
-    opt = r2p.options(lz77=args.lz77, wg_threads=args.wg, segment_bytes=args.seg)
+    opt = r2p.options(lz77=args.lz77, wg_threads=args.wg, segment_bytes=args.seg, slice_bytes=args.slice)
     fb = r2p.filtered_bytes(img)
     enc = r2p.Encoder(fb, opt)
     d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
@@ -202,7 +202,7 @@ def run_batch(args, r2p, synth, torch):
                        "assemble": round(kt[3], 3)},
         "filter_gbs": round((isum + fsum) / (kt[0] * 1e-3) / 1e9, 1),
         "deflate_gbs": round((fsum + png_total) / (kt[1] * 1e-3) / 1e9, 1),
-        "autotuned": {"segment_bytes": tuned.segment_bytes, "lane_bytes": tuned.lane_bytes,
+        "autotuned": {"segment_bytes": tuned.segment_bytes, "slice_bytes": tuned.slice_bytes,
                       "wg_threads": tuned.wg_threads,
                       "band_rows": tuned.band_rows, "best_ms": tuned_ms},
         "png_over_filtered": round(png_total / fsum, 4),
@@ -224,6 +224,7 @@ def main():
     ap.add_argument("--lz77", type=int, default=1)
     ap.add_argument("--wg", type=int, default=256, choices=[256, 512])
     ap.add_argument("--seg", type=int, default=65536, choices=[16384, 32768, 65536])
+    ap.add_argument("--slice", type=int, default=16384, choices=[4096, 8192, 16384, 32768, 65536])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times (profiler calibration)")
@@ -254,7 +255,7 @@ def main():
         "ms_per_step": round(r["wall"] / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"{args.workload} {r['s'].img.width}x{r['s'].img.height}",
-                   "form": "plain per-image", "segment_bytes": args.seg, "wg_threads": args.wg, "lz77": args.lz77,
+                   "form": "plain per-image", "segment_bytes": args.seg, "slice_bytes": args.slice, "wg_threads": args.wg, "lz77": args.lz77,
                    "bit_exact": r["ok"]},
         "latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},

@@ -91,7 +91,8 @@ size_t r2p_png_bound(const r2p_image* img, int segment_bytes, int idat_chunk_byt
 
 /* ---- encoder options ---------------------------------------------------- */
 typedef struct r2p_options {
-    int segment_bytes;     /* DEFLATE segment (independent fixed-Huffman block); default 65536 */
+    int segment_bytes;     /* DEFLATE segment (independent fixed-Huffman block, closed by a
+                              sync flush), a multiple of 4096; default 65536 */
     int lz77;              /* 0: literal-only fixed Huffman (the reference default,
                               pipeline.h:40); 1: LZ77 over a 32 KiB primed window */
     int idat_chunk_bytes;  /* IDAT chunk payload size; default 65536 */
@@ -99,6 +100,9 @@ typedef struct r2p_options {
     int band_rows;         /* rows per filter workgroup; 0 = auto */
     int lane_bytes;        /* LZ77 parse span per lane; 16 is the one supported value */
     int wg_threads;        /* DEFLATE workgroup size (256 or 512); 0 = auto (256) */
+    int slice_bytes;       /* DEFLATE work unit: one workgroup per slice of a segment (a power
+                              of two >= 4096 dividing segment_bytes); 0 = auto (16384).
+                              Changes speed only, never the decoded output */
 } r2p_options;
 
 void r2p_options_default(r2p_options* o);

@@ -45,15 +45,16 @@ struct r2p_batch {
     r2p_options opt{};
     int max_images = 0;
     size_t max_input = 0, max_filtered = 0;
-    size_t max_segs = 0, max_out = 0;
-    int seg_bytes_cap = 0;            // smallest segment size the arenas were sized for
+    size_t max_segs = 0, max_slices = 0, slice_arena_dw = 0, max_out = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     // device arenas
     uint8_t* d_filt = nullptr;
-    uint32_t* d_seg_out = nullptr;
-    uint32_t* d_seg_len = nullptr;
-    uint32_t* d_seg_adler = nullptr;
+    uint32_t* d_slice_out = nullptr;
+    uint32_t* d_slice_bits = nullptr;
+    uint32_t* d_slice_adler = nullptr;
+    uint32_t* d_slice_boff = nullptr;
+    uint32_t* d_seg_tbits = nullptr;
     uint64_t* d_seg_off = nullptr;
     uint8_t* d_out = nullptr;
     uint8_t* d_lut = nullptr;
@@ -62,7 +63,7 @@ struct r2p_batch {
     void* h_tab = nullptr;
     size_t tab_bytes = 0;
     FilterJob* d_fj = nullptr; DeflateJob* d_dj = nullptr; AssembleJob* d_aj = nullptr;
-    uint32_t* d_band_start = nullptr; uint32_t* d_seg_start = nullptr; uint32_t* d_chunk_start = nullptr;
+    uint32_t* d_band_start = nullptr; uint32_t* d_slice_start = nullptr; uint32_t* d_chunk_start = nullptr;
     uint32_t* d_counts = nullptr;      // [0..1] filter, [2..3] deflate, [4..5] assemble
     uint64_t* d_res = nullptr;         // per image: png length, adler
     uint64_t* h_res = nullptr;         // pinned
@@ -96,9 +97,9 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     if (timed) R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(launch_filter(L, 4096, lds, s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
-    // 3. deflate over all segments
+    // 3. deflate over all slices
     DeflateLaunch DL{};
-    DL.jobs = b->d_dj; DL.seg_start = b->d_seg_start; DL.counts = b->d_counts + 2;
+    DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 2;
     R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, b->opt.wg_threads, 1024, s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[2], s));
     // 4. per-image scan, 5. assemble all chunks
@@ -129,12 +130,12 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     b->max_images = max_images;
     b->max_input = max_input_bytes;
     b->max_filtered = max_filtered_bytes;
-    // arenas sized for segments as small as 32 KiB (the autotuner's smallest)
-    b->seg_bytes_cap = std::min(32768, b->opt.segment_bytes);
-    b->max_segs = max_filtered_bytes / b->seg_bytes_cap + (size_t)max_images + 1;
-    const size_t cap_dw = (size_t)seg_cap_dwords(b->seg_bytes_cap);
-    // largest per-segment capacity for any segment size <= 4x the smallest
-    const size_t seg_arena_dw = b->max_segs * cap_dw;
+    // arenas sized for any segment and slice size from 4 KiB up: per image
+    // ceil(len / slice) buffers of slice_cap_dwords(slice) dwords
+    b->max_segs = max_filtered_bytes / 4096 + (size_t)max_images + 1;
+    b->max_slices = b->max_segs;
+    b->slice_arena_dw = max_filtered_bytes * 9 / 32 + 9 * (max_filtered_bytes / 4096) +
+                        (size_t)max_images * (size_t)(slice_cap_dwords(65536) + 16) + 1024;
     b->max_out = max_filtered_bytes + max_filtered_bytes / 8 + 32 * b->max_segs + 4096 * (size_t)max_images;
     b->off_fj = 0;
     b->off_dj = b->off_fj + sizeof(FilterJob) * (size_t)max_images;
@@ -149,9 +150,11 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto& v : b->ev) TRY(hipEventCreate(&v));
     TRY(hipMalloc((void**)&b->d_filt, max_filtered_bytes + 256 * (size_t)max_images));
-    TRY(hipMalloc((void**)&b->d_seg_out, seg_arena_dw * 4));
-    TRY(hipMalloc((void**)&b->d_seg_len, b->max_segs * 4));
-    TRY(hipMalloc((void**)&b->d_seg_adler, b->max_segs * 4));
+    TRY(hipMalloc((void**)&b->d_slice_out, b->slice_arena_dw * 4));
+    TRY(hipMalloc((void**)&b->d_slice_bits, b->max_slices * 4));
+    TRY(hipMalloc((void**)&b->d_slice_adler, b->max_slices * 4));
+    TRY(hipMalloc((void**)&b->d_slice_boff, b->max_slices * 4));
+    TRY(hipMalloc((void**)&b->d_seg_tbits, b->max_segs * 4));
     TRY(hipMalloc((void**)&b->d_seg_off, (b->max_segs + (size_t)max_images + 1) * 8));
     TRY(hipMalloc((void**)&b->d_out, b->max_out));
     TRY(hipMalloc((void**)&b->d_lut, (size_t)kLutSlot * (size_t)max_images));
@@ -166,7 +169,7 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     b->d_dj = (DeflateJob*)(dt + b->off_dj);
     b->d_aj = (AssembleJob*)(dt + b->off_aj);
     b->d_band_start = (uint32_t*)(dt + b->off_bs);
-    b->d_seg_start = (uint32_t*)(dt + b->off_ss);
+    b->d_slice_start = (uint32_t*)(dt + b->off_ss);
     b->d_chunk_start = (uint32_t*)(dt + b->off_cs);
     b->d_counts = (uint32_t*)(dt + b->off_cnt);
     *out = b;
@@ -179,8 +182,9 @@ int r2p_batch_destroy(r2p_batch* b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    (void)hipFree(b->d_filt); (void)hipFree(b->d_seg_out); (void)hipFree(b->d_seg_len);
-    (void)hipFree(b->d_seg_adler); (void)hipFree(b->d_seg_off); (void)hipFree(b->d_out);
+    (void)hipFree(b->d_filt); (void)hipFree(b->d_slice_out); (void)hipFree(b->d_slice_bits);
+    (void)hipFree(b->d_slice_adler); (void)hipFree(b->d_slice_boff); (void)hipFree(b->d_seg_tbits);
+    (void)hipFree(b->d_seg_off); (void)hipFree(b->d_out);
     (void)hipFree(b->d_lut); (void)hipFree(b->d_tab); (void)hipFree(b->d_res);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
     if (b->h_res) (void)hipHostFree(b->h_res);
@@ -195,11 +199,6 @@ int r2p_batch_options(r2p_batch* b, const r2p_options* opt)
     if (!b || !opt) { set_error("bad options"); return R2P_ERR_ARG; }
     r2p_options o = *opt;
     fill_defaults(&o);
-    if (o.segment_bytes < b->seg_bytes_cap) {
-        set_error("segment_bytes %d below the arena's %d", o.segment_bytes, b->seg_bytes_cap);
-        return R2P_ERR_ARG;
-    }
-    if (o.idat_chunk_bytes > 65536) o.idat_chunk_bytes = 65536;
     b->opt = o;
     return R2P_OK;
 }
@@ -218,8 +217,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     uint32_t* ss = (uint32_t*)(ht + b->off_ss);
     uint32_t* cs = (uint32_t*)(ht + b->off_cs);
     uint32_t* cnt = (uint32_t*)(ht + b->off_cnt);
-    const int cap_dw = seg_cap_dwords(o.segment_bytes);
-    const size_t seg_arena_dw = b->max_segs * (size_t)seg_cap_dwords(b->seg_bytes_cap);
+    const int cap_dw = slice_cap_dwords(o.slice_bytes);
 
     // band height: auto from the batch's total rows unless fixed by the options
     int64_t total_rows = 0;
@@ -242,7 +240,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     // device tables, deduplicated by parameter set
     std::map<LutKey, int> lut_slot;
     b->luts.clear();
-    size_t foff = 0, segs = 0, ooff = 0;
+    size_t foff = 0, segs = 0, slices = 0, ooff = 0;
     uint32_t bands = 0, chunks = 0;
     int rowmax = 16, lutmax = 0;
     b->out_off.assign((size_t)n, 0);
@@ -275,28 +273,32 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         bs[i] = bands;
         bands += (uint32_t)((imgs[i].height + b->band_rows - 1) / b->band_rows);
 
-        const int nseg = (int)((g.filtered + o.segment_bytes - 1) / o.segment_bytes);
-        if ((segs + nseg) * (size_t)cap_dw > seg_arena_dw || segs + nseg > b->max_segs) {
-            set_error("segment arena exhausted"); return R2P_ERR_SIZE;
+        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
+        const int nseg = sg.nseg;
+        if ((slices + sg.nslice) * (size_t)cap_dw > b->slice_arena_dw || slices + sg.nslice > b->max_slices ||
+            segs + nseg > b->max_segs) {
+            set_error("slice arena exhausted"); return R2P_ERR_SIZE;
         }
         DeflateJob& D = dj[i];
         D = DeflateJob{};
         D.data = F.dst; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
-        D.seg_out = b->d_seg_out + segs * (size_t)cap_dw; D.seg_len = b->d_seg_len + segs;
-        D.seg_adler = b->d_seg_adler + segs; D.seg_stats = nullptr; D.seg_bytes = o.segment_bytes;
-        D.seg_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
-        D.lz77 = o.lz77; D.nseg = nseg;
-        ss[i] = (uint32_t)segs;
+        D.slice_out = b->d_slice_out + slices * (size_t)cap_dw; D.slice_bits = b->d_slice_bits + slices;
+        D.slice_adler = b->d_slice_adler + slices; D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes;
+        D.slice_bytes = o.slice_bytes; D.slice_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp;
+        D.row_stride = (int32_t)(g.row_bytes + 1); D.lz77 = o.lz77; D.nslice = sg.nslice;
+        ss[i] = (uint32_t)slices;
 
         const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
         if (ooff + bound > b->max_out) { set_error("output arena exhausted"); return R2P_ERR_SIZE; }
         AssembleJob& A = aj[i];
         A = AssembleJob{};
-        A.seg_out = D.seg_out; A.seg_len = D.seg_len; A.seg_adler = D.seg_adler;
+        A.slice_out = D.slice_out; A.slice_bits = D.slice_bits; A.slice_adler = D.slice_adler;
+        A.slice_boff = b->d_slice_boff + slices; A.seg_tbits = b->d_seg_tbits + segs;
         A.seg_off = b->d_seg_off + segs + (size_t)i; A.out = b->d_out + ooff;
         A.out_len = b->d_res + 2 * (size_t)i; A.adler_out = (uint32_t*)(b->d_res + 2 * (size_t)i + 1);
-        A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.seg_cap_dw = cap_dw;
-        A.seg_bytes = o.segment_bytes; A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1;
+        A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice;
+        A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = o.slice_bytes;
+        A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
         A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
         A.color_type = (uint8_t)g.color_type;
         const int lvl = o.zlib_level;
@@ -309,10 +311,11 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         ooff += (bound + 255) & ~(size_t)255;
         foff += (g.filtered + 255) & ~(size_t)255;
         segs += (size_t)nseg;
+        slices += (size_t)sg.nslice;
     }
-    bs[n] = bands; ss[n] = (uint32_t)segs; cs[n] = chunks;
+    bs[n] = bands; ss[n] = (uint32_t)slices; cs[n] = chunks;
     cnt[0] = (uint32_t)n; cnt[1] = bands;
-    cnt[2] = (uint32_t)n; cnt[3] = (uint32_t)segs;
+    cnt[2] = (uint32_t)n; cnt[3] = (uint32_t)slices;
     cnt[4] = (uint32_t)n; cnt[5] = chunks;
     b->row_cap = rowmax;
     b->lut_cap = lutmax;
@@ -409,7 +412,7 @@ int r2p_batch_timings(r2p_batch* b, r2p_timings* t)
     return R2P_OK;
 }
 
-// Autotuner: times each candidate (segment_bytes x wg_threads x band_rows)
+// Autotuner: times each candidate (slice_bytes x wg_threads x band_rows)
 // on the declared batch -- one warm-up graph replay, then the best of three
 // timed replays -- and keeps the fastest, re-declaring the batch with it.
 int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
@@ -421,15 +424,15 @@ int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
     const r2p_options base = b->opt;
     r2p_options best = base;
     double best_t = 1e30;
-    static const int segs[] = {32768, 65536};
+    static const int slices_c[] = {16384, 32768, 65536};
     static const int shapes[] = {256, 512};   // DEFLATE wg_threads
     static const int bands[] = {4, 8, 16};
-    for (int sg : segs) {
-        if (sg < b->seg_bytes_cap) continue;
+    for (int sl : slices_c) {
+        if (sl > base.segment_bytes || base.segment_bytes % sl) continue;
         for (int sh : shapes)
             for (int bd : bands) {
                 r2p_options o = base;
-                o.segment_bytes = sg; o.wg_threads = sh; o.band_rows = bd;
+                o.slice_bytes = sl; o.wg_threads = sh; o.band_rows = bd;
                 int rc = r2p_batch_options(b, &o);
                 if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
                 if (rc) continue;

@@ -290,6 +290,14 @@ int default_band_rows(int64_t total_rows)
 void fill_defaults(r2p_options* o)
 {
     if (o->segment_bytes <= 0) o->segment_bytes = 65536;
+    o->segment_bytes = std::max(4096, (o->segment_bytes + 4095) & ~4095);
+    // a slice is a power of two from 4 KiB up that divides the segment
+    int sl = o->slice_bytes > 0 ? o->slice_bytes : 16384;
+    if (sl > o->segment_bytes || (sl & (sl - 1)) || sl < 4096 || o->segment_bytes % sl) {
+        sl = 4096;
+        while (sl * 2 <= std::min(o->segment_bytes, 16384) && o->segment_bytes % (sl * 2) == 0) sl *= 2;
+    }
+    o->slice_bytes = sl;
     if (o->idat_chunk_bytes <= 0) o->idat_chunk_bytes = 65536;
     if (o->idat_chunk_bytes > 65536) o->idat_chunk_bytes = 65536;
     o->lane_bytes = 16;
@@ -379,6 +387,7 @@ void r2p_options_default(r2p_options* o)
     o->band_rows = 0;
     o->lane_bytes = 16;
     o->wg_threads = 256;
+    o->slice_bytes = 16384;
 }
 
 uint32_t r2p_adler32_combine(uint32_t a1, uint32_t a2, uint64_t len2)
@@ -494,11 +503,14 @@ struct r2p_encoder {
     hipEvent_t ev[5] = {};
     size_t max_filtered = 0;
     int max_seg = 0;
-    int seg_cap_dw = 0;
+    int slice_cap_dw = 0;
+    int max_slice = 0;
     uint8_t* d_filt = nullptr;
-    uint32_t* d_seg_out = nullptr;
-    uint32_t* d_seg_len = nullptr;
-    uint32_t* d_seg_adler = nullptr;
+    uint32_t* d_slice_out = nullptr;
+    uint32_t* d_slice_bits = nullptr;
+    uint32_t* d_slice_adler = nullptr;
+    uint32_t* d_slice_boff = nullptr;
+    uint32_t* d_seg_tbits = nullptr;
     uint64_t* d_seg_off = nullptr;
     uint64_t* d_misc = nullptr;      // [0] png length, [1] adler
     uint64_t* h_misc = nullptr;      // pinned mirror
@@ -533,16 +545,19 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
     fill_defaults(&e->opt);
     e->max_filtered = max_filtered_bytes;
     e->max_seg = (int)((max_filtered_bytes + e->opt.segment_bytes - 1) / e->opt.segment_bytes);
-    e->seg_cap_dw = seg_cap_dwords(e->opt.segment_bytes);
+    e->max_slice = (int)((max_filtered_bytes + e->opt.slice_bytes - 1) / e->opt.slice_bytes) + 1;
+    e->slice_cap_dw = slice_cap_dwords(e->opt.slice_bytes);
     hipError_t err;
 #define TRY(x) do { if ((err = (x)) != hipSuccess) { r2p_encoder_destroy(e); return hip_fail(err, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& v : e->ev) TRY(hipEventCreate(&v));
     TRY(hipMalloc((void**)&e->d_filt, max_filtered_bytes + kTailSlack));
-    TRY(hipMalloc((void**)&e->d_seg_out, (size_t)e->max_seg * e->seg_cap_dw * 4));
-    TRY(hipMalloc((void**)&e->d_seg_len, (size_t)e->max_seg * 4));
-    TRY(hipMalloc((void**)&e->d_seg_adler, (size_t)e->max_seg * 4));
-    TRY(hipMalloc((void**)&e->d_seg_off, ((size_t)e->max_seg + 1) * 8));
+    TRY(hipMalloc((void**)&e->d_slice_out, (size_t)e->max_slice * e->slice_cap_dw * 4));
+    TRY(hipMalloc((void**)&e->d_slice_bits, (size_t)e->max_slice * 4));
+    TRY(hipMalloc((void**)&e->d_slice_adler, (size_t)e->max_slice * 4));
+    TRY(hipMalloc((void**)&e->d_slice_boff, (size_t)e->max_slice * 4));
+    TRY(hipMalloc((void**)&e->d_seg_tbits, ((size_t)e->max_seg + 1) * 4));
+    TRY(hipMalloc((void**)&e->d_seg_off, ((size_t)e->max_seg + 2) * 8));
     TRY(hipMalloc((void**)&e->d_misc, 64));
     TRY(hipHostMalloc((void**)&e->h_misc, 64, hipHostMallocDefault));
 #undef TRY
@@ -554,8 +569,9 @@ int r2p_encoder_destroy(r2p_encoder* e)
 {
     if (!e) return R2P_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    (void)hipFree(e->d_filt); (void)hipFree(e->d_seg_out); (void)hipFree(e->d_seg_len);
-    (void)hipFree(e->d_seg_adler); (void)hipFree(e->d_seg_off); (void)hipFree(e->d_misc);
+    (void)hipFree(e->d_filt); (void)hipFree(e->d_slice_out); (void)hipFree(e->d_slice_bits);
+    (void)hipFree(e->d_slice_adler); (void)hipFree(e->d_slice_boff); (void)hipFree(e->d_seg_tbits);
+    (void)hipFree(e->d_seg_off); (void)hipFree(e->d_misc);
     if (e->h_misc) (void)hipHostFree(e->h_misc);
     (void)hipFree(e->d_src_stage); (void)hipFree(e->d_png_stage);
     for (auto& v : e->ev) if (v) (void)hipEventDestroy(v);
@@ -600,25 +616,29 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     if (d_filtered_out)
         R2P_HIP(hipMemcpyAsync(d_filtered_out, filt, g.filtered, hipMemcpyDeviceToDevice, s));
 
-    const int nseg = (int)((g.filtered + o.segment_bytes - 1) / o.segment_bytes);
+    const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
+    const int nseg = sg.nseg;
     DeflateLaunch DL{};
     DeflateJob& D = DL.single;
     D.data = filt; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
-    D.seg_out = e->d_seg_out; D.seg_len = e->d_seg_len; D.seg_adler = e->d_seg_adler;
-    D.seg_stats = nullptr; D.seg_bytes = o.segment_bytes; D.seg_cap_dw = e->seg_cap_dw;
+    D.slice_out = e->d_slice_out; D.slice_bits = e->d_slice_bits; D.slice_adler = e->d_slice_adler;
+    D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes; D.slice_bytes = o.slice_bytes;
+    D.slice_cap_dw = e->slice_cap_dw;
     D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
-    D.lz77 = o.lz77; D.nseg = nseg;
-    DL.jobs = nullptr; DL.njobs = 1; DL.total_segs = (uint32_t)nseg;
-    R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, nseg, s));
+    D.lz77 = o.lz77; D.nslice = sg.nslice;
+    DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
+    R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
     R2P_HIP(hipEventRecord(e->ev[2], s));
 
     AssembleLaunch AL{};
     AssembleJob& A = AL.single;
-    A.seg_out = e->d_seg_out; A.seg_len = e->d_seg_len; A.seg_adler = e->d_seg_adler;
+    A.slice_out = e->d_slice_out; A.slice_bits = e->d_slice_bits; A.slice_adler = e->d_slice_adler;
+    A.slice_boff = e->d_slice_boff; A.seg_tbits = e->d_seg_tbits;
     A.seg_off = e->d_seg_off; A.out = (uint8_t*)d_png; A.out_len = e->d_misc;
     A.adler_out = (uint32_t*)(e->d_misc + 1); A.out_cap = (int64_t)png_cap;
-    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.seg_cap_dw = e->seg_cap_dw;
-    A.seg_bytes = o.segment_bytes; A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1;
+    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
+    A.slice_cap_dw = e->slice_cap_dw; A.slice_bytes = o.slice_bytes;
+    A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
     A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
     A.color_type = (uint8_t)g.color_type;
     const int lvl = o.zlib_level;

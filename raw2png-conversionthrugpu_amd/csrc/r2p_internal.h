@@ -47,42 +47,54 @@ struct LutBuild {
                     // 1: a4 8-bit LUT -> u8
 };
 
-// One stream of filtered bytes to compress into independent segments.
+// One stream of filtered bytes to compress.  The stream is cut into
+// segments of seg_bytes -- the unit of the output format: one fixed-Huffman
+// block each, closed by EOB and a sync flush (or BFINAL on the stream's last
+// segment) -- and every segment into slices of slice_bytes, the unit of
+// work: one workgroup, with the 32 KiB before the slice as history.  A
+// slice's tokens go to its own buffer as a bit string (the first slice of a
+// segment starts with the 3-bit block header); k_scan and k_assemble join a
+// segment's slices at bit granularity and append the segment's trailer.
 struct DeflateJob {
     const uint8_t* data;       // base of the addressable bytes
     int64_t        first;      // offset of the first byte to compress
     int64_t        len;        // bytes to compress
     int64_t        dict_lo;    // lowest offset a match may reference
-    uint32_t*      seg_out;    // nseg * seg_cap_dw dwords of scratch
-    uint32_t*      seg_len;    // compressed bytes per segment
-    uint32_t*      seg_adler;  // adler32 of each segment's input
-    uint32_t*      seg_stats;  // optional: 4 u32 per segment (matches, matched bytes, dist sum lo, max len)
+    uint32_t*      slice_out;  // nslice * slice_cap_dw dwords of scratch
+    uint32_t*      slice_bits; // token bits per slice
+    uint32_t*      slice_adler;// adler32 of each slice's input
+    uint32_t*      slice_stats;// optional: 4 u32 per slice (matches, matched bytes, dist sum lo, max len)
     int32_t        seg_bytes;
-    int32_t        seg_cap_dw;
+    int32_t        slice_bytes;// divides seg_bytes
+    int32_t        slice_cap_dw;
     int32_t        final_block; // the job's last segment sets BFINAL (else sync flush)
     int32_t        bpp;
     int32_t        row_stride; // filtered row size (match candidate one row up), 0 = none
     int32_t        lz77;
-    int32_t        nseg;
-    int32_t        pad_;
+    int32_t        nslice;     // max(1, ceil(len / slice_bytes))
 };
 
 // Where a job's segments go once their sizes are known.
 struct AssembleJob {
-    const uint32_t* seg_out;
-    const uint32_t* seg_len;
-    const uint32_t* seg_adler;
-    uint64_t*       seg_off;     // scan output: nseg + 1 entries (byte offsets in the zlib payload)
+    const uint32_t* slice_out;
+    const uint32_t* slice_bits;
+    const uint32_t* slice_adler;
+    uint32_t*       slice_boff;  // scan output: bit offset of each slice inside its segment
+    uint32_t*       seg_tbits;   // scan output: token bits of each segment (header included)
+    uint64_t*       seg_off;     // scan output: nseg + 1 byte offsets in the deflate payload
     uint8_t*        out;         // PNG (or raw) destination
     uint64_t*       out_len;     // total bytes written (device)
     uint32_t*       adler_out;   // final adler32 (device), optional
     int64_t         out_cap;
     int64_t         total_in;    // uncompressed bytes (for the adler combine)
     int32_t         nseg;
-    int32_t         seg_cap_dw;
-    int32_t         seg_bytes;
+    int32_t         nslice;
+    int32_t         spb;         // slices per segment (seg_bytes / slice_bytes)
+    int32_t         slice_cap_dw;
+    int32_t         slice_bytes;
     int32_t         chunk_bytes; // IDAT payload bytes per chunk
     int32_t         mode;        // 0 = raw deflate concatenation, 1 = complete PNG
+    int32_t         final_block; // the last segment ends the stream (no sync flush)
     uint32_t        width, height;
     uint8_t         bit_depth, color_type, zflg, pad_;
     int32_t         max_chunks;  // chunks this job may use (grid sizing bound)
@@ -106,9 +118,9 @@ struct FilterLaunch {
 
 struct DeflateLaunch {
     const DeflateJob* jobs;
-    const uint32_t*   seg_start;   // prefix of segment counts per job (njobs + 1)
+    const uint32_t*   slice_start; // prefix of slice counts per job (njobs + 1)
     int32_t           njobs;
-    uint32_t          total_segs;
+    uint32_t          total_slices;
     const uint32_t*   counts;      // optional device [njobs, total]
     DeflateJob        single;
 };
@@ -134,8 +146,21 @@ constexpr int kWindow = 32768;           // DEFLATE window
 constexpr int kTailSlack = 64;
 constexpr int kChecksumBlock = 65536;    // bytes per checksum workgroup
 
-// Worst-case compressed dwords of one segment of `seg_bytes` input bytes:
-// 9 bits per literal, 3-bit header, 7-bit EOB, <=7 pad, 3+32 bits of sync flush.
-inline int seg_cap_dwords(int seg_bytes) { return (int)(((int64_t)seg_bytes * 9 + 64) / 32) + 4; }
+// Worst-case token dwords of one slice of `slice_bytes` input bytes (9 bits
+// per literal, the 3-bit block header), plus the slack k_assemble's 5-dword
+// reads may touch past the last written word.
+inline int slice_cap_dwords(int slice_bytes) { return (int)(((int64_t)slice_bytes * 9 + 3 + 31) / 32) + 8; }
+
+// Segment / slice counts of a stream of `len` bytes (an empty stream still
+// has one segment and one slice: the block header and its end).
+struct SliceGeom { int nseg, nslice, spb; };
+inline SliceGeom slice_geom(int64_t len, int seg_bytes, int slice_bytes)
+{
+    SliceGeom g;
+    g.nseg = len > 0 ? (int)((len + seg_bytes - 1) / seg_bytes) : 1;
+    g.nslice = len > 0 ? (int)((len + slice_bytes - 1) / slice_bytes) : 1;
+    g.spb = seg_bytes / slice_bytes;
+    return g;
+}
 
 }  // namespace r2p

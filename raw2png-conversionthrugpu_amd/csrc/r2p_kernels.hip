@@ -965,7 +965,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
     __shared__ uint32_t st[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    const uint32_t total = DL.counts ? DL.counts[1] : DL.total_segs;
+    const uint32_t total = DL.counts ? DL.counts[1] : DL.total_slices;
     const int njobs = DL.counts ? (int)DL.counts[0] : DL.njobs;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         int j = 0;
@@ -973,23 +973,26 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             int lo = 0, hi = njobs - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (DL.seg_start[mid] <= g) lo = mid; else hi = mid - 1;
+                if (DL.slice_start[mid] <= g) lo = mid; else hi = mid - 1;
             }
             j = lo;
         }
         const DeflateJob J = DL.jobs ? DL.jobs[j] : DL.single;
-        const int seg = (int)(g - (DL.jobs ? DL.seg_start[j] : 0u));
+        const int sl = (int)(g - (DL.jobs ? DL.slice_start[j] : 0u));
+        const int spb = J.seg_bytes / J.slice_bytes;
+        const int seg = sl / spb;
+        const int nseg = J.len > 0 ? (int)((J.len + J.seg_bytes - 1) / J.seg_bytes) : 1;
         const uint8_t* d = J.data;
-        const int64_t s0 = J.first + (int64_t)seg * J.seg_bytes;
-        const int64_t s1 = min64(s0 + J.seg_bytes, J.first + J.len);
+        const int64_t s0 = J.first + (int64_t)sl * J.slice_bytes;
+        const int64_t s1 = min64(s0 + J.slice_bytes, J.first + J.len);
         const int64_t dend = J.first + J.len;
-        const bool last = (seg == J.nseg - 1) && J.final_block;
+        const bool last = (seg == nseg - 1) && J.final_block;
         const int64_t wlo = max64(J.dict_lo, s0 - (int64_t)kWindow);
-        uint32_t* out = J.seg_out + (int64_t)seg * J.seg_cap_dw;
+        uint32_t* out = J.slice_out + (int64_t)sl * J.slice_cap_dw;
         // Positions below are relative to wb = d + wlo (window base).
         const uint8_t* wb = d + wlo;
         const int s0_rel = (int)(s0 - wlo);
-        const int s1_rel = (int)(s1 - wlo);
+        const int s1_rel = (int)max64(s1 - wlo, s0_rel);
         const int end_rel = (int)min64(dend - wlo, 0x7FFFFFF0);
         const bool lz = J.lz77 != 0;
         const int bpp = J.bpp;
@@ -1012,8 +1015,10 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             }
         }
 
-        uint32_t carry = last ? 3u : 2u;   // BFINAL | BTYPE=01 << 1
-        uint32_t carry_bits = 3;
+        // the segment's first slice opens its block: BFINAL | BTYPE=01 << 1
+        const bool opens = sl % spb == 0;
+        uint32_t carry = opens ? (last ? 3u : 2u) : 0u;
+        uint32_t carry_bits = opens ? 3u : 0u;
         int64_t out_dw = 0;
 
         for (int q = s0_rel; q < s1_rel; q += SUB) {
@@ -1077,7 +1082,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             // ---- block scan of bit counts ----------------------------------------------
             const uint32_t incl = wave_incl_scan(nbits);
             if (lane == 63) red[wave] = incl;
-            if (J.seg_stats) {
+            if (J.slice_stats) {
                 atomicAdd(&st[0], m_cnt); atomicAdd(&st[1], m_bytes);
                 atomicAdd(&st[2], m_dist); atomicMax(&st[3], m_max);
             }
@@ -1130,29 +1135,18 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             out_dw += full;
         }
 
-        // ---- segment end: EOB, then sync flush or final pad ------------------------
+        // ---- slice end: the partial last word and the bit count ---------------
         __syncthreads();
         if (tid == 0) {
-            uint64_t acc = carry;
-            uint32_t nb = carry_bits + 7;                 // EOB = 7 zero bits
-            int64_t o = out_dw;
-            if (!last) nb += 3;                           // stored block header, BFINAL=0
-            nb = (nb + 7) & ~7u;                          // byte align
-            if (nb >= 32) { out[o++] = (uint32_t)acc; acc >>= 32; nb -= 32; }
-            if (!last) {
-                acc |= (uint64_t)0xFFFF0000u << nb;       // LEN = 0000, NLEN = FFFF
-                nb += 32;
-                if (nb >= 32) { out[o++] = (uint32_t)acc; acc >>= 32; nb -= 32; }
-            }
-            if (nb > 0) out[o] = (uint32_t)acc;
-            J.seg_len[seg] = (uint32_t)(o * 4 + nb / 8);
-            if (J.seg_stats) {
-                uint32_t* ss = J.seg_stats + 4 * (int64_t)seg;
+            if (carry_bits) out[out_dw] = carry;
+            J.slice_bits[sl] = (uint32_t)(out_dw * 32 + carry_bits);
+            if (J.slice_stats) {
+                uint32_t* ss = J.slice_stats + 4 * (int64_t)sl;
                 ss[0] = st[0]; ss[1] = st[1]; ss[2] = st[2]; ss[3] = st[3];
             }
         }
         const uint32_t ad = block_adler(d + s0, s1 - s0, red);
-        if (tid == 0) J.seg_adler[seg] = ad;
+        if (tid == 0) J.slice_adler[sl] = ad;
     }
 }
 
@@ -1175,10 +1169,21 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (AL.counts && blockIdx.x >= AL.counts[0]) return;
     const AssembleJob A = AL.jobs ? AL.jobs[blockIdx.x] : AL.single;
+    // per segment: bit offsets of its slices, token bits, then the byte
+    // length with its trailer (EOB, and unless last the 3-bit stored header,
+    // pad, 00 00 FF FF); exclusive scan of the lengths
     uint64_t base = 0;
     for (int c0 = 0; c0 < A.nseg; c0 += 256) {
         const int i = c0 + tid;
-        const uint32_t v = i < A.nseg ? A.seg_len[i] : 0u;
+        uint32_t v = 0;
+        if (i < A.nseg) {
+            const int k0 = i * A.spb, k1 = min(k0 + A.spb, A.nslice);
+            uint32_t tb = 0;
+            for (int k = k0; k < k1; k++) { A.slice_boff[k] = tb; tb += A.slice_bits[k]; }
+            A.seg_tbits[i] = tb;
+            const bool last = (i == A.nseg - 1) && A.final_block;
+            v = (tb + 7 + (last ? 0u : 3u) + 7) / 8 + (last ? 0u : 4u);
+        }
         const uint32_t incl = wave_incl_scan(v);
         __syncthreads();
         if (lane == 63) wsum[wave] = incl;
@@ -1188,13 +1193,13 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
         if (i < A.nseg) A.seg_off[i] = base + wb + incl - v;
         base += tot;
     }
-    // adler combine: contiguous runs per thread, then a tree
-    const int per = (A.nseg + 255) / 256;
-    const int b = min(A.nseg, per * tid), e = min(A.nseg, b + per);
+    // adler combine over the slices: contiguous runs per thread, then a tree
+    const int per = (A.nslice + 255) / 256;
+    const int b = min(A.nslice, per * tid), e = min(A.nslice, b + per);
     uint32_t v = 1; uint64_t l = 0;
     for (int i = b; i < e; i++) {
-        const uint64_t li = (uint64_t)min64(A.seg_bytes, A.total_in - (int64_t)i * A.seg_bytes);
-        v = l ? adler_combine_dev(v, A.seg_adler[i], li) : A.seg_adler[i];
+        const uint64_t li = (uint64_t)max64(0, min64(A.slice_bytes, A.total_in - (int64_t)i * A.slice_bytes));
+        v = l ? adler_combine_dev(v, A.slice_adler[i], li) : A.slice_adler[i];
         l += li;
     }
     aval[tid] = v; alen[tid] = l;
@@ -1256,17 +1261,70 @@ __device__ __forceinline__ void put_be32(uint8_t* p, uint32_t v)
 typedef R2P_GLOBAL u32x4_ua gu32x4_ua_w;
 __device__ __forceinline__ void st128u(uint8_t* p, u32x4 v) { *(gu32x4_ua_w*)p = v; }
 
-// Payload byte zz (mode 1: [78 FLG][segments][adler BE]; mode 0: the
-// segments).  s is a segment index at or before the one holding it and is
-// advanced in place.
-__device__ __forceinline__ uint32_t payload_byte(const AssembleJob& A, int64_t hdr, int64_t segbytes, int64_t zz,
-                                                 int& s)
+// 32 bits of slice k's bit string from bit q (bits past its end are junk;
+// the slice buffers have slack words for the second read).
+__device__ __forceinline__ uint32_t slice_read32(const AssembleJob& A, int k, uint32_t q)
 {
-    if (zz < hdr) return zz == 0 ? 0x78u : (uint32_t)A.zflg;
-    const int64_t a = zz - hdr;
-    if (a >= segbytes) return (*A.adler_out >> (24 - 8 * (int)(a - segbytes))) & 0xFFu;
-    while ((int64_t)A.seg_off[s + 1] <= a) s++;
-    return g8((const uint8_t*)(A.seg_out + (int64_t)s * A.seg_cap_dw) + (a - (int64_t)A.seg_off[s]));
+    const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (q >> 5);
+    return __builtin_amdgcn_alignbit(w[1], w[0], q & 31);
+}
+
+__device__ __forceinline__ uint32_t low_bits(uint32_t v, uint32_t n) { return n >= 32 ? v : v & ((1u << n) - 1); }
+
+// 32 bits of segment s's token string from bit p (zero past its end): from
+// the slice holding p and, across its end, the next one (a slice shorter
+// than 32 bits can only be a segment's last).  k is a slice cursor at or
+// before that slice, advanced in place.
+__device__ uint32_t seg_bits32(const AssembleJob& A, int s, uint32_t p, int& k)
+{
+    const uint32_t tb = A.seg_tbits[s];
+    if (p >= tb) return 0u;
+    const int k1 = min(s * A.spb + A.spb, A.nslice);
+    k = max(k, s * A.spb);
+    while (k + 1 < k1 && A.slice_boff[k + 1] <= p) k++;
+    const uint32_t off = p - A.slice_boff[k];
+    const uint32_t avail = A.slice_bits[k] - off;
+    uint32_t v = low_bits(slice_read32(A, k, off), avail);
+    if (avail < 32 && k + 1 < k1) v |= low_bits(slice_read32(A, k + 1, 0), A.slice_bits[k + 1]) << avail;
+    return low_bits(v, tb - p);
+}
+
+// Payload bytes [q, q + n) (n <= 16) of mode 1: [78 FLG][segments][adler BE]
+// (mode 0: the segments), assembled 4 bytes at a time where they come from
+// one segment; s and k are segment and slice cursors at or before q.
+__device__ u32x4 payload16(const AssembleJob& A, int64_t hdr, int64_t segbytes, int64_t q, int n, int& s, int& k)
+{
+    unsigned __int128 acc = 0;
+    int t = 0;
+    while (t < n) {
+        const int64_t x = q + t;
+        if (x < hdr) {
+            acc |= (unsigned __int128)(x == 0 ? 0x78u : (uint32_t)A.zflg) << (8 * t);
+            t++;
+            continue;
+        }
+        const int64_t a = x - hdr;
+        if (a >= segbytes) {
+            acc |= (unsigned __int128)((*A.adler_out >> (24 - 8 * (int)(a - segbytes))) & 0xFFu) << (8 * t);
+            t++;
+            continue;
+        }
+        while ((int64_t)A.seg_off[s + 1] <= a) s++;
+        const uint32_t bo = (uint32_t)(a - (int64_t)A.seg_off[s]);
+        const uint32_t seglen = (uint32_t)(A.seg_off[s + 1] - A.seg_off[s]);
+        const bool last = (s == A.nseg - 1) && A.final_block;
+        const int m = (int)min((uint32_t)min(4, n - t), seglen - bo);     // bytes from this segment
+        uint32_t v = seg_bits32(A, s, 8 * bo, k);
+        if (!last && bo + m > seglen - 2) {                                  // ... 00 00 FF FF
+            for (int i = 0; i < m; i++)
+                if (bo + i >= seglen - 2) v |= 0xFFu << (8 * i);
+        }
+        acc |= (unsigned __int128)low_bits(v, 8 * m) << (8 * t);
+        t += m;
+    }
+    u32x4 r;
+    r[0] = (uint32_t)acc; r[1] = (uint32_t)(acc >> 32); r[2] = (uint32_t)(acc >> 64); r[3] = (uint32_t)(acc >> 96);
+    return r;
 }
 
 __device__ __forceinline__ uint32_t crc_step4(uint32_t c, uint32_t w, const uint32_t (*T)[256])
@@ -1330,27 +1388,54 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 }
                 s = lo;
             }
+            int k = s * A.spb;                         // slice cursor
             for (int64_t q = b; q < e; q += 16) {
                 const int64_t a = q - hdr;
                 bool fast = q + 16 <= e && a >= 0 && a + 16 <= segbytes;
+                uint32_t p = 0;
                 if (fast) {
                     while ((int64_t)A.seg_off[s + 1] <= a) s++;
-                    fast = a + 16 <= (int64_t)A.seg_off[s + 1];
+                    p = 8 * (uint32_t)(a - (int64_t)A.seg_off[s]);
+                    const int k1 = min(s * A.spb + A.spb, A.nslice);
+                    k = max(k, s * A.spb);
+                    while (k + 1 < k1 && A.slice_boff[k + 1] <= p) k++;
+                    // all 128 bits inside slice k's tokens
+                    fast = p + 128 <= A.slice_boff[k] + A.slice_bits[k];
                 }
                 if (fast) {
-                    const u32x4 v = g128u((const uint8_t*)(A.seg_out + (int64_t)s * A.seg_cap_dw) +
-                                          (a - (int64_t)A.seg_off[s]));
+                    const uint32_t off = p - A.slice_boff[k];
+                    const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
+                    const u32x4 lo4 = *(R2P_GLOBAL const u32x4_ua*)w;
+                    const uint32_t w4 = w[4];
+                    const uint32_t sh = off & 31;
+                    u32x4 v;
+                    v[0] = __builtin_amdgcn_alignbit(lo4[1], lo4[0], sh);
+                    v[1] = __builtin_amdgcn_alignbit(lo4[2], lo4[1], sh);
+                    v[2] = __builtin_amdgcn_alignbit(lo4[3], lo4[2], sh);
+                    v[3] = __builtin_amdgcn_alignbit(w4, lo4[3], sh);
                     st128u(dst + q, v);
                     r = crc_step4(r, v[0], tab);
                     r = crc_step4(r, v[1], tab);
                     r = crc_step4(r, v[2], tab);
                     r = crc_step4(r, v[3], tab);
                 } else {
-                    const int64_t qe = min64(q + 16, e);
-                    for (int64_t x = q; x < qe; x++) {
-                        const uint32_t by = payload_byte(A, hdr, segbytes, x, s);
-                        dst[x] = (uint8_t)by;
-                        r = tab[0][(r ^ by) & 0xFF] ^ (r >> 8);
+                    const int nb = (int)min64(16, e - q);
+                    const u32x4 v = payload16(A, hdr, segbytes, q, nb, s, k);
+                    if (nb == 16) {
+                        st128u(dst + q, v);
+                        r = crc_step4(r, v[0], tab);
+                        r = crc_step4(r, v[1], tab);
+                        r = crc_step4(r, v[2], tab);
+                        r = crc_step4(r, v[3], tab);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 16; i++) {
+                            if (i < nb) {
+                                const uint32_t by = (v[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                                dst[q + i] = (uint8_t)by;
+                                r = tab[0][(r ^ by) & 0xFF] ^ (r >> 8);
+                            }
+                        }
                     }
                 }
             }

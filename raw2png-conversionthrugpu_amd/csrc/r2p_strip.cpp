@@ -19,22 +19,24 @@ using namespace r2p;
 struct r2p_deflate {
     hipStream_t stream = nullptr;
     size_t max_strip = 0, max_total = 0;
-    int lz77 = 1, seg_bytes = 65536, seg_cap_dw = 0, max_seg = 0;
+    int lz77 = 1, seg_bytes = 65536, slice_bytes = 16384, slice_cap_dw = 0, max_seg = 0, max_slice = 0;
     uint8_t* d_win[2] = {nullptr, nullptr};
     int cur = 0;
     size_t hist = 0;                   // history bytes at the start of d_win[cur]
-    uint32_t* d_seg_out = nullptr;
-    uint32_t* d_seg_len = nullptr;
-    uint32_t* d_seg_adler = nullptr;
-    uint32_t* d_seg_stats = nullptr;
+    uint32_t* d_slice_out = nullptr;
+    uint32_t* d_slice_bits = nullptr;
+    uint32_t* d_slice_adler = nullptr;
+    uint32_t* d_slice_stats = nullptr;
+    uint32_t* d_slice_boff = nullptr;
+    uint32_t* d_seg_tbits = nullptr;
     uint64_t* d_seg_off = nullptr;
     uint8_t* d_out = nullptr;
     uint64_t* d_state = nullptr;       // [0] running bytes, [1] base slot, [2] out_len, [3] adler
     uint64_t* h_state = nullptr;       // pinned
-    uint32_t* h_stats = nullptr;       // pinned, 4 per segment
+    uint32_t* h_stats = nullptr;       // pinned, 4 per slice
     std::vector<uint32_t> strip_adler;
     std::vector<uint64_t> strip_len;
-    int pending_nseg = 0;
+    int pending_nslice = 0;
     bool pending = false;
     size_t out_bytes = 0;
     r2p_lz_stats stats{};
@@ -54,9 +56,14 @@ int r2p_deflate_create(size_t max_strip_bytes, size_t max_total_bytes, int lz77,
     r2p_deflate* d = new r2p_deflate();
     d->max_strip = max_strip_bytes;
     d->lz77 = lz77 ? 1 : 0;
-    d->seg_bytes = segment_bytes > 0 ? segment_bytes : 65536;
-    d->seg_cap_dw = seg_cap_dwords(d->seg_bytes);
-    d->max_seg = (int)((max_strip_bytes + d->seg_bytes - 1) / d->seg_bytes);
+    r2p_options o{};
+    o.segment_bytes = segment_bytes;
+    fill_defaults(&o);
+    d->seg_bytes = o.segment_bytes;
+    d->slice_bytes = o.slice_bytes;
+    d->slice_cap_dw = slice_cap_dwords(d->slice_bytes);
+    d->max_seg = (int)((max_strip_bytes + d->seg_bytes - 1) / d->seg_bytes) + 1;
+    d->max_slice = (int)((max_strip_bytes + d->slice_bytes - 1) / d->slice_bytes) + 1;
     // output bound: 9/8 of the input plus per-segment framing
     const size_t total_in = std::max(max_total_bytes, max_strip_bytes);
     d->max_total = total_in + total_in / 8 + 16 * (total_in / d->seg_bytes + 2) + 64;
@@ -64,15 +71,17 @@ int r2p_deflate_create(size_t max_strip_bytes, size_t max_total_bytes, int lz77,
 #define TRY(x) do { if ((e = (x)) != hipSuccess) { r2p_deflate_destroy(d); return hip_fail(e, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
     for (int i = 0; i < 2; i++) TRY(hipMalloc((void**)&d->d_win[i], kWindow + max_strip_bytes + kTailSlack));
-    TRY(hipMalloc((void**)&d->d_seg_out, (size_t)d->max_seg * d->seg_cap_dw * 4));
-    TRY(hipMalloc((void**)&d->d_seg_len, (size_t)d->max_seg * 4));
-    TRY(hipMalloc((void**)&d->d_seg_adler, (size_t)d->max_seg * 4));
-    TRY(hipMalloc((void**)&d->d_seg_stats, (size_t)d->max_seg * 16));
+    TRY(hipMalloc((void**)&d->d_slice_out, (size_t)d->max_slice * d->slice_cap_dw * 4));
+    TRY(hipMalloc((void**)&d->d_slice_bits, (size_t)d->max_slice * 4));
+    TRY(hipMalloc((void**)&d->d_slice_adler, (size_t)d->max_slice * 4));
+    TRY(hipMalloc((void**)&d->d_slice_stats, (size_t)d->max_slice * 16));
+    TRY(hipMalloc((void**)&d->d_slice_boff, (size_t)d->max_slice * 4));
+    TRY(hipMalloc((void**)&d->d_seg_tbits, (size_t)d->max_seg * 4));
     TRY(hipMalloc((void**)&d->d_seg_off, ((size_t)d->max_seg + 1) * 8));
     TRY(hipMalloc((void**)&d->d_out, d->max_total));
     TRY(hipMalloc((void**)&d->d_state, 64));
     TRY(hipHostMalloc((void**)&d->h_state, 64, hipHostMallocDefault));
-    TRY(hipHostMalloc((void**)&d->h_stats, (size_t)d->max_seg * 16, hipHostMallocDefault));
+    TRY(hipHostMalloc((void**)&d->h_stats, (size_t)d->max_slice * 16, hipHostMallocDefault));
 #undef TRY
     *out = d;
     return r2p_deflate_reset(d);
@@ -83,8 +92,9 @@ int r2p_deflate_destroy(r2p_deflate* d)
     if (!d) return R2P_OK;
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     for (auto p : d->d_win) (void)hipFree(p);
-    (void)hipFree(d->d_seg_out); (void)hipFree(d->d_seg_len); (void)hipFree(d->d_seg_adler);
-    (void)hipFree(d->d_seg_stats); (void)hipFree(d->d_seg_off); (void)hipFree(d->d_out);
+    (void)hipFree(d->d_slice_out); (void)hipFree(d->d_slice_bits); (void)hipFree(d->d_slice_adler);
+    (void)hipFree(d->d_slice_stats); (void)hipFree(d->d_slice_boff); (void)hipFree(d->d_seg_tbits);
+    (void)hipFree(d->d_seg_off); (void)hipFree(d->d_out);
     (void)hipFree(d->d_state);
     if (d->h_state) (void)hipHostFree(d->h_state);
     if (d->h_stats) (void)hipHostFree(d->h_stats);
@@ -99,7 +109,7 @@ int r2p_deflate_reset(r2p_deflate* d)
     R2P_HIP(hipStreamSynchronize(d->stream));
     R2P_HIP(hipMemsetAsync(d->d_state, 0, 64, d->stream));
     R2P_HIP(hipStreamSynchronize(d->stream));
-    d->cur = 0; d->hist = 0; d->out_bytes = 0; d->pending = false; d->pending_nseg = 0;
+    d->cur = 0; d->hist = 0; d->out_bytes = 0; d->pending = false; d->pending_nslice = 0;
     d->strip_adler.clear(); d->strip_len.clear();
     d->stats = r2p_lz_stats{};
     d->dist_sum = 0;
@@ -121,24 +131,28 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     hipStream_t s = d->stream;
     if (len) R2P_HIP(hipMemcpyAsync(win + d->hist, d_filtered, len, hipMemcpyDeviceToDevice, s));
 
-    const int nseg = len ? (int)((len + d->seg_bytes - 1) / d->seg_bytes) : 1;
+    const SliceGeom sg = slice_geom((int64_t)len, d->seg_bytes, d->slice_bytes);
+    const int nseg = sg.nseg;
     DeflateLaunch DL{};
     DeflateJob& D = DL.single;
     D.data = win; D.first = (int64_t)d->hist; D.len = (int64_t)len; D.dict_lo = 0;
-    D.seg_out = d->d_seg_out; D.seg_len = d->d_seg_len; D.seg_adler = d->d_seg_adler;
-    D.seg_stats = d->d_seg_stats; D.seg_bytes = d->seg_bytes; D.seg_cap_dw = d->seg_cap_dw;
+    D.slice_out = d->d_slice_out; D.slice_bits = d->d_slice_bits; D.slice_adler = d->d_slice_adler;
+    D.slice_stats = d->d_slice_stats; D.seg_bytes = d->seg_bytes; D.slice_bytes = d->slice_bytes;
+    D.slice_cap_dw = d->slice_cap_dw;
     D.final_block = is_last ? 1 : 0; D.bpp = bpp > 0 ? bpp : 1; D.row_stride = row_stride;
-    D.lz77 = d->lz77; D.nseg = nseg;
-    DL.jobs = nullptr; DL.njobs = 1; DL.total_segs = (uint32_t)nseg;
-    R2P_HIP(launch_deflate(DL, 16, 512, nseg, s));
+    D.lz77 = d->lz77; D.nslice = sg.nslice;
+    DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
+    R2P_HIP(launch_deflate(DL, 16, 256, sg.nslice, s));
 
     AssembleLaunch AL{};
     AssembleJob& A = AL.single;
-    A.seg_out = d->d_seg_out; A.seg_len = d->d_seg_len; A.seg_adler = d->d_seg_adler;
+    A.slice_out = d->d_slice_out; A.slice_bits = d->d_slice_bits; A.slice_adler = d->d_slice_adler;
+    A.slice_boff = d->d_slice_boff; A.seg_tbits = d->d_seg_tbits;
     A.seg_off = d->d_seg_off; A.out = d->d_out; A.out_len = d->d_state + 2;
     A.adler_out = (uint32_t*)(d->d_state + 3); A.out_cap = (int64_t)d->max_total;
-    A.total_in = (int64_t)len; A.nseg = nseg; A.seg_cap_dw = d->seg_cap_dw; A.seg_bytes = d->seg_bytes;
-    A.chunk_bytes = 65536; A.mode = 0;
+    A.total_in = (int64_t)len; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
+    A.slice_cap_dw = d->slice_cap_dw; A.slice_bytes = d->slice_bytes;
+    A.chunk_bytes = 65536; A.mode = 0; A.final_block = is_last ? 1 : 0;
     const size_t zmax = len + len / 8 + 16 * (size_t)nseg + 16;
     A.max_chunks = (int32_t)((zmax + 65535) / 65536);
     A.running = d->d_state; A.base_slot = d->d_state + 1;
@@ -151,11 +165,11 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     const size_t keep = std::min<size_t>(kWindow, have);
     if (keep) R2P_HIP(hipMemcpyAsync(d->d_win[d->cur ^ 1], win + (have - keep), keep, hipMemcpyDeviceToDevice, s));
     R2P_HIP(hipMemcpyAsync(d->h_state, d->d_state, 32, hipMemcpyDeviceToHost, s));
-    if (nseg > 0) R2P_HIP(hipMemcpyAsync(d->h_stats, d->d_seg_stats, (size_t)nseg * 16, hipMemcpyDeviceToHost, s));
+    R2P_HIP(hipMemcpyAsync(d->h_stats, d->d_slice_stats, (size_t)sg.nslice * 16, hipMemcpyDeviceToHost, s));
     d->cur ^= 1;
     d->hist = keep;
     d->pending = true;
-    d->pending_nseg = len ? nseg : 0;
+    d->pending_nslice = len ? sg.nslice : 0;
     d->strip_len.push_back(len);
     return R2P_OK;
 }
@@ -167,7 +181,7 @@ int r2p_deflate_sync(r2p_deflate* d)
     if (d->pending) {
         d->out_bytes = (size_t)d->h_state[0];
         d->strip_adler.push_back((uint32_t)d->h_state[3]);
-        for (int i = 0; i < d->pending_nseg; i++) {
+        for (int i = 0; i < d->pending_nslice; i++) {
             const uint32_t* s = d->h_stats + 4 * i;
             d->stats.matches += s[0];
             d->stats.matched_bytes += s[1];

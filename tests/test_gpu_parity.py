@@ -240,6 +240,28 @@ def test_encode_segment_and_chunk_sizes(torch_dev, r2p, oracle, seg, chunk):
     _check_png(oracle, r2p, png, px, img, filt)
 
 
+@pytest.mark.parametrize("seg,slc,w,h", [
+    (65536, 4096, 2340, 7),        # 16387 bytes: four full 4 KiB slices and one of 3 bytes
+    (8192, 4096, 2340, 7),         # three segments, the last one a 3-byte slice
+    (65536, 16384, 1500, 300),     # segments of four slices, ragged last segment
+    (65536, 65536, 1500, 300),     # one slice per segment
+])
+def test_encode_slices(torch_dev, r2p, oracle, seg, slc, w, h):
+    """Slices are a work split only: any slice size inflates to the same bytes,
+    including segments whose last slice holds a few bytes."""
+    log(f"case encode seg={seg} slice={slc} {w}x{h}")
+    rng = np.random.default_rng(w + h)
+    px = (np.cumsum(rng.integers(-3, 4, (h, w)), axis=1) % 256).astype(np.uint8)
+    px[h // 3: h // 2, :] = 200
+    img = r2p.Image.make(w, h, 1, 8, 0)
+    for lz77 in (0, 1):
+        opt = r2p.options(segment_bytes=seg, slice_bytes=slc, lz77=lz77)
+        assert (opt.segment_bytes, opt.slice_bytes) == (seg, slc)
+        png, filt, adler, nseg, t = encode_device(torch_dev, r2p, px, img, opt)
+        _check_png(oracle, r2p, png, px, img, filt)
+        assert adler == zlib.adler32(filt.tobytes())
+
+
 def test_encode_tiny_and_host_path(torch_dev, r2p, oracle):
     log("case encode tiny/host")
     for (w, h, ch, bits) in [(1, 1, 1, 8), (2, 1, 3, 8), (1, 3, 1, 16), (5, 4, 4, 16)]:
