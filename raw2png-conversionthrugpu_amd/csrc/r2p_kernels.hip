@@ -951,7 +951,10 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
     o[4] = g32u(p + 16);
 }
 
-template <int NT>
+// STATS: also accumulate per-slice match statistics (the strip form's
+// LzStats); a separate instantiation so that the plain and batched forms do
+// not carry their registers.
+template <int NT, bool STATS>
 __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 {
     constexpr int LANE = kLane;
@@ -967,6 +970,10 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 
     const uint32_t total = DL.counts ? DL.counts[1] : DL.total_slices;
     const int njobs = DL.counts ? (int)DL.counts[0] : DL.njobs;
+    // Slices are dealt to workgroups in order.  (Giving each of the 8 XCDs a
+    // contiguous run of slices, so that the 32 KiB windows hit its L2, cut
+    // the HBM reads to the input size but ran 35% slower on a radiograph:
+    // neighbouring slices cost alike, so the XCDs ended unevenly loaded.)
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         int j = 0;
         if (DL.jobs) {
@@ -1063,26 +1070,35 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                 lane_matches<R2P_MATCH_GROUP>(pk, wb, o, lb, le, end_rel, m1, mb, mr, bpp > 1 ? bpp : 1, row_ok ? dr : 1, T1, T2);
             }
             launder(pk, LANE);
-            // greedy parse: token starts and the lane's bit count
-            uint32_t nextp = 0, nbits = 0, tokm = 0;
-            uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
+            // greedy parse: each token's code (code | nbits << 26, 0 where no
+            // token starts) and the lane's bit count
+            uint32_t tk[LANE];
+            uint32_t nextp = 0, nbits = 0;
 #pragma unroll
             for (int i = 0; i < LANE; i++) {
+                if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
                 const uint32_t L = pk[i] >> 16;
                 const uint32_t D = (pk[i] & 0xFFFFu) + 1;
-                const uint32_t byte = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                 const bool tok = ((uint32_t)i == nextp) & (i < le);
-                const uint32_t nb = L ? match_bits(L, D) : (byte >= 144 ? 9u : 8u);
+                uint32_t mv, mn, lv, ln;
+                match_code(max(L, 3u), D, mv, mn);
+                lit_code((o[i >> 2] >> (8 * (i & 3))) & 0xFFu, lv, ln);
+                const uint32_t nb = L ? mn : ln;
+                tk[i] = tok ? ((L ? mv : lv) | (nb << 26)) : 0u;
                 nbits += tok ? nb : 0u;
-                tokm |= tok ? 1u << i : 0u;
                 nextp = tok ? (uint32_t)i + max(L, 1u) : nextp;
-                const uint32_t Lt = tok ? L : 0u;
-                m_cnt += Lt ? 1u : 0u; m_bytes += Lt; m_dist += Lt ? D : 0u; m_max = max(m_max, Lt);
             }
             // ---- block scan of bit counts ----------------------------------------------
             const uint32_t incl = wave_incl_scan(nbits);
             if (lane == 63) red[wave] = incl;
-            if (J.slice_stats) {
+            if (STATS && J.slice_stats) {          // match statistics (strip form only)
+                uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
+#pragma unroll
+                for (int i = 0; i < LANE; i++) {
+                    const uint32_t Lt = tk[i] ? pk[i] >> 16 : 0u;
+                    m_cnt += Lt ? 1u : 0u; m_bytes += Lt; m_dist += Lt ? (pk[i] & 0xFFFFu) + 1 : 0u;
+                    m_max = max(m_max, Lt);
+                }
                 atomicAdd(&st[0], m_cnt); atomicAdd(&st[1], m_bytes);
                 atomicAdd(&st[2], m_dist); atomicMax(&st[3], m_max);
             }
@@ -1090,7 +1106,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
-            launder(pk, LANE);
+            launder(tk, LANE);
             launder(o, LANE / 4 + 1);
             // the sub-block's positions enter T1; T2 is emptied for the next one
             if (lz) {
@@ -1109,15 +1125,8 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                 uint64_t acc = 0;
 #pragma unroll
                 for (int i = 0; i < LANE; i++) {
-                    if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
-                    const uint32_t L = pk[i] >> 16;
-                    const uint32_t D = (pk[i] & 0xFFFFu) + 1;
-                    uint32_t mv, mn, lv, ln;
-                    match_code(max(L, 3u), D, mv, mn);
-                    lit_code((o[i >> 2] >> (8 * (i & 3))) & 0xFFu, lv, ln);
-                    const bool tok = (tokm >> i) & 1u;
-                    acc |= (uint64_t)(tok ? (L ? mv : lv) : 0u) << nacc;
-                    nacc += tok ? (L ? mn : ln) : 0u;
+                    acc |= (uint64_t)(tk[i] & 0x3FFFFFFu) << nacc;
+                    nacc += tk[i] >> 26;
                     const bool fl = nacc >= 32;
                     atomicOr(&obuf[wi], fl ? (uint32_t)acc : 0u);
                     wi += fl ? 1u : 0u;
@@ -1140,7 +1149,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
         if (tid == 0) {
             if (carry_bits) out[out_dw] = carry;
             J.slice_bits[sl] = (uint32_t)(out_dw * 32 + carry_bits);
-            if (J.slice_stats) {
+            if (STATS && J.slice_stats) {
                 uint32_t* ss = J.slice_stats + 4 * (int64_t)sl;
                 ss[0] = st[0]; ss[1] = st[1]; ss[2] = st[2]; ss[3] = st[3];
             }
@@ -1341,10 +1350,10 @@ __device__ __forceinline__ uint32_t crc_shift(uint32_t c, const uint32_t (*S)[25
 __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL)
 {
     __shared__ uint32_t tab[4][256];
-    __shared__ uint32_t shf[kCrcShiftLevels][4][256];
     __shared__ uint32_t red[kAssembleThreads / 64];
+    __shared__ u32x4 stage[kAssembleThreads / 64][256];    // 4 KiB per wave
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < kCrcShiftLevels * 1024; i += kAssembleThreads) (&shf[0][0][0])[i] = (&c_crcshift[0][0][0])[i];
+    const uint32_t (*shf)[4][256] = c_crcshift;     // read through the caches: ~4 lookups per level
     load_crc_tables(tab);
 
     const uint32_t total = AL.counts ? AL.counts[1] : AL.total_chunks;
@@ -1389,7 +1398,10 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 s = lo;
             }
             int k = s * A.spb;                         // slice cursor
-            for (int64_t q = b; q < e; q += 16) {
+#pragma unroll 1
+            for (int jg = 0; jg < 4; jg++) {
+                const int64_t q = b + 16 * jg;
+                if (q >= e) break;
                 const int64_t a = q - hdr;
                 bool fast = q + 16 <= e && a >= 0 && a + 16 <= segbytes;
                 uint32_t p = 0;
@@ -1402,44 +1414,55 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                     // all 128 bits inside slice k's tokens
                     fast = p + 128 <= A.slice_boff[k] + A.slice_bits[k];
                 }
+                u32x4 v;
                 if (fast) {
                     const uint32_t off = p - A.slice_boff[k];
                     const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
                     const u32x4 lo4 = *(R2P_GLOBAL const u32x4_ua*)w;
                     const uint32_t w4 = w[4];
                     const uint32_t sh = off & 31;
-                    u32x4 v;
                     v[0] = __builtin_amdgcn_alignbit(lo4[1], lo4[0], sh);
                     v[1] = __builtin_amdgcn_alignbit(lo4[2], lo4[1], sh);
                     v[2] = __builtin_amdgcn_alignbit(lo4[3], lo4[2], sh);
                     v[3] = __builtin_amdgcn_alignbit(w4, lo4[3], sh);
-                    st128u(dst + q, v);
+                } else {
+                    v = payload16(A, hdr, segbytes, q, (int)min64(16, e - q), s, k);
+                }
+                stage[wave][lane * 4 + jg] = v;
+                const int nb = (int)min64(16, e - q);
+                if (nb == 16) {
                     r = crc_step4(r, v[0], tab);
                     r = crc_step4(r, v[1], tab);
                     r = crc_step4(r, v[2], tab);
                     r = crc_step4(r, v[3], tab);
                 } else {
-                    const int nb = (int)min64(16, e - q);
-                    const u32x4 v = payload16(A, hdr, segbytes, q, nb, s, k);
-                    if (nb == 16) {
-                        st128u(dst + q, v);
-                        r = crc_step4(r, v[0], tab);
-                        r = crc_step4(r, v[1], tab);
-                        r = crc_step4(r, v[2], tab);
-                        r = crc_step4(r, v[3], tab);
-                    } else {
 #pragma unroll
-                        for (int i = 0; i < 16; i++) {
-                            if (i < nb) {
-                                const uint32_t by = (v[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-                                dst[q + i] = (uint8_t)by;
-                                r = tab[0][(r ^ by) & 0xFF] ^ (r >> 8);
-                            }
-                        }
-                    }
+                    for (int i = 0; i < 16; i++)
+                        if (i < nb) r = tab[0][(r ^ ((v[i >> 2] >> (8 * (i & 3))) & 0xFFu)) & 0xFF] ^ (r >> 8);
                 }
             }
         }
+        // stores from the wave's LDS stage: a wave whose 64 pieces are all whole
+        // writes consecutive 16 bytes from consecutive lanes; otherwise piece-wise
+        __builtin_amdgcn_wave_barrier();
+        {
+            const int64_t wb0 = z1 - (int64_t)(kAssembleThreads - 64 * wave) * kAsmBytes;   // wave's first byte
+            if (wb0 >= z0) {
+#pragma unroll
+                for (int jg = 0; jg < 4; jg++) st128u(dst + wb0 + 1024 * jg + 16 * lane, stage[wave][64 * jg + lane]);
+            } else {
+                for (int jg = 0; jg < 4; jg++) {
+                    const int64_t q = b + 16 * jg;
+                    if (q >= e) break;
+                    const u32x4 v = stage[wave][lane * 4 + jg];
+                    const int nb = (int)min64(16, e - q);
+                    if (nb == 16) st128u(dst + q, v);
+                    else
+                        for (int i = 0; i < nb; i++) dst[q + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
         // tree: levels 0..5 inside the wave, 6..9 across the 16 waves
 #pragma unroll
         for (int k = 0; k < 6; k++) {
@@ -1546,8 +1569,10 @@ hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_thread
 {
     (void)lane_bytes;
     (void)hipGetLastError();
-    if (wg_threads == 256) k_deflate<256><<<grid, 256, 0, s>>>(DL);
-    else                   k_deflate<512><<<grid, 512, 0, s>>>(DL);
+    const bool stats = !DL.jobs && DL.single.slice_stats;
+    if (stats)                  k_deflate<256, true><<<grid, 256, 0, s>>>(DL);
+    else if (wg_threads == 256) k_deflate<256, false><<<grid, 256, 0, s>>>(DL);
+    else                        k_deflate<512, false><<<grid, 512, 0, s>>>(DL);
     return hipGetLastError();
 }
 
