@@ -210,13 +210,79 @@ def run_batch(args, r2p, synth, torch):
     }
 
 
+def run_serve(args, r2p, synth, torch):
+    """Served form: open-loop Poisson arrivals drawn from the mixed batch's
+    size/type mix plus 5% 8192x8192 uint16 radiographs, admitted against a
+    64 GB HBM budget; latency is arrival -> PNG bytes in host memory."""
+    import ctypes as C
+    L = r2p.lib()
+    rng = np.random.Generator(np.random.PCG64(4))
+    pool = synth.mixed_batch_device(torch, args.serve_pool, seed=4, max_side=args.batch_max_side)
+    big = synth.radiograph_device(torch, 8192, seed=44)
+    tmpl = pool + [big]
+    torch.cuda.synchronize()
+    # expected PNG length of every template from the plain form
+    opt = r2p.options()
+    expect = []
+    for im, t in tmpl:
+        enc = r2p.Encoder(r2p.filtered_bytes(im), opt)
+        cap = r2p.png_bound(im, opt)
+        d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
+        enc.encode_device(im, t.data_ptr(), d_png.data_ptr(), cap)
+        n, _, _ = enc.sync()
+        enc.close()
+        expect.append(n)
+    pins = [t.cpu().pin_memory() for _, t in tmpl]
+    imgs = (r2p.Image * len(tmpl))(*[im for im, _ in tmpl])
+    srcs = (C.c_void_p * len(tmpl))(*[C.c_void_p(p.data_ptr()) for p in pins])
+    mp_of = np.array([im.width * im.height / 1e6 for im, _ in tmpl])
+    out = {"seconds_per_rate": args.serve_seconds, "hbm_budget_gb": 64, "lanes": args.serve_lanes,
+           "max_batch_images": args.serve_batch, "pool": len(pool)}
+    for lam in args.serve_rates:
+        n = int(lam * args.serve_seconds)
+        arr = np.cumsum(rng.exponential(1.0 / lam, n))
+        kind = rng.integers(0, len(pool), n).astype(np.int32)
+        kind[rng.uniform(size=n) < 0.05] = len(tmpl) - 1
+        cfg = r2p.ServerConfig()
+        cfg.hbm_budget_bytes = 64 << 30
+        cfg.max_batch_images = args.serve_batch
+        cfg.inflight_batches = args.serve_lanes
+        cfg.opt = opt
+        h = C.c_void_p()
+        r2p.check(L.r2p_server_create(C.byref(cfg), C.byref(h)))
+        lat = np.zeros(n)
+        plen = np.zeros(n, np.uint64)
+        wall = C.c_double()
+        try:
+            r2p.check(L.r2p_server_loadtest(h, n, arr.ctypes.data_as(C.POINTER(C.c_double)),
+                                            kind.ctypes.data_as(C.POINTER(C.c_int)), imgs, srcs,
+                                            lat.ctypes.data_as(C.POINTER(C.c_double)),
+                                            plen.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(wall)))
+            st = r2p.ServerStats()
+            r2p.check(L.r2p_server_stats_get(h, C.byref(st)))
+        finally:
+            L.r2p_server_destroy(h)
+        mp = float(mp_of[kind].sum())
+        same = bool(np.all(plen == np.array(expect, np.uint64)[kind]))
+        out[str(lam)] = {
+            "requests": n, "offered_mp_per_s": round(mp / float(arr[-1]), 1),
+            "mp_per_s": round(mp / wall.value, 1), "wall_s": round(wall.value, 3),
+            "latency_ms": {q: round(float(np.percentile(lat, int(q[1:]))) * 1e3, 3) for q in ("p50", "p95", "p99")},
+            "batches": int(st.batches), "mean_batch_images": round(st.mean_batch_images, 2),
+            "peak_admitted_gb": round(st.peak_admitted_bytes / 1e9, 3),
+            "png_lengths_equal_plain": same,
+        }
+        log(f"bench: serve lambda={lam}: {json.dumps(out[str(lam)])}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="radiograph", choices=["ct_slice", "radiograph", "camera_rgb"])
-    ap.add_argument("--mode", default="all", choices=["plain", "batch", "all"])
+    ap.add_argument("--mode", default="all", choices=["plain", "batch", "serve", "all"])
     ap.add_argument("--batch-n", type=int, default=1024)
     ap.add_argument("--batch-max-side", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -226,6 +292,12 @@ def main():
     ap.add_argument("--seg", type=int, default=65536, choices=[16384, 32768, 65536])
     ap.add_argument("--slice", type=int, default=16384, choices=[4096, 8192, 16384, 32768, 65536])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 800.0, 2000.0])
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window per rate (the served workload names 60 s)")
+    ap.add_argument("--serve-pool", type=int, default=64, help="distinct request images of the mix")
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times (profiler calibration)")
     ap.add_argument("--cpu-level", type=int, default=3)
@@ -276,6 +348,8 @@ def main():
         out["ratio"]["zlib_level%d_png_over_filtered" % args.cpu_level] = round(cb["png_bytes"] / r["filtered_bytes"], 4)
     if args.mode in ("batch", "all"):
         out["batched"] = run_batch(args, r2p, synth, torch)
+    if args.mode in ("serve", "all"):
+        out["served"] = run_serve(args, r2p, synth, torch)
     if args.calib_copy_mib:
         # a plain device copy of known size, for scaling the HBM byte counters
         # of a profiler run (FETCH_SIZE / WRITE_SIZE) to bytes

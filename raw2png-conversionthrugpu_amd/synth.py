@@ -261,6 +261,17 @@ def mixed_batch_device(torch, n: int = 1024, seed: int = 3, max_side: int = 4096
     return out
 
 
+def radiograph_device(torch, side: int, seed: int):
+    """One radiograph-like image (the 3072x3072 workload's generator) of the
+    given side, generated in device memory: (Image, uint8 tensor)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    v, wc, ww = _dev_rad(torch, g, side)
+    px = v.to(torch.int16).view(torch.uint8).reshape(-1).contiguous()
+    img = Image.make(side, side, 1, 16, PIX_DICOM, DicomParams.make(16, 12, 11, 0, 0, 1.0, 0.0, 1, wc, ww))
+    return img, px
+
+
 # ---------------------------------------------------------------------------
 # Container writers for the workloads' specified formats: uncompressed
 # Explicit VR Little Endian DICOM Part 10 and uncompressed little-endian TIFF.
