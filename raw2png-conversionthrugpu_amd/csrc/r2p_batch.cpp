@@ -15,6 +15,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -487,7 +488,8 @@ struct Request {
     const void* h_src = nullptr;
     size_t in_bytes = 0, filtered = 0, footprint = 0;
     SClock::time_point arrival{}, done_at{};
-    std::vector<uint8_t> png;
+    uint8_t* png = nullptr;            // PNG bytes, a buffer of the server's pool
+    size_t png_len = 0, png_cap = 0;
     int status = R2P_OK;
     bool done = false;
 };
@@ -507,8 +509,46 @@ struct Lane {
 
 }  // namespace
 
+// Recycled host buffers for finished PNGs: fresh allocations of this size
+// come from mmap and fault in every page on first touch, which at serving
+// rates costs more than the copy itself.
+struct BufPool {
+    std::mutex mu;
+    std::multimap<size_t, uint8_t*> free;      // capacity -> buffer
+    size_t kept = 0;
+    static constexpr size_t kKeep = (size_t)8 << 30;
+    uint8_t* get(size_t n, size_t* cap)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = free.lower_bound(n);
+            if (it != free.end() && it->first <= 2 * n + (1 << 16)) {
+                uint8_t* p = it->second;
+                *cap = it->first;
+                kept -= it->first;
+                free.erase(it);
+                return p;
+            }
+        }
+        size_t c = (size_t)1 << 16;
+        while (c < n) c <<= 1;
+        *cap = c;
+        return new uint8_t[c];
+    }
+    void put(uint8_t* p, size_t cap)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu);
+        if (kept + cap > kKeep) { delete[] p; return; }
+        free.emplace(cap, p);
+        kept += cap;
+    }
+    ~BufPool() { for (auto& kv : free) delete[] kv.second; }
+};
+
 struct r2p_server {
     r2p_server_config cfg{};
+    BufPool pool;
     std::vector<Lane> lanes;
     size_t lane_filtered = 0;
     std::mutex mu;
@@ -523,6 +563,8 @@ struct r2p_server {
     uint64_t batch_images = 0;
     int dev = 0;
 };
+
+constexpr size_t kCopyThreads = 4;   // host threads copying a lane's PNGs out
 
 static size_t req_footprint(size_t in_b, size_t filt)
 {
@@ -550,7 +592,9 @@ static void lane_launch(r2p_server* s, Lane& L)
         if (hipMemcpyAsync(L.d_in + off, r->h_src, r->in_bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = R2P_ERR_HIP;
         off += (r->in_bytes + 255) & ~(size_t)255;
     }
-    if (!rc) rc = r2p_batch_run(L.b, 1);
+    // eager launches: the batch's shape (widest row, table size) changes from
+    // one admission to the next, and each change would re-capture a graph
+    if (!rc) rc = r2p_batch_run(L.b, 0);
     if (!rc && hipEventRecord(L.ev_run, st) != hipSuccess) rc = R2P_ERR_HIP;
     if (rc) {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -575,7 +619,7 @@ static void lane_copy(Lane& L)
         L.stage_off[i] = off;
         if (len && off + len <= L.stage_cap)
             (void)hipMemcpyAsync(L.h_stage + off, d, len, hipMemcpyDeviceToHost, st);
-        L.reqs[i]->png.resize(len);
+        L.reqs[i]->png_len = len;
         off += (len + 255) & ~(size_t)255;
     }
     (void)hipEventRecord(L.ev_copy, st);
@@ -584,10 +628,21 @@ static void lane_copy(Lane& L)
 
 static void lane_finish(r2p_server* s, Lane& L)
 {
-    for (size_t i = 0; i < L.reqs.size(); i++) {
-        Request* r = L.reqs[i];
-        if (!r->png.empty()) memcpy(r->png.data(), L.h_stage + L.stage_off[i], r->png.size());
-    }
+    // copy the PNGs out of the pinned stage, split over a few threads
+    const size_t n = L.reqs.size();
+    auto work = [&](size_t t, size_t nt) {
+        for (size_t i = t; i < n; i += nt) {
+            Request* r = L.reqs[i];
+            if (!r->png_len) continue;
+            r->png = s->pool.get(r->png_len, &r->png_cap);
+            memcpy(r->png, L.h_stage + L.stage_off[i], r->png_len);
+        }
+    };
+    const size_t nt = std::min<size_t>(kCopyThreads, n);
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work, t, nt);
+    work(0, std::max<size_t>(nt, 1));
+    for (auto& t : th) t.join();
     const auto now = SClock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     for (Request* r : L.reqs) { r->done_at = now; r->done = true; }
@@ -718,7 +773,7 @@ int r2p_server_destroy(r2p_server* s)
         if (L.ev_run) (void)hipEventDestroy(L.ev_run);
         if (L.ev_copy) (void)hipEventDestroy(L.ev_copy);
     }
-    for (auto& kv : s->all) delete kv.second;
+    for (auto& kv : s->all) { s->pool.put(kv.second->png, kv.second->png_cap); delete kv.second; }
     delete s;
     return R2P_OK;
 }
@@ -776,12 +831,13 @@ int r2p_server_wait(r2p_server* s, int64_t id, int timeout_ms, void* h_png, size
         return R2P_ERR_STATE;
     }
     if (r->status) { set_error("request failed with %d", r->status); return r->status; }
-    if (len) *len = r->png.size();
+    if (len) *len = r->png_len;
     if (h_png) {
-        if (r->png.size() > cap) { set_error("png of %zu bytes exceeds %zu", r->png.size(), cap); return R2P_ERR_SIZE; }
-        memcpy(h_png, r->png.data(), r->png.size());
+        if (r->png_len > cap) { set_error("png of %zu bytes exceeds %zu", r->png_len, cap); return R2P_ERR_SIZE; }
+        memcpy(h_png, r->png, r->png_len);
     }
     s->all.erase(it);
+    s->pool.put(r->png, r->png_cap);
     delete r;
     return R2P_OK;
 }
@@ -821,10 +877,11 @@ int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int
         Request* r = s->all[ids[(size_t)i]];
         s->cv_done.wait(lk, [r] { return r->done; });
         latency_s[i] = std::chrono::duration<double>(r->done_at - r->arrival).count();
-        if (png_len) png_len[i] = r->png.size();
+        if (png_len) png_len[i] = r->png_len;
         if (r->done_at > last) last = r->done_at;
         if (r->status) rc = r->status;
         s->all.erase(ids[(size_t)i]);
+        s->pool.put(r->png, r->png_cap);
         delete r;
     }
     if (wall_s) *wall_s = std::chrono::duration<double>(last - t0).count();
