@@ -236,6 +236,7 @@ def run_serve(args, r2p, synth, torch):
     imgs = (r2p.Image * len(tmpl))(*[im for im, _ in tmpl])
     srcs = (C.c_void_p * len(tmpl))(*[C.c_void_p(p.data_ptr()) for p in pins])
     mp_of = np.array([im.width * im.height / 1e6 for im, _ in tmpl])
+    in_of = np.array([r2p.input_bytes(im) for im, _ in tmpl], dtype=np.float64)
     out = {"seconds_per_rate": args.serve_seconds, "hbm_budget_gb": 64, "lanes": args.serve_lanes,
            "max_batch_images": args.serve_batch, "pool": len(pool)}
     for lam in args.serve_rates:
@@ -267,6 +268,8 @@ def run_serve(args, r2p, synth, torch):
         out[str(lam)] = {
             "requests": n, "offered_mp_per_s": round(mp / float(arr[-1]), 1),
             "mp_per_s": round(mp / wall.value, 1), "wall_s": round(wall.value, 3),
+            "h2d_gb_per_s": round(float(in_of[kind].sum()) / wall.value / 1e9, 2),
+            "d2h_gb_per_s": round(float(plen.astype(np.float64).sum()) / wall.value / 1e9, 2),
             "latency_ms": {q: round(float(np.percentile(lat, int(q[1:]))) * 1e3, 3) for q in ("p50", "p95", "p99")},
             "batches": int(st.batches), "mean_batch_images": round(st.mean_batch_images, 2),
             "peak_admitted_gb": round(st.peak_admitted_bytes / 1e9, 3),
