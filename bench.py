@@ -76,6 +76,22 @@ def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
             "png_bytes": png_len}
 
 
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 byte
+    counters (profiles/traffic_*.json, scaled by a copy calibration), when
+    they were taken on this workload and configuration; else None."""
+    path = os.path.join(REPO, "profiles", "traffic_radiograph3072.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if args.workload != "radiograph" or (t.get("segment_bytes"), t.get("slice_bytes"), t.get("wg_threads"),
+                                         t.get("lz77")) != (args.seg, args.slice, args.wg, args.lz77):
+        return None
+    k = t.get("kernels", {}).get("k_" + kernel)
+    return k["bytes"] if k else None
+
+
 def run_plain(args, r2p, synth, torch):
     s = synth.workload(args.workload)
     img = s.img
@@ -337,7 +353,7 @@ def main():
         "ratio": {"gpu_png_over_filtered": round(r["png_bytes"] / r["filtered_bytes"], 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic(args, dom)},
     }
     log("bench: plain " + json.dumps({k: out[k] for k in ("value", "ms_per_step", "latency_ms", "kernels_ms", "roofline")}))
     fk = "filter"
