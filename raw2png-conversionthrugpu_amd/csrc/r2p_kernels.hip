@@ -775,6 +775,14 @@ __global__ __launch_bounds__(256) void k_checksum_final(const uint32_t* part, in
 constexpr int kHashBits = 13;      // T1: 8 Ki entries
 constexpr int kHash2Bits = 12;     // T2: 4 Ki entries
 constexpr int kLane = 16;          // positions per parse lane
+// Table entries are tag << kPosBits | position (relative to the window base,
+// < 32 KiB + 64 KiB).  The tag names the slice (T1) or sub-block (T2) that
+// wrote the entry, so entries of earlier ones never match and the tables
+// are cleared only when the workgroup starts or the tags run out.
+constexpr int kPosBits = 17;
+constexpr uint32_t kPosMask = (1u << kPosBits) - 1;
+constexpr uint32_t kTagMax = (1u << (32 - kPosBits)) - 1;
+constexpr int kPrimeStride = 4;    // window positions entered into T1 at this stride
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
 
@@ -885,7 +893,7 @@ __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t* c, int KW
 template <int G>
 __device__ __forceinline__ void lane_matches(uint32_t* pk, const uint8_t* wb, const uint32_t* o, int lb, int le,
                                              int end_rel, uint32_t m1, uint32_t mb, uint32_t mr, int bpp, int dr,
-                                             const uint32_t* T1, const uint32_t* T2)
+                                             const uint32_t* T1, const uint32_t* T2, uint32_t tag1, uint32_t tag2)
 {
 #pragma unroll
     for (int g0 = 0; g0 < kLane; g0 += G) {
@@ -903,10 +911,11 @@ __device__ __forceinline__ void lane_matches(uint32_t* pk, const uint8_t* wb, co
             const uint32_t h = hash4(bytes_at(o, i));
             const uint32_t f = T2[h >> (kHashBits - kHash2Bits)];
             const uint32_t t = T1[h];
-            const bool fv = f < (uint32_t)P;
-            const bool tv = (t != 0) & (P - (int)(t - 1) <= kWindow);
+            const int fp = (int)(f & kPosMask), tp = (int)(t & kPosMask);
+            const bool fv = ((f >> kPosBits) == tag2) & (fp < P);
+            const bool tv = ((t >> kPosBits) == tag1) & (P - tp <= kWindow);
             const bool hv = (fv | tv) & (bl < cap) & (P + 4 <= end_rel);
-            C[u] = hv ? (fv ? (int)f : (int)t - 1) : -1;
+            C[u] = hv ? (fv ? fp : tp) : -1;
         }
         uint32_t c[G][4];
 #pragma unroll
@@ -961,8 +970,8 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
     constexpr int SUB = NT * LANE;               // positions per sub-block
     constexpr int OBUF = (SUB * 9 + 31) / 32 + 8;
     constexpr int NW = NT / 64;
-    __shared__ uint32_t T1[1 << kHashBits];      // most recent position before the sub-block, + 1 (0 = none)
-    __shared__ uint32_t T2[1 << kHash2Bits];     // first position inside the sub-block (~0 = none)
+    __shared__ uint32_t T1[1 << kHashBits];      // most recent position before the sub-block (tag: slice)
+    __shared__ uint32_t T2[1 << kHash2Bits];     // first position inside the sub-block (tag: ~sub-block)
     __shared__ uint32_t obuf[OBUF];
     __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
@@ -970,6 +979,9 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 
     const uint32_t total = DL.counts ? DL.counts[1] : DL.total_slices;
     const int njobs = DL.counts ? (int)DL.counts[0] : DL.njobs;
+    // T1 keeps the largest entry (atomicMax) and T2 the smallest (atomicMin),
+    // so T1's tag counts up per slice and T2's counts down per sub-block.
+    uint32_t tag1 = kTagMax, tag2 = 0;
     // Slices are dealt to workgroups in order.  (Giving each of the 8 XCDs a
     // contiguous run of slices, so that the 32 KiB windows hit its L2, cut
     // the HBM reads to the input size but ran 35% slower on a radiograph:
@@ -1006,19 +1018,32 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
         const int dr = J.row_stride;
 
         __syncthreads();
-        for (int i = tid; i < (1 << kHashBits); i += NT) T1[i] = 0;
-        for (int i = tid; i < (1 << kHash2Bits); i += NT) T2[i] = ~0u;
+        // a new slice: fresh tags (both tables cleared when they run out; a
+        // slice has at most 64 KiB / SUB sub-blocks)
+        if (tag1 >= kTagMax || tag2 <= (uint32_t)(65536 / SUB) + 1) {
+            for (int i = tid; i < (1 << kHashBits); i += NT) T1[i] = 0;
+            for (int i = tid; i < (1 << kHash2Bits); i += NT) T2[i] = ~0u;
+            tag1 = 0;
+            tag2 = kTagMax;
+        }
+        tag1++;
         if (tid < 4) st[tid] = 0;
         __syncthreads();
         if (lz) {
-            // prime T1 with the (up to) 32 KiB before the segment
-            for (int b = tid * LANE; b < s0_rel; b += SUB) {
-                uint32_t o[LANE / 4 + 1];
-                load_own(wb + b, o);
-                each_hash<0>(o, [&](int i, uint32_t h) {
-                    const int P = b + i;
-                    atomicMax(&T1[h], (P < s0_rel && P + 4 <= end_rel) ? (uint32_t)(P + 1) : 0u);
-                });
+            // prime T1 with the (up to) 32 KiB before the slice: the window
+            // positions P = 0 mod kPrimeStride, 16 per lane and pass
+            constexpr int PL = 16 * kPrimeStride;
+            static_assert(kPrimeStride == 4, "one hash per dword");
+            for (int b = tid * PL; b < s0_rel; b += NT * PL) {
+                // 64 bytes from b (in bounds: the slice and the tail slack follow)
+                u32x4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = g128u(wb + b + 16 * k);
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    const int P = b + 4 * k;
+                    if (P < s0_rel && P + 4 <= end_rel) atomicMax(&T1[hash4(v[k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
+                }
             }
         }
 
@@ -1030,6 +1055,7 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 
         for (int q = s0_rel; q < s1_rel; q += SUB) {
             const int n = min(SUB, s1_rel - q);
+            tag2--;
             __syncthreads();                       // previous flush done with obuf
             for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;
             const int lb = q + tid * LANE;         // this lane's first position
@@ -1039,7 +1065,8 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             // ---- A. the sub-block's first positions per hash --------------------
             if (lz && le > 0)
                 each_hash<0>(o, [&](int i, uint32_t h) {
-                    atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i) : ~0u);
+                    if (i < le && lb + i + 4 <= end_rel)
+                        atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 });
             __syncthreads();
             launder(o, LANE / 4 + 1);
@@ -1067,7 +1094,8 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 #pragma unroll
                     for (int k = 0; k < LANE / 4; k++) mr |= zero_nibble(o[k] ^ r[k]) << (4 * k);
                 }
-                lane_matches<R2P_MATCH_GROUP>(pk, wb, o, lb, le, end_rel, m1, mb, mr, bpp > 1 ? bpp : 1, row_ok ? dr : 1, T1, T2);
+                lane_matches<R2P_MATCH_GROUP>(pk, wb, o, lb, le, end_rel, m1, mb, mr, bpp > 1 ? bpp : 1, row_ok ? dr : 1, T1, T2,
+                                              tag1, tag2);
             }
             launder(pk, LANE);
             // greedy parse: each token's code (code | nbits << 26, 0 where no
@@ -1108,14 +1136,11 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             const uint32_t off = carry_bits + wbase + incl - nbits;
             launder(tk, LANE);
             launder(o, LANE / 4 + 1);
-            // the sub-block's positions enter T1; T2 is emptied for the next one
-            if (lz) {
-                if (le > 0)
-                    each_hash<0>(o, [&](int i, uint32_t h) {
-                        atomicMax(&T1[h], (i < le && lb + i + 4 <= end_rel) ? (uint32_t)(lb + i + 1) : 0u);
-                    });
-                for (int i = tid; i < (1 << kHash2Bits); i += NT) T2[i] = ~0u;
-            }
+            // the sub-block's positions enter T1 (T2's entries expire with its tag)
+            if (lz && le > 0)
+                each_hash<0>(o, [&](int i, uint32_t h) {
+                    if (i < le && lb + i + 4 <= end_rel) atomicMax(&T1[h], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                });
             // ---- emission ------------------------------------------------------------------
             // Every word is ORed in (obuf starts zeroed and each bit has one
             // writer), so the words shared with the neighbouring lanes need no
