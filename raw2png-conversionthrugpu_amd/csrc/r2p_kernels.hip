@@ -816,63 +816,55 @@ __device__ __forceinline__ uint32_t eq_mask(const uint32_t* X, int d)
 }
 
 // Fixed-Huffman codes (RFC 1951 3.2.6), bit-reversed for the LSB-first
-// stream.  Literal b: 8-bit codes 0x30+b (b < 144), 9-bit 0x190+b-144.
-__device__ __forceinline__ void lit_code(uint32_t b, uint32_t& v, uint32_t& nb)
+// stream, as code | nbits << 26.  Literal b: 8-bit codes 0x30+b (b < 144),
+// 9-bit 0x190+b-144.
+__device__ __forceinline__ uint32_t lit_tok(uint32_t b)
 {
     const bool hi = b >= 144;
-    nb = hi ? 9u : 8u;
-    v = __builtin_bitreverse32(hi ? b + 0x100u : b + 0x30u) >> (32 - nb);
+    return (__builtin_bitreverse32(b + (hi ? 0x100u : 0x30u)) >> (hi ? 23 : 24)) | ((hi ? 9u : 8u) << 26);
 }
 
-// Extra bits of a distance code: 0 for distances 1..4, else floor(log2(d-1)) - 1.
-__device__ __forceinline__ uint32_t dist_extra_bits(uint32_t dist)
+// A match as a key: len << 16 | (0xFFFF - dist).  The larger key is the
+// longer match, and of equal lengths the shorter distance.
+__device__ __forceinline__ uint32_t mkey(uint32_t len, uint32_t dist) { return (len << 16) | (0xFFFFu - dist); }
+
+// Smallest key of a match that is cheaper than its literals: lengths of 4 and
+// more always are (at most 7 + 1 + 5 + 13 = 26 bits against 32), length 3
+// only up to distance 16384 (12 + 12 bits; beyond, 13 extra bits make 25).
+constexpr uint32_t kMinKey = (3u << 16) | (0xFFFFu - 16384u);
+
+// The token of match key k (length 3..18, distance 1..32768): length symbols
+// 257..268 (7-bit codes, one extra bit from length 11), then the 5-bit
+// distance code and its extra bits.  Distance code of d: with dm = d - 1 and
+// e = max(0, floor(log2 dm) - 1) extra bits, code = 2e + (dm >> e).
+__device__ __forceinline__ uint32_t match_tok(uint32_t k)
 {
-    const uint32_t dm = dist - 1;
-    return dm < 4 ? 0u : 30u - __clz(dm);
+    const uint32_t m = (k >> 16) - 3;
+    const uint32_t dm = 0xFFFEu - (k & 0xFFFFu);
+    const uint32_t deb = 30u - __builtin_clz(dm | 2u);
+    const uint32_t dc = (deb << 1) + (dm >> deb);
+    const uint32_t dex = dm & ((1u << deb) - 1u);
+    const bool lhi = m >= 8;
+    const uint32_t sym = lhi ? 9u + ((m >> 1) & 3u) : m + 1u;
+    const uint32_t lnb = lhi ? 8u : 7u;
+    const uint32_t lcode = (__builtin_bitreverse32(sym) >> 25) | ((lhi ? m & 1u : 0u) << 7);
+    return lcode | ((__builtin_bitreverse32(dc) >> 27) << lnb) | (dex << (lnb + 5)) | ((lnb + 5 + deb) << 26);
 }
 
-// Length 3..18 (symbols 257..268, 7-bit codes) and distance 1..32768.
-__device__ __forceinline__ void match_code(uint32_t len, uint32_t dist, uint32_t& v, uint32_t& nb)
-{
-    const uint32_t m = len - 3;
-    const uint32_t leb = m >= 8 ? 1u : 0u;
-    const uint32_t sym = m < 8 ? m + 1 : 9 + ((m >> 1) & 3);      // symbol - 256
-    const uint32_t lex = m & leb;
-    const uint32_t dm = dist - 1;
-    const uint32_t deb = dist_extra_bits(dist);
-    const uint32_t dc = dm < 4 ? dm : 2 * (deb + 1) + ((dm >> deb) & 1);
-    const uint32_t dex = dm & ((1u << deb) - 1);
-    v = (__builtin_bitreverse32(sym) >> 25) | (lex << 7) | ((__builtin_bitreverse32(dc) >> 27) << (7 + leb)) |
-        (dex << (12 + leb));
-    nb = 12 + leb + deb;
-}
-
-// Bits of a match token, for the "cheaper than literals" test.
-__device__ __forceinline__ uint32_t match_bits(uint32_t len, uint32_t dist)
-{
-    return 12u + (len >= 11 ? 1u : 0u) + dist_extra_bits(dist);
-}
-
-__device__ __forceinline__ void take(uint32_t l, uint32_t d, uint32_t& bl, uint32_t& bd)
-{
-    const bool b = (l > bl) | ((l == bl) & (d < bd));
-    bl = b ? l : bl;
-    bd = b ? d : bd;
-}
-
-// Compares the lane's bytes from position i with the KW dwords c: the number
-// of leading equal bytes (up to 4 * KW).  i and KW are constants after
-// unrolling.
+// Bytes of the lane from position i equal to the KW dwords c, up to 4 * KW:
+// the first differing bit over the dwords (v_ffbl gives ~0 for a zero
+// dword, so min() carries the count over), in bytes.  i and KW are
+// constants after unrolling.
 __device__ __forceinline__ uint32_t cmp_run(const uint32_t* o, const uint32_t* c, int i, int KW)
 {
-    uint32_t l = 4 * KW;
+    uint32_t r = 0;
 #pragma unroll
     for (int k = 3; k >= 0; k--) {
         if (k >= KW) continue;
         const uint32_t x = bytes_at(o, i + 4 * k) ^ c[k];
-        l = x ? 4 * k + (__builtin_ctz(x) >> 3) : l;
+        r = min((uint32_t)__builtin_ctzg(x, -1), 32u + r);
     }
-    return l;
+    return r >> 3;
 }
 
 __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t* c, int KW)
@@ -885,50 +877,52 @@ __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t* c, int KW
     else c[0] = g32u(p);
 }
 
-// Best match of every position of the lane, as len << 16 | (dist - 1), 0 for
-// a literal.  Positions go in groups of G; per group three passes, so that
-// the group's table reads, then its candidate loads, are in flight together.
-// Branch-free: an unusable candidate is marked -1 and its load reads the
-// window base.  A scheduling barrier between groups bounds the registers.
+// The 16-bit hash of position i from the packed pairs hh.
+__device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return (hh[i >> 1] >> (16 * (i & 1))) & 0xFFFFu; }
+
+// Best match key of every position of the lane (0 for a literal).
+// Candidates: distance 1, bpp and one row up, whose runs come from the
+// equality masks n* (bit j set where byte j differs from the byte that far
+// back, or j is past the lane's end), and one hashed earlier position -- T2's
+// entry when it precedes the position, else T1's within the window.
+// Positions go in groups of G so that a group's table reads, then its
+// candidate loads, are in flight together; a scheduling barrier between
+// groups bounds the registers.  Branch-free: an unusable hashed candidate
+// loads from the window base and is then discarded.
 template <int G>
-__device__ __forceinline__ void lane_matches(uint32_t* pk, const uint8_t* wb, const uint32_t* o, int lb, int le,
-                                             int end_rel, uint32_t m1, uint32_t mb, uint32_t mr, int bpp, int dr,
-                                             const uint32_t* T1, const uint32_t* T2, uint32_t tag1, uint32_t tag2)
+__device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, const uint32_t* o, const uint32_t* hh,
+                                          int lb, int le, uint32_t hmask, uint32_t n1, uint32_t nb, uint32_t nr,
+                                          uint32_t kb, uint32_t kr, const uint32_t* T1, const uint32_t* T2,
+                                          uint32_t base1, uint32_t base2)
 {
 #pragma unroll
     for (int g0 = 0; g0 < kLane; g0 += G) {
         __builtin_amdgcn_sched_barrier(0);
-        int C[G];
+        uint32_t C[G];
+        bool hv[G];
 #pragma unroll
         for (int u = 0; u < G; u++) {
             const int i = g0 + u;
-            const uint32_t cap = (uint32_t)max(le - i, 0);
-            uint32_t bl = min((uint32_t)__builtin_ctz(~(m1 >> i)), cap), bd = 1;
-            take(min((uint32_t)__builtin_ctz(~(mb >> i)), cap), (uint32_t)bpp, bl, bd);
-            take(min((uint32_t)__builtin_ctz(~(mr >> i)), cap), (uint32_t)dr, bl, bd);
-            pk[i] = (bl << 16) | bd;               // dist <= 32768 fits 16 bits
-            const int P = lb + i;
-            const uint32_t h = hash4(bytes_at(o, i));
-            const uint32_t f = T2[h >> (kHashBits - kHash2Bits)];
-            const uint32_t t = T1[h];
-            const int fp = (int)(f & kPosMask), tp = (int)(t & kPosMask);
-            const bool fv = ((f >> kPosBits) == tag2) & (fp < P);
-            const bool tv = ((t >> kPosBits) == tag1) & (P - tp <= kWindow);
-            const bool hv = (fv | tv) & (bl < cap) & (P + 4 <= end_rel);
-            C[u] = hv ? (fv ? fp : tp) : -1;
+            const uint32_t r1 = __builtin_ctz(n1 >> i), rb = __builtin_ctz(nb >> i), rr = __builtin_ctz(nr >> i);
+            key[i] = max(max((r1 << 16) | (0xFFFFu - 1u), (rb << 16) | kb), (rr << 16) | kr);
+            const uint32_t h = hash_at(hh, i);
+            const uint32_t x2 = T2[h >> (kHashBits - kHash2Bits)] ^ base2;     // position if the tag matches
+            const uint32_t x1 = T1[h] ^ base1;
+            const uint32_t P = (uint32_t)(lb + i);
+            const bool fv = x2 < P;                                          // earlier in this sub-block
+            const bool tv = P - 1u - x1 < (uint32_t)kWindow;                 // 1 <= P - x1 <= 32768
+            hv[u] = (fv | tv) & (((hmask >> i) & 1u) != 0);
+            C[u] = hv[u] ? (fv ? x2 : x1) : 0u;
         }
         uint32_t c[G][4];
 #pragma unroll
-        for (int u = 0; u < G; u++) load_words(wb + max(C[u], 0), c[u], (kLane - (g0 + u) + 3) / 4);
+        for (int u = 0; u < G; u++) load_words(wb + C[u], c[u], (kLane - (g0 + u) + 3) / 4);
 #pragma unroll
         for (int u = 0; u < G; u++) {
             const int i = g0 + u;
-            const uint32_t cap = (uint32_t)max(le - i, 0);
-            uint32_t bl = pk[i] >> 16, bd = pk[i] & 0xFFFFu;
-            const uint32_t lh = C[u] >= 0 ? min(cmp_run(o, c[u], i, (kLane - i + 3) / 4), cap) : 0u;
-            take(lh, (uint32_t)(lb + i - C[u]), bl, bd);
-            const bool ok = (bl >= 3) & (match_bits(bl, bd) <= 8 * bl);
-            pk[i] = ok ? (bl << 16) | (bd - 1) : 0u;
+            const uint32_t lh = min(cmp_run(o, c[u], i, (kLane - i + 3) / 4), (uint32_t)max(le - i, 0));
+            const uint32_t k = max(key[i], hv[u] ? mkey(lh, (uint32_t)(lb + i) - C[u]) : 0u);
+            key[i] = k >= kMinKey ? k : 0u;
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -941,14 +935,6 @@ __device__ __forceinline__ void launder(uint32_t* v, int n)
 {
 #pragma unroll
     for (int i = 0; i < n; i++) asm volatile("" : "+v"(v[i]));
-}
-
-// Hash of every position I < kLane of the lane, applied to f(I, hash).
-template <int I, typename F>
-__device__ __forceinline__ void each_hash(const uint32_t* o, F&& f)
-{
-    f(I, hash4(bytes_at(o, I)));
-    if constexpr (I + 1 < kLane) each_hash<I + 1>(o, f);
 }
 
 // The lane's bytes [p, p + 20) (the 4 after it feed the hashes of its last
@@ -1062,19 +1048,29 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             const int le = max(0, min(LANE, n - tid * LANE));
             uint32_t o[LANE / 4 + 1] = {0, 0, 0, 0, 0};
             if (le > 0) load_own(wb + lb, o);
-            // ---- A. the sub-block's first positions per hash --------------------
-            if (lz && le > 0)
-                each_hash<0>(o, [&](int i, uint32_t h) {
-                    if (i < le && lb + i + 4 <= end_rel)
+            // ---- A. hashes (kept, two per register), the sub-block's first
+            // position per hash ----------------------------------------------------
+            // hashable positions: inside the lane, with 4 bytes before the end
+            const int hl = min(le, end_rel - 3 - lb);
+            const uint32_t hmask = hl <= 0 ? 0u : hl >= LANE ? 0xFFFFu : (1u << hl) - 1u;
+            uint32_t hh[LANE / 2];
+#pragma unroll
+            for (int i = 0; i < LANE / 2; i++) hh[i] = 0;
+            if (lz && le > 0) {
+#pragma unroll
+                for (int i = 0; i < LANE; i++) {
+                    const uint32_t h = hash4(bytes_at(o, i));
+                    hh[i >> 1] |= h << (16 * (i & 1));
+                    if ((hmask >> i) & 1u)
                         atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (tag2 << kPosBits) | (uint32_t)(lb + i));
-                });
+                }
+            }
             __syncthreads();
-            launder(o, LANE / 4 + 1);
             // ---- B. best match per position, greedy parse of the lane -----------
             if (tid == 0) obuf[0] = carry;
-            uint32_t pk[LANE];
+            uint32_t key[LANE];
 #pragma unroll
-            for (int i = 0; i < LANE; i++) pk[i] = 0;
+            for (int i = 0; i < LANE; i++) key[i] = 0;
             if (lz && le > 0) {
                 uint32_t X[LANE / 4 + 2];
                 uint64_t pr = 0;
@@ -1094,10 +1090,12 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 #pragma unroll
                     for (int k = 0; k < LANE / 4; k++) mr |= zero_nibble(o[k] ^ r[k]) << (4 * k);
                 }
-                lane_matches<R2P_MATCH_GROUP>(pk, wb, o, lb, le, end_rel, m1, mb, mr, bpp > 1 ? bpp : 1, row_ok ? dr : 1, T1, T2,
-                                              tag1, tag2);
+                // runs stop at the lane's end: past it every position "differs"
+                const uint32_t cap = le >= LANE ? 0u : ~0u << le;
+                lane_keys<R2P_MATCH_GROUP>(key, wb, o, hh, lb, le, hmask, ~m1 | cap, ~mb | cap, ~mr | cap,
+                                           0xFFFFu - (uint32_t)(bpp > 1 ? bpp : 1), 0xFFFFu - (uint32_t)(row_ok ? dr : 1),
+                                           T1, T2, tag1 << kPosBits, tag2 << kPosBits);
             }
-            launder(pk, LANE);
             // greedy parse: each token's code (code | nbits << 26, 0 where no
             // token starts) and the lane's bit count
             uint32_t tk[LANE];
@@ -1105,15 +1103,11 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
 #pragma unroll
             for (int i = 0; i < LANE; i++) {
                 if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
-                const uint32_t L = pk[i] >> 16;
-                const uint32_t D = (pk[i] & 0xFFFFu) + 1;
+                const uint32_t L = key[i] >> 16;       // 0: literal
                 const bool tok = ((uint32_t)i == nextp) & (i < le);
-                uint32_t mv, mn, lv, ln;
-                match_code(max(L, 3u), D, mv, mn);
-                lit_code((o[i >> 2] >> (8 * (i & 3))) & 0xFFu, lv, ln);
-                const uint32_t nb = L ? mn : ln;
-                tk[i] = tok ? ((L ? mv : lv) | (nb << 26)) : 0u;
-                nbits += tok ? nb : 0u;
+                const uint32_t t = L ? match_tok(key[i]) : lit_tok((o[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+                tk[i] = tok ? t : 0u;
+                nbits += tok ? t >> 26 : 0u;
                 nextp = tok ? (uint32_t)i + max(L, 1u) : nextp;
             }
             // ---- block scan of bit counts ----------------------------------------------
@@ -1123,8 +1117,8 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
                 uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
 #pragma unroll
                 for (int i = 0; i < LANE; i++) {
-                    const uint32_t Lt = tk[i] ? pk[i] >> 16 : 0u;
-                    m_cnt += Lt ? 1u : 0u; m_bytes += Lt; m_dist += Lt ? (pk[i] & 0xFFFFu) + 1 : 0u;
+                    const uint32_t Lt = tk[i] ? key[i] >> 16 : 0u;
+                    m_cnt += Lt ? 1u : 0u; m_bytes += Lt; m_dist += Lt ? 0xFFFFu - (key[i] & 0xFFFFu) : 0u;
                     m_max = max(m_max, Lt);
                 }
                 atomicAdd(&st[0], m_cnt); atomicAdd(&st[1], m_bytes);
@@ -1134,13 +1128,12 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
-            launder(tk, LANE);
-            launder(o, LANE / 4 + 1);
             // the sub-block's positions enter T1 (T2's entries expire with its tag)
-            if (lz && le > 0)
-                each_hash<0>(o, [&](int i, uint32_t h) {
-                    if (i < le && lb + i + 4 <= end_rel) atomicMax(&T1[h], (tag1 << kPosBits) | (uint32_t)(lb + i));
-                });
+            if (lz && le > 0) {
+#pragma unroll
+                for (int i = 0; i < LANE; i++)
+                    if ((hmask >> i) & 1u) atomicMax(&T1[hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+            }
             // ---- emission ------------------------------------------------------------------
             // Every word is ORed in (obuf starts zeroed and each bit has one
             // writer), so the words shared with the neighbouring lanes need no
