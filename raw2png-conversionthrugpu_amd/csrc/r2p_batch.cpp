@@ -94,7 +94,7 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     FilterLaunch L{};
     L.jobs = b->d_fj; L.band_start = b->d_band_start; L.njobs = 0; L.band_rows = b->band_rows;
     L.total_bands = 0; L.row_cap = b->row_cap; L.lut_cap = b->lut_cap; L.counts = b->d_counts;
-    const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 128;
+    const size_t lds = filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap);
     if (timed) R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(launch_filter(L, 4096, lds, s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
@@ -236,7 +236,14 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         if (!d_src[i]) { set_error("image %d has no source", i); return R2P_ERR_ARG; }
     }
     if (total_f > b->max_filtered) { set_error("batch needs %zu filtered bytes > %zu", total_f, b->max_filtered); return R2P_ERR_SIZE; }
-    b->band_rows = o.band_rows > 0 ? o.band_rows : (int)std::max<int64_t>(1, std::min<int64_t>(16, total_rows / 8192));
+    {
+        int rmax = 16, lmax = 0;
+        for (int i = 0; i < n; i++) {
+            rmax = std::max(rmax, (int)((geo[i].row_bytes + 15) & ~(size_t)15));
+            if (plan[i].lut_kind >= 0) lmax = std::max(lmax, (plan[i].lut_bytes + 15) & ~15);
+        }
+        b->band_rows = filter_band_rows(o.band_rows, total_rows, rmax, lmax);
+    }
 
     // device tables, deduplicated by parameter set
     std::map<LutKey, int> lut_slot;
@@ -300,6 +307,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice;
         A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = o.slice_bytes;
         A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
+        A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
         A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
         A.color_type = (uint8_t)g.color_type;
         const int lvl = o.zlib_level;

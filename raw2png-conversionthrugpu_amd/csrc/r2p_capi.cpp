@@ -10,6 +10,8 @@
 #include <mutex>
 #include <vector>
 
+#include <zlib.h>
+
 #include "r2p_common.h"
 
 namespace r2p {
@@ -280,11 +282,25 @@ size_t png_bound_bytes(size_t filtered, int segment_bytes, int chunk_bytes)
     return 8 + 25 + 12 * nch + z + 12 + 64;
 }
 
-int default_band_rows(int64_t total_rows)
+uint32_t idat_prefix_full(int chunk_bytes)
 {
-    // aim for ~4 bands per CU slot on 256 CUs; never fewer than 1 or more than 16 rows
-    int64_t b = total_rows / 1024;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(16, b));
+    const uint32_t g = ~(uint32_t)crc32(0L, (const Bytef*)"IDAT", 4);
+    return r2p_crc32_combine(g, 0u, (uint64_t)chunk_bytes);
+}
+
+size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap)
+{
+    return (size_t)lut_cap + (size_t)(band_rows + 1) * (size_t)(row_cap + 32);
+}
+
+int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap)
+{
+    // from the row count: about 768 bands (three workgroups on each of 256
+    // CUs), at least one row per wave of the workgroup, at most 16
+    int h = requested > 0 ? requested : (int)std::max<int64_t>(4, std::min<int64_t>(16, total_rows / 768));
+    // the band and the row above it must fit the LDS beside the table
+    while (h > 1 && filter_lds_bytes(h, row_cap, lut_cap) > (size_t)kFilterLdsMax) h--;
+    return h;
 }
 
 void fill_defaults(r2p_options* o)
@@ -431,13 +447,13 @@ static int launch_filter_job(const FilterJob& J, int band_rows, hipStream_t s)
     L.jobs = nullptr;
     L.band_start = nullptr;
     L.njobs = 1;
-    L.band_rows = band_rows > 0 ? band_rows : default_band_rows(J.rows);
-    const uint32_t bands = (uint32_t)((J.rows + L.band_rows - 1) / L.band_rows);
-    L.total_bands = bands;
     L.row_cap = (J.row_bytes + 15) & ~15;
     L.lut_cap = (J.lut_bytes + 15) & ~15;
+    L.band_rows = filter_band_rows(band_rows, J.rows, L.row_cap, L.lut_cap);
+    const uint32_t bands = (uint32_t)((J.rows + L.band_rows - 1) / L.band_rows);
+    L.total_bands = bands;
     L.single = J;
-    const size_t lds = (size_t)L.lut_cap + 2 * (size_t)(L.row_cap + 32) + 128;
+    const size_t lds = filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap);
     const int grid = (int)std::min<uint32_t>(bands, 8192);
     R2P_HIP(launch_filter(L, grid, lds, s));
     return R2P_OK;
@@ -639,6 +655,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
     A.slice_cap_dw = e->slice_cap_dw; A.slice_bytes = o.slice_bytes;
     A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
+    A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
     A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
     A.color_type = (uint8_t)g.color_type;
     const int lvl = o.zlib_level;

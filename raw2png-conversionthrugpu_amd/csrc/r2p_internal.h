@@ -98,6 +98,7 @@ struct AssembleJob {
     uint32_t        width, height;
     uint8_t         bit_depth, color_type, zflg, pad_;
     int32_t         max_chunks;  // chunks this job may use (grid sizing bound)
+    uint32_t        idat_full;   // CRC register after "IDAT" times x^(8 * chunk_bytes): a full chunk's prefix term
     uint64_t*       running;     // optional: running output offset (raw mode, strip appends)
     uint64_t*       base_slot;   // optional: where k_scan records this job's base offset
 };
@@ -139,7 +140,7 @@ constexpr int kDeflateThreads = 256;
 constexpr int kAssembleThreads = 1024;
 constexpr int kCrcShiftLevels = 10;       // CRC shift tables for 64 * 2^k zero bytes, k < 10
 constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
-constexpr int kFilterMaxK = kMaxRowBytesLds / (16 * kFilterThreads);
+constexpr int kFilterLdsMax = 160 * 1024 - 256;  // LDS one filter workgroup may use
 constexpr int kWindow = 32768;           // DEFLATE window
 // Every buffer k_deflate reads holds at least this many readable bytes after
 // the last data byte, so its 16-byte loads need no tail guard.

@@ -283,114 +283,128 @@ __global__ void k_build_lut(LutBuild lb, void* out)
 // ---------------------------------------------------------------------------
 
 
-// Window one raw row into LDS as PNG-ready bytes at row8[0..row_bytes).
-__device__ void window_row(const FilterJob& J, const uint8_t* src_row, uint8_t* row8,
-                           const uint8_t* lut_lds)
+// Raw input of 16-byte output item i of a row: 16 bytes, or 32 for the modes
+// that narrow 16-bit samples to 8 bits.
+struct RawItem { u32x4 v0, v1; };
+__device__ __forceinline__ bool narrows(int wmode) { return wmode == WM_Q15 || wmode == WM_LUT8_FROM16; }
+__device__ __forceinline__ RawItem load_raw(const FilterJob& J, const uint8_t* src_row, int i)
 {
-    const int rb = J.row_bytes;
-    const int nfull = rb >> 4;
-    uint32_t* row32 = (uint32_t*)row8;
-    for (int i = threadIdx.x; i < nfull; i += blockDim.x) {
-        u32x4 o;
-        switch (J.wmode) {
-        case WM_COPY: {
-            o = g128u(src_row + 16 * i);
-            break;
-        }
-        case WM_SWAP16: {
-            const u32x4 v = g128u(src_row + 16 * i);
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = ((v[k] & 0x00FF00FFu) << 8) | ((v[k] >> 8) & 0x00FF00FFu);
-            break;
-        }
-        case WM_LUT16: {
-            const u32x4 v = g128u(src_row + 16 * i);
-            const uint16_t* t16 = (const uint16_t*)lut_lds;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t s0 = t16[((v[k] & 0xFFFFu) >> J.shift) & J.mask];
-                const uint32_t s1 = t16[((v[k] >> 16) >> J.shift) & J.mask];
-                const uint32_t w = s0 | (s1 << 16);
-                o[k] = ((w & 0x00FF00FFu) << 8) | ((w >> 8) & 0x00FF00FFu);
-            }
-            break;
-        }
-        case WM_LUT8_FROM8: {
-            const u32x4 v = g128u(src_row + 16 * i);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                o[k] = (uint32_t)lut_lds[v[k] & 0xFF] | ((uint32_t)lut_lds[(v[k] >> 8) & 0xFF] << 8) |
-                       ((uint32_t)lut_lds[(v[k] >> 16) & 0xFF] << 16) | ((uint32_t)lut_lds[v[k] >> 24] << 24);
-            }
-            break;
-        }
-        case WM_Q15: {
-            const u32x4 v0 = g128u(src_row + 32 * i);
-            const u32x4 v1 = g128u(src_row + 32 * i + 16);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
-                const uint32_t w1 = k < 2 ? v0[2 * k + 1] : v1[2 * k - 3];
-                uint32_t r = 0;
-#pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
-                    const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
-                    r |= q15_clamp(x, J.q15_a, J.q15_b) << (8 * h);
-                }
-                o[k] = r;
-            }
-            break;
-        }
-        default: {  // WM_LUT8_FROM16
-            const u32x4 v0 = g128u(src_row + 32 * i);
-            const u32x4 v1 = g128u(src_row + 32 * i + 16);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
-                const uint32_t w1 = k < 2 ? v0[2 * k + 1] : v1[2 * k - 3];
-                uint32_t r = 0;
-#pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
-                    r |= (uint32_t)lut_lds[(raw >> J.shift) & J.mask] << (8 * h);
-                }
-                o[k] = r;
-            }
-            break;
-        }
-        }
-        *(u32x4*)(row32 + 4 * i) = o;
+    RawItem r;
+    if (narrows(J.wmode)) { r.v0 = g128u(src_row + 32 * i); r.v1 = g128u(src_row + 32 * i + 16); }
+    else { r.v0 = g128u(src_row + 16 * i); r.v1 = r.v0; }
+    return r;
+}
+
+// PNG-ready bytes of a full 16-byte item from its raw input.
+__device__ __forceinline__ u32x4 window_raw(const FilterJob& J, const RawItem& R, const uint8_t* lut_lds)
+{
+    u32x4 o;
+    switch (J.wmode) {
+    case WM_COPY: {
+        o = R.v0;
+        break;
     }
-    // Tail bytes (row_bytes not a multiple of 16): one thread per output byte.
-    const int tail0 = nfull << 4;
-    for (int j = tail0 + (int)threadIdx.x; j < rb; j += blockDim.x) {
-        uint32_t v;
-        switch (J.wmode) {
-        case WM_COPY: v = src_row[j]; break;
-        case WM_SWAP16: v = src_row[j ^ 1]; break;
-        case WM_LUT16: {
-            const int s = j >> 1;
-            const uint32_t raw = (uint32_t)src_row[2 * s] | ((uint32_t)src_row[2 * s + 1] << 8);
-            const uint32_t w = ((const uint16_t*)lut_lds)[(raw >> J.shift) & J.mask];
-            v = (j & 1) ? (w & 0xFF) : (w >> 8);
-            break;
-        }
-        case WM_LUT8_FROM8: v = lut_lds[src_row[j]]; break;
-        case WM_Q15: {
-            const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
-            const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
-            v = q15_clamp(x, J.q15_a, J.q15_b);
-            break;
-        }
-        default: {
-            const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
-            v = lut_lds[(raw >> J.shift) & J.mask];
-            break;
-        }
-        }
-        row8[j] = (uint8_t)v;
+    case WM_SWAP16: {
+        const u32x4 v = R.v0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = ((v[k] & 0x00FF00FFu) << 8) | ((v[k] >> 8) & 0x00FF00FFu);
+        break;
     }
+    case WM_LUT16: {
+        const u32x4 v = R.v0;
+        const uint16_t* t16 = (const uint16_t*)lut_lds;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t s0 = t16[((v[k] & 0xFFFFu) >> J.shift) & J.mask];
+            const uint32_t s1 = t16[((v[k] >> 16) >> J.shift) & J.mask];
+            const uint32_t w = s0 | (s1 << 16);
+            o[k] = ((w & 0x00FF00FFu) << 8) | ((w >> 8) & 0x00FF00FFu);
+        }
+        break;
+    }
+    case WM_LUT8_FROM8: {
+        const u32x4 v = R.v0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            o[k] = (uint32_t)lut_lds[v[k] & 0xFF] | ((uint32_t)lut_lds[(v[k] >> 8) & 0xFF] << 8) |
+                   ((uint32_t)lut_lds[(v[k] >> 16) & 0xFF] << 16) | ((uint32_t)lut_lds[v[k] >> 24] << 24);
+        }
+        break;
+    }
+    case WM_Q15: {
+        const u32x4 v0 = R.v0, v1 = R.v1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
+            const uint32_t w1 = k < 2 ? v0[2 * k + 1] : v1[2 * k - 3];
+            uint32_t r = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
+                const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
+                r |= q15_clamp(x, J.q15_a, J.q15_b) << (8 * h);
+            }
+            o[k] = r;
+        }
+        break;
+    }
+    default: {  // WM_LUT8_FROM16
+        const u32x4 v0 = R.v0, v1 = R.v1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t w0 = k < 2 ? v0[2 * k] : v1[2 * k - 4];
+            const uint32_t w1 = k < 2 ? v0[2 * k + 1] : v1[2 * k - 3];
+            uint32_t r = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const uint32_t raw = ((h < 2 ? w0 : w1) >> (16 * (h & 1))) & 0xFFFFu;
+                r |= (uint32_t)lut_lds[(raw >> J.shift) & J.mask] << (8 * h);
+            }
+            o[k] = r;
+        }
+        break;
+    }
+    }
+    return o;
+}
+
+// PNG-ready byte j of a raw row (the tail of a row whose length is not a
+// multiple of 16).
+__device__ __forceinline__ uint32_t window_byte(const FilterJob& J, const uint8_t* src_row, int j, const uint8_t* lut_lds)
+{
+    switch (J.wmode) {
+    case WM_COPY: return src_row[j];
+    case WM_SWAP16: return src_row[j ^ 1];
+    case WM_LUT16: {
+        const int s = j >> 1;
+        const uint32_t raw = (uint32_t)src_row[2 * s] | ((uint32_t)src_row[2 * s + 1] << 8);
+        const uint32_t w = ((const uint16_t*)lut_lds)[(raw >> J.shift) & J.mask];
+        return (j & 1) ? (w & 0xFF) : (w >> 8);
+    }
+    case WM_LUT8_FROM8: return lut_lds[src_row[j]];
+    case WM_Q15: {
+        const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
+        const int32_t x = J.pixel_rep == 1 ? (int32_t)(int16_t)raw : (int32_t)raw;
+        return q15_clamp(x, J.q15_a, J.q15_b);
+    }
+    default: {
+        const uint32_t raw = (uint32_t)src_row[2 * j] | ((uint32_t)src_row[2 * j + 1] << 8);
+        return lut_lds[(raw >> J.shift) & J.mask];
+    }
+    }
+}
+
+// Sum of v over the 64 lanes of the wave (DPP: quad swaps, half-row and row
+// mirrors, then the row broadcasts), returned uniform.
+__device__ __forceinline__ uint32_t wave_total(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);   // row_mirror
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // The 6 dwords around item i: d[0..1] = 8 bytes before, d[2..5] = the item.
@@ -415,15 +429,16 @@ __device__ __forceinline__ uint32_t byte_mask(int nvalid_in_dword)
     return nvalid_in_dword >= 4 ? 0xFFFFFFFFu : nvalid_in_dword <= 0 ? 0u : ((1u << (8 * nvalid_in_dword)) - 1u);
 }
 
-template <int MAXK>
-__global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
+template <int K>   // items per lane: ceil(16-byte items of the widest row / 64)
+__global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(K <= 6 ? 4 : 1))) void k_filter(FilterLaunch L)
 {
+    constexpr bool KEEP = K <= 8;                         // Paeth results stay in registers
     extern __shared__ u32x4 smem_filter[];
     uint8_t* lut_lds = (uint8_t*)smem_filter;
     uint8_t* bufs = lut_lds + L.lut_cap;
     const int rbuf = L.row_cap + 32;                      // 16 B front pad + row + 16 B slack
-    uint32_t* red = (uint32_t*)(bufs + 2 * rbuf);         // 5 scores x 4 waves
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    constexpr int NW = kFilterThreads / 64;
 
     int cur_job = -1;
     const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
@@ -441,10 +456,11 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
         const FilterJob J = L.jobs ? L.jobs[j] : L.single;
         const int band = (int)(g - (L.jobs ? L.band_start[j] : 0u));
         const int r0 = band * L.band_rows;
-        const int r1 = min(J.rows, r0 + L.band_rows);
+        const int nr = min(J.rows, r0 + L.band_rows) - r0;
         const int rb = J.row_bytes;
+        const int nfull = rb >> 4, nitems = (rb + 15) >> 4, tail = rb & 15;
 
-        __syncthreads();
+        __syncthreads();                                   // the previous band is done with LDS
         if (j != cur_job) {
             if (J.lut_bytes > 0) {
                 const int n16 = (J.lut_bytes + 15) >> 4;
@@ -453,109 +469,138 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(FilterLaunch L)
             }
             cur_job = j;
         }
-        // zero the front pads of both row buffers
-        if (tid < 8) ((uint32_t*)(bufs + (tid >> 2) * rbuf))[tid & 3] = 0;
+        // zero the front pads (left neighbours of every row's first item)
+        for (int b = tid; b <= nr; b += blockDim.x) *(u32x4*)(bufs + b * rbuf) = (u32x4){0, 0, 0, 0};
         __syncthreads();
 
-        // previous row of r0 -> buffer (r0 + 1) & 1
+        // ---- window the band and the row above it: buffer b holds row r0 - 1 + b
         {
-            uint8_t* prev = bufs + ((r0 + 1) & 1) * rbuf + 16;
-            if (r0 > 0) {
-                window_row(J, J.src + (int64_t)(r0 - 1) * J.src_pitch, prev, lut_lds);
-            } else if (J.prior) {
-                for (int i = tid; i < rb; i += blockDim.x) prev[i] = J.prior[i];
-            } else {
-                for (int i = tid; i < ((rb + 3) >> 2); i += blockDim.x) ((uint32_t*)prev)[i] = 0;
+            // row above: windowed from the source, or the caller's prior row, or zeros
+            const bool above_src = r0 > 0;
+            const int b_first = above_src ? 0 : 1;
+            if (!above_src) {
+                uint8_t* prev = bufs + 16;
+                if (J.prior) for (int i = tid; i < rb; i += blockDim.x) prev[i] = J.prior[i];
+                else for (int i = tid; i < ((rb + 3) >> 2); i += blockDim.x) ((uint32_t*)prev)[i] = 0;
             }
-        }
-
-        const int nitems = (rb + 15) >> 4;
-        const int near = J.bpp <= 4;                      // neighbour within 1 dword back
-        const uint32_t sh = (uint32_t)((4 - (J.bpp & 3)) & 3);
-        const int kmax = (nitems + (int)blockDim.x - 1) / (int)blockDim.x;
-
-        for (int r = r0; r < r1; r++) {
-            uint8_t* cur8 = bufs + (r & 1) * rbuf + 16;
-            const uint32_t* cur = (const uint32_t*)cur8;
-            const uint32_t* prv = (const uint32_t*)(bufs + ((r + 1) & 1) * rbuf + 16);
-            window_row(J, J.src + (int64_t)r * J.src_pitch, cur8, lut_lds);
-            __syncthreads();
-
-            uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-            uint32_t f4c[MAXK][4];
+            // full items of buffers b_first..nr, flattened over the rows; the
+            // raw loads of WU items are issued together (global latency)
+            const int nbuf = nr + 1 - b_first;
+            if (nfull > 0) {
+                constexpr int WU = 2;
+                const int tot = nbuf * nfull;
+                for (int t0 = tid; t0 < tot; t0 += WU * kFilterThreads) {
+                    RawItem raw[WU];
+                    int bi[WU], ii[WU];
 #pragma unroll
-            for (int k = 0; k < MAXK; k++) {
-                const int i = tid + k * (int)blockDim.x;
-                if (k < kmax && i < nitems) {
-                    uint32_t dc[6], dp[6], a[4], c[4];
-                    load_item(cur, i, dc);
-                    load_item(prv, i, dp);
-                    left4(dc, near, sh, a);
-                    left4(dp, near, sh, c);
-                    const int nv = min(16, rb - 16 * i);
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const uint32_t m = byte_mask(nv - 4 * q);
-                        const uint32_t x = dc[2 + q], b = dp[2 + q];
-                        const uint32_t f4 = sub8(x, paeth4(a[q], b, c[q]));
-                        f4c[k][q] = f4;
-                        s0 = cost8(x & m, s0);
-                        s1 = cost8(sub8(x, a[q]) & m, s1);
-                        s2 = cost8(sub8(x, b) & m, s2);
-                        s3 = cost8(sub8(x, avg8(a[q], b)) & m, s3);
-                        s4 = cost8(f4 & m, s4);
+                    for (int u = 0; u < WU; u++) {
+                        const int t = min(t0 + u * kFilterThreads, tot - 1);
+                        bi[u] = t / nfull; ii[u] = t - bi[u] * nfull;
+                        raw[u] = load_raw(J, J.src + (int64_t)(r0 - 1 + b_first + bi[u]) * J.src_pitch, ii[u]);
                     }
+#pragma unroll
+                    for (int u = 0; u < WU; u++)
+                        if (t0 + u * kFilterThreads < tot)
+                            *(u32x4*)(bufs + (b_first + bi[u]) * rbuf + 16 + 16 * ii[u]) = window_raw(J, raw[u], lut_lds);
                 }
             }
-            s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3); s4 = wave_sum(s4);
-            if (lane == 0) {
-                red[wave * 5 + 0] = s0; red[wave * 5 + 1] = s1; red[wave * 5 + 2] = s2;
-                red[wave * 5 + 3] = s3; red[wave * 5 + 4] = s4;
+            for (int t = tid; t < nbuf * tail; t += blockDim.x) {
+                const int b = t / tail, jj = (nfull << 4) + (t - b * tail);
+                const int r = r0 - 1 + b_first + b;
+                bufs[(b_first + b) * rbuf + 16 + jj] = (uint8_t)window_byte(J, J.src + (int64_t)r * J.src_pitch, jj, lut_lds);
             }
-            __syncthreads();
-            uint32_t sc[5] = {0, 0, 0, 0, 0};
-            const int nw = (int)blockDim.x >> 6;
-            for (int w = 0; w < nw; w++)
+        }
+        __syncthreads();
+
+        // ---- each wave filters whole rows of the band
+        const int near = J.bpp <= 4;                      // neighbour within 1 dword back
+        const uint32_t sh = (uint32_t)((4 - (J.bpp & 3)) & 3);
+        for (int jr = wave; jr < nr; jr += NW) {
+            const uint8_t* cur8 = bufs + (jr + 1) * rbuf + 16;
+            const uint32_t* cur = (const uint32_t*)cur8;
+            const uint32_t* prv = (const uint32_t*)(bufs + jr * rbuf + 16);
+            uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+            uint32_t f4c[KEEP ? K : 1][4], f4t[4];
+            // scores of the item at i; the first nv bytes count (a row's
+            // last item may be partial: only its lane takes the masked path)
+            auto score = [&](int i, int nv, uint32_t* f4o) {
+                uint32_t dc[6], dp[6], a[4], c[4];
+                load_item(cur, i, dc);
+                load_item(prv, i, dp);
+                left4(dc, near, sh, a);
+                left4(dp, near, sh, c);
 #pragma unroll
-                for (int f = 0; f < 5; f++) sc[f] += red[w * 5 + f];
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t m = nv >= 16 ? ~0u : byte_mask(nv - 4 * q);
+                    const uint32_t x = dc[2 + q], b = dp[2 + q];
+                    const uint32_t f4 = sub8(x, paeth4(a[q], b, c[q]));
+                    f4o[q] = f4;
+                    s0 = cost8(x & m, s0);
+                    s1 = cost8(sub8(x, a[q]) & m, s1);
+                    s2 = cost8(sub8(x, b) & m, s2);
+                    s3 = cost8(sub8(x, avg8(a[q], b)) & m, s3);
+                    s4 = cost8(f4 & m, s4);
+                }
+            };
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = lane + 64 * k;
+                // one item in flight at a time bounds the registers; the
+                // other waves of the SIMD hide the LDS latency
+                __builtin_amdgcn_sched_barrier(0);
+                if (i < nfull) score(i, 16, f4c[KEEP ? k : 0]);
+            }
+            const bool ragged = tail != 0;
+            const int tl = nfull & 63;                    // lane of the partial item
+            if (ragged && lane == tl) score(nfull, tail, f4t);
+            const uint32_t sc[5] = {wave_total(s0), wave_total(s1), wave_total(s2), wave_total(s3), wave_total(s4)};
             int best = 0; uint32_t bsc = sc[0];
 #pragma unroll
             for (int f = 1; f < 5; f++) if (sc[f] < bsc) { bsc = sc[f]; best = f; }
+            // the write pass reads the rows from LDS again rather than keeping
+            // every item of the cost pass in registers
+            asm volatile("" ::: "memory");
 
+            const int r = r0 + jr;
             uint8_t* dst = J.dst + (int64_t)r * (rb + 1);
-            if (tid == 0) dst[0] = (uint8_t)best;
+            if (lane == 0) dst[0] = (uint8_t)best;
+            // filtered bytes of the item at i (f4: its Paeth result, if kept)
+            auto filtered = [&](int i, const uint32_t* f4, bool kept, uint32_t* o) {
+                if (kept && best == 4) {
 #pragma unroll
-            for (int k = 0; k < MAXK; k++) {
-                const int i = tid + k * (int)blockDim.x;
-                if (k < kmax && i < nitems) {
+                    for (int q = 0; q < 4; q++) o[q] = f4[q];
+                    return;
+                }
+                uint32_t dc[6], dp[6], a[4], c[4];
+                load_item(cur, i, dc);
+                load_item(prv, i, dp);
+                left4(dc, near, sh, a);
+                if (best == 4) left4(dp, near, sh, c);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t x = dc[2 + q], b = dp[2 + q];
+                    o[q] = best == 0 ? x : best == 1 ? sub8(x, a[q]) : best == 2 ? sub8(x, b)
+                         : best == 3 ? sub8(x, avg8(a[q], b)) : sub8(x, paeth4(a[q], b, c[q]));
+                }
+            };
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = lane + 64 * k;
+                __builtin_amdgcn_sched_barrier(0);
+                if (i < nfull) {
                     uint32_t o[4];
-                    if (best == 4) {
-#pragma unroll
-                        for (int q = 0; q < 4; q++) o[q] = f4c[k][q];
-                    } else {
-                        uint32_t dc[6], dp[6], a[4];
-                        load_item(cur, i, dc);
-                        load_item(prv, i, dp);
-                        left4(dc, near, sh, a);
-#pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const uint32_t x = dc[2 + q], b = dp[2 + q];
-                            o[q] = best == 0 ? x : best == 1 ? sub8(x, a[q])
-                                 : best == 2 ? sub8(x, b) : sub8(x, avg8(a[q], b));
-                        }
-                    }
-                    uint8_t* p = dst + 1 + 16 * i;
-                    const int nv = min(16, rb - 16 * i);
-                    if (nv == 16) {
-                        *(u32x4_ua*)p = (u32x4){o[0], o[1], o[2], o[3]};
-                    } else {
-                        for (int t = 0; t < nv; t++) p[t] = (uint8_t)(o[t >> 2] >> (8 * (t & 3)));
-                    }
+                    filtered(i, f4c[KEEP ? k : 0], KEEP, o);
+                    *(u32x4_ua*)(dst + 1 + 16 * i) = (u32x4){o[0], o[1], o[2], o[3]};
                 }
             }
+            if (ragged && lane == tl) {
+                uint32_t o[4];
+                filtered(nfull, f4t, true, o);
+                uint8_t* p = dst + 1 + 16 * nfull;
+                for (int t = 0; t < tail; t++) p[t] = (uint8_t)(o[t >> 2] >> (8 * (t & 3)));
+            }
             if (J.last_row && r == J.rows - 1)
-                for (int i = tid; i < rb; i += blockDim.x) J.last_row[i] = cur8[i];
-            __syncthreads();
+                for (int i = lane; i < rb; i += 64) J.last_row[i] = cur8[i];
         }
     }
 }
@@ -772,7 +817,7 @@ __global__ __launch_bounds__(256) void k_checksum_final(const uint32_t* part, in
 #ifndef R2P_MATCH_GROUP
 #define R2P_MATCH_GROUP 4
 #endif
-constexpr int kHashBits = 13;      // T1: 8 Ki entries
+constexpr int kHashBits = 12;      // T1: 4 Ki entries
 constexpr int kHash2Bits = 12;     // T2: 4 Ki entries
 constexpr int kLane = 16;          // positions per parse lane
 // Table entries are tag << kPosBits | position (relative to the window base,
@@ -782,7 +827,8 @@ constexpr int kLane = 16;          // positions per parse lane
 constexpr int kPosBits = 17;
 constexpr uint32_t kPosMask = (1u << kPosBits) - 1;
 constexpr uint32_t kTagMax = (1u << (32 - kPosBits)) - 1;
-constexpr int kPrimeStride = 4;    // window positions entered into T1 at this stride
+constexpr int kPrimeStride = 8;    // window positions entered into T1 at this stride
+constexpr int kInsertStride = 2;   // the slice's own positions entered into T1 at this stride
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
 
@@ -950,7 +996,7 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
 // LzStats); a separate instantiation so that the plain and batched forms do
 // not carry their registers.
 template <int NT, bool STATS>
-__global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_deflate(DeflateLaunch DL)
 {
     constexpr int LANE = kLane;
     constexpr int SUB = NT * LANE;               // positions per sub-block
@@ -1017,16 +1063,16 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
         __syncthreads();
         if (lz) {
             // prime T1 with the (up to) 32 KiB before the slice: the window
-            // positions P = 0 mod kPrimeStride, 16 per lane and pass
-            constexpr int PL = 16 * kPrimeStride;
-            static_assert(kPrimeStride == 4, "one hash per dword");
+            // positions P = 0 mod kPrimeStride, 8 per lane and pass
+            static_assert(kPrimeStride == 8, "every other dword of 64 bytes");
+            constexpr int PL = 64;
             for (int b = tid * PL; b < s0_rel; b += NT * PL) {
                 // 64 bytes from b (in bounds: the slice and the tail slack follow)
                 u32x4 v[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) v[k] = g128u(wb + b + 16 * k);
 #pragma unroll
-                for (int k = 0; k < 16; k++) {
+                for (int k = 0; k < 16; k += 2) {
                     const int P = b + 4 * k;
                     if (P < s0_rel && P + 4 <= end_rel) atomicMax(&T1[hash4(v[k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
                 }
@@ -1128,10 +1174,11 @@ __global__ __launch_bounds__(NT) void k_deflate(DeflateLaunch DL)
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
-            // the sub-block's positions enter T1 (T2's entries expire with its tag)
+            // the sub-block's positions (every kInsertStride-th) enter T1; T2's
+            // entries expire with its tag
             if (lz && le > 0) {
 #pragma unroll
-                for (int i = 0; i < LANE; i++)
+                for (int i = 0; i < LANE; i += kInsertStride)
                     if ((hmask >> i) & 1u) atomicMax(&T1[hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
             }
             // ---- emission ------------------------------------------------------------------
@@ -1354,6 +1401,26 @@ __device__ u32x4 payload16(const AssembleJob& A, int64_t hdr, int64_t segbytes, 
     return r;
 }
 
+// Last segment s in [0, nseg) with seg_off[s] <= t (seg_off[0] = 0 <= t),
+// found by the whole wave: each round probes 64 evenly spaced candidates
+// and keeps the span after the last one at or below t, so a few dependent
+// loads replace a per-lane binary search.  t must be the same on all lanes.
+__device__ int wave_find_seg(const uint64_t* seg_off, int nseg, int64_t t)
+{
+    const int lane = threadIdx.x & 63;
+    int lo = 0, n = nseg;
+    while (n > 1) {
+        const int step = (n + 63) >> 6;
+        const int idx = lo + lane * step;
+        const bool p = idx < lo + n && (int64_t)seg_off[idx] <= t;
+        const int cnt = __popcll(__ballot(p));          // a prefix of the lanes (lane 0 always)
+        const int nlo = lo + (cnt - 1) * step;
+        n = min(step, lo + n - nlo);
+        lo = nlo;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ uint32_t crc_step4(uint32_t c, uint32_t w, const uint32_t (*T)[256])
 {
     c ^= w;
@@ -1403,18 +1470,15 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         // piece tid: [e - 64, e) with e counted back from the chunk end
         const int64_t e = z1 - (int64_t)(kAssembleThreads - 1 - tid) * kAsmBytes;
         const int64_t b = max64(z0, e - kAsmBytes);
+        // the segment of the wave's first byte, then each lane's by a short walk
+        const int64_t wfirst = max64(z0, z1 - (int64_t)(kAssembleThreads - 64 * wave) * kAsmBytes);
+        const int sw = wave_find_seg(A.seg_off, A.nseg, min64(max64(wfirst - hdr, 0), max64(segbytes - 1, 0)));
         uint32_t r = 0;
         if (b < e) {
-            int s = 0;
+            int s = sw;
             const int64_t a0 = b - hdr;
-            if (a0 > 0 && a0 < segbytes) {             // last segment starting at or before a0
-                int lo = 0, hi = A.nseg - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if ((int64_t)A.seg_off[mid] <= a0) lo = mid; else hi = mid - 1;
-                }
-                s = lo;
-            }
+            if (a0 > 0 && a0 < segbytes)
+                while ((int64_t)A.seg_off[s + 1] <= a0) s++;
             int k = s * A.spb;                         // slice cursor
 #pragma unroll 1
             for (int jg = 0; jg < 4; jg++) {
@@ -1509,13 +1573,17 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 for (int i = 0; i < 17; i++) A.out[12 + i] = ih[i];
                 put_be32(A.out + 29, crc_bytes(0, ih, 17, tab));
             }
-            // g * x^(8n): n = 64q + rem, as rem zero-byte steps and the set bits of q
-            uint32_t gsh = ~crc_bytes(0, (const uint8_t*)"IDAT", 4, tab);
-            int q = n / kAsmBytes, rem = n % kAsmBytes;
-            if (q >= (1 << kCrcShiftLevels)) { q -= 1; rem += kAsmBytes; }
-            for (int i = 0; i < rem; i++) gsh = tab[0][gsh & 0xFF] ^ (gsh >> 8);
-            for (int k = 0; k < kCrcShiftLevels; k++)
-                if (q & (1 << k)) gsh = crc_shift(gsh, shf[k]);
+            // g * x^(8n), g the register after "IDAT": given for a full chunk,
+            // else n = 64q + rem as rem zero-byte steps and the set bits of q
+            uint32_t gsh = A.idat_full;
+            if (n != A.chunk_bytes) {
+                gsh = ~crc_bytes(0, (const uint8_t*)"IDAT", 4, tab);
+                int q = n / kAsmBytes, rem = n % kAsmBytes;
+                if (q >= (1 << kCrcShiftLevels)) { q -= 1; rem += kAsmBytes; }
+                for (int i = 0; i < rem; i++) gsh = tab[0][gsh & 0xFF] ^ (gsh >> 8);
+                for (int k = 0; k < kCrcShiftLevels; k++)
+                    if (q & (1 << k)) gsh = crc_shift(gsh, shf[k]);
+            }
             put_be32(o, (uint32_t)n);
             o[4] = 'I'; o[5] = 'D'; o[6] = 'A'; o[7] = 'T';
             put_be32(o + 8 + n, ~(gsh ^ r));
@@ -1562,22 +1630,23 @@ static hipError_t launch_filter_k(const FilterLaunch& L, int grid, size_t lds, h
     return hipGetLastError();
 }
 
-// Instantiation by the widest row: K 16-byte items per thread cover rows of
-// up to K * 4096 bytes, and K sets how many Paeth results stay in registers.
+// Instantiation by the widest row: lane l of a wave takes 16-byte items
+// l, l + 64, ... of a row, K of them for rows of up to K * 1024 bytes.
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
 {
     (void)hipGetLastError();
     const int items = (L.row_cap + 15) / 16;
-    const int k = (items + kFilterThreads - 1) / kFilterThreads;
-    switch (k) {
-    case 1: return launch_filter_k<1>(L, grid, lds, s);
-    case 2: return launch_filter_k<2>(L, grid, lds, s);
-    case 3: return launch_filter_k<3>(L, grid, lds, s);
-    case 4: return launch_filter_k<4>(L, grid, lds, s);
-    case 5: return launch_filter_k<5>(L, grid, lds, s);
-    case 6: return launch_filter_k<6>(L, grid, lds, s);
-    default: return launch_filter_k<8>(L, grid, lds, s);
-    }
+    const int k = (items + 63) / 64;
+    if (k <= 1) return launch_filter_k<1>(L, grid, lds, s);
+    if (k <= 2) return launch_filter_k<2>(L, grid, lds, s);
+    if (k <= 3) return launch_filter_k<3>(L, grid, lds, s);
+    if (k <= 4) return launch_filter_k<4>(L, grid, lds, s);
+    if (k <= 6) return launch_filter_k<6>(L, grid, lds, s);
+    if (k <= 8) return launch_filter_k<8>(L, grid, lds, s);
+    if (k <= 12) return launch_filter_k<12>(L, grid, lds, s);
+    if (k <= 16) return launch_filter_k<16>(L, grid, lds, s);
+    if (k <= 24) return launch_filter_k<24>(L, grid, lds, s);
+    return launch_filter_k<32>(L, grid, lds, s);
 }
 
 // Workgroup sizes: 256 (4 K positions per sub-block, three workgroups per
