@@ -146,6 +146,42 @@ def run_plain(args, r2p, synth, torch):
     return res
 
 
+def run_checksums(args, r2p, torch):
+    """The standalone Adler-32 / CRC-32 kernels (per-block partials, then one
+    combine workgroup) over a large device buffer of random bytes; the HBM
+    rate is the bytes read over the time of both launches (HIP events on
+    the stream they run on)."""
+    n = args.checksum_mib << 20
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    d = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda", generator=g)
+    work = torch.empty(r2p.checksum_work_bytes(n), dtype=torch.uint8, device="cuda")
+    res = torch.zeros(2, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    out = {"bytes": n, "data": "random bytes, device-resident"}
+    for name, fn, off in (("adler32", r2p.adler32_device, 0), ("crc32", r2p.crc32_device, 8)):
+        for _ in range(3):
+            fn(d.data_ptr(), n, work.data_ptr(), res.data_ptr() + off, sp)
+        ts = []
+        for _ in range(10):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn(d.data_ptr(), n, work.data_ptr(), res.data_ptr() + off, sp)
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = statistics.median(ts)
+        gbs = n / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "gb_per_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    torch.cuda.synchronize()
+    v = res.cpu().numpy().view(np.uint32)
+    host = d.cpu().numpy().tobytes()
+    out["zlib_equal"] = bool(v[0] == zlib.adler32(host) and v[2] == zlib.crc32(host))
+    log("bench: checksums " + json.dumps(out))
+    return out
+
+
 def run_batch(args, r2p, synth, torch):
     """Mixed batch: the batched form (graph replays) against the plain form on the same images."""
     t0 = time.perf_counter()
@@ -320,6 +356,7 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times (profiler calibration)")
     ap.add_argument("--cpu-level", type=int, default=3)
+    ap.add_argument("--checksum-mib", type=int, default=1024, help="buffer for the standalone checksum kernels (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this encoder is measured on one GPU only; --gpus must be 1 "
@@ -365,6 +402,8 @@ def main():
         cb = cpu_baseline(O, r["s"], args.cpu_level, threads)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["ratio"]["zlib_level%d_png_over_filtered" % args.cpu_level] = round(cb["png_bytes"] / r["filtered_bytes"], 4)
+    if args.checksum_mib > 0:
+        out["checksum_kernels"] = run_checksums(args, r2p, torch)
     if args.mode in ("batch", "all"):
         out["batched"] = run_batch(args, r2p, synth, torch)
     if args.mode in ("serve", "all"):

@@ -145,7 +145,7 @@ constexpr int kWindow = 32768;           // DEFLATE window
 // Every buffer k_deflate reads holds at least this many readable bytes after
 // the last data byte, so its 16-byte loads need no tail guard.
 constexpr int kTailSlack = 64;
-constexpr int kChecksumBlock = 65536;    // bytes per checksum workgroup
+constexpr int kChecksumBlock = 16384;    // bytes per checksum workgroup
 
 // Worst-case token dwords of one slice of `slice_bytes` input bytes (9 bits
 // per literal, the 3-bit block header), plus the slack k_assemble's 5-dword

@@ -609,7 +609,7 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
 // Adler-32 of a byte range, computed by a whole workgroup.
 // Per 16-byte item at offset o: S += sum(x), T += o*sum(x) + sum(j*x_j);
 // then A = 1 + S, B = n + n*S - T  (mod 65521), B = n + sum((n-i) x_i).
-// Returns the adler value on every thread.  `red` needs 4*nwaves u32.
+// Returns the adler value on every thread.  `red` needs 2*nwaves u32.
 // ---------------------------------------------------------------------------
 __device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
 {
@@ -617,36 +617,47 @@ __device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
     uint32_t S = 0;
     uint64_t T = 0;
     const int64_t nfull = n >> 4;
-    for (int64_t i = tid; i < nfull; i += blockDim.x) {
-        const u32x4 v = g128u(p + 16 * i);
-        uint32_t s16 = 0, w16 = 0;
+    for (int64_t i0 = tid; i0 < nfull; i0 += 4 * (int64_t)blockDim.x) {
+        u32x4 v[4];                                   // four loads in flight
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t sk = __builtin_amdgcn_udot4(v[k], 0x01010101u, 0u, false);
-            w16 = __builtin_amdgcn_udot4(v[k], 0x03020100u, w16, false) + (uint32_t)(4 * k) * sk;
-            s16 += sk;
+        for (int u = 0; u < 4; u++) {
+            const int64_t i = i0 + u * (int64_t)blockDim.x;
+            v[u] = i < nfull ? g128u(p + 16 * i) : (u32x4){0, 0, 0, 0};
         }
-        S += s16;
-        T += (uint64_t)(16 * i) * s16 + w16;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t i = i0 + u * (int64_t)blockDim.x;
+            uint32_t s16 = 0, w16 = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t sk = __builtin_amdgcn_udot4(v[u][k], 0x01010101u, 0u, false);
+                w16 = __builtin_amdgcn_udot4(v[u][k], 0x03020100u, w16, false) + (uint32_t)(4 * k) * sk;
+                s16 += sk;
+            }
+            S += s16;
+            T += (uint64_t)(16 * (i < nfull ? i : 0)) * s16 + w16;
+        }
     }
     for (int64_t j = (nfull << 4) + tid; j < n; j += blockDim.x) {
         const uint32_t b = g8(p + j);
         S += b;
         T += (uint64_t)j * b;
     }
-    S = wave_sum(S);
-    T = wave_sum64(T);
+    // S < 2^32 (n <= 64 KiB); T mod M from its 32-bit halves (2^32 = 225 mod M)
+    constexpr uint32_t M = 65521;
+    const uint32_t t32 = ((((uint32_t)(T >> 32)) % M) * 225u + ((uint32_t)T) % M) % M;
+    const uint32_t Sw = wave_total(S) % M;         // 64 lanes: no overflow
+    const uint32_t Tw = wave_total(t32) % M;
     __syncthreads();
-    if (lane == 0) { red[wave * 3] = S; red[wave * 3 + 1] = (uint32_t)T; red[wave * 3 + 2] = (uint32_t)(T >> 32); }
+    if (lane == 0) { red[wave * 2] = Sw; red[wave * 2 + 1] = Tw; }
     __syncthreads();
-    uint64_t St = 0, Tt = 0;
-    for (int w = 0; w < nw; w++) { St += red[w * 3]; Tt += (uint64_t)red[w * 3 + 1] | ((uint64_t)red[w * 3 + 2] << 32); }
-    const uint64_t M = 65521;
-    const uint64_t A = (1 + St) % M;
-    const uint64_t nn = (uint64_t)n;
-    const uint64_t W = ((nn % M) * (St % M) + M * M - (Tt % M)) % M;   // (n*S - T) mod M
-    const uint64_t B = (nn % M + W) % M;
-    return (uint32_t)((B << 16) | A);
+    uint32_t St = 0, Tt = 0;
+    for (int w = 0; w < nw; w++) { St += red[w * 2]; Tt += red[w * 2 + 1]; }
+    St %= M; Tt %= M;
+    const uint32_t A = (1u + St) % M;
+    const uint32_t nm = (uint32_t)((uint64_t)n % M);
+    const uint32_t B = (nm * A + M - Tt) % M;      // n (1 + S) - T
+    return (B << 16) | A;
 }
 
 __device__ __forceinline__ uint32_t adler_combine_dev(uint32_t a1, uint32_t a2, uint64_t len2)
@@ -721,6 +732,19 @@ __device__ uint32_t block_crc(const uint8_t* p, int64_t n, const uint32_t (*tab)
     return r;
 }
 
+// One slice-by-4 step: 4 more bytes (w, little-endian) into register c.
+__device__ __forceinline__ uint32_t crc_step4(uint32_t c, uint32_t w, const uint32_t (*T)[256])
+{
+    c ^= w;
+    return T[3][c & 0xFF] ^ T[2][(c >> 8) & 0xFF] ^ T[1][(c >> 16) & 0xFF] ^ T[0][c >> 24];
+}
+
+// c times the fixed polynomial a shift table S holds, by input byte.
+__device__ __forceinline__ uint32_t crc_shift(uint32_t c, const uint32_t (*S)[256])
+{
+    return S[0][c & 0xFF] ^ S[1][(c >> 8) & 0xFF] ^ S[2][(c >> 16) & 0xFF] ^ S[3][c >> 24];
+}
+
 __device__ void load_crc_tables(uint32_t (*tab)[256])
 {
     for (int i = threadIdx.x; i < 1024; i += blockDim.x) (&tab[0][0])[i] = (&c_crc[0][0])[i];
@@ -728,7 +752,8 @@ __device__ void load_crc_tables(uint32_t (*tab)[256])
 }
 
 // ---------------------------------------------------------------------------
-// Standalone checksums: partial per 64 KiB block, then one combine block.
+// Standalone checksums: a partial per kChecksumBlock-byte block, then one
+// combine workgroup.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_adler_partial(const uint8_t* p, int64_t n, uint32_t* part)
 {
@@ -739,50 +764,131 @@ __global__ __launch_bounds__(256) void k_adler_partial(const uint8_t* p, int64_t
     if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
+// XOR over the 64 lanes of the wave (the DPP pattern of wave_total), uniform.
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v)
+{
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// CRC register of one block, run from 0 with no final inversion.  Thread t
+// takes the 64 bytes [e - 64, e) with e counted back from the block's end
+// (a short block's first pieces are partly or wholly empty: leading zeros
+// leave a register that starts at 0 unchanged), so the pieces combine in a
+// tree whose level-k shift is the fixed x^(8*64*2^k).
+static_assert(kChecksumBlock == 256 * 64, "one 64-byte piece per thread");
 __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n, uint32_t* part)
 {
     __shared__ uint32_t tab[4][256];
-    __shared__ uint32_t red[512];
+    __shared__ uint32_t red[4];
     load_crc_tables(tab);
-    const int64_t b = (int64_t)blockIdx.x * kChecksumBlock;
-    const int64_t len = min64(kChecksumBlock, n - b);
-    const uint32_t c = block_crc(p + b, len, tab, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = c;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * kChecksumBlock;
+    const int64_t b1 = min64(b0 + kChecksumBlock, n);
+    const int64_t e = b1 - (int64_t)(255 - tid) * 64;
+    const int64_t b = max64(b0, e - 64);
+    uint32_t r = 0;
+    if (e - b == 64) {
+        u32x4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = g128u(p + b + 16 * q);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) r = crc_step4(r, v[q][k], tab);
+    } else {
+        for (int64_t i = b; i < e; i++) r = tab[0][(r ^ p[i]) & 0xFF] ^ (r >> 8);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const uint32_t right = __shfl_down(r, 1 << k, 64);
+        if ((lane & ((2 << k) - 1)) == 0) r = crc_shift(r, c_crcshift[k]) ^ right;
+    }
+    if (lane == 0) red[wave] = r;
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t v01 = crc_shift(red[0], c_crcshift[6]) ^ red[1];
+        const uint32_t v23 = crc_shift(red[2], c_crcshift[6]) ^ red[3];
+        part[blockIdx.x] = crc_shift(v01, c_crcshift[7]) ^ v23;
+    }
 }
 
-// Combine nb partials in order (each block kChecksumBlock bytes but the last).
-// kind 0 = adler, 1 = crc.  One workgroup: per-thread runs, then a tree.
-__global__ __launch_bounds__(256) void k_checksum_final(const uint32_t* part, int nb, int64_t n, int kind,
-                                                         uint32_t* result)
+// Combines the nb partials (every block kChecksumBlock bytes but the last).
+// kind 0 = adler: for blocks with sums A_i - 1 and R_i bytes after them,
+//   A = 1 + sum(A_i - 1), B = sum(B_i + (A_i - 1) R_i) (mod 65521).
+// kind 1 = crc: registers p_i from 0; each thread folds a run of blocks
+//   Horner-wise (h = h * x^(8*block) + p_i, the block shift as table
+//   lookups), shifts its run by the bytes after it, and the runs XOR
+//   together with the initial register's term ~0 * x^(8n).
+constexpr int kFinalThreads = 1024;
+__global__ __launch_bounds__(kFinalThreads) void k_checksum_final(const uint32_t* part, int nb, int64_t n, int kind,
+                                                                   uint32_t* result)
 {
-    __shared__ uint32_t val[256];
-    __shared__ uint64_t len[256];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int per = (nb + nt - 1) / nt;
-    const int b = min(nb, per * tid), e = min(nb, b + per);
-    uint32_t v = kind ? 0u : 1u;
-    uint64_t l = 0;
-    for (int i = b; i < e; i++) {
-        const uint64_t li = (uint64_t)min64(kChecksumBlock, n - (int64_t)i * kChecksumBlock);
-        if (l == 0) v = part[i];
-        else v = kind ? crc_combine_dev(v, part[i], li) : adler_combine_dev(v, part[i], li);
-        l += li;
-    }
-    val[tid] = v; len[tid] = l;
-    __syncthreads();
-    for (int s = 1; s < nt; s <<= 1) {
-        if ((tid & (2 * s - 1)) == 0 && tid + s < nt) {
-            const uint64_t l2 = len[tid + s];
-            if (l2) {
-                if (len[tid] == 0) val[tid] = val[tid + s];
-                else val[tid] = kind ? crc_combine_dev(val[tid], val[tid + s], l2)
-                                     : adler_combine_dev(val[tid], val[tid + s], l2);
-                len[tid] += l2;
+    constexpr int NW = kFinalThreads / 64;
+    __shared__ uint32_t shb[4][256];
+    __shared__ uint32_t red[2 * NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (kind == 0) {
+        constexpr uint32_t M = 65521;
+        const uint32_t nm = (uint32_t)((uint64_t)n % M), bm = (uint32_t)kChecksumBlock % M;
+        uint32_t sa = 0, sb = 0;
+        for (int i0 = tid; i0 < nb; i0 += 8 * kFinalThreads) {
+            uint32_t a[8];                                // eight loads in flight
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] = i0 + u * kFinalThreads < nb ? part[i0 + u * kFinalThreads] : 1u;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u * kFinalThreads;
+                // bytes after block i, mod M: n - (i + 1) * block (0 for the last)
+                const uint32_t x = (uint32_t)((i + 1) % (int)M) * bm % M;
+                const uint32_t R = i >= nb - 1 ? 0u : (nm + M - x) % M;
+                const uint32_t am = (a[u] & 0xFFFFu) ? (a[u] & 0xFFFFu) - 1u : M - 1u;
+                sa += am;
+                sa = sa >= M ? sa - M : sa;
+                sb = (sb + (a[u] >> 16) + (am * R) % M) % M;
             }
         }
+        sa = wave_total(sa) % M;
+        sb = wave_total(sb) % M;
+        if (lane == 0) { red[wave] = sa; red[NW + wave] = sb; }
         __syncthreads();
+        if (tid == 0) {
+            uint32_t A = 1, B = 0;
+            for (int w = 0; w < NW; w++) { A += red[w]; B += red[NW + w]; }
+            *result = ((B % M) << 16) | (A % M);
+        }
+        return;
     }
-    if (tid == 0) *result = (n == 0) ? (kind ? 0u : 1u) : val[0];
+    // a full block's shift: level log2(kChecksumBlock / 64) of the tables
+    constexpr int kBlockLevel = 8;
+    static_assert((64 << kBlockLevel) == kChecksumBlock, "block shift table");
+    for (int i = tid; i < 1024; i += kFinalThreads) (&shb[0][0])[i] = (&c_crcshift[kBlockLevel][0][0])[i];
+    __syncthreads();
+    const int per = (nb + kFinalThreads - 1) / kFinalThreads;
+    const int i0 = min(nb, per * tid), i1 = min(nb, i0 + per);
+    uint32_t term = 0;
+    if (i0 < i1) {
+        uint32_t h = 0;
+        for (int i = i0; i < i1; i++) {
+            const int64_t len = min64(kChecksumBlock, n - (int64_t)i * kChecksumBlock);
+            h = (len == kChecksumBlock ? crc_shift(h, shb) : gf2_mulmod(gf2_x8n((uint64_t)len), h)) ^ part[i];
+        }
+        const int64_t after = n - min64((int64_t)i1 * kChecksumBlock, n);
+        term = after ? gf2_mulmod(gf2_x8n((uint64_t)after), h) : h;
+    }
+    term = wave_xor(term);
+    if (lane == 0) red[wave] = term;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t x = gf2_mulmod(gf2_x8n((uint64_t)n), 0xFFFFFFFFu);
+        for (int w = 0; w < NW; w++) x ^= red[w];
+        *result = ~x;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1238,8 +1344,7 @@ __device__ __forceinline__ int64_t png_payload(const AssembleJob& A, uint64_t se
 __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
 {
     __shared__ uint64_t wsum[4];
-    __shared__ uint32_t aval[256];
-    __shared__ uint64_t alen[256];
+    __shared__ uint32_t ared[8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (AL.counts && blockIdx.x >= AL.counts[0]) return;
     const AssembleJob A = AL.jobs ? AL.jobs[blockIdx.x] : AL.single;
@@ -1253,7 +1358,14 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
         if (i < A.nseg) {
             const int k0 = i * A.spb, k1 = min(k0 + A.spb, A.nslice);
             uint32_t tb = 0;
-            for (int k = k0; k < k1; k++) { A.slice_boff[k] = tb; tb += A.slice_bits[k]; }
+            for (int kb = k0; kb < k1; kb += 8) {          // eight loads in flight
+                uint32_t bits[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) bits[u] = kb + u < k1 ? A.slice_bits[kb + u] : 0u;
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (kb + u < k1) { A.slice_boff[kb + u] = tb; tb += bits[u]; }
+            }
             A.seg_tbits[i] = tb;
             const bool last = (i == A.nseg - 1) && A.final_block;
             v = (tb + 7 + (last ? 0u : 3u) + 7) / 8 + (last ? 0u : 4u);
@@ -1267,30 +1379,30 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
         if (i < A.nseg) A.seg_off[i] = base + wb + incl - v;
         base += tot;
     }
-    // adler combine over the slices: contiguous runs per thread, then a tree
-    const int per = (A.nslice + 255) / 256;
-    const int b = min(A.nslice, per * tid), e = min(A.nslice, b + per);
-    uint32_t v = 1; uint64_t l = 0;
-    for (int i = b; i < e; i++) {
-        const uint64_t li = (uint64_t)max64(0, min64(A.slice_bytes, A.total_in - (int64_t)i * A.slice_bytes));
-        v = l ? adler_combine_dev(v, A.slice_adler[i], li) : A.slice_adler[i];
-        l += li;
+    // Adler-32 of the whole input from the slices' values: for pieces with
+    // sums (A_i - 1) and R_i bytes after them, A = 1 + sum(A_i - 1) and
+    // B = sum(B_i + (A_i - 1) R_i) (mod 65521) -- a plain sum, so every
+    // slice is one independent term
+    constexpr uint32_t M = 65521;
+    uint32_t sa = 0, sb = 0;
+    for (int i = tid; i < A.nslice; i += 256) {
+        const int64_t end = min64((int64_t)(i + 1) * A.slice_bytes, A.total_in);
+        const uint32_t R = (uint32_t)((uint64_t)(A.total_in - end) % M);
+        const uint32_t a = A.slice_adler[i];
+        const uint32_t am = (a & 0xFFFFu) ? (a & 0xFFFFu) - 1u : M - 1u;
+        sa += am;
+        sa = sa >= M ? sa - M : sa;
+        sb = (sb + (a >> 16) + (am * R) % M) % M;
     }
-    aval[tid] = v; alen[tid] = l;
+    sa = wave_total(sa) % M;                       // 64 terms < M: no overflow
+    sb = wave_total(sb) % M;
+    if (lane == 0) { ared[wave] = sa; ared[4 + wave] = sb; }
     __syncthreads();
-    for (int s = 1; s < 256; s <<= 1) {
-        if ((tid & (2 * s - 1)) == 0 && tid + s < 256) {
-            const uint64_t l2 = alen[tid + s];
-            if (l2) {
-                aval[tid] = alen[tid] ? adler_combine_dev(aval[tid], aval[tid + s], l2) : aval[tid + s];
-                alen[tid] += l2;
-            }
-        }
-        __syncthreads();
-    }
+    const uint32_t adler_a = (1u + ared[0] + ared[1] + ared[2] + ared[3]) % M;
+    const uint32_t adler_b = (ared[4] + ared[5] + ared[6] + ared[7]) % M;
     if (tid == 0) {
         A.seg_off[A.nseg] = base;
-        const uint32_t adler = A.total_in ? aval[0] : 1u;
+        const uint32_t adler = A.total_in ? (adler_b << 16) | adler_a : 1u;
         if (A.adler_out) *A.adler_out = adler;
         const int64_t z = png_payload(A, base);
         if (A.running) {
@@ -1421,16 +1533,6 @@ __device__ int wave_find_seg(const uint64_t* seg_off, int nseg, int64_t t)
     return lo;
 }
 
-__device__ __forceinline__ uint32_t crc_step4(uint32_t c, uint32_t w, const uint32_t (*T)[256])
-{
-    c ^= w;
-    return T[3][c & 0xFF] ^ T[2][(c >> 8) & 0xFF] ^ T[1][(c >> 16) & 0xFF] ^ T[0][c >> 24];
-}
-
-__device__ __forceinline__ uint32_t crc_shift(uint32_t c, const uint32_t (*S)[256])
-{
-    return S[0][c & 0xFF] ^ S[1][(c >> 8) & 0xFF] ^ S[2][(c >> 16) & 0xFF] ^ S[3][c >> 24];
-}
 
 __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL)
 {
@@ -1682,7 +1784,7 @@ hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* r
     (void)hipGetLastError();
     const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
     if (nb > 0) k_adler_partial<<<nb, 256, 0, s>>>(p, n, part);
-    k_checksum_final<<<1, 256, 0, s>>>(part, nb, n, 0, result);
+    k_checksum_final<<<1, kFinalThreads, 0, s>>>(part, nb, n, 0, result);
     return hipGetLastError();
 }
 
@@ -1691,7 +1793,7 @@ hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* res
     (void)hipGetLastError();
     const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
     if (nb > 0) k_crc_partial<<<nb, 256, 0, s>>>(p, n, part);
-    k_checksum_final<<<1, 256, 0, s>>>(part, nb, n, 1, result);
+    k_checksum_final<<<1, kFinalThreads, 0, s>>>(part, nb, n, 1, result);
     return hipGetLastError();
 }
 
