@@ -138,14 +138,15 @@ struct AssembleLaunch {
 constexpr int kFilterThreads = 256;
 constexpr int kDeflateThreads = 256;
 constexpr int kAssembleThreads = 1024;
-constexpr int kCrcShiftLevels = 10;       // CRC shift tables for 64 * 2^k zero bytes, k < 10
+constexpr int kCrcShiftLevels = 11;       // CRC shift tables for 64 * 2^k zero bytes, k < 11
 constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
 constexpr int kFilterLdsMax = 160 * 1024 - 256;  // LDS one filter workgroup may use
 constexpr int kWindow = 32768;           // DEFLATE window
 // Every buffer k_deflate reads holds at least this many readable bytes after
 // the last data byte, so its 16-byte loads need no tail guard.
 constexpr int kTailSlack = 64;
-constexpr int kChecksumBlock = 16384;    // bytes per checksum workgroup
+constexpr int kChecksumBlock = 16384;    // bytes per Adler-32 partial
+constexpr int kCrcBlock = 32768;         // bytes per CRC-32 partial
 
 // Worst-case token dwords of one slice of `slice_bytes` input bytes (9 bits
 // per literal, the 3-bit block header), plus the slack k_assemble's 5-dword

@@ -781,40 +781,58 @@ __device__ __forceinline__ uint32_t wave_xor(uint32_t v)
 // (a short block's first pieces are partly or wholly empty: leading zeros
 // leave a register that starts at 0 unchanged), so the pieces combine in a
 // tree whose level-k shift is the fixed x^(8*64*2^k).
-static_assert(kChecksumBlock == 256 * 64, "one 64-byte piece per thread");
+// CRC partial of a kCrcBlock-byte block.  Thread t takes the 128 bytes
+// [e - 128, e), e counted back from the block's end (a short block's first
+// pieces are partly or wholly empty: leading zeros leave a register that
+// starts at 0 unchanged), as two independent 64-byte chains joined by the
+// 64-byte shift; the threads' pieces then combine in a tree from the
+// 128-byte shift up.  One table lookup per byte: the kernel is bound by
+// LDS lookups and by the strided 16-byte loads a per-thread chain needs.
+static_assert(kCrcBlock == 256 * 128, "two 64-byte chains per thread");
 __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n, uint32_t* part)
 {
     __shared__ uint32_t tab[4][256];
-    __shared__ uint32_t red[4];
+    __shared__ uint32_t red[2][4];
     load_crc_tables(tab);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t b0 = (int64_t)blockIdx.x * kChecksumBlock;
-    const int64_t b1 = min64(b0 + kChecksumBlock, n);
-    const int64_t e = b1 - (int64_t)(255 - tid) * 64;
-    const int64_t b = max64(b0, e - 64);
-    uint32_t r = 0;
-    if (e - b == 64) {
-        u32x4 v[4];
+    const int nb = (int)((n + kCrcBlock - 1) / kCrcBlock);
+    // persistent: the tables are loaded once, blocks are dealt round-robin
+    int it = 0;
+    for (int blk = blockIdx.x; blk < nb; blk += gridDim.x, it ^= 1) {
+        const int64_t b0 = (int64_t)blk * kCrcBlock;
+        const int64_t b1 = min64(b0 + kCrcBlock, n);
+        const int64_t e = b1 - (int64_t)(255 - tid) * 128;
+        const int64_t b = max64(b0, e - 128);
+        uint32_t r = 0;
+        if (e - b == 128) {
+            u32x4 v[8];
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[q] = g128u(p + b + 16 * q);
+            for (int q = 0; q < 8; q++) v[q] = g128u(p + b + 16 * q);
+            uint32_t r0 = 0, r1 = 0;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+            for (int q = 0; q < 4; q++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) r = crc_step4(r, v[q][k], tab);
-    } else {
-        for (int64_t i = b; i < e; i++) r = tab[0][(r ^ p[i]) & 0xFF] ^ (r >> 8);
-    }
+                for (int k = 0; k < 4; k++) {
+                    r0 = crc_step4(r0, v[q][k], tab);
+                    r1 = crc_step4(r1, v[q + 4][k], tab);
+                }
+            r = crc_shift(r0, c_crcshift[0]) ^ r1;
+        } else {
+            for (int64_t i = b; i < e; i++) r = tab[0][(r ^ p[i]) & 0xFF] ^ (r >> 8);
+        }
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const uint32_t right = __shfl_down(r, 1 << k, 64);
-        if ((lane & ((2 << k) - 1)) == 0) r = crc_shift(r, c_crcshift[k]) ^ right;
-    }
-    if (lane == 0) red[wave] = r;
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t v01 = crc_shift(red[0], c_crcshift[6]) ^ red[1];
-        const uint32_t v23 = crc_shift(red[2], c_crcshift[6]) ^ red[3];
-        part[blockIdx.x] = crc_shift(v01, c_crcshift[7]) ^ v23;
+        for (int k = 0; k < 6; k++) {
+            const uint32_t right = __shfl_down(r, 1 << k, 64);
+            if ((lane & ((2 << k) - 1)) == 0) r = crc_shift(r, c_crcshift[k + 1]) ^ right;
+        }
+        // red is double-buffered by iteration, so one barrier per block
+        if (lane == 0) red[it][wave] = r;
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t v01 = crc_shift(red[it][0], c_crcshift[7]) ^ red[it][1];
+            const uint32_t v23 = crc_shift(red[it][2], c_crcshift[7]) ^ red[it][3];
+            part[blk] = crc_shift(v01, c_crcshift[8]) ^ v23;
+        }
     }
 }
 
@@ -826,6 +844,7 @@ __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n
 //   lookups), shifts its run by the bytes after it, and the runs XOR
 //   together with the initial register's term ~0 * x^(8n).
 constexpr int kFinalThreads = 1024;
+constexpr int kCrcPartialGrid = 2048;   // 8 workgroups on each of 256 CUs
 __global__ __launch_bounds__(kFinalThreads) void k_checksum_final(const uint32_t* part, int nb, int64_t n, int kind,
                                                                    uint32_t* result)
 {
@@ -864,9 +883,9 @@ __global__ __launch_bounds__(kFinalThreads) void k_checksum_final(const uint32_t
         }
         return;
     }
-    // a full block's shift: level log2(kChecksumBlock / 64) of the tables
-    constexpr int kBlockLevel = 8;
-    static_assert((64 << kBlockLevel) == kChecksumBlock, "block shift table");
+    // a full block's shift: level log2(kCrcBlock / 64) of the tables
+    constexpr int kBlockLevel = 9;
+    static_assert((64 << kBlockLevel) == kCrcBlock, "block shift table");
     for (int i = tid; i < 1024; i += kFinalThreads) (&shb[0][0])[i] = (&c_crcshift[kBlockLevel][0][0])[i];
     __syncthreads();
     const int per = (nb + kFinalThreads - 1) / kFinalThreads;
@@ -875,10 +894,10 @@ __global__ __launch_bounds__(kFinalThreads) void k_checksum_final(const uint32_t
     if (i0 < i1) {
         uint32_t h = 0;
         for (int i = i0; i < i1; i++) {
-            const int64_t len = min64(kChecksumBlock, n - (int64_t)i * kChecksumBlock);
-            h = (len == kChecksumBlock ? crc_shift(h, shb) : gf2_mulmod(gf2_x8n((uint64_t)len), h)) ^ part[i];
+            const int64_t len = min64(kCrcBlock, n - (int64_t)i * kCrcBlock);
+            h = (len == kCrcBlock ? crc_shift(h, shb) : gf2_mulmod(gf2_x8n((uint64_t)len), h)) ^ part[i];
         }
-        const int64_t after = n - min64((int64_t)i1 * kChecksumBlock, n);
+        const int64_t after = n - min64((int64_t)i1 * kCrcBlock, n);
         term = after ? gf2_mulmod(gf2_x8n((uint64_t)after), h) : h;
     }
     term = wave_xor(term);
@@ -1658,8 +1677,10 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         __syncthreads();
         if (wave == 0) {
             r = lane < kAssembleThreads / 64 ? red[lane] : 0u;
+            // 16 waves of 4 KiB: the shifts by 4, 8, 16 and 32 KiB
+            static_assert(kAssembleThreads == 1024, "four cross-wave levels");
 #pragma unroll
-            for (int k = 6; k < kCrcShiftLevels; k++) {
+            for (int k = 6; k < 10; k++) {
                 const uint32_t right = __shfl_down(r, 1 << (k - 6), 64);
                 if ((lane & ((2 << (k - 6)) - 1)) == 0) r = crc_shift(r, shf[k]) ^ right;
             }
@@ -1791,8 +1812,8 @@ hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* r
 hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s)
 {
     (void)hipGetLastError();
-    const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
-    if (nb > 0) k_crc_partial<<<nb, 256, 0, s>>>(p, n, part);
+    const int nb = (int)((n + kCrcBlock - 1) / kCrcBlock);
+    if (nb > 0) k_crc_partial<<<nb < kCrcPartialGrid ? nb : kCrcPartialGrid, 256, 0, s>>>(p, n, part);
     k_checksum_final<<<1, kFinalThreads, 0, s>>>(part, nb, n, 1, result);
     return hipGetLastError();
 }
