@@ -32,7 +32,7 @@ __constant__ uint32_t c_len[259];      // length L: (revcode | extra << cbits) |
 __constant__ uint32_t c_dist_rev[32];  // 5-bit reversed distance codes
 __constant__ uint32_t c_crc[4][256];   // slice-by-4 CRC-32 tables
 __constant__ uint32_t c_x8pow[64];     // x^(8 * 2^k) mod P, reflected
-__constant__ uint32_t c_crcshift[kCrcShiftLevels][4][256];   // c -> c * x^(8*64*2^k) mod P, by input byte
+__device__ uint32_t c_crcshift[kCrcShiftLevels][4][256];     // c -> c * x^(8*64*2^k) mod P, by input byte
 
 // ---------------------------------------------------------------------------
 // Small helpers
@@ -755,13 +755,21 @@ __device__ void load_crc_tables(uint32_t (*tab)[256])
 // Standalone checksums: a partial per kChecksumBlock-byte block, then one
 // combine workgroup.
 // ---------------------------------------------------------------------------
+// Per block its two terms of the sum form of the Adler-32 combine (see
+// k_checksum_final), A_i - 1 and B_i + (A_i - 1) R_i with R_i the bytes
+// after the block, both mod 65521, packed as low | high << 16.
 __global__ __launch_bounds__(256) void k_adler_partial(const uint8_t* p, int64_t n, uint32_t* part)
 {
     __shared__ uint32_t red[16];
+    constexpr uint32_t M = 65521;
     const int64_t b = (int64_t)blockIdx.x * kChecksumBlock;
     const int64_t len = min64(kChecksumBlock, n - b);
     const uint32_t a = block_adler(p + b, len, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
+    if (threadIdx.x == 0) {
+        const uint32_t R = (uint32_t)((uint64_t)(n - b - len) % M);
+        const uint32_t am = (a & 0xFFFFu) ? (a & 0xFFFFu) - 1u : M - 1u;
+        part[blockIdx.x] = am | ((((a >> 16) + (am * R) % M) % M) << 16);
+    }
 }
 
 // XOR over the 64 lanes of the wave (the DPP pattern of wave_total), uniform.
@@ -781,13 +789,17 @@ __device__ __forceinline__ uint32_t wave_xor(uint32_t v)
 // (a short block's first pieces are partly or wholly empty: leading zeros
 // leave a register that starts at 0 unchanged), so the pieces combine in a
 // tree whose level-k shift is the fixed x^(8*64*2^k).
-// CRC partial of a kCrcBlock-byte block.  Thread t takes the 128 bytes
-// [e - 128, e), e counted back from the block's end (a short block's first
-// pieces are partly or wholly empty: leading zeros leave a register that
-// starts at 0 unchanged), as two independent 64-byte chains joined by the
-// 64-byte shift; the threads' pieces then combine in a tree from the
-// 128-byte shift up.  One table lookup per byte: the kernel is bound by
-// LDS lookups and by the strided 16-byte loads a per-thread chain needs.
+constexpr int kFinalThreads = 1024;     // the combining workgroup
+constexpr int kCrcPartialGrid = 2048;   // 8 workgroups on each of 256 CUs
+
+// CRC partials.  Blocks are counted back from the end: block j covers
+// [n - (j + 1) B, n - j B) (the data's first block may be short: leading
+// zeros leave a register that starts at 0 unchanged), so the register of
+// the whole input is XOR_j p_j * x^(8 j B) with every shift a table.  In a
+// block, thread t takes the 128 bytes [e - 128, e), e counted back from the
+// block's end, as two independent 64-byte chains joined by the 64-byte
+// shift; the threads' pieces then combine in a tree from the 128-byte shift
+// up.  One table lookup per byte.
 static_assert(kCrcBlock == 256 * 128, "two 64-byte chains per thread");
 __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n, uint32_t* part)
 {
@@ -799,8 +811,8 @@ __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n
     // persistent: the tables are loaded once, blocks are dealt round-robin
     int it = 0;
     for (int blk = blockIdx.x; blk < nb; blk += gridDim.x, it ^= 1) {
-        const int64_t b0 = (int64_t)blk * kCrcBlock;
-        const int64_t b1 = min64(b0 + kCrcBlock, n);
+        const int64_t b1 = n - (int64_t)blk * kCrcBlock;
+        const int64_t b0 = max64(0, b1 - kCrcBlock);
         const int64_t e = b1 - (int64_t)(255 - tid) * 128;
         const int64_t b = max64(b0, e - 128);
         uint32_t r = 0;
@@ -836,41 +848,65 @@ __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n
     }
 }
 
-// Combines the nb partials (every block kChecksumBlock bytes but the last).
-// kind 0 = adler: for blocks with sums A_i - 1 and R_i bytes after them,
-//   A = 1 + sum(A_i - 1), B = sum(B_i + (A_i - 1) R_i) (mod 65521).
-// kind 1 = crc: registers p_i from 0; each thread folds a run of blocks
-//   Horner-wise (h = h * x^(8*block) + p_i, the block shift as table
-//   lookups), shifts its run by the bytes after it, and the runs XOR
-//   together with the initial register's term ~0 * x^(8n).
-constexpr int kFinalThreads = 1024;
-constexpr int kCrcPartialGrid = 2048;   // 8 workgroups on each of 256 CUs
-__global__ __launch_bounds__(kFinalThreads) void k_checksum_final(const uint32_t* part, int nb, int64_t n, int kind,
-                                                                   uint32_t* result)
+// Total register XOR_j p_j * x^(8 j B) of the CRC partials: thread t folds
+// the blocks [t m, (t + 1) m) Horner-wise from the top (h = h * x^(8B) + p_j),
+// then the threads combine in a tree whose level-k shift is x^(8 m B 2^k);
+// m is a power of two, so every shift is one of the tables.  The result is
+// ~(init ^ total) with init = ~0 * x^(8n), the initial register's term.
+constexpr int kBlockLevel = 9;                         // x^(8 B): 64 * 2^9 = B
+static_assert((64 << kBlockLevel) == kCrcBlock, "block shift table");
+__global__ __launch_bounds__(kFinalThreads) void k_crc_final(const uint32_t* part, int nb, int lm, uint32_t init,
+                                                             uint32_t* result)
 {
     constexpr int NW = kFinalThreads / 64;
     __shared__ uint32_t shb[4][256];
+    __shared__ uint32_t red[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < 1024; i += kFinalThreads) (&shb[0][0])[i] = (&c_crcshift[kBlockLevel][0][0])[i];
+    __syncthreads();
+    const int m = 1 << lm;
+    const int lo = min(nb, tid << lm), hi = min(nb, lo + m);
+    uint32_t h = 0;
+    for (int j = hi - 1; j >= lo; j--) h = crc_shift(h, shb) ^ part[j];
+    // thread t's register is worth x^(8 t m B): level k shifts by m B 2^k
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const uint32_t right = __shfl_down(h, 1 << k, 64);
+        if ((lane & ((2 << k) - 1)) == 0) h ^= crc_shift(right, c_crcshift[kBlockLevel + lm + k]);
+    }
+    if (lane == 0) red[wave] = h;
+    __syncthreads();
+    if (wave == 0) {
+        h = lane < NW ? red[lane] : 0u;
+#pragma unroll
+        for (int k = 6; k < 10; k++) {
+            const uint32_t right = __shfl_down(h, 1 << (k - 6), 64);
+            if ((lane & ((2 << (k - 6)) - 1)) == 0) h ^= crc_shift(right, c_crcshift[kBlockLevel + lm + k]);
+        }
+        if (lane == 0) *result = ~(init ^ h);
+    }
+}
+
+// Adler-32 of the nb blocks (every block kChecksumBlock bytes but the last):
+// for blocks with sums A_i - 1 and R_i bytes after them, A = 1 + sum(A_i - 1)
+// and B = sum(B_i + (A_i - 1) R_i) (mod 65521); the partials hold the two
+// terms of each block, so this is a plain sum.
+__global__ __launch_bounds__(kFinalThreads) void k_adler_final(const uint32_t* part, int nb, uint32_t* result)
+{
+    constexpr int NW = kFinalThreads / 64;
     __shared__ uint32_t red[2 * NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (kind == 0) {
+    {
+        // 16 terms < 2^16 per round, reduced mod M after each round
         constexpr uint32_t M = 65521;
-        const uint32_t nm = (uint32_t)((uint64_t)n % M), bm = (uint32_t)kChecksumBlock % M;
         uint32_t sa = 0, sb = 0;
-        for (int i0 = tid; i0 < nb; i0 += 8 * kFinalThreads) {
-            uint32_t a[8];                                // eight loads in flight
+        for (int i0 = tid; i0 < nb; i0 += 16 * kFinalThreads) {
+            uint32_t a[16];                               // sixteen loads in flight
 #pragma unroll
-            for (int u = 0; u < 8; u++) a[u] = i0 + u * kFinalThreads < nb ? part[i0 + u * kFinalThreads] : 1u;
+            for (int u = 0; u < 16; u++) a[u] = i0 + u * kFinalThreads < nb ? part[i0 + u * kFinalThreads] : 0u;
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int i = i0 + u * kFinalThreads;
-                // bytes after block i, mod M: n - (i + 1) * block (0 for the last)
-                const uint32_t x = (uint32_t)((i + 1) % (int)M) * bm % M;
-                const uint32_t R = i >= nb - 1 ? 0u : (nm + M - x) % M;
-                const uint32_t am = (a[u] & 0xFFFFu) ? (a[u] & 0xFFFFu) - 1u : M - 1u;
-                sa += am;
-                sa = sa >= M ? sa - M : sa;
-                sb = (sb + (a[u] >> 16) + (am * R) % M) % M;
-            }
+            for (int u = 0; u < 16; u++) { sa += a[u] & 0xFFFFu; sb += a[u] >> 16; }
+            sa %= M; sb %= M;
         }
         sa = wave_total(sa) % M;
         sb = wave_total(sb) % M;
@@ -882,31 +918,6 @@ __global__ __launch_bounds__(kFinalThreads) void k_checksum_final(const uint32_t
             *result = ((B % M) << 16) | (A % M);
         }
         return;
-    }
-    // a full block's shift: level log2(kCrcBlock / 64) of the tables
-    constexpr int kBlockLevel = 9;
-    static_assert((64 << kBlockLevel) == kCrcBlock, "block shift table");
-    for (int i = tid; i < 1024; i += kFinalThreads) (&shb[0][0])[i] = (&c_crcshift[kBlockLevel][0][0])[i];
-    __syncthreads();
-    const int per = (nb + kFinalThreads - 1) / kFinalThreads;
-    const int i0 = min(nb, per * tid), i1 = min(nb, i0 + per);
-    uint32_t term = 0;
-    if (i0 < i1) {
-        uint32_t h = 0;
-        for (int i = i0; i < i1; i++) {
-            const int64_t len = min64(kCrcBlock, n - (int64_t)i * kCrcBlock);
-            h = (len == kCrcBlock ? crc_shift(h, shb) : gf2_mulmod(gf2_x8n((uint64_t)len), h)) ^ part[i];
-        }
-        const int64_t after = n - min64((int64_t)i1 * kCrcBlock, n);
-        term = after ? gf2_mulmod(gf2_x8n((uint64_t)after), h) : h;
-    }
-    term = wave_xor(term);
-    if (lane == 0) red[wave] = term;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t x = gf2_mulmod(gf2_x8n((uint64_t)n), 0xFFFFFFFFu);
-        for (int w = 0; w < NW; w++) x ^= red[w];
-        *result = ~x;
     }
 }
 
@@ -1805,16 +1816,19 @@ hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* r
     (void)hipGetLastError();
     const int nb = (int)((n + kChecksumBlock - 1) / kChecksumBlock);
     if (nb > 0) k_adler_partial<<<nb, 256, 0, s>>>(p, n, part);
-    k_checksum_final<<<1, kFinalThreads, 0, s>>>(part, nb, n, 0, result);
+    k_adler_final<<<1, kFinalThreads, 0, s>>>(part, nb, result);
     return hipGetLastError();
 }
 
 hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s)
 {
     (void)hipGetLastError();
-    const int nb = (int)((n + kCrcBlock - 1) / kCrcBlock);
-    if (nb > 0) k_crc_partial<<<nb < kCrcPartialGrid ? nb : kCrcPartialGrid, 256, 0, s>>>(p, n, part);
-    k_checksum_final<<<1, kFinalThreads, 0, s>>>(part, nb, n, 1, result);
+    const int64_t nb = (n + kCrcBlock - 1) / kCrcBlock;
+    int lm = 0;                                     // blocks per combine thread: 2^lm
+    while (((int64_t)kFinalThreads << lm) < nb) lm++;
+    if (kBlockLevel + lm + 9 >= kCrcShiftLevels) return hipErrorInvalidValue;
+    if (nb > 0) k_crc_partial<<<nb < kCrcPartialGrid ? (int)nb : kCrcPartialGrid, 256, 0, s>>>(p, n, part);
+    k_crc_final<<<1, kFinalThreads, 0, s>>>(part, (int)nb, lm, r2p_crc32_combine(0xFFFFFFFFu, 0u, (uint64_t)n), result);
     return hipGetLastError();
 }
 

@@ -143,7 +143,8 @@ def test_window_exhaustive(torch_dev, r2p, oracle, case):
     assert np.array_equal(out, oracle.filter_rows(ready, h, rb, bpp))
 
 
-@pytest.mark.parametrize("n", [0, 1, 3, 15, 16, 17, 4095, 65535, 65536, 65537, 1 << 20, (1 << 22) + 7])
+@pytest.mark.parametrize("n", [0, 1, 3, 15, 16, 17, 4095, 65535, 65536, 65537, 1 << 20, (1 << 22) + 7,
+                               (40 << 20) + 3])
 def test_checksums_vs_zlib(torch_dev, r2p, n):
     log(f"case checksum n={n}")
     rng = np.random.default_rng(n)
