@@ -39,6 +39,7 @@ struct LutKey {
 };
 
 constexpr int kLutSlot = 65536;      // bytes per table slot (the largest table)
+constexpr int kNarrowRow = 4096;     // rows up to this many bytes go to the first filter launch
 
 }  // namespace
 
@@ -73,11 +74,14 @@ struct r2p_batch {
     int n = 0;
     std::vector<size_t> out_off;
     std::vector<std::pair<LutKey, int>> luts;
+    // filter launches: rows up to kNarrowRow bytes, then wider rows, each
+    // with its own items per lane, band height and LDS size
     int band_rows = 1, row_cap = 16, lut_cap = 0;
+    int band_rows_w = 1, row_cap_w = 16;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1, g_wg = -1;
+    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1, g_wg = -1, g_row_cap_w = -1, g_band_w = -1;
     bool ran = false;
     bool timed_kernels = false;
     std::vector<r2p_image> shadow_img;
@@ -94,18 +98,20 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     FilterLaunch L{};
     L.jobs = b->d_fj; L.band_start = b->d_band_start; L.njobs = 0; L.band_rows = b->band_rows;
     L.total_bands = 0; L.row_cap = b->row_cap; L.lut_cap = b->lut_cap; L.counts = b->d_counts;
-    const size_t lds = filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap);
     if (timed) R2P_HIP(hipEventRecord(b->ev[0], s));
-    R2P_HIP(launch_filter(L, 4096, lds, s));
+    R2P_HIP(launch_filter(L, 4096, filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap), s));
+    FilterLaunch LW = L;                          // the wide rows: bands after the narrow ones
+    LW.band_rows = b->band_rows_w; LW.row_cap = b->row_cap_w; LW.counts = b->d_counts + 3;
+    R2P_HIP(launch_filter(LW, 4096, filter_lds_bytes(LW.band_rows, LW.row_cap, LW.lut_cap), s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
     // 3. deflate over all slices
     DeflateLaunch DL{};
-    DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 2;
+    DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 6;
     R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, b->opt.wg_threads, 1024, s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[2], s));
     // 4. per-image scan, 5. assemble all chunks
     AssembleLaunch AL{};
-    AL.jobs = b->d_aj; AL.chunk_start = b->d_chunk_start; AL.counts = b->d_counts + 4;
+    AL.jobs = b->d_aj; AL.chunk_start = b->d_chunk_start; AL.counts = b->d_counts + 8;
     R2P_HIP(launch_scan(AL, b->max_images, s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[3], s));
     R2P_HIP(launch_assemble(AL, 2048, s));
@@ -236,21 +242,32 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         if (!d_src[i]) { set_error("image %d has no source", i); return R2P_ERR_ARG; }
     }
     if (total_f > b->max_filtered) { set_error("batch needs %zu filtered bytes > %zu", total_f, b->max_filtered); return R2P_ERR_SIZE; }
+    // the filter's two classes (narrow rows first in the job table) and
+    // their band heights
+    std::vector<int> forder;
     {
-        int rmax = 16, lmax = 0;
+        int rn = 16, rw = 16, lmax = 0;
+        int64_t rows_n = 0, rows_w = 0;
         for (int i = 0; i < n; i++) {
-            rmax = std::max(rmax, (int)((geo[i].row_bytes + 15) & ~(size_t)15));
+            const int rc = (int)((geo[i].row_bytes + 15) & ~(size_t)15);
+            if (rc <= kNarrowRow) { rn = std::max(rn, rc); rows_n += imgs[i].height; }
+            else { rw = std::max(rw, rc); rows_w += imgs[i].height; }
             if (plan[i].lut_kind >= 0) lmax = std::max(lmax, (plan[i].lut_bytes + 15) & ~15);
         }
-        b->band_rows = filter_band_rows(o.band_rows, total_rows, rmax, lmax);
+        for (int i = 0; i < n; i++) if ((int)((geo[i].row_bytes + 15) & ~(size_t)15) <= kNarrowRow) forder.push_back(i);
+        for (int i = 0; i < n; i++) if ((int)((geo[i].row_bytes + 15) & ~(size_t)15) > kNarrowRow) forder.push_back(i);
+        b->row_cap = rn; b->row_cap_w = rw; b->lut_cap = lmax;
+        b->band_rows = filter_band_rows(o.band_rows, rows_n, rn, lmax);
+        b->band_rows_w = filter_band_rows(o.band_rows, rows_w, rw, lmax);
     }
+    std::vector<int> fslot((size_t)n);
+    for (int k = 0; k < n; k++) fslot[(size_t)forder[(size_t)k]] = k;
 
     // device tables, deduplicated by parameter set
     std::map<LutKey, int> lut_slot;
     b->luts.clear();
     size_t foff = 0, segs = 0, slices = 0, ooff = 0;
-    uint32_t bands = 0, chunks = 0;
-    int rowmax = 16, lutmax = 0;
+    uint32_t chunks = 0;
     b->out_off.assign((size_t)n, 0);
     for (int i = 0; i < n; i++) {
         const Geometry& g = geo[i];
@@ -268,18 +285,15 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
                 slot = it->second;
             }
             lut = b->d_lut + (size_t)slot * kLutSlot;
-            lutmax = std::max(lutmax, (wp.lut_bytes + 15) & ~15);
         }
-        rowmax = std::max(rowmax, (int)((g.row_bytes + 15) & ~(size_t)15));
         if (foff + g.filtered > b->max_filtered + 256 * (size_t)i) { set_error("filtered arena exhausted"); return R2P_ERR_SIZE; }
-        FilterJob& F = fj[i];
+        FilterJob& F = fj[fslot[(size_t)i]];
         F = FilterJob{};
         F.src = (const uint8_t*)d_src[i]; F.dst = b->d_filt + foff; F.prior = nullptr; F.last_row = nullptr;
         F.lut = lut; F.src_pitch = g.pitch; F.row_bytes = (int32_t)g.row_bytes; F.rows = (int32_t)imgs[i].height;
         F.bpp = g.bpp; F.wmode = wp.wmode; F.shift = wp.shift; F.mask = wp.mask; F.q15_a = wp.q15_a;
         F.q15_b = wp.q15_b; F.pixel_rep = wp.pixel_rep; F.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
-        bs[i] = bands;
-        bands += (uint32_t)((imgs[i].height + b->band_rows - 1) / b->band_rows);
+        F.band_rows = (int)((g.row_bytes + 15) & ~(size_t)15) <= kNarrowRow ? b->band_rows : b->band_rows_w;
 
         const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
         const int nseg = sg.nseg;
@@ -322,12 +336,18 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         segs += (size_t)nseg;
         slices += (size_t)sg.nslice;
     }
+    // band prefix in job-table order: the narrow class, then the wide one
+    uint32_t bands = 0, bands_n = 0;
+    for (int k = 0; k < n; k++) {
+        bs[k] = bands;
+        bands += (uint32_t)((fj[k].rows + fj[k].band_rows - 1) / fj[k].band_rows);
+        if (fj[k].row_bytes <= kNarrowRow) bands_n = bands;
+    }
     bs[n] = bands; ss[n] = (uint32_t)slices; cs[n] = chunks;
-    cnt[0] = (uint32_t)n; cnt[1] = bands;
-    cnt[2] = (uint32_t)n; cnt[3] = (uint32_t)slices;
-    cnt[4] = (uint32_t)n; cnt[5] = chunks;
-    b->row_cap = rowmax;
-    b->lut_cap = lutmax;
+    cnt[0] = (uint32_t)n; cnt[1] = bands_n; cnt[2] = 0;            // filter, narrow rows
+    cnt[3] = (uint32_t)n; cnt[4] = bands; cnt[5] = bands_n;         // filter, wide rows
+    cnt[6] = (uint32_t)n; cnt[7] = (uint32_t)slices;
+    cnt[8] = (uint32_t)n; cnt[9] = chunks;
     b->n = n;
     // build the distinct tables now (outside the graph)
     for (auto& kv : b->luts) {
@@ -357,7 +377,8 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
     // graph keyed by the launch parameters that are baked into its nodes
     const int lane = b->opt.lane_bytes;
     if (!b->gexec || b->g_row_cap != b->row_cap || b->g_lut_cap != b->lut_cap ||
-        b->g_band != b->band_rows || b->g_lane != lane || b->g_wg != b->opt.wg_threads) {
+        b->g_band != b->band_rows || b->g_lane != lane || b->g_wg != b->opt.wg_threads ||
+        b->g_row_cap_w != b->row_cap_w || b->g_band_w != b->band_rows_w) {
         if (b->gexec) { R2P_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
         if (b->graph) { R2P_HIP(hipGraphDestroy(b->graph)); b->graph = nullptr; }
         R2P_HIP(hipStreamSynchronize(s));
@@ -370,6 +391,7 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
         b->graph = g;
         R2P_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
         b->g_row_cap = b->row_cap; b->g_lut_cap = b->lut_cap; b->g_band = b->band_rows; b->g_lane = lane; b->g_wg = b->opt.wg_threads;
+        b->g_row_cap_w = b->row_cap_w; b->g_band_w = b->band_rows_w;
     }
     R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(hipGraphLaunch(b->gexec, s));

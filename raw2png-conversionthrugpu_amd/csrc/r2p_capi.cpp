@@ -453,6 +453,7 @@ static int launch_filter_job(const FilterJob& J, int band_rows, hipStream_t s)
     const uint32_t bands = (uint32_t)((J.rows + L.band_rows - 1) / L.band_rows);
     L.total_bands = bands;
     L.single = J;
+    L.single.band_rows = L.band_rows;
     const size_t lds = filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap);
     const int grid = (int)std::min<uint32_t>(bands, 8192);
     R2P_HIP(launch_filter(L, grid, lds, s));

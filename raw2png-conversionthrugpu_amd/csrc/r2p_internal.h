@@ -37,6 +37,7 @@ struct FilterJob {
     uint32_t       mask;       // mask after the shift (LUT index range - 1)
     int32_t        q15_a, q15_b, pixel_rep;
     int32_t        lut_bytes;  // size of the table in bytes (0 = none)
+    int32_t        band_rows;  // rows per band (workgroup work item) of this job
 };
 
 // Parameters of the DICOM table builder (one table per parameter set).
@@ -113,7 +114,8 @@ struct FilterLaunch {
     uint32_t          total_bands;
     int32_t           row_cap;     // LDS row capacity (bytes, multiple of 16)
     int32_t           lut_cap;     // LDS table capacity (bytes, multiple of 16)
-    const uint32_t*   counts;      // optional device [njobs, total]: overrides the two fields
+    const uint32_t*   counts;      // optional device [njobs, band_end, band_begin]: overrides njobs and
+                                   // total_bands, and the launch takes the bands [band_begin, band_end)
     FilterJob         single;
 };
 

@@ -442,8 +442,9 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
 
     int cur_job = -1;
     const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
+    const uint32_t first = L.counts ? L.counts[2] : 0u;
     const int njobs = L.counts ? (int)L.counts[0] : L.njobs;
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+    for (uint32_t g = first + blockIdx.x; g < total; g += gridDim.x) {
         int j = 0;
         if (L.jobs) {
             int lo = 0, hi = njobs - 1;
@@ -455,8 +456,8 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
         }
         const FilterJob J = L.jobs ? L.jobs[j] : L.single;
         const int band = (int)(g - (L.jobs ? L.band_start[j] : 0u));
-        const int r0 = band * L.band_rows;
-        const int nr = min(J.rows, r0 + L.band_rows) - r0;
+        const int r0 = band * J.band_rows;
+        const int nr = min(J.rows, r0 + J.band_rows) - r0;
         const int rb = J.row_bytes;
         const int nfull = rb >> 4, nitems = (rb + 15) >> 4, tail = rb & 15;
 
@@ -922,7 +923,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_adler_final(const uint32_t* p
 }
 
 // ---------------------------------------------------------------------------
-// k_deflate: one workgroup per segment (64 KiB by default).
+// k_deflate: one workgroup per slice.
 //
 // Segment layout on the wire: [BFINAL|BTYPE=01 fixed-Huffman block][EOB]
 // then, unless it is the stream's final segment, an empty stored block
@@ -930,23 +931,32 @@ __global__ __launch_bounds__(kFinalThreads) void k_adler_final(const uint32_t* p
 // therefore a byte-aligned, independently decodable piece whose history
 // (the 32 KiB before it) is already in memory, so all segments of all
 // images run in parallel and are joined by a prefix sum of their lengths.
+// A segment (64 KiB by default) is cut into slices (16 KiB by default), the
+// unit of work: each slice is one workgroup, emits a bit string, and
+// k_scan / k_assemble join a segment's slices at bit granularity.
 //
-// Inside a segment, sub-blocks of NT * 16 positions; thread t owns the 16
-// consecutive positions of its parse lane and keeps them in registers:
-//   A. every position's 4-byte hash enters T2 (first position of the
-//      sub-block with that hash, atomicMin);
+// A slice first enters its 32 KiB window into T1 (every kPrimeStride-th
+// position, atomicMax), then runs sub-blocks of NT * 16 positions; thread t
+// owns the 16 consecutive positions of its parse lane and keeps them in
+// registers:
+//   A. the positions' 4-byte hashes (kept, two per register); every
+//      kInsertStride-th position enters T2 (first position of the sub-block
+//      with that hash, atomicMin);
 //   B. candidates per position: distance 1, distance bpp and one row up
 //      (equality bitmasks from the lane's own bytes, no extra loads), and
 //      one hashed earlier position -- T2's entry when it precedes the
-//      position, else T1's (most recent position before the sub-block).
-//      The longest match wins, ties to the shorter distance; matches stop
-//      at the lane end, so each lane parses greedily on its own;
+//      position, else T1's (most recent position before the sub-block)
+//      within the window.  The longest match wins, ties to the shorter
+//      distance (a max over length << 16 | (0xFFFF - distance) keys);
+//      matches stop at the lane end, so each lane parses greedily on its own;
 //   C. per-lane bit counts, block exclusive scan, then each lane packs its
 //      tokens into 32-bit words at its bit offset (the two words shared
 //      with neighbouring lanes by LDS atomicOr) and full words stream out;
-//      the sub-block's positions then enter T1 (atomicMax) and leave T2.
+//      every kInsertStride-th position of the sub-block then enters T1.
+// Table entries carry a tag (the slice for T1, the sub-block for T2), so
+// stale entries never match and the tables are not cleared per slice.
 // The result depends only on the input bytes: every table update is an
-// order-independent min/max or a plain reset, fenced by barriers.
+// order-independent min/max, fenced by barriers.
 // ---------------------------------------------------------------------------
 
 
@@ -964,7 +974,9 @@ constexpr int kPosBits = 17;
 constexpr uint32_t kPosMask = (1u << kPosBits) - 1;
 constexpr uint32_t kTagMax = (1u << (32 - kPosBits)) - 1;
 constexpr int kPrimeStride = 8;    // window positions entered into T1 at this stride
-constexpr int kInsertStride = 2;   // the slice's own positions entered into T1 at this stride
+constexpr int kInsertStride = 2;   // the slice's own positions entered into T1 and T2, and searched
+                                   // for a hashed candidate, at this stride
+constexpr int kHashCmpDwords = 2;  // a hashed candidate is compared over at most 8 bytes
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
 
@@ -1065,8 +1077,9 @@ __device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return 
 // Best match key of every position of the lane (0 for a literal).
 // Candidates: distance 1, bpp and one row up, whose runs come from the
 // equality masks n* (bit j set where byte j differs from the byte that far
-// back, or j is past the lane's end), and one hashed earlier position -- T2's
-// entry when it precedes the position, else T1's within the window.
+// back, or j is past the lane's end), and at every kInsertStride-th position
+// one hashed earlier position -- T2's entry when it precedes the position,
+// else T1's within the window.
 // Positions go in groups of G so that a group's table reads, then its
 // candidate loads, are in flight together; a scheduling barrier between
 // groups bounds the registers.  Branch-free: an unusable hashed candidate
@@ -1093,16 +1106,16 @@ __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, cons
             const uint32_t P = (uint32_t)(lb + i);
             const bool fv = x2 < P;                                          // earlier in this sub-block
             const bool tv = P - 1u - x1 < (uint32_t)kWindow;                 // 1 <= P - x1 <= 32768
-            hv[u] = (fv | tv) & (((hmask >> i) & 1u) != 0);
+            hv[u] = (fv | tv) & (((hmask >> i) & 1u) != 0) & (i % kInsertStride == 0);
             C[u] = hv[u] ? (fv ? x2 : x1) : 0u;
         }
         uint32_t c[G][4];
 #pragma unroll
-        for (int u = 0; u < G; u++) load_words(wb + C[u], c[u], (kLane - (g0 + u) + 3) / 4);
+        for (int u = 0; u < G; u++) load_words(wb + C[u], c[u], min(kHashCmpDwords, (kLane - (g0 + u) + 3) / 4));
 #pragma unroll
         for (int u = 0; u < G; u++) {
             const int i = g0 + u;
-            const uint32_t lh = min(cmp_run(o, c[u], i, (kLane - i + 3) / 4), (uint32_t)max(le - i, 0));
+            const uint32_t lh = min(cmp_run(o, c[u], i, min(kHashCmpDwords, (kLane - i + 3) / 4)), (uint32_t)max(le - i, 0));
             const uint32_t k = max(key[i], hv[u] ? mkey(lh, (uint32_t)(lb + i) - C[u]) : 0u);
             key[i] = k >= kMinKey ? k : 0u;
         }
@@ -1243,7 +1256,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 for (int i = 0; i < LANE; i++) {
                     const uint32_t h = hash4(bytes_at(o, i));
                     hh[i >> 1] |= h << (16 * (i & 1));
-                    if ((hmask >> i) & 1u)
+                    if (((hmask >> i) & 1u) && i % kInsertStride == 0)
                         atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 }
             }

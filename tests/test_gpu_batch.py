@@ -97,3 +97,33 @@ def test_batch_redeclare_and_autotune(torch_dev, r2p, synth):
     for i, s in enumerate(ims):
         assert b.copy_result(i) == plain_png(r2p, torch, s.img, srcs[i], o)
     b.close()
+
+
+def test_batch_wide_and_narrow_rows(torch_dev, r2p):
+    """Rows on both sides of the filter's narrow/wide split (4096 bytes), in
+    interleaved order, with ragged widths: each PNG equals the plain path's."""
+    torch = torch_dev
+    log("case batch wide+narrow rows")
+    rng = np.random.default_rng(11)
+    shapes = [(1500, 6, 3, 8, r2p.PIX_READY), (300, 9, 1, 8, r2p.PIX_READY), (2100, 5, 1, 16, r2p.PIX_SWAP16),
+              (1366, 4, 3, 8, r2p.PIX_READY), (1365, 3, 3, 8, r2p.PIX_READY), (77, 13, 3, 16, r2p.PIX_SWAP16),
+              (4096, 3, 1, 8, r2p.PIX_READY), (4097, 2, 1, 8, r2p.PIX_READY), (2001, 7, 3, 16, r2p.PIX_SWAP16)]
+    imgs, srcs = [], []
+    for w, h, ch, bits, mode in shapes:
+        n = w * h * ch * (bits // 8)
+        px = (np.cumsum(rng.integers(-3, 4, n), dtype=np.int64) % 256).astype(np.uint8)
+        px[: n // 4] = rng.integers(0, 256, n // 4, dtype=np.uint8)
+        imgs.append(r2p.Image.make(w, h, ch, bits, mode))
+        srcs.append(torch.from_numpy(px).cuda())
+    opt = r2p.options()
+    fsum = sum(r2p.filtered_bytes(im) for im in imgs)
+    isum = sum(r2p.input_bytes(im) for im in imgs)
+    b = r2p.Batch(16, isum, fsum, opt)
+    b.set(imgs, [t.data_ptr() for t in srcs])
+    for graph in (False, True):
+        b.run(graph=graph)
+        b.sync()
+        for i, im in enumerate(imgs):
+            got = b.copy_result(i)
+            assert got == plain_png(r2p, torch, im, srcs[i], opt), f"image {i} ({im.width}x{im.height}) differs"
+    b.close()
