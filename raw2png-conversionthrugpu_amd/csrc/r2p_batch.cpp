@@ -17,6 +17,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
+#include <set>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -502,12 +504,6 @@ int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
 // soon as a lane frees up.  A finished lane copies its PNGs back, completes
 // its requests and returns their footprint to the budget.
 // ===========================================================================
-#include <condition_variable>
-#include <deque>
-#include <mutex>
-#include <thread>
-#include <unordered_map>
-
 namespace {
 
 using SClock = std::chrono::steady_clock;
@@ -528,23 +524,21 @@ struct Lane {
     r2p_batch* b = nullptr;
     uint8_t* d_in = nullptr;
     size_t in_cap = 0;
-    uint8_t* h_stage = nullptr;
-    size_t stage_cap = 0;
     hipEvent_t ev_run = nullptr, ev_copy = nullptr;
     int state = 0;  // 0 free, 1 running, 2 copying
     std::vector<Request*> reqs;
-    std::vector<size_t> stage_off;
     size_t footprint = 0;
 };
 
 }  // namespace
 
-// Recycled host buffers for finished PNGs: fresh allocations of this size
-// come from mmap and fault in every page on first touch, which at serving
-// rates costs more than the copy itself.
+// Recycled pinned host buffers for finished PNGs: the device copies each
+// PNG straight into its request's buffer (no staging copy on the host), and
+// pinning a fresh buffer costs far more than reusing one.
 struct BufPool {
     std::mutex mu;
     std::multimap<size_t, uint8_t*> free;      // capacity -> buffer
+    std::set<uint8_t*> pageable;               // buffers that could not be pinned
     size_t kept = 0;
     static constexpr size_t kKeep = (size_t)8 << 30;
     uint8_t* get(size_t n, size_t* cap)
@@ -563,17 +557,29 @@ struct BufPool {
         size_t c = (size_t)1 << 16;
         while (c < n) c <<= 1;
         *cap = c;
-        return new uint8_t[c];
+        uint8_t* p = nullptr;
+        if (hipHostMalloc((void**)&p, c, hipHostMallocDefault) == hipSuccess) return p;
+        // pinned memory exhausted: a pageable buffer (a slower copy, same bytes)
+        (void)hipGetLastError();
+        p = new (std::nothrow) uint8_t[c];
+        if (p) { std::lock_guard<std::mutex> lk(mu); pageable.insert(p); }
+        return p;
+    }
+    void release(uint8_t* p)            // with mu held
+    {
+        auto it = pageable.find(p);
+        if (it != pageable.end()) { pageable.erase(it); delete[] p; }
+        else (void)hipHostFree(p);
     }
     void put(uint8_t* p, size_t cap)
     {
         if (!p) return;
         std::lock_guard<std::mutex> lk(mu);
-        if (kept + cap > kKeep) { delete[] p; return; }
+        if (kept + cap > kKeep) { release(p); return; }
         free.emplace(cap, p);
         kept += cap;
     }
-    ~BufPool() { for (auto& kv : free) delete[] kv.second; }
+    ~BufPool() { for (auto& kv : free) release(kv.second); }
 };
 
 struct r2p_server {
@@ -593,8 +599,6 @@ struct r2p_server {
     uint64_t batch_images = 0;
     int dev = 0;
 };
-
-constexpr size_t kCopyThreads = 4;   // host threads copying a lane's PNGs out
 
 static size_t req_footprint(size_t in_b, size_t filt)
 {
@@ -637,20 +641,23 @@ static void lane_launch(r2p_server* s, Lane& L)
     L.state = 1;
 }
 
-static void lane_copy(Lane& L)
+// The batch is done: each PNG goes from the device straight into its
+// request's pinned buffer.
+static void lane_copy(r2p_server* s, Lane& L)
 {
     hipStream_t st = L.b->stream;
-    L.stage_off.assign(L.reqs.size(), 0);
-    size_t off = 0;
     for (size_t i = 0; i < L.reqs.size(); i++) {
+        Request* r = L.reqs[i];
         const void* d = nullptr;
         size_t len = 0;
         r2p_batch_result(L.b, (int)i, &d, &len);
-        L.stage_off[i] = off;
-        if (len && off + len <= L.stage_cap)
-            (void)hipMemcpyAsync(L.h_stage + off, d, len, hipMemcpyDeviceToHost, st);
-        L.reqs[i]->png_len = len;
-        off += (len + 255) & ~(size_t)255;
+        r->png_len = len;
+        if (!len) continue;
+        r->png = s->pool.get(len, &r->png_cap);
+        if (!r->png || hipMemcpyAsync(r->png, d, len, hipMemcpyDeviceToHost, st) != hipSuccess) {
+            r->status = R2P_ERR_HIP;
+            r->png_len = 0;
+        }
     }
     (void)hipEventRecord(L.ev_copy, st);
     L.state = 2;
@@ -658,21 +665,6 @@ static void lane_copy(Lane& L)
 
 static void lane_finish(r2p_server* s, Lane& L)
 {
-    // copy the PNGs out of the pinned stage, split over a few threads
-    const size_t n = L.reqs.size();
-    auto work = [&](size_t t, size_t nt) {
-        for (size_t i = t; i < n; i += nt) {
-            Request* r = L.reqs[i];
-            if (!r->png_len) continue;
-            r->png = s->pool.get(r->png_len, &r->png_cap);
-            memcpy(r->png, L.h_stage + L.stage_off[i], r->png_len);
-        }
-    };
-    const size_t nt = std::min<size_t>(kCopyThreads, n);
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; t++) th.emplace_back(work, t, nt);
-    work(0, std::max<size_t>(nt, 1));
-    for (auto& t : th) t.join();
     const auto now = SClock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     for (Request* r : L.reqs) { r->done_at = now; r->done = true; }
@@ -690,7 +682,7 @@ static void server_loop(r2p_server* s)
     for (;;) {
         bool progressed = false;
         for (Lane& L : s->lanes) {
-            if (L.state == 1 && hipEventQuery(L.ev_run) == hipSuccess) { lane_copy(L); progressed = true; }
+            if (L.state == 1 && hipEventQuery(L.ev_run) == hipSuccess) { lane_copy(s, L); progressed = true; }
             if (L.state == 2 && hipEventQuery(L.ev_copy) == hipSuccess) { lane_finish(s, L); progressed = true; }
         }
         // admission into free lanes
@@ -770,9 +762,7 @@ int r2p_server_create(const r2p_server_config* cfg, r2p_server** out)
         rc = r2p_batch_create(cfg->max_batch_images, in_cap, s->lane_filtered, &o, &L.b);
         if (rc) { r2p_server_destroy(s); return rc; }
         L.in_cap = in_cap;
-        L.stage_cap = L.b->max_out;
         if (hipMalloc((void**)&L.d_in, in_cap) != hipSuccess ||
-            hipHostMalloc((void**)&L.h_stage, L.stage_cap, hipHostMallocDefault) != hipSuccess ||
             hipEventCreateWithFlags(&L.ev_run, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&L.ev_copy, hipEventDisableTiming) != hipSuccess) {
             set_error("server lane allocation failed");
@@ -799,7 +789,6 @@ int r2p_server_destroy(r2p_server* s)
     for (Lane& L : s->lanes) {
         if (L.b) r2p_batch_destroy(L.b);
         if (L.d_in) (void)hipFree(L.d_in);
-        if (L.h_stage) (void)hipHostFree(L.h_stage);
         if (L.ev_run) (void)hipEventDestroy(L.ev_run);
         if (L.ev_copy) (void)hipEventDestroy(L.ev_copy);
     }
@@ -888,6 +877,8 @@ int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int
     std::vector<int64_t> ids((size_t)n, -1);
     const auto t0 = SClock::now() + std::chrono::milliseconds(5);
     int rc = R2P_OK;
+    int submitted = 0;                  // guarded by s->mu
+    bool client_failed = false;
     std::thread client([&] {
         for (int i = 0; i < n; i++) {
             const auto at = t0 + std::chrono::duration_cast<SClock::duration>(std::chrono::duration<double>(arrival_s[i]));
@@ -895,15 +886,20 @@ int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int
             int64_t id = -1;
             const int k = kind[i];
             int e = submit_at(s, &imgs[k], h_src[k], at, &id);
-            if (e) { rc = e; break; }
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (e) { rc = e; client_failed = true; s->cv_done.notify_all(); break; }
             ids[(size_t)i] = id;
+            submitted = i + 1;
+            s->cv_done.notify_all();
         }
     });
-    client.join();
-    if (rc) return rc;
+    // results are consumed (and their buffers recycled) as they complete, in
+    // arrival order, while the client keeps submitting
     SClock::time_point last = t0;
     for (int i = 0; i < n; i++) {
         std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return submitted > i || client_failed; });
+        if (submitted <= i) break;
         Request* r = s->all[ids[(size_t)i]];
         s->cv_done.wait(lk, [r] { return r->done; });
         latency_s[i] = std::chrono::duration<double>(r->done_at - r->arrival).count();
@@ -914,6 +910,7 @@ int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int
         s->pool.put(r->png, r->png_cap);
         delete r;
     }
+    client.join();
     if (wall_s) *wall_s = std::chrono::duration<double>(last - t0).count();
     return rc;
 }

@@ -861,9 +861,11 @@ __global__ __launch_bounds__(kFinalThreads) void k_crc_final(const uint32_t* par
 {
     constexpr int NW = kFinalThreads / 64;
     __shared__ uint32_t shb[4][256];
+    __shared__ uint32_t lvl[10][4][256];                   // the tree's ten shifts, from x^(8 m B) up
     __shared__ uint32_t red[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < 1024; i += kFinalThreads) (&shb[0][0])[i] = (&c_crcshift[kBlockLevel][0][0])[i];
+    for (int i = tid; i < 10 * 1024; i += kFinalThreads) (&lvl[0][0][0])[i] = (&c_crcshift[kBlockLevel + lm][0][0])[i];
     __syncthreads();
     const int m = 1 << lm;
     const int lo = min(nb, tid << lm), hi = min(nb, lo + m);
@@ -873,7 +875,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_crc_final(const uint32_t* par
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         const uint32_t right = __shfl_down(h, 1 << k, 64);
-        if ((lane & ((2 << k) - 1)) == 0) h ^= crc_shift(right, c_crcshift[kBlockLevel + lm + k]);
+        if ((lane & ((2 << k) - 1)) == 0) h ^= crc_shift(right, lvl[k]);
     }
     if (lane == 0) red[wave] = h;
     __syncthreads();
@@ -882,7 +884,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_crc_final(const uint32_t* par
 #pragma unroll
         for (int k = 6; k < 10; k++) {
             const uint32_t right = __shfl_down(h, 1 << (k - 6), 64);
-            if ((lane & ((2 << (k - 6)) - 1)) == 0) h ^= crc_shift(right, c_crcshift[kBlockLevel + lm + k]);
+            if ((lane & ((2 << (k - 6)) - 1)) == 0) h ^= crc_shift(right, lvl[k]);
         }
         if (lane == 0) *result = ~(init ^ h);
     }
