@@ -1165,6 +1165,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // T1 keeps the largest entry (atomicMax) and T2 the smallest (atomicMin),
     // so T1's tag counts up per slice and T2's counts down per sub-block.
     uint32_t tag1 = kTagMax, tag2 = 0;
+    for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;   // then kept zero by every flush
     // Slices are dealt to workgroups in order.  (Giving each of the 8 XCDs a
     // contiguous run of slices, so that the 32 KiB windows hit its L2, cut
     // the HBM reads to the input size but ran 35% slower on a radiograph:
@@ -1238,9 +1239,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
         for (int q = s0_rel; q < s1_rel; q += SUB) {
             const int n = min(SUB, s1_rel - q);
-            tag2--;
-            __syncthreads();                       // previous flush done with obuf
-            for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;
+            tag2--;                                // (obuf is all zero: the last flush cleared it)
             const int lb = q + tid * LANE;         // this lane's first position
             const int le = max(0, min(LANE, n - tid * LANE));
             uint32_t o[LANE / 4 + 1] = {0, 0, 0, 0, 0};
@@ -1333,29 +1332,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if ((hmask >> i) & 1u) atomicMax(&T1[hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
             }
             // ---- emission ------------------------------------------------------------------
-            // Every word is ORed in (obuf starts zeroed and each bit has one
-            // writer), so the words shared with the neighbouring lanes need no
-            // special case and a position without a full word ORs in zero.
+            // A lane's first word may hold the previous lane's last bits and
+            // its last partial word the next lane's first ones: those two are
+            // ORed in (obuf starts zeroed, each bit has one writer); every
+            // other word the lane completes is its own and is stored.
             {
-                uint32_t wi = off >> 5, nacc = off & 31;
+                const uint32_t w0 = off >> 5;
+                uint32_t wi = w0, nacc = off & 31;
                 uint64_t acc = 0;
 #pragma unroll
                 for (int i = 0; i < LANE; i++) {
                     acc |= (uint64_t)(tk[i] & 0x3FFFFFFu) << nacc;
                     nacc += tk[i] >> 26;
-                    const bool fl = nacc >= 32;
-                    atomicOr(&obuf[wi], fl ? (uint32_t)acc : 0u);
-                    wi += fl ? 1u : 0u;
-                    acc = fl ? acc >> 32 : acc;
-                    nacc -= fl ? 32u : 0u;
+                    if (nacc >= 32) {
+                        if (wi == w0) atomicOr(&obuf[wi], (uint32_t)acc);
+                        else obuf[wi] = (uint32_t)acc;
+                        wi++;
+                        acc >>= 32;
+                        nacc -= 32;
+                    }
                 }
                 if (nacc > 0) atomicOr(&obuf[wi], (uint32_t)acc);
             }
             __syncthreads();
             const uint32_t tb = carry_bits + tot;
             const uint32_t full = tb >> 5;
-            for (uint32_t i = tid; i < full; i += NT) out[out_dw + i] = obuf[i];
-            carry = (tb & 31) ? obuf[full] : 0u;
+            // the flush clears what it read, so the next sub-block (past its
+            // first barrier) finds obuf zero; only thread 0 keeps the carry
+            for (uint32_t i = tid; i < full; i += NT) { out[out_dw + i] = obuf[i]; obuf[i] = 0; }
+            if (tid == 0) { carry = (tb & 31) ? obuf[full] : 0u; obuf[full] = 0; }
             carry_bits = tb & 31;
             out_dw += full;
         }
@@ -1627,8 +1632,42 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             if (a0 > 0 && a0 < segbytes)
                 while ((int64_t)A.seg_off[s + 1] <= a0) s++;
             int k = s * A.spb;                         // slice cursor
+            // the usual case: all 64 bytes from one slice's token bits, read
+            // with one cursor lookup and four loads in flight
+            bool whole = e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
+                         (int64_t)A.seg_off[s + 1] >= a0 + kAsmBytes;
+            uint32_t pw = 0;
+            if (whole) {
+                pw = 8 * (uint32_t)(a0 - (int64_t)A.seg_off[s]);
+                const int k1 = min(s * A.spb + A.spb, A.nslice);
+                while (k + 1 < k1 && A.slice_boff[k + 1] <= pw) k++;
+                whole = pw + 8 * kAsmBytes <= A.slice_boff[k] + A.slice_bits[k];
+            }
+            if (whole) {
+                const uint32_t off = pw - A.slice_boff[k];
+                const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
+                u32x4 t[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) t[q] = *(R2P_GLOBAL const u32x4_ua*)(w + 4 * q);
+                const uint32_t t16 = w[16];
+                const uint32_t sh = off & 31;
+#pragma unroll
+                for (int jg = 0; jg < 4; jg++) {
+                    const uint32_t nx = jg < 3 ? t[jg + 1][0] : t16;
+                    u32x4 v;
+                    v[0] = __builtin_amdgcn_alignbit(t[jg][1], t[jg][0], sh);
+                    v[1] = __builtin_amdgcn_alignbit(t[jg][2], t[jg][1], sh);
+                    v[2] = __builtin_amdgcn_alignbit(t[jg][3], t[jg][2], sh);
+                    v[3] = __builtin_amdgcn_alignbit(nx, t[jg][3], sh);
+                    stage[wave][lane * 4 + jg] = v;
+                    r = crc_step4(r, v[0], tab);
+                    r = crc_step4(r, v[1], tab);
+                    r = crc_step4(r, v[2], tab);
+                    r = crc_step4(r, v[3], tab);
+                }
+            }
 #pragma unroll 1
-            for (int jg = 0; jg < 4; jg++) {
+            for (int jg = 0; jg < 4 && !whole; jg++) {
                 const int64_t q = b + 16 * jg;
                 if (q >= e) break;
                 const int64_t a = q - hdr;
