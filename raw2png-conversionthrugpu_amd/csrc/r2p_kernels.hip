@@ -1073,8 +1073,8 @@ __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t* c, int KW
     else c[0] = g32u(p);
 }
 
-// The 16-bit hash of position i from the packed pairs hh.
-__device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return (hh[i >> 1] >> (16 * (i & 1))) & 0xFFFFu; }
+// The hash of position i (a multiple of kInsertStride: only those are hashed).
+__device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return hh[i / kInsertStride]; }
 
 // Best match key of every position of the lane (0 for a literal).
 // Candidates: distance 1, bpp and one row up, whose runs come from the
@@ -1102,21 +1102,26 @@ __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, cons
             const int i = g0 + u;
             const uint32_t r1 = __builtin_ctz(n1 >> i), rb = __builtin_ctz(nb >> i), rr = __builtin_ctz(nr >> i);
             key[i] = max(max((r1 << 16) | (0xFFFFu - 1u), (rb << 16) | kb), (rr << 16) | kr);
+            hv[u] = false;
+            C[u] = 0;
+            if (i % kInsertStride) continue;        // (i is a constant: no code for these positions)
             const uint32_t h = hash_at(hh, i);
             const uint32_t x2 = T2[h >> (kHashBits - kHash2Bits)] ^ base2;     // position if the tag matches
             const uint32_t x1 = T1[h] ^ base1;
             const uint32_t P = (uint32_t)(lb + i);
             const bool fv = x2 < P;                                          // earlier in this sub-block
             const bool tv = P - 1u - x1 < (uint32_t)kWindow;                 // 1 <= P - x1 <= 32768
-            hv[u] = (fv | tv) & (((hmask >> i) & 1u) != 0) & (i % kInsertStride == 0);
+            hv[u] = (fv | tv) & (((hmask >> i) & 1u) != 0);
             C[u] = hv[u] ? (fv ? x2 : x1) : 0u;
         }
         uint32_t c[G][4];
 #pragma unroll
-        for (int u = 0; u < G; u++) load_words(wb + C[u], c[u], min(kHashCmpDwords, (kLane - (g0 + u) + 3) / 4));
+        for (int u = 0; u < G; u++)
+            if ((g0 + u) % kInsertStride == 0) load_words(wb + C[u], c[u], min(kHashCmpDwords, (kLane - (g0 + u) + 3) / 4));
 #pragma unroll
         for (int u = 0; u < G; u++) {
             const int i = g0 + u;
+            if (i % kInsertStride) { key[i] = key[i] >= kMinKey ? key[i] : 0u; continue; }
             const uint32_t lh = min(cmp_run(o, c[u], i, min(kHashCmpDwords, (kLane - i + 3) / 4)), (uint32_t)max(le - i, 0));
             const uint32_t k = max(key[i], hv[u] ? mkey(lh, (uint32_t)(lb + i) - C[u]) : 0u);
             key[i] = k >= kMinKey ? k : 0u;
@@ -1244,20 +1249,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int le = max(0, min(LANE, n - tid * LANE));
             uint32_t o[LANE / 4 + 1] = {0, 0, 0, 0, 0};
             if (le > 0) load_own(wb + lb, o);
-            // ---- A. hashes (kept, two per register), the sub-block's first
-            // position per hash ----------------------------------------------------
+            // ---- A. hashes of every kInsertStride-th position (kept), the
+            // sub-block's first of those positions per hash ------------------------
             // hashable positions: inside the lane, with 4 bytes before the end
             const int hl = min(le, end_rel - 3 - lb);
             const uint32_t hmask = hl <= 0 ? 0u : hl >= LANE ? 0xFFFFu : (1u << hl) - 1u;
-            uint32_t hh[LANE / 2];
+            uint32_t hh[LANE / kInsertStride];
 #pragma unroll
-            for (int i = 0; i < LANE / 2; i++) hh[i] = 0;
+            for (int i = 0; i < LANE / kInsertStride; i++) hh[i] = 0;
             if (lz && le > 0) {
 #pragma unroll
-                for (int i = 0; i < LANE; i++) {
+                for (int i = 0; i < LANE; i += kInsertStride) {
                     const uint32_t h = hash4(bytes_at(o, i));
-                    hh[i >> 1] |= h << (16 * (i & 1));
-                    if (((hmask >> i) & 1u) && i % kInsertStride == 0)
+                    hh[i / kInsertStride] = h;
+                    if ((hmask >> i) & 1u)
                         atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 }
             }
