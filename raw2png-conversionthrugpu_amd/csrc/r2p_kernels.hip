@@ -72,33 +72,8 @@ __device__ __forceinline__ uint32_t cost8(uint32_t v, uint32_t acc)
 {
     return __builtin_amdgcn_sad_u8(v ^ 0x80808080u, 0x80808080u, acc);
 }
-// Paeth predictor (PNG spec 9.4) for 4 packed bytes.  The bytes are split
-// into even/odd 16-bit lanes and processed with packed 16-bit min/max/sub:
-//   pa = |b-c|, pb = |a-c|, pc = |a+b-2c|   (all <= 510, so 16 bits suffice)
-//   a if pa <= min(pb, pc), else b if pb <= pc, else c.
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 typedef short ss2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ uint32_t paeth_lanes(uint32_t a32, uint32_t b32, uint32_t c32)
-{
-    const us2 a = __builtin_bit_cast(us2, a32), b = __builtin_bit_cast(us2, b32), c = __builtin_bit_cast(us2, c32);
-    const us2 pa = __builtin_elementwise_max(b, c) - __builtin_elementwise_min(b, c);
-    const us2 pb = __builtin_elementwise_max(a, c) - __builtin_elementwise_min(a, c);
-    const us2 s = a + b, t = c + c;
-    const us2 pc = __builtin_elementwise_max(s, t) - __builtin_elementwise_min(s, t);
-    const us2 m = __builtin_elementwise_min(pb, pc);
-    const uint32_t not_a = __builtin_bit_cast(uint32_t, (ss2)(__builtin_bit_cast(ss2, m - pa) >> 15));
-    const uint32_t take_c = __builtin_bit_cast(uint32_t, (ss2)(__builtin_bit_cast(ss2, pc - pb) >> 15));
-    const uint32_t bc = (c32 & take_c) | (b32 & ~take_c);
-    return (bc & not_a) | (a32 & ~not_a);
-}
-
-__device__ __forceinline__ uint32_t paeth4(uint32_t a, uint32_t b, uint32_t c)
-{
-    const uint32_t lo = paeth_lanes(a & 0x00FF00FFu, b & 0x00FF00FFu, c & 0x00FF00FFu);
-    const uint32_t hi = paeth_lanes((a >> 8) & 0x00FF00FFu, (b >> 8) & 0x00FF00FFu, (c >> 8) & 0x00FF00FFu);
-    return lo | (hi << 8);
-}
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
@@ -271,13 +246,58 @@ __global__ void k_build_lut(LutBuild lb, void* out)
     }
 }
 
+// The even bytes (0, 2) and the odd bytes (1, 3) of a dword as two 16-bit lanes.
+__device__ __forceinline__ us2 lanes_even(uint32_t v) { return __builtin_bit_cast(us2, v & 0x00FF00FFu); }
+__device__ __forceinline__ us2 lanes_odd(uint32_t v) { return __builtin_bit_cast(us2, __builtin_amdgcn_perm(0u, v, 0x0C030C01u)); }
+// The low bytes of the lanes of ev and od, back in byte order (ev0 od0 ev1 od1).
+__device__ __forceinline__ uint32_t join_lanes(us2 ev, us2 od)
+{
+    return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, od), __builtin_bit_cast(uint32_t, ev), 0x06020400u);
+}
+// Paeth predictor (PNG spec 9.4) on two 16-bit lanes of bytes:
+//   p - a = b - c, p - b = a - c, p - c = (b - c) + (a - c)
+//   a if |p-a| <= min(|p-b|, |p-c|), else b if |p-b| <= |p-c|, else c.
+__device__ __forceinline__ us2 paeth_pred(us2 a, us2 b, us2 c)
+{
+    const ss2 da = __builtin_bit_cast(ss2, (us2)(b - c));
+    const ss2 db = __builtin_bit_cast(ss2, (us2)(a - c));
+    const ss2 dc = da + db;
+    const ss2 pa = __builtin_elementwise_max(da, (ss2)(-da));
+    const ss2 pb = __builtin_elementwise_max(db, (ss2)(-db));
+    const ss2 pc = __builtin_elementwise_max(dc, (ss2)(-dc));
+    uint32_t na = __builtin_bit_cast(uint32_t, (ss2)((__builtin_elementwise_min(pb, pc) - pa) >> 15));
+    uint32_t tc = __builtin_bit_cast(uint32_t, (ss2)((pc - pb) >> 15));
+    // the lane masks stay masks (v_bfi_b32): as selects the compiler splits
+    // them into per-lane compares
+    asm("" : "+v"(na), "+v"(tc));
+    const uint32_t a32 = __builtin_bit_cast(uint32_t, a), b32 = __builtin_bit_cast(uint32_t, b);
+    const uint32_t bc = (__builtin_bit_cast(uint32_t, c) & tc) | (b32 & ~tc);
+    return __builtin_bit_cast(us2, (bc & na) | (a32 & ~na));
+}
+// Costs of the five filters on one dword (x; a, b, c: its left, up and
+// upper-left neighbours) added to s[0..4], the bytes outside m not counted;
+// returns the Paeth-filtered dword.
+__device__ __forceinline__ uint32_t cost_dword(uint32_t x, uint32_t a, uint32_t b, uint32_t c, uint32_t m, uint32_t* s)
+{
+    const us2 xe = lanes_even(x), xo = lanes_odd(x), ae = lanes_even(a), ao = lanes_odd(a);
+    const us2 be = lanes_even(b), bo = lanes_odd(b), ce = lanes_even(c), co = lanes_odd(c);
+    const uint32_t fp = join_lanes(xe - paeth_pred(ae, be, ce), xo - paeth_pred(ao, bo, co));
+    s[0] = cost8(x & m, s[0]);
+    s[1] = cost8(join_lanes(xe - ae, xo - ao) & m, s[1]);
+    s[2] = cost8(join_lanes(xe - be, xo - bo) & m, s[2]);
+    s[3] = cost8(join_lanes(xe - ((ae + be) >> 1), xo - ((ao + bo) >> 1)) & m, s[3]);
+    s[4] = cost8(fp & m, s[4]);
+    return fp;
+}
+
 // ---------------------------------------------------------------------------
 // k_filter: fused window + adaptive filter
 //
 // A workgroup owns a band of consecutive rows.  For each row it windows the
 // raw input into a PNG-ready row in LDS (the previous row stays resident in
-// the other LDS buffer), scores the five PNG filters on 16-byte items with
-// SWAR byte arithmetic (v_sad_u8 for the |int8| cost), reduces the scores
+// the other LDS buffer), scores the five PNG filters on 16-byte items (the
+// even and odd bytes of each dword as packed 16-bit lanes, v_sad_u8 for the
+// |int8| cost), reduces the scores
 // across the workgroup, and writes [filter byte | chosen row].  Ties go to
 // the lowest filter index, as filter_select_kernel (gpu_filter.cu:267-273).
 // ---------------------------------------------------------------------------
@@ -533,14 +553,9 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const uint32_t m = nv >= 16 ? ~0u : byte_mask(nv - 4 * q);
-                    const uint32_t x = dc[2 + q], b = dp[2 + q];
-                    const uint32_t f4 = sub8(x, paeth4(a[q], b, c[q]));
-                    f4o[q] = f4;
-                    s0 = cost8(x & m, s0);
-                    s1 = cost8(sub8(x, a[q]) & m, s1);
-                    s2 = cost8(sub8(x, b) & m, s2);
-                    s3 = cost8(sub8(x, avg8(a[q], b)) & m, s3);
-                    s4 = cost8(f4 & m, s4);
+                    uint32_t sv[5] = {s0, s1, s2, s3, s4};
+                    f4o[q] = cost_dword(dc[2 + q], a[q], dp[2 + q], c[q], m, sv);
+                    s0 = sv[0]; s1 = sv[1]; s2 = sv[2]; s3 = sv[3]; s4 = sv[4];
                 }
             };
 #pragma unroll
@@ -581,7 +596,9 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
                 for (int q = 0; q < 4; q++) {
                     const uint32_t x = dc[2 + q], b = dp[2 + q];
                     o[q] = best == 0 ? x : best == 1 ? sub8(x, a[q]) : best == 2 ? sub8(x, b)
-                         : best == 3 ? sub8(x, avg8(a[q], b)) : sub8(x, paeth4(a[q], b, c[q]));
+                         : best == 3 ? sub8(x, avg8(a[q], b))
+                         : join_lanes(lanes_even(x) - paeth_pred(lanes_even(a[q]), lanes_even(b), lanes_even(c[q])),
+                                      lanes_odd(x) - paeth_pred(lanes_odd(a[q]), lanes_odd(b), lanes_odd(c[q])));
                 }
             };
 #pragma unroll
