@@ -131,10 +131,14 @@ int r2p_encoder_sync(r2p_encoder* enc, size_t* png_len, uint32_t* adler, uint32_
 int r2p_encode_host(r2p_encoder* enc, const r2p_image* img, const void* h_src,
                     void* h_png, size_t png_cap, size_t* png_len);
 
-/* Per-kernel device timings of the last encode (ms, from HIP events). */
+/* Per-kernel device timings of the last encode (ms, from HIP events).  The
+ * events are recorded only while timing is on (r2p_encoder_set_timing; off
+ * by default): each is one more packet on the stream, about 3 us of every
+ * encode. */
 typedef struct r2p_timings {
     float filter_ms, deflate_ms, scan_ms, assemble_ms, total_ms;
 } r2p_timings;
+int r2p_encoder_set_timing(r2p_encoder* enc, int on);
 int r2p_encoder_timings(r2p_encoder* enc, r2p_timings* t);
 
 /* ======================================================================== */

@@ -112,6 +112,7 @@ def lib():
             "r2p_encode_device": ([vp, imgp, vp, vp, sz, vp], ip),
             "r2p_encoder_sync": ([vp, C.POINTER(sz), C.POINTER(u32), C.POINTER(u32)], ip),
             "r2p_encode_host": ([vp, imgp, vp, vp, sz, C.POINTER(sz)], ip),
+            "r2p_encoder_set_timing": ([vp, ip], ip),
             "r2p_encoder_timings": ([vp, C.POINTER(Timings)], ip),
             "r2p_filter_rows": ([imgp, vp, ip, vp, vp, vp, ip, vp], ip),
             "r2p_checksum_work_bytes": ([sz], sz),
@@ -192,11 +193,18 @@ def png_bound(img: Image, opt: Options | None = None) -> int:
 class Encoder:
     """Plain per-image device encoder (r2p_encoder_*)."""
 
-    def __init__(self, max_filtered_bytes: int, opt: Options | None = None):
+    def __init__(self, max_filtered_bytes: int, opt: Options | None = None, timing: bool = False):
         self.opt = opt or options()
         h = C.c_void_p()
         check(lib().r2p_encoder_create(max_filtered_bytes, C.byref(self.opt), C.byref(h)))
         self.h = h
+        if timing:
+            self.set_timing(True)
+
+    def set_timing(self, on: bool):
+        """Per-kernel HIP events on (timings()) or off (the default: each event
+        costs the stream about 3 us per encode)."""
+        check(lib().r2p_encoder_set_timing(self.h, 1 if on else 0))
 
     def close(self):
         if self.h:

@@ -537,7 +537,8 @@ struct r2p_encoder {
     size_t png_stage_cap = 0;
     LutCache lut;
     uint32_t last_nseg = 0;
-    bool timed = false;
+    bool timing = false;             // record the per-kernel events
+    bool timed = false;              // the last encode recorded them
 };
 
 static int enc_grow(uint8_t** p, size_t* cap, size_t need)
@@ -621,7 +622,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     // The deflate stage reads past the data end (kTailSlack), so it always
     // works on the encoder's own buffer; a caller's copy is made afterwards.
     uint8_t* filt = e->d_filt;
-    R2P_HIP(hipEventRecord(e->ev[0], s));
+    if (e->timing) R2P_HIP(hipEventRecord(e->ev[0], s));
     FilterJob J{};
     J.src = (const uint8_t*)d_src; J.dst = filt; J.prior = nullptr; J.last_row = nullptr;
     J.lut = e->lut.d_lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes;
@@ -629,7 +630,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     J.mask = wp.mask; J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
     J.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
     if ((rc = launch_filter_job(J, o.band_rows, s))) return rc;
-    R2P_HIP(hipEventRecord(e->ev[1], s));
+    if (e->timing) R2P_HIP(hipEventRecord(e->ev[1], s));
     if (d_filtered_out)
         R2P_HIP(hipMemcpyAsync(d_filtered_out, filt, g.filtered, hipMemcpyDeviceToDevice, s));
 
@@ -645,7 +646,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     D.lz77 = o.lz77; D.nslice = sg.nslice;
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
-    R2P_HIP(hipEventRecord(e->ev[2], s));
+    if (e->timing) R2P_HIP(hipEventRecord(e->ev[2], s));
 
     AssembleLaunch AL{};
     AssembleJob& A = AL.single;
@@ -666,12 +667,12 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     A.running = nullptr; A.base_slot = nullptr;
     AL.jobs = nullptr; AL.njobs = 1; AL.total_chunks = (uint32_t)A.max_chunks;
     R2P_HIP(launch_scan(AL, 1, s));
-    R2P_HIP(hipEventRecord(e->ev[3], s));
+    if (e->timing) R2P_HIP(hipEventRecord(e->ev[3], s));
     R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
-    R2P_HIP(hipEventRecord(e->ev[4], s));
+    if (e->timing) R2P_HIP(hipEventRecord(e->ev[4], s));
     R2P_HIP(hipMemcpyAsync(e->h_misc, e->d_misc, 16, hipMemcpyDeviceToHost, s));
     e->last_nseg = (uint32_t)nseg;
-    e->timed = true;
+    e->timed = e->timing;
     return R2P_OK;
 }
 
@@ -706,9 +707,16 @@ int r2p_encode_host(r2p_encoder* e, const r2p_image* img, const void* h_src, voi
     return R2P_OK;
 }
 
+int r2p_encoder_set_timing(r2p_encoder* e, int on)
+{
+    if (!e) { set_error("null encoder"); return R2P_ERR_ARG; }
+    e->timing = on != 0;
+    return R2P_OK;
+}
+
 int r2p_encoder_timings(r2p_encoder* e, r2p_timings* t)
 {
-    if (!e || !t || !e->timed) { set_error("no timings"); return R2P_ERR_STATE; }
+    if (!e || !t || !e->timed) { set_error("no timings: the last encode ran with timing off"); return R2P_ERR_STATE; }
     R2P_HIP(hipEventSynchronize(e->ev[4]));
     R2P_HIP(hipEventElapsedTime(&t->filter_ms, e->ev[0], e->ev[1]));
     R2P_HIP(hipEventElapsedTime(&t->deflate_ms, e->ev[1], e->ev[2]));

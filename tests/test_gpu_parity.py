@@ -165,7 +165,7 @@ def test_checksums_vs_zlib(torch_dev, r2p, n):
 def encode_device(torch, r2p, px, img, opt=None):
     opt = opt or r2p.options()
     fb = r2p.filtered_bytes(img)
-    enc = r2p.Encoder(fb, opt)
+    enc = r2p.Encoder(fb, opt, timing=True)
     d_src = dev(torch, px)
     cap = r2p.png_bound(img, opt)
     d_png = empty(torch, cap)
