@@ -666,9 +666,11 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     A.max_chunks = (int32_t)((zmax + o.idat_chunk_bytes - 1) / o.idat_chunk_bytes);
     A.running = nullptr; A.base_slot = nullptr;
     AL.jobs = nullptr; AL.njobs = 1; AL.total_chunks = (uint32_t)A.max_chunks;
-    R2P_HIP(launch_scan(AL, 1, s));
+    const bool self_scan = assemble_scans_itself(A);
+    if (!self_scan) R2P_HIP(launch_scan(AL, 1, s));
     if (e->timing) R2P_HIP(hipEventRecord(e->ev[3], s));
-    R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
+    if (self_scan) R2P_HIP(launch_assemble_self_scan(AL, std::min(A.max_chunks, 4096), s));
+    else R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
     if (e->timing) R2P_HIP(hipEventRecord(e->ev[4], s));
     R2P_HIP(hipMemcpyAsync(e->h_misc, e->d_misc, 16, hipMemcpyDeviceToHost, s));
     e->last_nseg = (uint32_t)nseg;

@@ -31,6 +31,10 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s);
 hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s);
 hipError_t launch_assemble(const AssembleLaunch& AL, int grid, hipStream_t s);
+// A single job small enough for k_assemble to scan it itself (no k_scan
+// launch; the scan arrays stay in LDS): launch_assemble_self_scan.
+bool assemble_scans_itself(const AssembleJob& A);
+hipError_t launch_assemble_self_scan(const AssembleLaunch& AL, int grid, hipStream_t s);
 hipError_t launch_adler(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s);
 hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* result, hipStream_t s);
 hipError_t launch_put_bytes(uint8_t* dst, const uint8_t* bytes16, int n, const uint32_t* crc_be_src,

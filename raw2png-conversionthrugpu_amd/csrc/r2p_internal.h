@@ -140,6 +140,8 @@ struct AssembleLaunch {
 constexpr int kFilterThreads = 256;
 constexpr int kDeflateThreads = 256;
 constexpr int kAssembleThreads = 1024;
+constexpr int kAsmScanSlices = 2048;  // a single job of up to this many slices and
+constexpr int kAsmScanSegs = 1024;    // segments is scanned by k_assemble itself
 constexpr int kCrcShiftLevels = 29;       // CRC shift tables for 64 * 2^k zero bytes, k < 29
 constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
 constexpr int kFilterLdsMax = 160 * 1024 - 256;  // LDS one filter workgroup may use

@@ -1412,6 +1412,37 @@ __device__ __forceinline__ int64_t png_payload(const AssembleJob& A, uint64_t se
 {
     return A.mode == 1 ? (int64_t)segbytes + 6 : (int64_t)segbytes;
 }
+// Bytes written for a payload of z bytes: mode 1 adds the signature, IHDR,
+// the 12 bytes of every IDAT chunk and IEND.
+__device__ __forceinline__ uint64_t out_total(const AssembleJob& A, int64_t z)
+{
+    if (A.mode != 1) return (uint64_t)z;
+    const int64_t nch = (z + A.chunk_bytes - 1) / A.chunk_bytes;
+    return (uint64_t)(8 + 25 + nch * 12 + z + 12);
+}
+
+// Byte length of a segment with tb token bits (block header included): EOB
+// (7 bits), and unless it is the stream's last, the 3-bit stored-block
+// header, the pad to a byte and 00 00 FF FF.
+__device__ __forceinline__ uint32_t seg_len_bytes(uint32_t tb, bool last)
+{
+    return (tb + 7 + (last ? 0u : 3u) + 7) / 8 + (last ? 0u : 4u);
+}
+
+// Slice i's terms of the whole input's Adler-32 (sum form, mod 65521): for
+// a slice with sums A_i (A_i - 1 = am) and B_i and R bytes after it,
+// A = 1 + sum(am) and B = sum(B_i + am * R).
+__device__ __forceinline__ void adler_terms(const AssembleJob& A, int i, uint32_t& sa, uint32_t& sb)
+{
+    constexpr uint32_t M = 65521;
+    const int64_t end = min64((int64_t)(i + 1) * A.slice_bytes, A.total_in);
+    const uint32_t R = (uint32_t)((uint64_t)(A.total_in - end) % M);
+    const uint32_t a = A.slice_adler[i];
+    const uint32_t am = (a & 0xFFFFu) ? (a & 0xFFFFu) - 1u : M - 1u;
+    sa += am;
+    sa = sa >= M ? sa - M : sa;
+    sb = (sb + (a >> 16) + (am * R) % M) % M;
+}
 
 __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
 {
@@ -1439,8 +1470,7 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
                     if (kb + u < k1) { A.slice_boff[kb + u] = tb; tb += bits[u]; }
             }
             A.seg_tbits[i] = tb;
-            const bool last = (i == A.nseg - 1) && A.final_block;
-            v = (tb + 7 + (last ? 0u : 3u) + 7) / 8 + (last ? 0u : 4u);
+            v = seg_len_bytes(tb, (i == A.nseg - 1) && A.final_block);
         }
         const uint32_t incl = wave_incl_scan(v);
         __syncthreads();
@@ -1457,15 +1487,7 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
     // slice is one independent term
     constexpr uint32_t M = 65521;
     uint32_t sa = 0, sb = 0;
-    for (int i = tid; i < A.nslice; i += 256) {
-        const int64_t end = min64((int64_t)(i + 1) * A.slice_bytes, A.total_in);
-        const uint32_t R = (uint32_t)((uint64_t)(A.total_in - end) % M);
-        const uint32_t a = A.slice_adler[i];
-        const uint32_t am = (a & 0xFFFFu) ? (a & 0xFFFFu) - 1u : M - 1u;
-        sa += am;
-        sa = sa >= M ? sa - M : sa;
-        sb = (sb + (a >> 16) + (am * R) % M) % M;
-    }
+    for (int i = tid; i < A.nslice; i += 256) adler_terms(A, i, sa, sb);
     sa = wave_total(sa) % M;                       // 64 terms < M: no overflow
     sb = wave_total(sb) % M;
     if (lane == 0) { ared[wave] = sa; ared[4 + wave] = sb; }
@@ -1482,12 +1504,7 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
             *A.base_slot = b0;
             *A.running = b0 + (uint64_t)z;
         }
-        int64_t total = z;
-        if (A.mode == 1) {
-            const int64_t nch = (z + A.chunk_bytes - 1) / A.chunk_bytes;
-            total = 8 + 25 + nch * 12 + z + 12;
-        }
-        *A.out_len = (uint64_t)total;
+        *A.out_len = out_total(A, z);
     }
 }
 
@@ -1606,6 +1623,12 @@ __device__ int wave_find_seg(const uint64_t* seg_off, int nseg, int64_t t)
 }
 
 
+// SELF: one job of at most kAsmScanSlices slices and kAsmScanSegs segments,
+// whose scan every workgroup computes for itself into LDS (k_scan's work,
+// without its launch): slice bit counts and offsets, segment token bits and
+// byte offsets, and the Adler-32; the workgroup of the first chunk stores
+// the Adler-32 and the output length.
+template <bool SELF>
 __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL)
 {
     __shared__ uint32_t tab[4][256];
@@ -1614,6 +1637,54 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t (*shf)[4][256] = c_crcshift;     // read through the caches: ~4 lookups per level
     load_crc_tables(tab);
+
+    AssembleJob S = AL.single;                     // SELF: the job, its scan arrays in LDS
+    if constexpr (SELF) {
+        __shared__ uint32_t s_bits[kAsmScanSlices], s_boff[kAsmScanSlices], s_tbits[kAsmScanSegs];
+        __shared__ uint64_t s_off[kAsmScanSegs + 1];
+        __shared__ uint64_t s_wsum[kAssembleThreads / 64];
+        __shared__ uint32_t s_ared[2 * (kAssembleThreads / 64)], s_adler;
+        constexpr uint32_t M = 65521;
+        const int i = tid;                         // segment i (nseg <= kAsmScanSegs == threads)
+        uint32_t v = 0, sa = 0, sb = 0;
+        if (i < S.nseg) {
+            const int k0 = i * S.spb, k1 = min(k0 + S.spb, S.nslice);
+            uint32_t tb = 0;
+            for (int kb = k0; kb < k1; kb += 8) {          // eight loads in flight
+                uint32_t bits[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) bits[u] = kb + u < k1 ? S.slice_bits[kb + u] : 0u;
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (kb + u < k1) { s_bits[kb + u] = bits[u]; s_boff[kb + u] = tb; tb += bits[u]; }
+            }
+            s_tbits[i] = tb;
+            v = seg_len_bytes(tb, (i == S.nseg - 1) && S.final_block);
+        }
+        for (int k = tid; k < S.nslice; k += kAssembleThreads) adler_terms(S, k, sa, sb);
+        const uint32_t incl = wave_incl_scan(v);
+        sa = wave_total(sa) % M;
+        sb = wave_total(sb) % M;
+        if (lane == 63) s_wsum[wave] = incl;
+        if (lane == 0) { s_ared[wave] = sa; s_ared[kAssembleThreads / 64 + wave] = sb; }
+        __syncthreads();
+        uint64_t wb = 0, tot = 0;
+        for (int w = 0; w < kAssembleThreads / 64; w++) { if (w < wave) wb += s_wsum[w]; tot += s_wsum[w]; }
+        if (i < S.nseg) s_off[i] = wb + incl - v;
+        if (tid == 0) {
+            s_off[S.nseg] = tot;
+            uint32_t aa = 1u, ab = 0u;
+            for (int w = 0; w < kAssembleThreads / 64; w++) { aa += s_ared[w]; ab += s_ared[kAssembleThreads / 64 + w]; }
+            s_adler = S.total_in ? ((ab % M) << 16) | (aa % M) : 1u;
+            if (blockIdx.x == 0) {
+                if (S.adler_out) *S.adler_out = s_adler;
+                *S.out_len = out_total(S, png_payload(S, tot));
+            }
+        }
+        __syncthreads();
+        S.slice_bits = s_bits; S.slice_boff = s_boff; S.seg_tbits = s_tbits; S.seg_off = s_off;
+        S.adler_out = &s_adler;
+    }
 
     const uint32_t total = AL.counts ? AL.counts[1] : AL.total_chunks;
     const int njobs = AL.counts ? (int)AL.counts[0] : AL.njobs;
@@ -1627,7 +1698,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             }
             j = lo;
         }
-        const AssembleJob A = AL.jobs ? AL.jobs[j] : AL.single;
+        const AssembleJob A = SELF ? S : AL.jobs ? AL.jobs[j] : AL.single;
         const int c = (int)(g - (AL.jobs ? AL.chunk_start[j] : 0u));
         const int64_t segbytes = (int64_t)A.seg_off[A.nseg];
         const int64_t z = png_payload(A, (uint64_t)segbytes);
@@ -1883,7 +1954,19 @@ hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
 hipError_t launch_assemble(const AssembleLaunch& AL, int grid, hipStream_t s)
 {
     (void)hipGetLastError();
-    k_assemble<<<grid, kAssembleThreads, 0, s>>>(AL);
+    k_assemble<false><<<grid, kAssembleThreads, 0, s>>>(AL);
+    return hipGetLastError();
+}
+
+bool assemble_scans_itself(const AssembleJob& A)
+{
+    return A.nslice <= kAsmScanSlices && A.nseg <= kAsmScanSegs && !A.running;
+}
+
+hipError_t launch_assemble_self_scan(const AssembleLaunch& AL, int grid, hipStream_t s)
+{
+    (void)hipGetLastError();
+    k_assemble<true><<<grid, kAssembleThreads, 0, s>>>(AL);
     return hipGetLastError();
 }
 

@@ -246,6 +246,9 @@ def test_encode_segment_and_chunk_sizes(torch_dev, r2p, oracle, seg, chunk):
     (8192, 4096, 2340, 7),         # three segments, the last one a 3-byte slice
     (65536, 16384, 1500, 300),     # segments of four slices, ragged last segment
     (65536, 65536, 1500, 300),     # one slice per segment
+    (65536, 4096, 4096, 2047),     # 2048 slices: the most the assembly scans itself
+    (65536, 4096, 4000, 2200),     # 2149 slices: scanned by its own launch
+    (4096, 4096, 3000, 2000),      # 1466 one-slice segments: over the self-scan's 1024
 ])
 def test_encode_slices(torch_dev, r2p, oracle, seg, slc, w, h):
     """Slices are a work split only: any slice size inflates to the same bytes,
