@@ -295,9 +295,11 @@ size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap)
 
 int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap)
 {
-    // from the row count: about 768 bands (three workgroups on each of 256
-    // CUs), at least one row per wave of the workgroup, at most 16
-    int h = requested > 0 ? requested : (int)std::max<int64_t>(4, std::min<int64_t>(16, total_rows / 768));
+    // default: one row per wave of the workgroup -- the band's rows are
+    // filtered in one pass, and the small LDS footprint keeps several
+    // workgroups per CU (on the mixed batch, 8 and 16 rows measured slower)
+    (void)total_rows;
+    int h = requested > 0 ? requested : kFilterThreads / 64;
     // the band and the row above it must fit the LDS beside the table
     while (h > 1 && filter_lds_bytes(h, row_cap, lut_cap) > (size_t)kFilterLdsMax) h--;
     return h;
