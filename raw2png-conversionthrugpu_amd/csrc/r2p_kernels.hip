@@ -629,6 +629,8 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
 // then A = 1 + S, B = n + n*S - T  (mod 65521), B = n + sum((n-i) x_i).
 // Returns the adler value on every thread.  `red` needs 2*nwaves u32.
 // ---------------------------------------------------------------------------
+__device__ uint32_t adler_finish(uint32_t S, uint32_t t32, int64_t n, uint32_t* red);
+
 __device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
@@ -664,6 +666,16 @@ __device__ uint32_t block_adler(const uint8_t* p, int64_t n, uint32_t* red)
     // S < 2^32 (n <= 64 KiB); T mod M from its 32-bit halves (2^32 = 225 mod M)
     constexpr uint32_t M = 65521;
     const uint32_t t32 = ((((uint32_t)(T >> 32)) % M) * 225u + ((uint32_t)T) % M) % M;
+    return adler_finish(S, t32, n, red);
+}
+
+// The Adler-32 of n bytes from every thread's terms: S (its bytes' sum,
+// < 2^26) and T (the sum of offset * byte, mod 65521).  Returns the value
+// on every thread.  `red` needs 2*nwaves u32.
+__device__ uint32_t adler_finish(uint32_t S, uint32_t t32, int64_t n, uint32_t* red)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr uint32_t M = 65521;
     const uint32_t Sw = wave_total(S) % M;         // 64 lanes: no overflow
     const uint32_t Tw = wave_total(t32) % M;
     __syncthreads();
@@ -1240,18 +1252,32 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // positions P = 0 mod kPrimeStride, 8 per lane and pass
             static_assert(kPrimeStride == 8, "every other dword of 64 bytes");
             constexpr int PL = 64;
-            for (int b = tid * PL; b < s0_rel; b += NT * PL) {
-                // 64 bytes from b (in bounds: the slice and the tail slack follow)
-                u32x4 v[4];
+            constexpr int NP = (kWindow + NT * PL - 1) / (NT * PL);   // passes over the window
+            for (int b0 = tid * PL; b0 < s0_rel; b0 += NP * NT * PL) {
+                // 64 bytes from each pass's b (in bounds when b < s0_rel: the
+                // slice and the tail slack follow), all loads in flight at once
+                u32x4 v[NP][4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) v[k] = g128u(wb + b + 16 * k);
+                for (int p = 0; p < NP; p++) {
+                    const int b = b0 + p * NT * PL;
 #pragma unroll
-                for (int k = 0; k < 16; k += 2) {
-                    const int P = b + 4 * k;
-                    if (P < s0_rel && P + 4 <= end_rel) atomicMax(&T1[hash4(v[k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
+                    for (int k = 0; k < 4; k++) v[p][k] = b < s0_rel ? g128u(wb + b + 16 * k) : (u32x4){0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+#pragma unroll
+                    for (int k = 0; k < 16; k += 2) {
+                        const int P = b0 + p * NT * PL + 4 * k;
+                        if (P < s0_rel && P + 4 <= end_rel)
+                            atomicMax(&T1[hash4(v[p][k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
+                    }
                 }
             }
         }
+
+        // this thread's terms of the slice's Adler-32: its bytes' sum and
+        // sum of (offset in the slice) * byte, mod 65521
+        uint32_t ad_s = 0, ad_t = 0;
 
         // the segment's first slice opens its block: BFINAL | BTYPE=01 << 1
         const bool opens = sl % spb == 0;
@@ -1265,7 +1291,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int lb = q + tid * LANE;         // this lane's first position
             const int le = max(0, min(LANE, n - tid * LANE));
             uint32_t o[LANE / 4 + 1] = {0, 0, 0, 0, 0};
-            if (le > 0) load_own(wb + lb, o);
+            if (le > 0) {
+                load_own(wb + lb, o);
+                uint32_t s16 = 0, w16 = 0;
+#pragma unroll
+                for (int k = 0; k < LANE / 4; k++) {
+                    const uint32_t v = le >= LANE ? o[k] : o[k] & byte_mask(le - 4 * k);
+                    const uint32_t sk = __builtin_amdgcn_udot4(v, 0x01010101u, 0u, false);
+                    w16 = __builtin_amdgcn_udot4(v, 0x03020100u, w16, false) + (uint32_t)(4 * k) * sk;
+                    s16 += sk;
+                }
+                ad_s += s16;                           // < 16 * 4080 per slice
+                ad_t = (ad_t + (uint32_t)(lb - s0_rel) * s16 + w16) % 65521u;   // < 2^29 before the mod
+            }
             // ---- A. hashes of every kInsertStride-th position (kept), the
             // sub-block's first of those positions per hash ------------------------
             // hashable positions: inside the lane, with 4 bytes before the end
@@ -1397,7 +1435,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 ss[0] = st[0]; ss[1] = st[1]; ss[2] = st[2]; ss[3] = st[3];
             }
         }
-        const uint32_t ad = block_adler(d + s0, s1 - s0, red);
+        const uint32_t ad = adler_finish(ad_s, ad_t, s1 - s0, red);
         if (tid == 0) J.slice_adler[sl] = ad;
     }
 }
