@@ -507,8 +507,8 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
             // full items of buffers b_first..nr, flattened over the rows; the
             // raw loads of WU items are issued together (global latency)
             const int nbuf = nr + 1 - b_first;
-            if (nfull > 0) {
-                constexpr int WU = 2;
+            if (nfull > 0 && narrows(J.wmode)) {
+                constexpr int WU = 2;                  // 32 raw bytes per item
                 const int tot = nbuf * nfull;
                 for (int t0 = tid; t0 < tot; t0 += WU * kFilterThreads) {
                     RawItem raw[WU];
@@ -523,6 +523,26 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
                     for (int u = 0; u < WU; u++)
                         if (t0 + u * kFilterThreads < tot)
                             *(u32x4*)(bufs + (b_first + bi[u]) * rbuf + 16 + 16 * ii[u]) = window_raw(J, raw[u], lut_lds);
+                }
+            } else if (nfull > 0) {
+                // 16 raw bytes per item: twice as many in flight (but for K = 6,
+                // whose registers are at the 4-waves-per-SIMD limit)
+                constexpr int WU = K == 6 ? 2 : 4;
+                const int tot = nbuf * nfull;
+                for (int t0 = tid; t0 < tot; t0 += WU * kFilterThreads) {
+                    u32x4 raw[WU];
+                    int bi[WU], ii[WU];
+#pragma unroll
+                    for (int u = 0; u < WU; u++) {
+                        const int t = min(t0 + u * kFilterThreads, tot - 1);
+                        bi[u] = t / nfull; ii[u] = t - bi[u] * nfull;
+                        raw[u] = g128u(J.src + (int64_t)(r0 - 1 + b_first + bi[u]) * J.src_pitch + 16 * ii[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < WU; u++)
+                        if (t0 + u * kFilterThreads < tot)
+                            *(u32x4*)(bufs + (b_first + bi[u]) * rbuf + 16 + 16 * ii[u]) =
+                                window_raw(J, RawItem{raw[u], raw[u]}, lut_lds);
                 }
             }
             for (int t = tid; t < nbuf * tail; t += blockDim.x) {
