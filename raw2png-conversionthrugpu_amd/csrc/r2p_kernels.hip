@@ -1138,7 +1138,7 @@ __device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return 
 template <int G>
 __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, const uint32_t* o, const uint32_t* hh,
                                           int lb, int le, uint32_t hmask, uint32_t n1, uint32_t nb, uint32_t nr,
-                                          uint32_t kb, uint32_t kr, const uint32_t* T1, const uint32_t* T2,
+                                          uint32_t kb, uint32_t kr, const uint32_t* T,
                                           uint32_t base1, uint32_t base2)
 {
 #pragma unroll
@@ -1155,8 +1155,9 @@ __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, cons
             C[u] = 0;
             if (i % kInsertStride) continue;        // (i is a constant: no code for these positions)
             const uint32_t h = hash_at(hh, i);
-            const uint32_t x2 = T2[h >> (kHashBits - kHash2Bits)] ^ base2;     // position if the tag matches
-            const uint32_t x1 = T1[h] ^ base1;
+            const u32x2 e = *(const u32x2*)(T + 2 * h);                       // T1 and T2 entries, one read
+            const uint32_t x2 = e[1] ^ base2;                                // position if the tag matches
+            const uint32_t x1 = e[0] ^ base1;
             const uint32_t P = (uint32_t)(lb + i);
             const bool fv = x2 < P;                                          // earlier in this sub-block
             const bool tv = P - 1u - x1 < (uint32_t)kWindow;                 // 1 <= P - x1 <= 32768
@@ -1207,8 +1208,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int SUB = NT * LANE;               // positions per sub-block
     constexpr int OBUF = (SUB * 9 + 31) / 32 + 8;
     constexpr int NW = NT / 64;
-    __shared__ uint32_t T1[1 << kHashBits];      // most recent position before the sub-block (tag: slice)
-    __shared__ uint32_t T2[1 << kHash2Bits];     // first position inside the sub-block (tag: ~sub-block)
+    // per hash h: T[2h] "T1", the most recent position before the sub-block
+    // (tag: slice), and T[2h + 1] "T2", the first position inside the
+    // sub-block (tag: ~sub-block) -- side by side, so a lookup is one read
+    static_assert(kHash2Bits == kHashBits, "T1 and T2 share the index");
+    __shared__ __attribute__((aligned(8))) uint32_t T[2 << kHashBits];
     __shared__ uint32_t obuf[OBUF];
     __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
@@ -1259,8 +1263,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // a new slice: fresh tags (both tables cleared when they run out; a
         // slice has at most 64 KiB / SUB sub-blocks)
         if (tag1 >= kTagMax || tag2 <= (uint32_t)(65536 / SUB) + 1) {
-            for (int i = tid; i < (1 << kHashBits); i += NT) T1[i] = 0;
-            for (int i = tid; i < (1 << kHash2Bits); i += NT) T2[i] = ~0u;
+            for (int i = tid; i < (1 << kHashBits); i += NT) { T[2 * i] = 0; T[2 * i + 1] = ~0u; }
             tag1 = 0;
             tag2 = kTagMax;
         }
@@ -1289,7 +1292,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     for (int k = 0; k < 16; k += 2) {
                         const int P = b0 + p * NT * PL + 4 * k;
                         if (P < s0_rel && P + 4 <= end_rel)
-                            atomicMax(&T1[hash4(v[p][k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
+                            atomicMax(&T[2 * hash4(v[p][k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
                     }
                 }
             }
@@ -1338,7 +1341,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const uint32_t h = hash4(bytes_at(o, i));
                     hh[i / kInsertStride] = h;
                     if ((hmask >> i) & 1u)
-                        atomicMin(&T2[h >> (kHashBits - kHash2Bits)], (tag2 << kPosBits) | (uint32_t)(lb + i));
+                        atomicMin(&T[2 * h + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 }
             }
             __syncthreads();
@@ -1370,7 +1373,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const uint32_t cap = le >= LANE ? 0u : ~0u << le;
                 lane_keys<R2P_MATCH_GROUP>(key, wb, o, hh, lb, le, hmask, ~m1 | cap, ~mb | cap, ~mr | cap,
                                            0xFFFFu - (uint32_t)(bpp > 1 ? bpp : 1), 0xFFFFu - (uint32_t)(row_ok ? dr : 1),
-                                           T1, T2, tag1 << kPosBits, tag2 << kPosBits);
+                                           T, tag1 << kPosBits, tag2 << kPosBits);
             }
             // greedy parse: each token's code (code | nbits << 26, 0 where no
             // token starts) and the lane's bit count
@@ -1409,7 +1412,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (lz && le > 0) {
 #pragma unroll
                 for (int i = 0; i < LANE; i += kInsertStride)
-                    if ((hmask >> i) & 1u) atomicMax(&T1[hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                    if ((hmask >> i) & 1u) atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
             }
             // ---- emission ------------------------------------------------------------------
             // A lane's first word may hold the previous lane's last bits and
