@@ -103,6 +103,9 @@ typedef struct r2p_options {
     int slice_bytes;       /* DEFLATE work unit: one workgroup per slice of a segment (a power
                               of two >= 4096 dividing segment_bytes); 0 = auto (16384).
                               Changes speed only, never the decoded output */
+    int huffman;           /* 0: fixed-Huffman blocks (the reference's, gpu_deflate_backend.h:3);
+                              1: each segment a dynamic-Huffman block where that is smaller
+                              (plain encoder only; the batched and served forms reject it) */
 } r2p_options;
 
 void r2p_options_default(r2p_options* o);

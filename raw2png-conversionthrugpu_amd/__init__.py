@@ -67,7 +67,7 @@ class Options(C.Structure):
     _fields_ = [
         ("segment_bytes", C.c_int), ("lz77", C.c_int), ("idat_chunk_bytes", C.c_int),
         ("zlib_level", C.c_int), ("band_rows", C.c_int), ("lane_bytes", C.c_int),
-        ("wg_threads", C.c_int), ("slice_bytes", C.c_int),
+        ("wg_threads", C.c_int), ("slice_bytes", C.c_int), ("huffman", C.c_int),
     ]
 
 

@@ -136,6 +136,7 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     r2p_batch* b = new r2p_batch();
     if (opt) b->opt = *opt; else r2p_options_default(&b->opt);
     fill_defaults(&b->opt);
+    if (b->opt.huffman) { delete b; set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
     b->max_images = max_images;
     b->max_input = max_input_bytes;
     b->max_filtered = max_filtered_bytes;
@@ -208,6 +209,7 @@ int r2p_batch_options(r2p_batch* b, const r2p_options* opt)
     if (!b || !opt) { set_error("bad options"); return R2P_ERR_ARG; }
     r2p_options o = *opt;
     fill_defaults(&o);
+    if (o.huffman) { set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
     b->opt = o;
     return R2P_OK;
 }
@@ -751,6 +753,7 @@ int r2p_server_create(const r2p_server_config* cfg, r2p_server** out)
     s->cfg = *cfg;
     s->dev = dev;
     fill_defaults(&s->cfg.opt);
+    if (s->cfg.opt.huffman) { delete s; set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
     const int nl = cfg->inflight_batches;
     const size_t lane_budget = cfg->hbm_budget_bytes / (size_t)nl;
     // per lane: input arena 1/4, filtered 1/4 of what remains after the input

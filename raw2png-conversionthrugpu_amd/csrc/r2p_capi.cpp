@@ -321,6 +321,7 @@ void fill_defaults(r2p_options* o)
     o->lane_bytes = 16;
     if (o->wg_threads != 256 && o->wg_threads != 512) o->wg_threads = 256;
     if (o->lz77 < 0) o->lz77 = 1;
+    o->huffman = o->huffman ? 1 : 0;
 }
 
 }  // namespace r2p
@@ -406,6 +407,7 @@ void r2p_options_default(r2p_options* o)
     o->lane_bytes = 16;
     o->wg_threads = 256;
     o->slice_bytes = 16384;
+    o->huffman = 0;
 }
 
 uint32_t r2p_adler32_combine(uint32_t a1, uint32_t a2, uint64_t len2)
@@ -531,6 +533,12 @@ struct r2p_encoder {
     uint32_t* d_slice_boff = nullptr;
     uint32_t* d_seg_tbits = nullptr;
     uint64_t* d_seg_off = nullptr;
+    uint32_t* d_tokens = nullptr;    // dynamic-Huffman mode: tokens, counts, histograms,
+    uint32_t* d_tok_count = nullptr; // per-segment codes and block headers
+    uint32_t* d_hist = nullptr;
+    uint32_t* d_codes = nullptr;
+    uint32_t* d_hdr = nullptr;
+    uint32_t* d_hdr_bits = nullptr;
     uint64_t* d_misc = nullptr;      // [0] png length, [1] adler
     uint64_t* h_misc = nullptr;      // pinned mirror
     uint8_t* d_src_stage = nullptr;  // encode_host staging
@@ -566,7 +574,11 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
     e->max_filtered = max_filtered_bytes;
     e->max_seg = (int)((max_filtered_bytes + e->opt.segment_bytes - 1) / e->opt.segment_bytes);
     e->max_slice = (int)((max_filtered_bytes + e->opt.slice_bytes - 1) / e->opt.slice_bytes) + 1;
-    e->slice_cap_dw = slice_cap_dwords(e->opt.slice_bytes);
+    // a dynamic-Huffman slice may take up to 16 bits per byte (15-bit
+    // literal codes; a match of 3 or more bytes is at most 48 bits) plus the
+    // segment's header
+    e->slice_cap_dw = e->opt.huffman ? (16 * e->opt.slice_bytes + 32 * kHuffHdrWords + 31) / 32 + 8
+                                     : slice_cap_dwords(e->opt.slice_bytes);
     hipError_t err;
 #define TRY(x) do { if ((err = (x)) != hipSuccess) { r2p_encoder_destroy(e); return hip_fail(err, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
@@ -578,6 +590,14 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
     TRY(hipMalloc((void**)&e->d_slice_boff, (size_t)e->max_slice * 4));
     TRY(hipMalloc((void**)&e->d_seg_tbits, ((size_t)e->max_seg + 1) * 4));
     TRY(hipMalloc((void**)&e->d_seg_off, ((size_t)e->max_seg + 2) * 8));
+    if (e->opt.huffman) {
+        TRY(hipMalloc((void**)&e->d_tokens, (size_t)e->max_slice * e->opt.slice_bytes * 4));
+        TRY(hipMalloc((void**)&e->d_tok_count, (size_t)e->max_slice * 4));
+        TRY(hipMalloc((void**)&e->d_hist, (size_t)e->max_slice * kHuffSyms * 4));
+        TRY(hipMalloc((void**)&e->d_codes, ((size_t)e->max_seg + 1) * kHuffSyms * 4));
+        TRY(hipMalloc((void**)&e->d_hdr, ((size_t)e->max_seg + 1) * kHuffHdrWords * 4));
+        TRY(hipMalloc((void**)&e->d_hdr_bits, ((size_t)e->max_seg + 1) * 4));
+    }
     TRY(hipMalloc((void**)&e->d_misc, 64));
     TRY(hipHostMalloc((void**)&e->h_misc, 64, hipHostMallocDefault));
 #undef TRY
@@ -592,6 +612,8 @@ int r2p_encoder_destroy(r2p_encoder* e)
     (void)hipFree(e->d_filt); (void)hipFree(e->d_slice_out); (void)hipFree(e->d_slice_bits);
     (void)hipFree(e->d_slice_adler); (void)hipFree(e->d_slice_boff); (void)hipFree(e->d_seg_tbits);
     (void)hipFree(e->d_seg_off); (void)hipFree(e->d_misc);
+    (void)hipFree(e->d_tokens); (void)hipFree(e->d_tok_count); (void)hipFree(e->d_hist);
+    (void)hipFree(e->d_codes); (void)hipFree(e->d_hdr); (void)hipFree(e->d_hdr_bits);
     if (e->h_misc) (void)hipHostFree(e->h_misc);
     (void)hipFree(e->d_src_stage); (void)hipFree(e->d_png_stage);
     for (auto& v : e->ev) if (v) (void)hipEventDestroy(v);
@@ -646,8 +668,17 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     D.slice_cap_dw = e->slice_cap_dw;
     D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
     D.lz77 = o.lz77; D.nslice = sg.nslice;
+    if (o.huffman) { D.tokens = e->d_tokens; D.tok_count = e->d_tok_count; D.hist = e->d_hist; }
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
+    if (o.huffman) {                       // the segments' codes, then the slices' bits
+        HuffJob H{};
+        H.tokens = e->d_tokens; H.tok_count = e->d_tok_count; H.hist = e->d_hist;
+        H.codes = e->d_codes; H.hdr = e->d_hdr; H.hdr_bits = e->d_hdr_bits;
+        H.slice_out = e->d_slice_out; H.slice_bits = e->d_slice_bits; H.slice_cap_dw = e->slice_cap_dw;
+        H.slice_bytes = o.slice_bytes; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg; H.final_block = 1;
+        R2P_HIP(launch_huffman(H, s));
+    }
     if (e->timing) R2P_HIP(hipEventRecord(e->ev[2], s));
 
     AssembleLaunch AL{};
@@ -658,7 +689,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     A.adler_out = (uint32_t*)(e->d_misc + 1); A.out_cap = (int64_t)png_cap;
     A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
     A.slice_cap_dw = e->slice_cap_dw; A.slice_bytes = o.slice_bytes;
-    A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
+    A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
     A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
     A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
     A.color_type = (uint8_t)g.color_type;

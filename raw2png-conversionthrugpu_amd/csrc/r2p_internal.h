@@ -73,6 +73,14 @@ struct DeflateJob {
     int32_t        row_stride; // filtered row size (match candidate one row up), 0 = none
     int32_t        lz77;
     int32_t        nslice;     // max(1, ceil(len / slice_bytes))
+    // dynamic-Huffman mode (single job; k_deflate<.., true>): instead of
+    // bits, each slice writes its tokens (slice_bytes u32 per slice:
+    // literal b, or 1 << 31 | (length - 3) << 16 | (distance - 1)), their
+    // count, and a histogram of kHuffSyms u32 (literal/length symbols
+    // 0..285, then distance codes 0..29)
+    uint32_t*      tokens;
+    uint32_t*      tok_count;
+    uint32_t*      hist;
 };
 
 // Where a job's segments go once their sizes are known.
@@ -100,6 +108,7 @@ struct AssembleJob {
     uint8_t         bit_depth, color_type, zflg, pad_;
     int32_t         max_chunks;  // chunks this job may use (grid sizing bound)
     uint32_t        idat_full;   // CRC register after "IDAT" times x^(8 * chunk_bytes): a full chunk's prefix term
+    int32_t         eob_in_bits; // the segments' bit strings end with their end-of-block code (dynamic blocks)
     uint64_t*       running;     // optional: running output offset (raw mode, strip appends)
     uint64_t*       base_slot;   // optional: where k_scan records this job's base offset
 };
@@ -128,6 +137,25 @@ struct DeflateLaunch {
     DeflateJob        single;
 };
 
+// Dynamic-Huffman blocks of one job (after k_deflate<.., true>): per
+// segment the code tables and the block header, then per slice the bits.
+struct HuffJob {
+    const uint32_t* tokens;      // as DeflateJob
+    const uint32_t* tok_count;
+    const uint32_t* hist;
+    uint32_t*       codes;       // nseg * kHuffSyms: bit-reversed code | length << 16 (0: unused)
+    uint32_t*       hdr;         // nseg * kHuffHdrWords: the block header, BFINAL/BTYPE first
+    uint32_t*       hdr_bits;    // nseg
+    uint32_t*       slice_out;   // as DeflateJob: the slices' bit strings
+    uint32_t*       slice_bits;
+    int32_t         slice_cap_dw;
+    int32_t         slice_bytes;
+    int32_t         spb;         // slices per segment
+    int32_t         nslice;
+    int32_t         nseg;
+    int32_t         final_block;
+};
+
 struct AssembleLaunch {
     const AssembleJob* jobs;
     const uint32_t*    chunk_start; // prefix of max_chunks per job (njobs + 1)
@@ -146,6 +174,8 @@ constexpr int kCrcShiftLevels = 29;       // CRC shift tables for 64 * 2^k zero 
 constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
 constexpr int kFilterLdsMax = 160 * 1024 - 256;  // LDS one filter workgroup may use
 constexpr int kWindow = 32768;           // DEFLATE window
+constexpr int kHuffSyms = 286 + 30;      // literal/length symbols, then distance codes
+constexpr int kHuffHdrWords = 96;        // a dynamic block header: at most 3 + 14 + 57 + 316 * 7 bits
 // Every buffer k_deflate reads holds at least this many readable bytes after
 // the last data byte, so its 16-byte loads need no tail guard.
 constexpr int kTailSlack = 64;

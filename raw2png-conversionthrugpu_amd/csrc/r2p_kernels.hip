@@ -1069,6 +1069,32 @@ __device__ __forceinline__ uint32_t lit_tok(uint32_t b)
     return (__builtin_bitreverse32(b + (hi ? 0x100u : 0x30u)) >> (hi ? 23 : 24)) | ((hi ? 9u : 8u) << 26);
 }
 
+// Length symbol (257..285) of a match length 3..258 and its extra bits
+// (RFC 1951 3.2.5): 257 + m for m = L - 3 < 8, then four symbols per extra
+// bit count e with base (4 + i) << e.
+__device__ __forceinline__ uint32_t huff_len_sym(uint32_t L)
+{
+    if (L == 258) return 285;
+    const uint32_t m = L - 3;
+    if (m < 8) return 257 + m;
+    const uint32_t e = 29u - __builtin_clz(m);          // floor(log2 m) - 2
+    return 257 + 4 * e + (m >> e);
+}
+__device__ __forceinline__ uint32_t huff_len_extra_bits(uint32_t L)
+{
+    const uint32_t m = L - 3;
+    return (L == 258 || m < 8) ? 0u : 29u - __builtin_clz(m);
+}
+// Distance code (0..29) of a distance 1..32768: with dm = d - 1 and
+// e = max(0, floor(log2 dm) - 1) extra bits, code = 2e + (dm >> e).
+__device__ __forceinline__ uint32_t huff_dist_sym(uint32_t d)
+{
+    const uint32_t dm = d - 1;
+    const uint32_t e = 30u - __builtin_clz(dm | 2u);
+    return (e << 1) + (dm >> e);
+}
+constexpr uint32_t kNoToken = 0x7FFFFFFFu;   // (a parse position where no token starts)
+
 // A match as a key: len << 16 | (0xFFFF - dist).  The larger key is the
 // longer match, and of equal lengths the shorter distance.
 __device__ __forceinline__ uint32_t mkey(uint32_t len, uint32_t dist) { return (len << 16) | (0xFFFFu - dist); }
@@ -1200,8 +1226,10 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
 
 // STATS: also accumulate per-slice match statistics (the strip form's
 // LzStats); a separate instantiation so that the plain and batched forms do
-// not carry their registers.
-template <int NT, bool STATS>
+// not carry their registers.  DYN: the parse's tokens and their symbol
+// histogram go to memory (DeflateJob::tokens / tok_count / hist) for a
+// dynamic-Huffman block per segment; no bits are emitted here.
+template <int NT, bool STATS, bool DYN = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_deflate(DeflateLaunch DL)
 {
     constexpr int LANE = kLane;
@@ -1307,6 +1335,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         uint32_t carry = opens ? (last ? 3u : 2u) : 0u;
         uint32_t carry_bits = opens ? 3u : 0u;
         int64_t out_dw = 0;
+        uint32_t ntok_slice = 0;                   // DYN: tokens written so far
+        uint32_t* hist = obuf;                     // DYN: the slice's histogram (obuf is zero)
 
         for (int q = s0_rel; q < s1_rel; q += SUB) {
             const int n = min(SUB, s1_rel - q);
@@ -1346,7 +1376,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             __syncthreads();
             // ---- B. best match per position, greedy parse of the lane -----------
-            if (tid == 0) obuf[0] = carry;
+            if (!DYN && tid == 0) obuf[0] = carry;
             uint32_t key[LANE];
 #pragma unroll
             for (int i = 0; i < LANE; i++) key[i] = 0;
@@ -1384,9 +1414,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
                 const uint32_t L = key[i] >> 16;       // 0: literal
                 const bool tok = ((uint32_t)i == nextp) & (i < le);
-                const uint32_t t = L ? match_tok(key[i]) : lit_tok((o[i >> 2] >> (8 * (i & 3))) & 0xFFu);
-                tk[i] = tok ? t : 0u;
-                nbits += tok ? t >> 26 : 0u;
+                const uint32_t b = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                if constexpr (DYN) {                   // raw token; nbits counts tokens
+                    tk[i] = !tok ? kNoToken : L ? 0x80000000u | ((L - 3u) << 16) | (0xFFFEu - (key[i] & 0xFFFFu)) : b;
+                    nbits += tok ? 1u : 0u;
+                } else {
+                    const uint32_t t = L ? match_tok(key[i]) : lit_tok(b);
+                    tk[i] = tok ? t : 0u;
+                    nbits += tok ? t >> 26 : 0u;
+                }
                 nextp = tok ? (uint32_t)i + max(L, 1u) : nextp;
             }
             // ---- block scan of bit counts ----------------------------------------------
@@ -1413,6 +1449,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
                 for (int i = 0; i < LANE; i += kInsertStride)
                     if ((hmask >> i) & 1u) atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+            }
+            if constexpr (DYN) {
+                // ---- tokens in parse order, and their symbols' histogram --------
+                uint32_t* tp = J.tokens + (int64_t)sl * J.slice_bytes + ntok_slice + (wbase + incl - nbits);
+#pragma unroll
+                for (int i = 0; i < LANE; i++) {
+                    if (tk[i] == kNoToken) continue;
+                    *tp++ = tk[i];
+                    if (tk[i] >> 31) {
+                        atomicAdd(&hist[huff_len_sym(((tk[i] >> 16) & 0xFFu) + 3u)], 1u);
+                        atomicAdd(&hist[286 + huff_dist_sym((tk[i] & 0xFFFFu) + 1u)], 1u);
+                    } else {
+                        atomicAdd(&hist[tk[i]], 1u);
+                    }
+                }
+                ntok_slice += tot;
+                __syncthreads();                   // the sub-block's T1/T2 and red uses done
+                continue;
             }
             // ---- emission ------------------------------------------------------------------
             // A lane's first word may hold the previous lane's last bits and
@@ -1450,7 +1504,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
         // ---- slice end: the partial last word and the bit count ---------------
         __syncthreads();
-        if (tid == 0) {
+        if constexpr (DYN) {
+            for (int i = tid; i < kHuffSyms; i += NT) {    // the histogram out, obuf zero again
+                J.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
+                hist[i] = 0;
+            }
+            if (tid == 0) J.tok_count[sl] = ntok_slice;
+        }
+        if (!DYN && tid == 0) {
             if (carry_bits) out[out_dw] = carry;
             J.slice_bits[sl] = (uint32_t)(out_dw * 32 + carry_bits);
             if (STATS && J.slice_stats) {
@@ -1460,6 +1521,351 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         const uint32_t ad = adler_finish(ad_s, ad_t, s1 - s0, red);
         if (tid == 0) J.slice_adler[sl] = ad;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dynamic-Huffman blocks (RFC 1951 3.2.7), one per segment
+//
+// k_huff_tables: one wave per segment sums its slices' histograms, builds
+// length-limited Huffman codes for the literal/length and distance
+// alphabets and for the code-length alphabet (the lengths are sent one
+// symbol each, without run-length codes: a little more header for a
+// header written in parallel), and writes the code tables and the block
+// header -- or the fixed code and its 3-bit header when that is no longer
+// (so a segment never grows past its fixed-Huffman size).
+// k_huff_emit: one workgroup per slice turns its tokens into bits with the
+// segment's table (the segment's first slice starts with the header, its
+// last ends with the end-of-block code), in rounds of 16 tokens per thread:
+// per-thread bit counts, a block scan, packing into LDS words, flush.
+// ---------------------------------------------------------------------------
+
+// Code lengths (at most LIMIT bits) of a Huffman code for freq[0..n) into
+// len[0..n), by one wave: at least two symbols get a code (a used one alone,
+// or none, gets partners of count 1); the used symbols are ranked by (count,
+// symbol) and merged by the two-queue method on lane 0; the depths come
+// from the parent links by pointer jumping; lengths over the limit are
+// repaired on the length counts (moving the Kraft sum back to 1, one unit
+// of 2^-LIMIT per step), longest codes going to the least frequent symbols.
+template <int LIMIT>
+__device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sorted, uint32_t* w, uint32_t* link,
+                             uint32_t* cnt)
+{
+    const int tid = threadIdx.x;
+    __syncthreads();
+    uint32_t used = 0;
+    for (int s = tid; s < n; s += 64) used += freq[s] != 0;
+    used = wave_total(used);
+    if (used < 2) {
+        if (tid == 0)
+            for (int i = 0; used < 2 && i < n; i++)
+                if (freq[i] == 0) { freq[i] = 1; used++; }
+        used = 2;
+    }
+    __syncthreads();
+    const int m = (int)used;
+    for (int s = tid; s < n; s += 64) {
+        len[s] = 0;
+        const uint32_t f = freq[s];
+        if (f == 0) continue;
+        int r = 0;
+        for (int t = 0; t < n; t++) r += freq[t] != 0 && (freq[t] < f || (freq[t] == f && t < s));
+        sorted[r] = (uint16_t)s;
+    }
+    if (tid < 16) cnt[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += 64) w[i] = freq[sorted[i]];
+    __syncthreads();
+    const int root = 2 * m - 2;
+    if (tid == 0) {                                          // the two-queue merge
+        int li = 0, ii = m, next = m;
+        uint32_t wl = w[0];
+        for (int k = 0; k < m - 1; k++) {
+            int c[2];
+            uint32_t wc[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const uint32_t wi = ii < next ? w[ii] : 0xFFFFFFFFu;
+                if (li < m && wl <= wi) { c[q] = li; wc[q] = wl; li++; wl = li < m ? w[li] : 0xFFFFFFFFu; }
+                else { c[q] = ii; wc[q] = wi; ii++; }
+            }
+            w[next] = wc[0] + wc[1];
+            link[c[0]] = (1u << 16) | (uint32_t)next;        // distance 1 to the parent
+            link[c[1]] = (1u << 16) | (uint32_t)next;
+            next++;
+        }
+        link[root] = (uint32_t)root;                         // the root: distance 0 to itself
+    }
+    __syncthreads();
+    // depths: link = distance << 16 | ancestor, jumped to the root (each entry
+    // read and written whole, so a round may mix old and new entries safely)
+    for (int round = 0; round < 10; round++) {
+        for (int v = tid; v <= root; v += 64) {
+            const uint32_t a = link[v], b = link[a & 0xFFFFu];
+            link[v] = ((a >> 16) + (b >> 16)) << 16 | (b & 0xFFFFu);
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < m; i += 64) atomicAdd(&cnt[min((int)(link[i] >> 16), LIMIT)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t blc[LIMIT + 1];
+#pragma unroll
+        for (int b = 0; b <= LIMIT; b++) blc[b] = cnt[b];
+        uint32_t kraft = 0;                                  // in units of 2^-LIMIT
+#pragma unroll
+        for (int b = 1; b <= LIMIT; b++) kraft += blc[b] << (LIMIT - b);
+        while (kraft > (1u << LIMIT)) {
+            blc[LIMIT]--;                                    // one code at the limit goes
+            bool done = false;
+#pragma unroll
+            for (int b = LIMIT - 1; b > 0; b--)              // one shorter code splits in two
+                if (!done && blc[b]) { blc[b]--; blc[b + 1] += 2; done = true; }
+            kraft--;
+        }
+        // sorted positions [start[b], start[b] + blc[b]) get length b, longest first
+        uint32_t at = 0;
+#pragma unroll
+        for (int b = LIMIT; b >= 1; b--) { cnt[b] = at; at += blc[b]; }
+        cnt[0] = at;
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += 64) {
+        int b = LIMIT;
+        while (b > 1 && (uint32_t)i >= (b > 1 ? cnt[b - 1] : cnt[0])) b--;
+        len[sorted[i]] = (uint8_t)b;
+    }
+    __syncthreads();
+}
+
+// Canonical codes (RFC 1951 3.2.2) for len[0..n), bit-reversed for the
+// LSB-first stream: out[s] = code | length << 16 (0 where unused); by the
+// wave, each symbol's code is its length's first code plus the number of
+// earlier symbols of that length.
+__device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* first)
+{
+    const int tid = threadIdx.x;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t blc[16];
+        for (int b = 0; b < 16; b++) blc[b] = 0;
+        for (int s = 0; s < n; s++) blc[len[s]]++;
+        blc[0] = 0;
+        uint32_t code = 0;
+        for (int b = 1; b < 16; b++) { code = (code + blc[b - 1]) << 1; first[b] = code; }
+    }
+    __syncthreads();
+    for (int s = tid; s < n; s += 64) {
+        const uint32_t l = len[s];
+        uint32_t r = 0;
+        for (int t = 0; t < s; t++) r += len[t] == l;
+        out[s] = l ? (__builtin_bitreverse32(first[l] + r) >> (32 - l)) | (l << 16) : 0u;
+    }
+    __syncthreads();
+}
+
+// Fixed-code length of literal/length symbol s (RFC 1951 3.2.6).
+__device__ __forceinline__ uint32_t fixed_len(int s) { return s < 144 ? 8u : s < 256 ? 9u : s < 280 ? 7u : 8u; }
+
+__global__ __launch_bounds__(64) void k_huff_tables(HuffJob H)
+{
+    __shared__ uint32_t freq[kHuffSyms];
+    __shared__ uint8_t len[kHuffSyms];
+    __shared__ uint16_t sorted[286];
+    __shared__ uint32_t w[2 * 286], link[2 * 286];
+    __shared__ uint32_t cnt[16], first[16];
+    __shared__ uint32_t cfreq[19], ccode[19], table[kHuffSyms];
+    __shared__ uint8_t clen[19];
+    __shared__ uint32_t hdr[kHuffHdrWords];
+    __shared__ uint32_t s_hlit, s_hdist;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int seg = blockIdx.x;
+    const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
+    const bool bfinal = seg == H.nseg - 1 && H.final_block;
+    for (int s = tid; s < kHuffSyms; s += 64) {
+        uint32_t f = 0;
+        for (int k = k0; k < k1; k++) f += H.hist[(int64_t)k * kHuffSyms + s];
+        freq[s] = f + (s == 256 ? 1u : 0u);                  // the end-of-block code
+    }
+    for (int i = tid; i < 19; i += 64) cfreq[i] = 0;
+    for (int i = tid; i < kHuffHdrWords; i += 64) hdr[i] = 0;
+    if (tid == 0) { s_hlit = 257; s_hdist = 1; }
+    huff_lengths<15>(freq, 286, len, sorted, w, link, cnt);
+    huff_lengths<15>(freq + 286, 30, len + 286, sorted, w, link, cnt);
+    // code sizes of this code and of the fixed one (extra bits are the same
+    // for both); the sent alphabet sizes; the code-length histogram
+    uint32_t dyn = 0, fix = 0;
+    for (int s = tid; s < kHuffSyms; s += 64) {
+        const uint32_t f = freq[s], l = len[s];
+        dyn += f * l;
+        fix += f * (s < 286 ? fixed_len(s) : 5u);
+        if (l) atomicMax(s < 286 ? &s_hlit : &s_hdist, s < 286 ? (uint32_t)s + 1 : (uint32_t)(s - 286) + 1);
+    }
+    dyn = wave_total(dyn);
+    fix = wave_total(fix) + 3;
+    __syncthreads();
+    const int hlit = (int)s_hlit, hdist = (int)s_hdist, N = hlit + hdist;
+    // the code lengths are sent one symbol each (no run-length codes)
+    for (int j = tid; j < N; j += 64) atomicAdd(&cfreq[j < hlit ? len[j] : len[286 + j - hlit]], 1u);
+    huff_lengths<7>(cfreq, 19, clen, sorted, w, link, cnt);
+    huff_codes(clen, 19, ccode, first);
+    const uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+    int hclen = 19;
+    while (hclen > 4 && clen[order[hclen - 1]] == 0) hclen--;
+    uint32_t hbits = 0;                                      // the code lengths' bits
+    for (int j = tid; j < N; j += 64) hbits += ccode[j < hlit ? len[j] : len[286 + j - hlit]] >> 16;
+    const uint32_t fixed_part = 3 + 14 + 3 * (uint32_t)hclen;
+    const bool dynamic = dyn + fixed_part + wave_total(hbits) < fix;
+    if (dynamic) {
+        if (tid == 0) {
+            uint64_t acc = (bfinal ? 1u : 0u) | (2u << 1) | (uint64_t)(hlit - 257) << 3 | (uint64_t)(hdist - 1) << 8 |
+                           (uint64_t)(hclen - 4) << 13;
+            uint32_t na = 17;
+            int nw = 0;
+            for (int k = 0; k < hclen; k++) {
+                acc |= (uint64_t)clen[order[k]] << na;
+                na += 3;
+                if (na >= 32) { hdr[nw++] = (uint32_t)acc; acc >>= 32; na -= 32; }
+            }
+            if (na) hdr[nw] = (uint32_t)acc;
+        }
+        __syncthreads();
+        // the N code lengths, packed by the wave at prefix-summed offsets
+        uint32_t base = fixed_part;
+        for (int j0 = 0; j0 < N; j0 += 64) {
+            const int j = j0 + lane;
+            const uint32_t c = j < N ? ccode[j < hlit ? len[j] : len[286 + j - hlit]] : 0u;
+            const uint32_t nb = c >> 16;
+            const uint32_t incl = wave_incl_scan(nb);
+            const uint32_t off = base + incl - nb;
+            if (nb) {
+                const uint64_t v = (uint64_t)(c & 0xFFFFu) << (off & 31);
+                atomicOr(&hdr[off >> 5], (uint32_t)v);
+                if ((off & 31) + nb > 32) atomicOr(&hdr[(off >> 5) + 1], (uint32_t)(v >> 32));
+            }
+            base += __shfl(incl, 63);
+        }
+        if (tid == 0) H.hdr_bits[seg] = base;
+        huff_codes(len, 286, table, first);
+        huff_codes(len + 286, 30, table + 286, first);
+    } else {
+        // the fixed code (RFC 1951 3.2.6): its canonical codes directly
+        if (tid == 0) { hdr[0] = (bfinal ? 1u : 0u) | (1u << 1); H.hdr_bits[seg] = 3; }
+        for (int s = tid; s < 286; s += 64) {
+            const uint32_t l = fixed_len(s);
+            const uint32_t c = s < 144 ? 0x30u + s : s < 256 ? 0x190u + (s - 144) : s < 280 ? (uint32_t)(s - 256) : 0xC0u + (s - 280);
+            table[s] = (__builtin_bitreverse32(c) >> (32 - l)) | (l << 16);
+        }
+        for (int d = tid; d < 30; d += 64) table[286 + d] = (__builtin_bitreverse32((uint32_t)d) >> 27) | (5u << 16);
+    }
+    __syncthreads();
+    for (int i = tid; i < kHuffHdrWords; i += 64) H.hdr[(int64_t)seg * kHuffHdrWords + i] = hdr[i];
+    for (int s = tid; s < kHuffSyms; s += 64) H.codes[(int64_t)seg * kHuffSyms + s] = table[s];
+}
+
+// The bits of token v with table T: up to two (value, count) parts of at
+// most 28 bits each.  v == kNoToken stands for the end-of-block code.
+__device__ __forceinline__ uint32_t token_bits(const uint32_t* T, uint32_t v, uint32_t& v1, uint32_t& n1, uint32_t& n0)
+{
+    if (v == kNoToken) { const uint32_t c = T[256]; n0 = c >> 16; n1 = 0; v1 = 0; return c & 0xFFFFu; }
+    if (!(v >> 31)) { const uint32_t c = T[v]; n0 = c >> 16; n1 = 0; v1 = 0; return c & 0xFFFFu; }
+    const uint32_t L = ((v >> 16) & 0xFFu) + 3u, d = (v & 0xFFFFu) + 1u;
+    const uint32_t lc = T[huff_len_sym(L)], le = huff_len_extra_bits(L);
+    const uint32_t lx = (L - 3u) & ((1u << le) - 1u);
+    const uint32_t dc = T[286 + huff_dist_sym(d)], de = 30u - __builtin_clz((d - 1u) | 2u);
+    const uint32_t dx = (d - 1u) & ((1u << de) - 1u);
+    n0 = (lc >> 16) + le;
+    n1 = (dc >> 16) + de;
+    v1 = (dc & 0xFFFFu) | (dx << (dc >> 16));
+    return (lc & 0xFFFFu) | (lx << (lc >> 16));
+}
+
+constexpr int kEmitThreads = 256;
+constexpr int kEmitPer = 8;                                    // tokens per thread and round
+constexpr int kEmitWords = kEmitThreads * kEmitPer * 48 / 32 + kHuffHdrWords + 8;
+
+__global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
+{
+    constexpr int NT = kEmitThreads, NW = NT / 64;
+    __shared__ uint32_t T[kHuffSyms];
+    __shared__ uint32_t obuf[kEmitWords];
+    __shared__ uint32_t red[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < kEmitWords; i += NT) obuf[i] = 0;
+    for (int sl = blockIdx.x; sl < H.nslice; sl += gridDim.x) {
+        const int seg = sl / H.spb;
+        const bool first = sl % H.spb == 0;
+        const bool lastsl = sl % H.spb == H.spb - 1 || sl == H.nslice - 1;
+        __syncthreads();                                       // the previous slice is done with T
+        for (int i = tid; i < kHuffSyms; i += NT) T[i] = H.codes[(int64_t)seg * kHuffSyms + i];
+        // bits already in obuf: the segment's header before its first slice
+        uint32_t pre = 0;
+        if (first) {
+            pre = H.hdr_bits[seg];
+            for (uint32_t i = tid; i < (pre + 31) / 32; i += NT) obuf[i] = H.hdr[(int64_t)seg * kHuffHdrWords + i];
+        }
+        __syncthreads();
+        const uint32_t nreal = H.tok_count[sl];
+        const uint32_t ntok = nreal + (lastsl ? 1u : 0u);   // + the end-of-block code
+        const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes;
+        uint32_t* out = H.slice_out + (int64_t)sl * H.slice_cap_dw;
+        int64_t out_dw = 0;
+        uint32_t carry_bits = pre;
+        const uint32_t rounds = max(1u, (ntok + NT * kEmitPer - 1) / (NT * kEmitPer));
+        for (uint32_t r = 0; r < rounds; r++) {
+            const uint32_t t0 = r * NT * kEmitPer + tid * kEmitPer;
+            uint32_t a[kEmitPer], na[kEmitPer], b[kEmitPer], nb[kEmitPer];
+            uint32_t nbits = 0;
+#pragma unroll
+            for (int i = 0; i < kEmitPer; i++) {
+                const uint32_t t = t0 + i;
+                na[i] = nb[i] = 0; a[i] = b[i] = 0;
+                if (t < ntok) {
+                    const uint32_t v = t < nreal ? tk[t] : kNoToken;
+                    a[i] = token_bits(T, v, b[i], nb[i], na[i]);
+                }
+                nbits += na[i] + nb[i];
+            }
+            const uint32_t incl = wave_incl_scan(nbits);
+            if (lane == 63) red[wave] = incl;
+            __syncthreads();
+            uint32_t wbase = 0, tot = 0;
+            for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
+            const uint32_t off = carry_bits + wbase + incl - nbits;
+            {   // as k_deflate: the lane's first and last words are shared (atomicOr)
+                const uint32_t w0 = off >> 5;
+                uint32_t wi = w0, nacc = off & 31;
+                uint64_t acc = 0;
+                auto push = [&](uint32_t v, uint32_t n) {
+                    acc |= (uint64_t)v << nacc;
+                    nacc += n;
+                    if (nacc >= 32) {
+                        if (wi == w0) atomicOr(&obuf[wi], (uint32_t)acc);
+                        else obuf[wi] = (uint32_t)acc;
+                        wi++;
+                        acc >>= 32;
+                        nacc -= 32;
+                    }
+                };
+#pragma unroll
+                for (int i = 0; i < kEmitPer; i++) { push(a[i], na[i]); push(b[i], nb[i]); }
+                if (nacc > 0) atomicOr(&obuf[wi], (uint32_t)acc);
+            }
+            __syncthreads();
+            const uint32_t tb = carry_bits + tot;
+            const uint32_t full = tb >> 5;
+            for (uint32_t i = tid; i < full; i += NT) { out[out_dw + i] = obuf[i]; obuf[i] = 0; }
+            __syncthreads();
+            if (tid == 0) { obuf[0] = (tb & 31) ? obuf[full] : 0u; if (full) obuf[full] = 0; }
+            __syncthreads();
+            carry_bits = tb & 31;
+            out_dw += full;
+        }
+        if (tid == 0) {
+            if (carry_bits) out[out_dw] = obuf[0];
+            obuf[0] = 0;
+            H.slice_bits[sl] = (uint32_t)(out_dw * 32 + carry_bits);
+        }
     }
 }
 
@@ -1483,11 +1889,12 @@ __device__ __forceinline__ uint64_t out_total(const AssembleJob& A, int64_t z)
 }
 
 // Byte length of a segment with tb token bits (block header included): EOB
-// (7 bits), and unless it is the stream's last, the 3-bit stored-block
-// header, the pad to a byte and 00 00 FF FF.
-__device__ __forceinline__ uint32_t seg_len_bytes(uint32_t tb, bool last)
+// (7 bits, unless the bits hold it: dynamic blocks), and unless it is the
+// stream's last, the 3-bit stored-block header, the pad to a byte and
+// 00 00 FF FF.
+__device__ __forceinline__ uint32_t seg_len_bytes(uint32_t tb, bool last, bool eob_in_bits)
 {
-    return (tb + 7 + (last ? 0u : 3u) + 7) / 8 + (last ? 0u : 4u);
+    return (tb + (eob_in_bits ? 0u : 7u) + (last ? 0u : 3u) + 7) / 8 + (last ? 0u : 4u);
 }
 
 // Slice i's terms of the whole input's Adler-32 (sum form, mod 65521): for
@@ -1531,7 +1938,7 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
                     if (kb + u < k1) { A.slice_boff[kb + u] = tb; tb += bits[u]; }
             }
             A.seg_tbits[i] = tb;
-            v = seg_len_bytes(tb, (i == A.nseg - 1) && A.final_block);
+            v = seg_len_bytes(tb, (i == A.nseg - 1) && A.final_block, A.eob_in_bits);
         }
         const uint32_t incl = wave_incl_scan(v);
         __syncthreads();
@@ -1720,7 +2127,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                     if (kb + u < k1) { s_bits[kb + u] = bits[u]; s_boff[kb + u] = tb; tb += bits[u]; }
             }
             s_tbits[i] = tb;
-            v = seg_len_bytes(tb, (i == S.nseg - 1) && S.final_block);
+            v = seg_len_bytes(tb, (i == S.nseg - 1) && S.final_block, S.eob_in_bits);
         }
         for (int k = tid; k < S.nslice; k += kAssembleThreads) adler_terms(S, k, sa, sb);
         const uint32_t incl = wave_incl_scan(v);
@@ -1999,7 +2406,8 @@ hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_thread
     (void)lane_bytes;
     (void)hipGetLastError();
     const bool stats = !DL.jobs && DL.single.slice_stats;
-    if (stats)                  k_deflate<256, true><<<grid, 256, 0, s>>>(DL);
+    if (!DL.jobs && DL.single.tokens) k_deflate<256, false, true><<<grid, 256, 0, s>>>(DL);
+    else if (stats)             k_deflate<256, true><<<grid, 256, 0, s>>>(DL);
     else if (wg_threads == 256) k_deflate<256, false><<<grid, 256, 0, s>>>(DL);
     else                        k_deflate<512, false><<<grid, 512, 0, s>>>(DL);
     return hipGetLastError();
@@ -2009,6 +2417,14 @@ hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
 {
     (void)hipGetLastError();
     k_scan<<<njobs, 256, 0, s>>>(AL);
+    return hipGetLastError();
+}
+
+hipError_t launch_huffman(const HuffJob& H, hipStream_t s)
+{
+    (void)hipGetLastError();
+    k_huff_tables<<<H.nseg, 64, 0, s>>>(H);
+    k_huff_emit<<<std::min(H.nslice, 8192), kEmitThreads, 0, s>>>(H);
     return hipGetLastError();
 }
 
