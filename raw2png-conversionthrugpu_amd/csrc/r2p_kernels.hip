@@ -1527,7 +1527,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // ---------------------------------------------------------------------------
 // Dynamic-Huffman blocks (RFC 1951 3.2.7), one per segment
 //
-// k_huff_tables: one wave per segment sums its slices' histograms, builds
+// k_huff_tables: one workgroup per segment sums its slices' histograms, builds
 // length-limited Huffman codes for the literal/length and distance
 // alphabets and for the code-length alphabet (the lengths are sent one
 // symbol each, without run-length codes: a little more header for a
@@ -1540,8 +1540,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // per-thread bit counts, a block scan, packing into LDS words, flush.
 // ---------------------------------------------------------------------------
 
+constexpr int kHuffThreads = 256;        // k_huff_tables workgroup
+
+// Sum of v over the workgroup (all threads call it); red holds a u32 per wave.
+__device__ uint32_t block_sum(uint32_t v, uint32_t* red)
+{
+    v = wave_total(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+    __syncthreads();
+    return t;
+}
+
 // Code lengths (at most LIMIT bits) of a Huffman code for freq[0..n) into
-// len[0..n), by one wave: at least two symbols get a code (a used one alone,
+// len[0..n), by the workgroup: at least two symbols get a code (a used one alone,
 // or none, gets partners of count 1); the used symbols are ranked by (count,
 // symbol) and merged by the two-queue method on lane 0; the depths come
 // from the parent links by pointer jumping; lengths over the limit are
@@ -1554,8 +1569,8 @@ __device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sort
     const int tid = threadIdx.x;
     __syncthreads();
     uint32_t used = 0;
-    for (int s = tid; s < n; s += 64) used += freq[s] != 0;
-    used = wave_total(used);
+    for (int s = tid; s < n; s += kHuffThreads) used += freq[s] != 0;
+    used = block_sum(used, cnt);
     if (used < 2) {
         if (tid == 0)
             for (int i = 0; used < 2 && i < n; i++)
@@ -1564,7 +1579,7 @@ __device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sort
     }
     __syncthreads();
     const int m = (int)used;
-    for (int s = tid; s < n; s += 64) {
+    for (int s = tid; s < n; s += kHuffThreads) {
         len[s] = 0;
         const uint32_t f = freq[s];
         if (f == 0) continue;
@@ -1574,7 +1589,7 @@ __device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sort
     }
     if (tid < 16) cnt[tid] = 0;
     __syncthreads();
-    for (int i = tid; i < m; i += 64) w[i] = freq[sorted[i]];
+    for (int i = tid; i < m; i += kHuffThreads) w[i] = freq[sorted[i]];
     __syncthreads();
     const int root = 2 * m - 2;
     if (tid == 0) {                                          // the two-queue merge
@@ -1600,13 +1615,13 @@ __device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sort
     // depths: link = distance << 16 | ancestor, jumped to the root (each entry
     // read and written whole, so a round may mix old and new entries safely)
     for (int round = 0; round < 10; round++) {
-        for (int v = tid; v <= root; v += 64) {
+        for (int v = tid; v <= root; v += kHuffThreads) {
             const uint32_t a = link[v], b = link[a & 0xFFFFu];
             link[v] = ((a >> 16) + (b >> 16)) << 16 | (b & 0xFFFFu);
         }
         __syncthreads();
     }
-    for (int i = tid; i < m; i += 64) atomicAdd(&cnt[min((int)(link[i] >> 16), LIMIT)], 1u);
+    for (int i = tid; i < m; i += kHuffThreads) atomicAdd(&cnt[min((int)(link[i] >> 16), LIMIT)], 1u);
     __syncthreads();
     if (tid == 0) {
         uint32_t blc[LIMIT + 1];
@@ -1630,7 +1645,7 @@ __device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sort
         cnt[0] = at;
     }
     __syncthreads();
-    for (int i = tid; i < m; i += 64) {
+    for (int i = tid; i < m; i += kHuffThreads) {
         int b = LIMIT;
         while (b > 1 && (uint32_t)i >= (b > 1 ? cnt[b - 1] : cnt[0])) b--;
         len[sorted[i]] = (uint8_t)b;
@@ -1640,22 +1655,27 @@ __device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sort
 
 // Canonical codes (RFC 1951 3.2.2) for len[0..n), bit-reversed for the
 // LSB-first stream: out[s] = code | length << 16 (0 where unused); by the
-// wave, each symbol's code is its length's first code plus the number of
+// workgroup, each symbol's code is its length's first code plus the number of
 // earlier symbols of that length.
 __device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* first)
 {
     const int tid = threadIdx.x;
     __syncthreads();
+    if (tid < 16) first[tid] = 0;
+    __syncthreads();
+    for (int s = tid; s < n; s += kHuffThreads) atomicAdd(&first[len[s]], 1u);   // codes per length
+    __syncthreads();
     if (tid == 0) {
         uint32_t blc[16];
-        for (int b = 0; b < 16; b++) blc[b] = 0;
-        for (int s = 0; s < n; s++) blc[len[s]]++;
+#pragma unroll
+        for (int b = 0; b < 16; b++) blc[b] = first[b];
         blc[0] = 0;
         uint32_t code = 0;
+#pragma unroll
         for (int b = 1; b < 16; b++) { code = (code + blc[b - 1]) << 1; first[b] = code; }
     }
     __syncthreads();
-    for (int s = tid; s < n; s += 64) {
+    for (int s = tid; s < n; s += kHuffThreads) {
         const uint32_t l = len[s];
         uint32_t r = 0;
         for (int t = 0; t < s; t++) r += len[t] == l;
@@ -1667,7 +1687,7 @@ __device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* f
 // Fixed-code length of literal/length symbol s (RFC 1951 3.2.6).
 __device__ __forceinline__ uint32_t fixed_len(int s) { return s < 144 ? 8u : s < 256 ? 9u : s < 280 ? 7u : 8u; }
 
-__global__ __launch_bounds__(64) void k_huff_tables(HuffJob H)
+__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
 {
     __shared__ uint32_t freq[kHuffSyms];
     __shared__ uint8_t len[kHuffSyms];
@@ -1682,40 +1702,40 @@ __global__ __launch_bounds__(64) void k_huff_tables(HuffJob H)
     const int seg = blockIdx.x;
     const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
     const bool bfinal = seg == H.nseg - 1 && H.final_block;
-    for (int s = tid; s < kHuffSyms; s += 64) {
+    for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
         uint32_t f = 0;
         for (int k = k0; k < k1; k++) f += H.hist[(int64_t)k * kHuffSyms + s];
         freq[s] = f + (s == 256 ? 1u : 0u);                  // the end-of-block code
     }
-    for (int i = tid; i < 19; i += 64) cfreq[i] = 0;
-    for (int i = tid; i < kHuffHdrWords; i += 64) hdr[i] = 0;
+    for (int i = tid; i < 19; i += kHuffThreads) cfreq[i] = 0;
+    for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) hdr[i] = 0;
     if (tid == 0) { s_hlit = 257; s_hdist = 1; }
     huff_lengths<15>(freq, 286, len, sorted, w, link, cnt);
     huff_lengths<15>(freq + 286, 30, len + 286, sorted, w, link, cnt);
     // code sizes of this code and of the fixed one (extra bits are the same
     // for both); the sent alphabet sizes; the code-length histogram
     uint32_t dyn = 0, fix = 0;
-    for (int s = tid; s < kHuffSyms; s += 64) {
+    for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
         const uint32_t f = freq[s], l = len[s];
         dyn += f * l;
         fix += f * (s < 286 ? fixed_len(s) : 5u);
         if (l) atomicMax(s < 286 ? &s_hlit : &s_hdist, s < 286 ? (uint32_t)s + 1 : (uint32_t)(s - 286) + 1);
     }
-    dyn = wave_total(dyn);
-    fix = wave_total(fix) + 3;
+    dyn = block_sum(dyn, cnt);
+    fix = block_sum(fix, cnt) + 3;
     __syncthreads();
     const int hlit = (int)s_hlit, hdist = (int)s_hdist, N = hlit + hdist;
     // the code lengths are sent one symbol each (no run-length codes)
-    for (int j = tid; j < N; j += 64) atomicAdd(&cfreq[j < hlit ? len[j] : len[286 + j - hlit]], 1u);
+    for (int j = tid; j < N; j += kHuffThreads) atomicAdd(&cfreq[j < hlit ? len[j] : len[286 + j - hlit]], 1u);
     huff_lengths<7>(cfreq, 19, clen, sorted, w, link, cnt);
     huff_codes(clen, 19, ccode, first);
     const uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
     int hclen = 19;
     while (hclen > 4 && clen[order[hclen - 1]] == 0) hclen--;
     uint32_t hbits = 0;                                      // the code lengths' bits
-    for (int j = tid; j < N; j += 64) hbits += ccode[j < hlit ? len[j] : len[286 + j - hlit]] >> 16;
+    for (int j = tid; j < N; j += kHuffThreads) hbits += ccode[j < hlit ? len[j] : len[286 + j - hlit]] >> 16;
     const uint32_t fixed_part = 3 + 14 + 3 * (uint32_t)hclen;
-    const bool dynamic = dyn + fixed_part + wave_total(hbits) < fix;
+    const bool dynamic = dyn + fixed_part + block_sum(hbits, cnt) < fix;
     if (dynamic) {
         if (tid == 0) {
             uint64_t acc = (bfinal ? 1u : 0u) | (2u << 1) | (uint64_t)(hlit - 257) << 3 | (uint64_t)(hdist - 1) << 8 |
@@ -1730,9 +1750,9 @@ __global__ __launch_bounds__(64) void k_huff_tables(HuffJob H)
             if (na) hdr[nw] = (uint32_t)acc;
         }
         __syncthreads();
-        // the N code lengths, packed by the wave at prefix-summed offsets
+        // the N code lengths, packed by wave 0 at prefix-summed offsets
         uint32_t base = fixed_part;
-        for (int j0 = 0; j0 < N; j0 += 64) {
+        for (int j0 = 0; j0 < N && tid < 64; j0 += 64) {
             const int j = j0 + lane;
             const uint32_t c = j < N ? ccode[j < hlit ? len[j] : len[286 + j - hlit]] : 0u;
             const uint32_t nb = c >> 16;
@@ -1746,21 +1766,22 @@ __global__ __launch_bounds__(64) void k_huff_tables(HuffJob H)
             base += __shfl(incl, 63);
         }
         if (tid == 0) H.hdr_bits[seg] = base;
+        __syncthreads();
         huff_codes(len, 286, table, first);
         huff_codes(len + 286, 30, table + 286, first);
     } else {
         // the fixed code (RFC 1951 3.2.6): its canonical codes directly
         if (tid == 0) { hdr[0] = (bfinal ? 1u : 0u) | (1u << 1); H.hdr_bits[seg] = 3; }
-        for (int s = tid; s < 286; s += 64) {
+        for (int s = tid; s < 286; s += kHuffThreads) {
             const uint32_t l = fixed_len(s);
             const uint32_t c = s < 144 ? 0x30u + s : s < 256 ? 0x190u + (s - 144) : s < 280 ? (uint32_t)(s - 256) : 0xC0u + (s - 280);
             table[s] = (__builtin_bitreverse32(c) >> (32 - l)) | (l << 16);
         }
-        for (int d = tid; d < 30; d += 64) table[286 + d] = (__builtin_bitreverse32((uint32_t)d) >> 27) | (5u << 16);
+        for (int d = tid; d < 30; d += kHuffThreads) table[286 + d] = (__builtin_bitreverse32((uint32_t)d) >> 27) | (5u << 16);
     }
     __syncthreads();
-    for (int i = tid; i < kHuffHdrWords; i += 64) H.hdr[(int64_t)seg * kHuffHdrWords + i] = hdr[i];
-    for (int s = tid; s < kHuffSyms; s += 64) H.codes[(int64_t)seg * kHuffSyms + s] = table[s];
+    for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) H.hdr[(int64_t)seg * kHuffHdrWords + i] = hdr[i];
+    for (int s = tid; s < kHuffSyms; s += kHuffThreads) H.codes[(int64_t)seg * kHuffSyms + s] = table[s];
 }
 
 // The bits of token v with table T: up to two (value, count) parts of at
@@ -2423,7 +2444,7 @@ hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
 hipError_t launch_huffman(const HuffJob& H, hipStream_t s)
 {
     (void)hipGetLastError();
-    k_huff_tables<<<H.nseg, 64, 0, s>>>(H);
+    k_huff_tables<<<H.nseg, kHuffThreads, 0, s>>>(H);
     k_huff_emit<<<std::min(H.nslice, 8192), kEmitThreads, 0, s>>>(H);
     return hipGetLastError();
 }
