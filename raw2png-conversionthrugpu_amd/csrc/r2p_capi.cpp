@@ -668,7 +668,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     D.slice_cap_dw = e->slice_cap_dw;
     D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
     D.lz77 = o.lz77; D.nslice = sg.nslice;
-    if (o.huffman) { D.tokens = e->d_tokens; D.tok_count = e->d_tok_count; D.hist = e->d_hist; }
+    if (o.huffman) { DL.tokens = e->d_tokens; DL.tok_count = e->d_tok_count; DL.hist = e->d_hist; }
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
     if (o.huffman) {                       // the segments' codes, then the slices' bits

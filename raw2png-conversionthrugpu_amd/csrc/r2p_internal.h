@@ -73,14 +73,6 @@ struct DeflateJob {
     int32_t        row_stride; // filtered row size (match candidate one row up), 0 = none
     int32_t        lz77;
     int32_t        nslice;     // max(1, ceil(len / slice_bytes))
-    // dynamic-Huffman mode (single job; k_deflate<.., true>): instead of
-    // bits, each slice writes its tokens (slice_bytes u32 per slice:
-    // literal b, or 1 << 31 | (length - 3) << 16 | (distance - 1)), their
-    // count, and a histogram of kHuffSyms u32 (literal/length symbols
-    // 0..285, then distance codes 0..29)
-    uint32_t*      tokens;
-    uint32_t*      tok_count;
-    uint32_t*      hist;
 };
 
 // Where a job's segments go once their sizes are known.
@@ -135,6 +127,14 @@ struct DeflateLaunch {
     uint32_t          total_slices;
     const uint32_t*   counts;      // optional device [njobs, total]
     DeflateJob        single;
+    // dynamic-Huffman mode (single job; k_deflate<.., true>): instead of
+    // bits, each slice writes its tokens (slice_bytes u32 per slice:
+    // literal b, or 1 << 31 | (length - 3) << 16 | (distance - 1)), their
+    // count, and a histogram of kHuffSyms u32 (literal/length symbols
+    // 0..285, then distance codes 0..29)
+    uint32_t*         tokens;
+    uint32_t*         tok_count;
+    uint32_t*         hist;
 };
 
 // Dynamic-Huffman blocks of one job (after k_deflate<.., true>): per

@@ -1227,7 +1227,7 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
 // STATS: also accumulate per-slice match statistics (the strip form's
 // LzStats); a separate instantiation so that the plain and batched forms do
 // not carry their registers.  DYN: the parse's tokens and their symbol
-// histogram go to memory (DeflateJob::tokens / tok_count / hist) for a
+// histogram go to memory (DeflateLaunch::tokens / tok_count / hist) for a
 // dynamic-Huffman block per segment; no bits are emitted here.
 template <int NT, bool STATS, bool DYN = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_deflate(DeflateLaunch DL)
@@ -1414,12 +1414,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
                 const uint32_t L = key[i] >> 16;       // 0: literal
                 const bool tok = ((uint32_t)i == nextp) & (i < le);
-                const uint32_t b = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                 if constexpr (DYN) {                   // raw token; nbits counts tokens
+                    const uint32_t b = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                     tk[i] = !tok ? kNoToken : L ? 0x80000000u | ((L - 3u) << 16) | (0xFFFEu - (key[i] & 0xFFFFu)) : b;
                     nbits += tok ? 1u : 0u;
                 } else {
-                    const uint32_t t = L ? match_tok(key[i]) : lit_tok(b);
+                    const uint32_t t = L ? match_tok(key[i]) : lit_tok((o[i >> 2] >> (8 * (i & 3))) & 0xFFu);
                     tk[i] = tok ? t : 0u;
                     nbits += tok ? t >> 26 : 0u;
                 }
@@ -1452,7 +1452,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             if constexpr (DYN) {
                 // ---- tokens in parse order, and their symbols' histogram --------
-                uint32_t* tp = J.tokens + (int64_t)sl * J.slice_bytes + ntok_slice + (wbase + incl - nbits);
+                uint32_t* tp = DL.tokens + (int64_t)sl * J.slice_bytes + ntok_slice + (wbase + incl - nbits);
 #pragma unroll
                 for (int i = 0; i < LANE; i++) {
                     if (tk[i] == kNoToken) continue;
@@ -1506,10 +1506,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         __syncthreads();
         if constexpr (DYN) {
             for (int i = tid; i < kHuffSyms; i += NT) {    // the histogram out, obuf zero again
-                J.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
+                DL.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
                 hist[i] = 0;
             }
-            if (tid == 0) J.tok_count[sl] = ntok_slice;
+            if (tid == 0) DL.tok_count[sl] = ntok_slice;
         }
         if (!DYN && tid == 0) {
             if (carry_bits) out[out_dw] = carry;
@@ -2427,7 +2427,7 @@ hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_thread
     (void)lane_bytes;
     (void)hipGetLastError();
     const bool stats = !DL.jobs && DL.single.slice_stats;
-    if (!DL.jobs && DL.single.tokens) k_deflate<256, false, true><<<grid, 256, 0, s>>>(DL);
+    if (!DL.jobs && DL.tokens) k_deflate<256, false, true><<<grid, 256, 0, s>>>(DL);
     else if (stats)             k_deflate<256, true><<<grid, 256, 0, s>>>(DL);
     else if (wg_threads == 256) k_deflate<256, false><<<grid, 256, 0, s>>>(DL);
     else                        k_deflate<512, false><<<grid, 512, 0, s>>>(DL);
