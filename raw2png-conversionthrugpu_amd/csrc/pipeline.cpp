@@ -7,6 +7,11 @@
 //   otherwise: strips are windowed+filtered on the GPU (gpu_filter.h) and
 //     deflated by the CPU zlib backend into 32 KiB IDAT chunks, which is the
 //     reference's default path (run_one, :298-443).
+// Output stage (use_gpu_deflate): the PNG leaves the device into a pooled
+// pinned host buffer and a writer thread writes the file, so the frames of
+// a multi-frame DICOM encode while the previous ones are written (the
+// reference's writer stage, pipeline.cpp:830-853); single-file calls wait
+// for their file.
 // Readers: uncompressed little-endian TIFF (image_loader.cpp:59-214 for the
 // strip/byteswap/MINISWHITE semantics) and uncompressed little-endian DICOM
 // Part 10 (dicom_loader.cpp:293-451 for the DicomPixelParams derivation).
@@ -16,10 +21,14 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "compression_backend.h"
@@ -336,10 +345,120 @@ bool grow(uint8_t** p, size_t* cap, size_t need)
     return true;
 }
 
+// One writer thread per encoding thread: jobs of (path, pinned buffer),
+// at most kMaxInFlight of them, buffers reused once written.
+class FileWriteQueue {
+public:
+    static constexpr int kMaxInFlight = 3;
+
+    ~FileWriteQueue()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+        // (pinned buffers are left to process exit: the runtime may be gone)
+    }
+
+    // A host buffer of at least `need` bytes (pinned when possible), once
+    // fewer than kMaxInFlight writes are pending.
+    uint8_t* acquire(size_t need, size_t* cap)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ < kMaxInFlight; });
+        for (size_t i = 0; i < free_.size(); i++)
+            if (free_[i].second >= need) {
+                auto b = free_[i];
+                free_.erase(free_.begin() + (long)i);
+                *cap = b.second;
+                return b.first;
+            }
+        lk.unlock();
+        uint8_t* p = nullptr;
+        const size_t c = need + need / 8 + 4096;
+        if (hipHostMalloc((void**)&p, c, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = (uint8_t*)malloc(c);                // pageable fallback
+            if (!p) return nullptr;
+            std::lock_guard<std::mutex> g(mu_);
+            pageable_.push_back(p);
+        }
+        *cap = c;
+        return p;
+    }
+
+    void submit(const char* path, uint8_t* buf, size_t len, size_t cap)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back({path, buf, len, cap});
+            pending_++;
+            if (!th_.joinable()) th_ = std::thread([this] { loop(); });
+        }
+        cv_.notify_all();
+    }
+
+    // Waits for every submitted write; false if any failed since the last wait.
+    bool wait()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        const bool ok = !failed_;
+        failed_ = false;
+        return ok;
+    }
+
+private:
+    struct Job { std::string path; uint8_t* buf; size_t len, cap; };
+
+    void loop()
+    {
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                j = std::move(q_.front());
+                q_.pop_front();
+            }
+            FILE* f = fopen(j.path.c_str(), "wb");
+            bool ok = f != nullptr;
+            if (!f) fprintf(stderr, "Cannot open output: %s\n", j.path.c_str());
+            else {
+                ok = fwrite(j.buf, 1, j.len, f) == j.len;
+                ok = (fclose(f) == 0) && ok;
+                if (!ok) fprintf(stderr, "write failed: %s\n", j.path.c_str());
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                free_.push_back({j.buf, j.cap});
+                pending_--;
+                if (!ok) failed_ = true;
+            }
+            done_.notify_all();
+        }
+    }
+
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::deque<Job> q_;
+    std::vector<std::pair<uint8_t*, size_t>> free_;
+    std::vector<uint8_t*> pageable_;
+    int pending_ = 0;
+    bool failed_ = false, stop_ = false;
+    std::thread th_;
+};
+thread_local FileWriteQueue tl_writer;
+
 std::atomic<long long> g_files{0}, g_dicom_ms{0}, g_total_ms{0};
 
+// The PNG of one image; with `async` the file is written by the writer
+// thread after this returns (tl_writer.wait() collects the result).
 bool encode_gpu_whole(const ImageInfo& info, const uint8_t* px, const DicomPixelParams* dicom,
-                      const PipelineConfig& cfg, const char* out_path)
+                      const PipelineConfig& cfg, const char* out_path, bool async)
 {
     r2p_image im{};
     im.width = info.width; im.height = info.height; im.channels = info.channels;
@@ -371,13 +490,12 @@ bool encode_gpu_whole(const ImageInfo& info, const uint8_t* px, const DicomPixel
     }
     size_t len = 0;
     if (r2p_encoder_sync(c.enc, &len, nullptr, nullptr) != R2P_OK) return false;
-    std::vector<uint8_t> png(len);
-    if (hipMemcpy(png.data(), c.d_png, len, hipMemcpyDeviceToHost) != hipSuccess) return false;
-    FILE* f = fopen(out_path, "wb");
-    if (!f) { fprintf(stderr, "Cannot open output: %s\n", out_path); return false; }
-    const bool ok = fwrite(png.data(), 1, len, f) == len;
-    fclose(f);
-    return ok;
+    size_t cap = 0;
+    uint8_t* hb = tl_writer.acquire(len, &cap);
+    if (!hb) { fprintf(stderr, "encode: out of host memory\n"); return false; }
+    if (hipMemcpy(hb, c.d_png, len, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    tl_writer.submit(out_path, hb, len, cap);
+    return async ? true : tl_writer.wait();
 }
 
 // strips: GPU filter + CPU zlib (the reference's default path)
@@ -425,7 +543,7 @@ bool encode_gpu_filter_cpu_deflate(ImageSource& src, const PipelineConfig& cfg, 
     return ok;
 }
 
-bool run(ImageSource& src, const char* out_path, const PipelineConfig& cfg)
+bool run(ImageSource& src, const char* out_path, const PipelineConfig& cfg, bool async = false)
 {
     if (cfg.format != ExportFormat::PNG) { fprintf(stderr, "only PNG export is implemented\n"); return false; }
     const ImageInfo& info = src.info();
@@ -438,7 +556,7 @@ bool run(ImageSource& src, const char* out_path, const PipelineConfig& cfg)
         if (r <= 0) return false;
         got += r;
     }
-    return encode_gpu_whole(info, px.data(), src.dicom_params(), cfg, out_path);
+    return encode_gpu_whole(info, px.data(), src.dicom_params(), cfg, out_path, async);
 }
 
 }  // namespace
@@ -489,9 +607,9 @@ bool encode_dicom_all_frames_to_png(const char* in, const char* out_dir, const P
         s.next = 0;
         char path[4096];
         snprintf(path, sizeof(path), "%s/slice_%04d.png", out_dir, i);
-        if (!run(s, path, cfg)) return false;
+        if (!run(s, path, cfg, true)) { (void)tl_writer.wait(); return false; }
     }
-    return true;
+    return tl_writer.wait();                       // every frame's file written
 }
 
 void pipeline_reset_gpu_batch_stats() { g_files = 0; g_dicom_ms = 0; g_total_ms = 0; }

@@ -55,14 +55,19 @@ def test_file_readers_through_pipeline(torch_dev, r2p, synth, oracle, tmp_path, 
     p = oracle.DicomParams(*[getattr(ct.img.dicom, f) for f, _ in ct.img.dicom._fields_])
     exp = oracle.dicom_window(ct.pixels.reshape(-1), p).view(">u2").reshape(128, 128)
     assert np.array_equal(np.array(PILImage.open(str(tmp_path / "ct.png"))).astype(np.int64), exp.astype(np.int64))
-    # multi-frame
-    frames = np.stack([synth.ct_slice(k, 64).pixels for k in range(3)])
-    synth.write_dicom(str(tmp_path / "mf.dcm"), frames, ct.img.dicom, frames=3)
+    # multi-frame: more frames than the output stage keeps in flight
+    nfr = 7
+    frames = np.stack([synth.ct_slice(k, 64).pixels for k in range(nfr)])
+    synth.write_dicom(str(tmp_path / "mf.dcm"), frames, ct.img.dicom, frames=nfr)
     os.makedirs(str(tmp_path / "mf"), exist_ok=True)
     r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "mf"), str(gpu_deflate)],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
-    for k in range(3):
+    # a directory that does not exist: the (asynchronous) writes fail and the call says so
+    r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "missing"), str(gpu_deflate)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    for k in range(nfr):
         got = np.array(PILImage.open(str(tmp_path / "mf" / f"slice_{k:04d}.png"))).astype(np.int64)
         exp = oracle.dicom_window(frames[k].reshape(-1), p).view(">u2").reshape(64, 64)
         assert np.array_equal(got, exp.astype(np.int64))
