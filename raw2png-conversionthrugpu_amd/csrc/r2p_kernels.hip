@@ -1031,12 +1031,17 @@ constexpr int kHashCmpDwords = 2;  // a hashed candidate is compared over at mos
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
 
-// Bits 0..3 flag the zero bytes 0..3 of v.
-__device__ __forceinline__ uint32_t zero_nibble(uint32_t v)
+// Bit 4k + i flags a zero byte i of x[k] (k, i < 4).  Each byte's flag is
+// its bit 7 (0x80 or 0); full-rate dot products against the byte weights
+// 2^i and 2^(4 + i) gather two dwords' flags into one byte, times 128.
+__device__ __forceinline__ uint32_t zero_bytes16(const uint32_t* x)
 {
-    const uint32_t nz = ((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v;   // bit 7 of a byte: byte != 0
-    const uint32_t z = (~nz & 0x80808080u) >> 7;                  // bits 0, 8, 16, 24
-    return ((z * 0x00204081u) >> 21) & 0xFu;                      // gathered into bits 0..3
+    uint32_t z[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) z[k] = ~(((x[k] & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x[k]) & 0x80808080u;
+    const uint32_t lo = __builtin_amdgcn_udot4(z[1], 0x80402010u, __builtin_amdgcn_udot4(z[0], 0x08040201u, 0u, false), false);
+    const uint32_t hi = __builtin_amdgcn_udot4(z[3], 0x80402010u, __builtin_amdgcn_udot4(z[2], 0x08040201u, 0u, false), false);
+    return (lo + (hi << 8)) >> 7;
 }
 
 // The 4 bytes starting at byte i (< 16) of the dword array o; i is a
@@ -1050,14 +1055,15 @@ __device__ __forceinline__ uint32_t bytes_at(const uint32_t* o, int i)
 // the 8 bytes before the lane, X[2..] the lane's own bytes.
 __device__ __forceinline__ uint32_t eq_mask(const uint32_t* X, int d)
 {
-    uint32_t m = 0;
+    static_assert(kLane == 16, "four dwords per lane");
+    uint32_t x[kLane / 4];
 #pragma unroll
     for (int k = 0; k < kLane / 4; k++) {
         const uint32_t prev = d <= 4 ? __builtin_amdgcn_alignbyte(X[2 + k], X[1 + k], 4 - d)
                                      : __builtin_amdgcn_alignbyte(X[1 + k], X[k], 8 - d);
-        m |= zero_nibble(X[2 + k] ^ prev) << (4 * k);
+        x[k] = X[2 + k] ^ prev;
     }
-    return m;
+    return zero_bytes16(x);
 }
 
 // Fixed-Huffman codes (RFC 1951 3.2.6), bit-reversed for the LSB-first
@@ -1108,18 +1114,23 @@ constexpr uint32_t kMinKey = (3u << 16) | (0xFFFFu - 16384u);
 // 257..268 (7-bit codes, one extra bit from length 11), then the 5-bit
 // distance code and its extra bits.  Distance code of d: with dm = d - 1 and
 // e = max(0, floor(log2 dm) - 1) extra bits, code = 2e + (dm >> e).
+// The length part is one byte, looked up by v_perm in two 8-byte tables
+// (lengths 3..10: the reversed 7-bit codes of symbols 257..264; lengths
+// 11..18: those of 265..268 with the extra bit above them); selector bytes
+// 0x0C give zero bytes, so the result is that byte alone.
 __device__ __forceinline__ uint32_t match_tok(uint32_t k)
 {
-    const uint32_t m = (k >> 16) - 3;
+    const uint32_t L = k >> 16;
+    const bool lhi = L >= 11;
+    const uint32_t lcode = lhi ? __builtin_amdgcn_perm(0x9818E868u, 0xA828C848u, L + (0x0C0C0C00u - 11u))
+                               : __builtin_amdgcn_perm(0x08703050u, 0x10602040u, L + (0x0C0C0C00u - 3u));
+    const uint32_t lnb = lhi ? 8u : 7u;
     const uint32_t dm = 0xFFFEu - (k & 0xFFFFu);
     const uint32_t deb = 30u - __builtin_clz(dm | 2u);
     const uint32_t dc = (deb << 1) + (dm >> deb);
     const uint32_t dex = dm & ((1u << deb) - 1u);
-    const bool lhi = m >= 8;
-    const uint32_t sym = lhi ? 9u + ((m >> 1) & 3u) : m + 1u;
-    const uint32_t lnb = lhi ? 8u : 7u;
-    const uint32_t lcode = (__builtin_bitreverse32(sym) >> 25) | ((lhi ? m & 1u : 0u) << 7);
-    return lcode | ((__builtin_bitreverse32(dc) >> 27) << lnb) | (dex << (lnb + 5)) | ((lnb + 5 + deb) << 26);
+    const uint32_t dpart = (__builtin_bitreverse32(dc) >> 27) | (dex << 5);
+    return lcode | (dpart << lnb) | ((lnb + 5 + deb) << 26);
 }
 
 // Bytes of the lane from position i equal to the KW dwords c, up to 4 * KW:
@@ -1396,8 +1407,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const bool row_ok = dr > 1 && dr != bpp && dr <= kWindow && lb >= dr;
                 if (row_ok) {
                     const u32x4 r = g128u(wb + lb - dr);
+                    uint32_t x[LANE / 4];
 #pragma unroll
-                    for (int k = 0; k < LANE / 4; k++) mr |= zero_nibble(o[k] ^ r[k]) << (4 * k);
+                    for (int k = 0; k < LANE / 4; k++) x[k] = o[k] ^ r[k];
+                    mr = zero_bytes16(x);
                 }
                 // runs stop at the lane's end: past it every position "differs"
                 const uint32_t cap = le >= LANE ? 0u : ~0u << le;
