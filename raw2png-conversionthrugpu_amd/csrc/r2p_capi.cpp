@@ -57,7 +57,8 @@ static uint32_t fixed_code(int s, int* nb)
 }
 
 static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev[32],
-                         uint32_t crc4[4][256], uint32_t x8pow[64], uint32_t shift[kCrcShiftLevels][4][256])
+                         uint32_t crc4[4][256], uint32_t x8pow[64], uint32_t shift[kCrcShiftLevels][4][256],
+                         uint32_t shift32[4][256])
 {
     for (int b = 0; b < 256; b++) {
         int nb;
@@ -103,6 +104,9 @@ static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev
     for (int k = 0; k < kCrcShiftLevels; k++)
         for (int b = 0; b < 4; b++)
             for (uint32_t v = 0; v < 256; v++) shift[k][b][v] = mulmod(x8pow[6 + k], v << (8 * b));
+    // the same for 32 zero bytes (the assembly joins two 32-byte CRC chains)
+    for (int b = 0; b < 4; b++)
+        for (uint32_t v = 0; v < 256; v++) shift32[b][v] = mulmod(x8pow[5], v << (8 * b));
 }
 
 static std::mutex g_init_mu;
@@ -351,9 +355,10 @@ int r2p_init(int device)
         set_error("device %d is %s, this build targets gfx950", device, prop.gcnArchName);
         return R2P_ERR_NODEV;
     }
-    static uint32_t lit[256], len[259], dist_rev[32], crc4[4][256], x8pow[64], shift[kCrcShiftLevels][4][256];
-    build_tables(lit, len, dist_rev, crc4, x8pow, shift);
-    R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow, &shift[0][0][0]));
+    static uint32_t lit[256], len[259], dist_rev[32], crc4[4][256], x8pow[64], shift[kCrcShiftLevels][4][256],
+        shift32[4][256];
+    build_tables(lit, len, dist_rev, crc4, x8pow, shift, shift32);
+    R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow, &shift[0][0][0], &shift32[0][0]));
     g_dev_ready[device] = true;
     return R2P_OK;
 }

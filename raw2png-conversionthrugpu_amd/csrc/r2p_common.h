@@ -42,7 +42,8 @@ hipError_t launch_crc(const uint8_t* p, int64_t n, uint32_t* part, uint32_t* res
 hipError_t launch_put_bytes(uint8_t* dst, const uint8_t* bytes16, int n, const uint32_t* crc_be_src,
                             hipStream_t s);
 hipError_t upload_tables(const uint32_t* lit, const uint32_t* len, const uint32_t* dist_rev,
-                         const uint32_t* crc4, const uint32_t* x8pow, const uint32_t* crc_shift);
+                         const uint32_t* crc4, const uint32_t* x8pow, const uint32_t* crc_shift,
+                         const uint32_t* crc_shift32);
 
 // Image geometry resolved from an r2p_image.
 struct Geometry {

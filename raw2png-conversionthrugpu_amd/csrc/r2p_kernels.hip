@@ -33,6 +33,7 @@ __constant__ uint32_t c_dist_rev[32];  // 5-bit reversed distance codes
 __constant__ uint32_t c_crc[4][256];   // slice-by-4 CRC-32 tables
 __constant__ uint32_t c_x8pow[64];     // x^(8 * 2^k) mod P, reflected
 __device__ uint32_t c_crcshift[kCrcShiftLevels][4][256];     // c -> c * x^(8*64*2^k) mod P, by input byte
+__device__ uint32_t c_crcshift32[4][256];                   // c -> c * x^(8*32) mod P, by input byte
 
 // ---------------------------------------------------------------------------
 // Small helpers
@@ -2125,6 +2126,12 @@ __device__ int wave_find_seg(const uint64_t* seg_off, int nseg, int64_t t)
 }
 
 
+// Stage slot of 16-byte group jg of lane l's piece (slots are 16 bytes, four
+// per lane): the group index is XORed with bits 1..2 of the lane, so the
+// lanes' writes of one group (64 bytes apart) and the coalesced reads of
+// consecutive groups both fall on distinct LDS banks within each eight lanes.
+__device__ __forceinline__ int stage_slot(int l, int jg) { return l * 4 + (jg ^ ((l >> 1) & 3)); }
+
 // SELF: one job of at most kAsmScanSlices slices and kAsmScanSegs segments,
 // whose scan every workgroup computes for itself into LDS (k_scan's work,
 // without its launch): slice bit counts and offsets, segment token bits and
@@ -2134,11 +2141,13 @@ template <bool SELF>
 __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL)
 {
     __shared__ uint32_t tab[4][256];
+    __shared__ uint32_t tab32[4][256];                     // the 32-byte shift
     __shared__ uint32_t red[kAssembleThreads / 64];
-    __shared__ u32x4 stage[kAssembleThreads / 64][256];    // 4 KiB per wave
+    __shared__ u32x4 stage[kAssembleThreads / 64][256];    // 4 KiB per wave, by stage_slot
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t (*shf)[4][256] = c_crcshift;     // read through the caches: ~4 lookups per level
-    load_crc_tables(tab);
+    for (int i = tid; i < 1024; i += kAssembleThreads) (&tab32[0][0])[i] = (&c_crcshift32[0][0])[i];
+    load_crc_tables(tab);                          // (its barrier also covers tab32)
 
     AssembleJob S = AL.single;                     // SELF: the job, its scan arrays in LDS
     if constexpr (SELF) {
@@ -2221,6 +2230,9 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         const int64_t wfirst = max64(z0, z1 - (int64_t)(kAssembleThreads - 64 * wave) * kAsmBytes);
         const int sw = wave_find_seg(A.seg_off, A.nseg, min64(max64(wfirst - hdr, 0), max64(segbytes - 1, 0)));
         uint32_t r = 0;
+        bool whole = false;
+        uint32_t slow = 0;                         // this piece's groups that the wave gathers
+        int sc = 0, kc = 0;                        // segment and slice cursors at the first of them
         if (b < e) {
             int s = sw;
             const int64_t a0 = b - hdr;
@@ -2229,7 +2241,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             int k = s * A.spb;                         // slice cursor
             // the usual case: all 64 bytes from one slice's token bits, read
             // with one cursor lookup and four loads in flight
-            bool whole = e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
+            whole = e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
                          (int64_t)A.seg_off[s + 1] >= a0 + kAsmBytes;
             uint32_t pw = 0;
             if (whole) {
@@ -2246,21 +2258,30 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 for (int q = 0; q < 4; q++) t[q] = *(R2P_GLOBAL const u32x4_ua*)(w + 4 * q);
                 const uint32_t t16 = w[16];
                 const uint32_t sh = off & 31;
+                u32x4 v[4];
 #pragma unroll
                 for (int jg = 0; jg < 4; jg++) {
                     const uint32_t nx = jg < 3 ? t[jg + 1][0] : t16;
-                    u32x4 v;
-                    v[0] = __builtin_amdgcn_alignbit(t[jg][1], t[jg][0], sh);
-                    v[1] = __builtin_amdgcn_alignbit(t[jg][2], t[jg][1], sh);
-                    v[2] = __builtin_amdgcn_alignbit(t[jg][3], t[jg][2], sh);
-                    v[3] = __builtin_amdgcn_alignbit(nx, t[jg][3], sh);
-                    stage[wave][lane * 4 + jg] = v;
-                    r = crc_step4(r, v[0], tab);
-                    r = crc_step4(r, v[1], tab);
-                    r = crc_step4(r, v[2], tab);
-                    r = crc_step4(r, v[3], tab);
+                    v[jg][0] = __builtin_amdgcn_alignbit(t[jg][1], t[jg][0], sh);
+                    v[jg][1] = __builtin_amdgcn_alignbit(t[jg][2], t[jg][1], sh);
+                    v[jg][2] = __builtin_amdgcn_alignbit(t[jg][3], t[jg][2], sh);
+                    v[jg][3] = __builtin_amdgcn_alignbit(nx, t[jg][3], sh);
+                    stage[wave][stage_slot(lane, jg)] = v[jg];
                 }
+                // two independent 32-byte chains, joined by the 32-byte shift
+                uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+                for (int jg = 0; jg < 2; jg++)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        r0 = crc_step4(r0, v[jg][u], tab);
+                        r1 = crc_step4(r1, v[jg + 2][u], tab);
+                    }
+                r = crc_shift(r0, tab32) ^ r1;
             }
+            // otherwise 16-byte groups, each from one slice by one read where
+            // it can be; the others (across a slice or segment boundary, or
+            // with framing bytes) are gathered by the wave below
 #pragma unroll 1
             for (int jg = 0; jg < 4 && !whole; jg++) {
                 const int64_t q = b + 16 * jg;
@@ -2277,21 +2298,51 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                     // all 128 bits inside slice k's tokens
                     fast = p + 128 <= A.slice_boff[k] + A.slice_bits[k];
                 }
-                u32x4 v;
                 if (fast) {
                     const uint32_t off = p - A.slice_boff[k];
                     const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
                     const u32x4 lo4 = *(R2P_GLOBAL const u32x4_ua*)w;
                     const uint32_t w4 = w[4];
                     const uint32_t sh = off & 31;
+                    u32x4 v;
                     v[0] = __builtin_amdgcn_alignbit(lo4[1], lo4[0], sh);
                     v[1] = __builtin_amdgcn_alignbit(lo4[2], lo4[1], sh);
                     v[2] = __builtin_amdgcn_alignbit(lo4[3], lo4[2], sh);
                     v[3] = __builtin_amdgcn_alignbit(w4, lo4[3], sh);
+                    stage[wave][stage_slot(lane, jg)] = v;
                 } else {
-                    v = payload16(A, hdr, segbytes, q, (int)min64(16, e - q), s, k);
+                    if (!slow) { sc = s; kc = k; }
+                    slow |= 1u << jg;
                 }
-                stage[wave][lane * 4 + jg] = v;
+            }
+        }
+        // The gathered groups, one at a time for the whole wave: lane i looks
+        // up payload byte i of the group (the lanes' loads in flight
+        // together, where one lane would wait for each 4 bytes in turn), the
+        // bytes are packed by shuffles, and lanes 0, 4, 8, 12 store the
+        // group's dwords into its owner's stage slot.
+        for (uint32_t pend = slow;;) {
+            const uint64_t m = __ballot(pend != 0);
+            if (m == 0) break;
+            const int src = __builtin_ctzll(m);
+            const int jg = __builtin_ctz((uint32_t)__shfl((int)pend, src, 64));
+            if (lane == src) pend &= pend - 1;
+            const int64_t q = (int64_t)__shfl((long long)b, src, 64) + 16 * jg;
+            const int nn = (int)min64(16, (int64_t)__shfl((long long)e, src, 64) - q);
+            int s2 = __shfl(sc, src, 64), k2 = __shfl(kc, src, 64);
+            uint32_t x = 0;
+            if (lane < nn) x = (payload16(A, hdr, segbytes, q + lane, 1, s2, k2)[0] & 0xFFu) << (8 * (lane & 3));
+            x |= (uint32_t)__shfl_down((int)x, 1, 64);
+            x |= (uint32_t)__shfl_down((int)x, 2, 64);
+            if (lane < nn && (lane & 3) == 0) ((uint32_t*)&stage[wave][stage_slot(src, jg)])[lane >> 2] = x;
+        }
+        // the CRC of a piece that is not whole, over its staged groups in order
+        if (b < e && !whole) {
+#pragma unroll 1
+            for (int jg = 0; jg < 4; jg++) {
+                const int64_t q = b + 16 * jg;
+                if (q >= e) break;
+                const u32x4 v = stage[wave][stage_slot(lane, jg)];
                 const int nb = (int)min64(16, e - q);
                 if (nb == 16) {
                     r = crc_step4(r, v[0], tab);
@@ -2312,12 +2363,12 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             const int64_t wb0 = z1 - (int64_t)(kAssembleThreads - 64 * wave) * kAsmBytes;   // wave's first byte
             if (wb0 >= z0) {
 #pragma unroll
-                for (int jg = 0; jg < 4; jg++) st128u(dst + wb0 + 1024 * jg + 16 * lane, stage[wave][64 * jg + lane]);
+                for (int jg = 0; jg < 4; jg++) st128u(dst + wb0 + 1024 * jg + 16 * lane, stage[wave][stage_slot(16 * jg + (lane >> 2), lane & 3)]);
             } else {
                 for (int jg = 0; jg < 4; jg++) {
                     const int64_t q = b + 16 * jg;
                     if (q >= e) break;
-                    const u32x4 v = stage[wave][lane * 4 + jg];
+                    const u32x4 v = stage[wave][stage_slot(lane, jg)];
                     const int nb = (int)min64(16, e - q);
                     if (nb == 16) st128u(dst + q, v);
                     else
@@ -2513,7 +2564,8 @@ hipError_t launch_put_bytes(uint8_t* dst, const uint8_t* bytes16, int n, const u
 }
 
 hipError_t upload_tables(const uint32_t* lit, const uint32_t* len, const uint32_t* dist_rev,
-                         const uint32_t* crc4, const uint32_t* x8pow, const uint32_t* crc_shift)
+                         const uint32_t* crc4, const uint32_t* x8pow, const uint32_t* crc_shift,
+                         const uint32_t* crc_shift32)
 {
     hipError_t e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_lit), lit, sizeof(uint32_t) * 256)) != hipSuccess) return e;
@@ -2522,6 +2574,7 @@ hipError_t upload_tables(const uint32_t* lit, const uint32_t* len, const uint32_
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_crc), crc4, sizeof(uint32_t) * 1024)) != hipSuccess) return e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_crcshift), crc_shift, sizeof(uint32_t) * kCrcShiftLevels * 1024)) != hipSuccess)
         return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_crcshift32), crc_shift32, sizeof(uint32_t) * 1024)) != hipSuccess) return e;
     return hipMemcpyToSymbol(HIP_SYMBOL(c_x8pow), x8pow, sizeof(uint32_t) * 64);
 }
 
