@@ -241,6 +241,27 @@ def test_encode_segment_and_chunk_sizes(torch_dev, r2p, oracle, seg, chunk):
     _check_png(oracle, r2p, png, px, img, filt)
 
 
+@pytest.mark.parametrize("seg,chunk,w,h", [
+    (4096, 333, 1500, 700),        # 257 segments: self-scanned assembly
+    (4096, 777, 3000, 2000),       # 1466 segments: assembly after its own scan launch
+    (8192, 65536, 1500, 700),      # two slices per segment, full-size chunks
+])
+def test_assembly_boundaries(torch_dev, r2p, oracle, seg, chunk, w, h):
+    """Incompressible rows with small segments and odd IDAT chunk sizes: slice,
+    segment and chunk boundaries every few hundred bytes, so most waves of the
+    assembly gather 16-byte groups across them."""
+    log(f"case assembly boundaries seg={seg} chunk={chunk} {w}x{h}")
+    rng = np.random.default_rng(seg + chunk)
+    px = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    px[h // 2: h // 2 + 40, :] = 9                 # a compressible band between random ones
+    img = r2p.Image.make(w, h, 1, 8, 0)
+    for lz77 in (0, 1):
+        opt = r2p.options(segment_bytes=seg, slice_bytes=4096, idat_chunk_bytes=chunk, lz77=lz77)
+        png, filt, adler, nseg, t = encode_device(torch_dev, r2p, px, img, opt)
+        _check_png(oracle, r2p, png, px, img, filt)
+        assert adler == zlib.adler32(filt.tobytes())
+
+
 @pytest.mark.parametrize("seg,slc,w,h", [
     (65536, 4096, 2340, 7),        # 16387 bytes: four full 4 KiB slices and one of 3 bytes
     (8192, 4096, 2340, 7),         # three segments, the last one a 3-byte slice
