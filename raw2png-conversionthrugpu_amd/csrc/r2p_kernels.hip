@@ -2316,25 +2316,45 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 }
             }
         }
-        // The gathered groups, one at a time for the whole wave: lane i looks
-        // up payload byte i of the group (the lanes' loads in flight
-        // together, where one lane would wait for each 4 bytes in turn), the
-        // bytes are packed by shuffles, and lanes 0, 4, 8, 12 store the
-        // group's dwords into its owner's stage slot.
+        // The gathered groups, up to four at a time for the whole wave: lanes
+        // 16j .. 16j + 15 take the j-th pending group (in lane, then group
+        // order), lane 16j + i looking up payload byte i of it (the lanes'
+        // loads in flight together, where one lane would wait for each 4
+        // bytes in turn); the bytes are packed by shuffles and lanes 16j + 0,
+        // 4, 8, 12 store the group's dwords into its owner's stage slot.
         for (uint32_t pend = slow;;) {
-            const uint64_t m = __ballot(pend != 0);
-            if (m == 0) break;
-            const int src = __builtin_ctzll(m);
-            const int jg = __builtin_ctz((uint32_t)__shfl((int)pend, src, 64));
-            if (lane == src) pend &= pend - 1;
+            uint64_t ml = __ballot(pend != 0);
+            if (ml == 0) break;
+            int srcs[4] = {0, 0, 0, 0}, jgs[4] = {0, 0, 0, 0}, cnt = 0;
+            uint32_t cur = 0;
+            int cl = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {          // wave-uniform picks
+                if (cur == 0 && ml) {
+                    cl = __builtin_ctzll(ml);
+                    ml &= ml - 1;
+                    cur = (uint32_t)__builtin_amdgcn_readlane((int)pend, cl);
+                }
+                if (cur == 0) break;
+                srcs[j] = cl; jgs[j] = __builtin_ctz(cur); cur &= cur - 1; cnt++;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < cnt && lane == srcs[j]) pend &= ~(1u << jgs[j]);
+            const int js = lane >> 4, bi = lane & 15;
+            const int src = js == 0 ? srcs[0] : js == 1 ? srcs[1] : js == 2 ? srcs[2] : srcs[3];
+            const int jg = js == 0 ? jgs[0] : js == 1 ? jgs[1] : js == 2 ? jgs[2] : jgs[3];
+            // (every shuffle runs on all lanes: ds_bpermute reads zero from a
+            // source lane that is off in EXEC)
             const int64_t q = (int64_t)__shfl((long long)b, src, 64) + 16 * jg;
-            const int nn = (int)min64(16, (int64_t)__shfl((long long)e, src, 64) - q);
+            const int64_t e_src = (int64_t)__shfl((long long)e, src, 64);
             int s2 = __shfl(sc, src, 64), k2 = __shfl(kc, src, 64);
+            const int nn = js < cnt ? (int)min64(16, e_src - q) : 0;
             uint32_t x = 0;
-            if (lane < nn) x = (payload16(A, hdr, segbytes, q + lane, 1, s2, k2)[0] & 0xFFu) << (8 * (lane & 3));
+            if (bi < nn) x = (payload16(A, hdr, segbytes, q + bi, 1, s2, k2)[0] & 0xFFu) << (8 * (bi & 3));
             x |= (uint32_t)__shfl_down((int)x, 1, 64);
             x |= (uint32_t)__shfl_down((int)x, 2, 64);
-            if (lane < nn && (lane & 3) == 0) ((uint32_t*)&stage[wave][stage_slot(src, jg)])[lane >> 2] = x;
+            if (bi < nn && (bi & 3) == 0) ((uint32_t*)&stage[wave][stage_slot(src, jg)])[bi >> 2] = x;
         }
         // the CRC of a piece that is not whole, over its staged groups in order
         if (b < e && !whole) {
