@@ -545,7 +545,8 @@ struct r2p_encoder {
     uint32_t* d_hdr = nullptr;
     uint32_t* d_hdr_bits = nullptr;
     uint64_t* d_misc = nullptr;      // [0] png length, [1] adler
-    uint64_t* h_misc = nullptr;      // pinned mirror
+    uint64_t* h_misc = nullptr;      // pinned mirror, written by the last kernel (coherent, mapped)
+    uint64_t* dh_misc = nullptr;     // its device address
     uint8_t* d_src_stage = nullptr;  // encode_host staging
     size_t src_stage_cap = 0;
     uint8_t* d_png_stage = nullptr;
@@ -604,7 +605,8 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
         TRY(hipMalloc((void**)&e->d_hdr_bits, ((size_t)e->max_seg + 1) * 4));
     }
     TRY(hipMalloc((void**)&e->d_misc, 64));
-    TRY(hipHostMalloc((void**)&e->h_misc, 64, hipHostMallocDefault));
+    TRY(hipHostMalloc((void**)&e->h_misc, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    TRY(hipHostGetDevicePointer((void**)&e->dh_misc, e->h_misc, 0));
 #undef TRY
     *out = e;
     return R2P_OK;
@@ -692,6 +694,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     A.slice_boff = e->d_slice_boff; A.seg_tbits = e->d_seg_tbits;
     A.seg_off = e->d_seg_off; A.out = (uint8_t*)d_png; A.out_len = e->d_misc;
     A.adler_out = (uint32_t*)(e->d_misc + 1); A.out_cap = (int64_t)png_cap;
+    A.host_res = e->dh_misc;             // the results reach h_misc without a copy
     A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
     A.slice_cap_dw = e->slice_cap_dw; A.slice_bytes = o.slice_bytes;
     A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
@@ -710,7 +713,6 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     if (self_scan) R2P_HIP(launch_assemble_self_scan(AL, std::min(A.max_chunks, 4096), s));
     else R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
     if (e->timing) R2P_HIP(hipEventRecord(e->ev[4], s));
-    R2P_HIP(hipMemcpyAsync(e->h_misc, e->d_misc, 16, hipMemcpyDeviceToHost, s));
     e->last_nseg = (uint32_t)nseg;
     e->timed = e->timing;
     return R2P_OK;

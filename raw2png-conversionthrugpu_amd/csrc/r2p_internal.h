@@ -86,6 +86,7 @@ struct AssembleJob {
     uint8_t*        out;         // PNG (or raw) destination
     uint64_t*       out_len;     // total bytes written (device)
     uint32_t*       adler_out;   // final adler32 (device), optional
+    uint64_t*       host_res;    // optional: the same two results into mapped host memory ([0] length, [1] adler)
     int64_t         out_cap;
     int64_t         total_in;    // uncompressed bytes (for the adler combine)
     int32_t         nseg;

@@ -2007,7 +2007,9 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
             *A.base_slot = b0;
             *A.running = b0 + (uint64_t)z;
         }
-        *A.out_len = out_total(A, z);
+        const uint64_t len = out_total(A, z);
+        *A.out_len = len;
+        if (A.host_res) { A.host_res[0] = len; A.host_res[1] = adler; }
     }
 }
 
@@ -2189,7 +2191,9 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             s_adler = S.total_in ? ((ab % M) << 16) | (aa % M) : 1u;
             if (blockIdx.x == 0) {
                 if (S.adler_out) *S.adler_out = s_adler;
-                *S.out_len = out_total(S, png_payload(S, tot));
+                const uint64_t len = out_total(S, png_payload(S, tot));
+                *S.out_len = len;
+                if (S.host_res) { S.host_res[0] = len; S.host_res[1] = s_adler; }
             }
         }
         __syncthreads();
