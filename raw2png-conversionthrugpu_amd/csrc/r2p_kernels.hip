@@ -1328,11 +1328,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
+                    const int bp = b0 + p * NT * PL;
+                    // all eight positions inside the window with their 4 bytes
+                    // (every lane but those at the window's edges): no test each
+                    if (bp + PL <= s0_rel && bp + PL <= end_rel) {
 #pragma unroll
-                    for (int k = 0; k < 16; k += 2) {
-                        const int P = b0 + p * NT * PL + 4 * k;
-                        if (P < s0_rel && P + 4 <= end_rel)
-                            atomicMax(&T[2 * hash4(v[p][k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
+                        for (int k = 0; k < 16; k += 2)
+                            atomicMax(&T[2 * hash4(v[p][k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)(bp + 4 * k));
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 16; k += 2) {
+                            const int P = bp + 4 * k;
+                            if (P < s0_rel && P + 4 <= end_rel)
+                                atomicMax(&T[2 * hash4(v[p][k >> 2][k & 3])], (tag1 << kPosBits) | (uint32_t)P);
+                        }
                     }
                 }
             }
