@@ -1388,11 +1388,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             for (int i = 0; i < LANE / kInsertStride; i++) hh[i] = 0;
             if (lz && le > 0) {
 #pragma unroll
-                for (int i = 0; i < LANE; i += kInsertStride) {
-                    const uint32_t h = hash4(bytes_at(o, i));
-                    hh[i / kInsertStride] = h;
-                    if ((hmask >> i) & 1u)
-                        atomicMin(&T[2 * h + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
+                for (int i = 0; i < LANE; i += kInsertStride) hh[i / kInsertStride] = hash4(bytes_at(o, i));
+                if (hmask == 0xFFFFu) {            // every position hashable: no test each
+#pragma unroll
+                    for (int i = 0; i < LANE; i += kInsertStride)
+                        atomicMin(&T[2 * hash_at(hh, i) + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < LANE; i += kInsertStride)
+                        if ((hmask >> i) & 1u)
+                            atomicMin(&T[2 * hash_at(hh, i) + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 }
             }
             __syncthreads();
@@ -1469,9 +1474,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // the sub-block's positions (every kInsertStride-th) enter T1; T2's
             // entries expire with its tag
             if (lz && le > 0) {
+                if (hmask == 0xFFFFu) {
 #pragma unroll
-                for (int i = 0; i < LANE; i += kInsertStride)
-                    if ((hmask >> i) & 1u) atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                    for (int i = 0; i < LANE; i += kInsertStride)
+                        atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < LANE; i += kInsertStride)
+                        if ((hmask >> i) & 1u) atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                }
             }
             if constexpr (DYN) {
                 // ---- tokens in parse order, and their symbols' histogram --------
