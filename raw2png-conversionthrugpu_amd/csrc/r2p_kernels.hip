@@ -2041,9 +2041,12 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
 // mode 0 (raw): chunk c copies payload bytes [c*C, (c+1)*C) to out.
 // The chunk's data is cut into 1024 pieces of 64 bytes counted back from its
 // end (the first pieces of a short chunk are partly or wholly empty).
-// Thread t moves piece t straight from the segment buffers to the output in
-// 16-byte pieces (unaligned loads and stores) and folds it into R(piece),
-// the CRC register run from 0 with no final inversion.  R is linear and
+// Thread t moves piece t from the segment buffers to the output in 16-byte
+// groups (unaligned loads, stores coalesced through the wave's LDS stage;
+// groups across a slice or segment boundary are gathered by the wave) and
+// folds it into R(piece), the CRC register run from 0 with no final
+// inversion (a whole piece as two 32-byte chains, R(A|B) below with
+// |B| = 32).  R is linear and
 // leading zeros do not change it, so every piece acts as exactly 64 bytes
 // and the pieces combine in a tree R(A|B) = R(A) * x^(8|B|) + R(B) whose
 // shift at level k is one fixed multiplication by x^(8*64*2^k), done as four
