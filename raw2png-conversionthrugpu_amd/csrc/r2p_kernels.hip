@@ -2171,8 +2171,16 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
     __shared__ u32x4 stage[kAssembleThreads / 64][256];    // 4 KiB per wave, by stage_slot
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t (*shf)[4][256] = c_crcshift;     // read through the caches: ~4 lookups per level
-    for (int i = tid; i < 1024; i += kAssembleThreads) (&tab32[0][0])[i] = (&c_crcshift32[0][0])[i];
-    load_crc_tables(tab);                          // (its barrier also covers tab32)
+    // the CRC tables (one entry of each per thread): SELF stores them in its
+    // scan, before the scan's first barrier, so that their loads and the
+    // scan's are in flight together; otherwise they are stored here
+    static_assert(kAssembleThreads == 1024, "one entry of each table per thread");
+    const uint32_t t_crc = (&c_crc[0][0])[tid], t_s32 = (&c_crcshift32[0][0])[tid];
+    if constexpr (!SELF) {
+        (&tab[0][0])[tid] = t_crc;
+        (&tab32[0][0])[tid] = t_s32;
+        __syncthreads();
+    }
 
     AssembleJob S = AL.single;                     // SELF: the job, its scan arrays in LDS
     if constexpr (SELF) {
@@ -2198,6 +2206,8 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             v = seg_len_bytes(tb, (i == S.nseg - 1) && S.final_block, S.eob_in_bits);
         }
         for (int k = tid; k < S.nslice; k += kAssembleThreads) adler_terms(S, k, sa, sb);
+        (&tab[0][0])[tid] = t_crc;
+        (&tab32[0][0])[tid] = t_s32;
         const uint32_t incl = wave_incl_scan(v);
         sa = wave_total(sa) % M;
         sb = wave_total(sb) % M;
