@@ -1358,6 +1358,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int64_t out_dw = 0;
         uint32_t ntok_slice = 0;                   // DYN: tokens written so far
         uint32_t* hist = obuf;                     // DYN: the slice's histogram (obuf is zero)
+        // The flush of a sub-block's bits waits for the next sub-block's first
+        // barrier (or the slice end's), which then also orders it before the
+        // next emission: one barrier fewer per sub-block.
+        bool pending = false;                      // a sub-block's bits not yet flushed
+        uint32_t pend_tot = 0;                     // their count (carry excluded)
+        auto flush = [&]() {
+            const uint32_t tb = carry_bits + pend_tot;
+            const uint32_t full = tb >> 5;
+            // the flush clears what it read, so the next emission finds obuf
+            // zero; only thread 0 keeps the carry
+            for (uint32_t i = tid; i < full; i += NT) { out[out_dw + i] = obuf[i]; obuf[i] = 0; }
+            if (tid == 0) { carry = (tb & 31) ? obuf[full] : 0u; obuf[full] = 0; }
+            carry_bits = tb & 31;
+            out_dw += full;
+            pending = false;
+        };
 
         for (int q = s0_rel; q < s1_rel; q += SUB) {
             const int n = min(SUB, s1_rel - q);
@@ -1402,7 +1418,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             __syncthreads();
             // ---- B. best match per position, greedy parse of the lane -----------
-            if (!DYN && tid == 0) obuf[0] = carry;
+            if constexpr (!DYN) {
+                if (pending) flush();
+                if (tid == 0) obuf[0] = carry;
+            }
             uint32_t key[LANE];
 #pragma unroll
             for (int i = 0; i < LANE; i++) key[i] = 0;
@@ -1525,19 +1544,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 if (nacc > 0) atomicOr(&obuf[wi], (uint32_t)acc);
             }
-            __syncthreads();
-            const uint32_t tb = carry_bits + tot;
-            const uint32_t full = tb >> 5;
-            // the flush clears what it read, so the next sub-block (past its
-            // first barrier) finds obuf zero; only thread 0 keeps the carry
-            for (uint32_t i = tid; i < full; i += NT) { out[out_dw + i] = obuf[i]; obuf[i] = 0; }
-            if (tid == 0) { carry = (tb & 31) ? obuf[full] : 0u; obuf[full] = 0; }
-            carry_bits = tb & 31;
-            out_dw += full;
+            pending = true;
+            pend_tot = tot;
         }
 
         // ---- slice end: the partial last word and the bit count ---------------
         __syncthreads();
+        if constexpr (!DYN) {
+            if (pending) flush();
+        }
         if constexpr (DYN) {
             for (int i = tid; i < kHuffSyms; i += NT) {    // the histogram out, obuf zero again
                 DL.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
