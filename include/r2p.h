@@ -182,6 +182,10 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
 int r2p_deflate_sync(r2p_deflate* d);
 const void* r2p_deflate_output(const r2p_deflate* d);
 size_t r2p_deflate_output_bytes(const r2p_deflate* d);
+/* Bytes the output buffer holds (the bound fixed at create time).  No
+ * reference counterpart: lets the facade register the buffer for its
+ * cross-stream ordering. */
+size_t r2p_deflate_output_capacity(const r2p_deflate* d);
 uint32_t r2p_deflate_adler(const r2p_deflate* d);     /* adler32 of all input so far */
 int r2p_deflate_copy_to_host(const r2p_deflate* d, void* h_dst, size_t n);
 /* Drops the output delivered so far: the next strip's bytes start at offset 0

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "compression_backend.h"
@@ -71,6 +72,111 @@ r2p_image strip_image(int width, int bpp, int rows, const DicomPixelParams* dico
     return im;
 }
 
+// ---------------------------------------------------------------------------
+// Cross-stream ordering of device buffers handed between contexts.
+//
+// Every context owns its stream, and the reference hands device buffers from
+// one context to another -- the filter output to DEFLATE and Adler-32, the
+// DEFLATE output to IDAT assembly -- with no event between the streams
+// (pipeline.cpp:618-623, 689-698: the last strip's filter can still be
+// running when the consumers start).  Here a producer registers its output
+// buffer; after each write it records `written` on its stream.  A consumer
+// whose input lies in a registered buffer makes its own stream wait on
+// `written`, and after enqueuing its reads records its own event, which the
+// producer's next write waits on.  The reference's call order, unchanged,
+// is then free of both races (read-before-write and overwrite-while-read).
+// ---------------------------------------------------------------------------
+struct BufferOrder {
+    const uint8_t* base = nullptr;
+    size_t size = 0;
+    hipEvent_t written = nullptr;
+    std::vector<hipEvent_t> readers;   // consumers' events, recorded after their reads
+};
+std::mutex g_order_mu;
+std::vector<BufferOrder*> g_orders;
+
+BufferOrder* order_register(const void* base, size_t size)
+{
+    auto* o = new BufferOrder();
+    o->base = (const uint8_t*)base;
+    o->size = size;
+    hchk(hipEventCreateWithFlags(&o->written, hipEventDisableTiming), "order_register");
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    g_orders.push_back(o);
+    return o;
+}
+
+void order_unregister(BufferOrder* o)
+{
+    if (!o) return;
+    {
+        std::lock_guard<std::mutex> lk(g_order_mu);
+        g_orders.erase(std::remove(g_orders.begin(), g_orders.end(), o), g_orders.end());
+    }
+    (void)hipEventDestroy(o->written);
+    delete o;
+}
+
+// A producer about to write its buffer on stream s: wait for the reads
+// enqueued so far.
+void order_before_write(BufferOrder* o, hipStream_t s)
+{
+    std::vector<hipEvent_t> r;
+    {
+        std::lock_guard<std::mutex> lk(g_order_mu);
+        r.swap(o->readers);
+    }
+    for (hipEvent_t e : r) hchk(hipStreamWaitEvent(s, e, 0), "order_before_write");
+}
+
+// A producer's write is enqueued on stream s.
+void order_after_write(BufferOrder* o, hipStream_t s)
+{
+    hchk(hipEventRecord(o->written, s), "order_after_write");
+}
+
+// A consumer on stream s about to read [p, p + n): its stream waits for the
+// producer's last write.  Returns the buffer's entry (nullptr: not a
+// registered buffer, e.g. the caller's own device memory).
+BufferOrder* order_before_read(const void* p, size_t n, hipStream_t s)
+{
+    const uint8_t* q = (const uint8_t*)p;
+    BufferOrder* hit = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_order_mu);
+        for (BufferOrder* o : g_orders)
+            if (q >= o->base && q < o->base + o->size && n <= (size_t)(o->base + o->size - q)) { hit = o; break; }
+    }
+    if (hit) hchk(hipStreamWaitEvent(s, hit->written, 0), "order_before_read");
+    return hit;
+}
+
+// The consumer's reads are enqueued on stream s: `mine` marks their end for
+// the producer's next write.
+void order_after_read(BufferOrder* o, hipEvent_t mine, hipStream_t s)
+{
+    if (!o) return;
+    hchk(hipEventRecord(mine, s), "order_after_read");
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    if (std::find(o->readers.begin(), o->readers.end(), mine) == o->readers.end()) o->readers.push_back(mine);
+}
+
+// A consumer context goes away: its event leaves every reader list.
+void order_forget_reader(hipEvent_t mine)
+{
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    for (BufferOrder* o : g_orders)
+        o->readers.erase(std::remove(o->readers.begin(), o->readers.end(), mine), o->readers.end());
+}
+
+// An event for a consumer context (no timing).
+hipEvent_t reader_event(const char* where)
+{
+    hipEvent_t e = nullptr;
+    hchk(hipEventCreateWithFlags(&e, hipEventDisableTiming), where);
+    return e;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -85,6 +191,8 @@ struct GpuFilterContext {
     uint8_t* d_selected = nullptr;
     uint8_t* h_output = nullptr;
     hipEvent_t ev[4] = {};
+    BufferOrder* out_order = nullptr;   // d_selected, read by other contexts' streams
+    hipEvent_t ev_read = nullptr;       // end of this context's reads of other contexts' buffers
 };
 
 GpuFilterContext* gpu_filter_create(int width, int bpp, int strip_height, bool device_output_only)
@@ -102,6 +210,8 @@ GpuFilterContext* gpu_filter_create(int width, int bpp, int strip_height, bool d
     hchk(hipMalloc((void**)&c->d_selected, out_b + 16), "gpu_filter_create");
     if (!device_output_only) hchk(hipHostMalloc((void**)&c->h_output, out_b + 16, hipHostMallocDefault), "gpu_filter_create");
     for (auto& e : c->ev) hchk(hipEventCreate(&e), "gpu_filter_create");
+    c->out_order = order_register(c->d_selected, out_b + 16);
+    c->ev_read = reader_event("gpu_filter_create");
     gpu_filter_reset(c);
     hchk(hipStreamSynchronize(c->stream), "gpu_filter_create");
     return c;
@@ -111,6 +221,9 @@ void gpu_filter_destroy(GpuFilterContext* c)
 {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
+    order_unregister(c->out_order);
+    order_forget_reader(c->ev_read);
+    (void)hipEventDestroy(c->ev_read);
     (void)hipFree(c->d_input); (void)hipFree(c->d_prior[0]); (void)hipFree(c->d_prior[1]);
     (void)hipFree(c->d_selected);
     if (c->h_output) (void)hipHostFree(c->h_output);
@@ -141,11 +254,13 @@ static const uint8_t* filter_run(GpuFilterContext* c, int rows, GpuTimings* t, c
                                  bool want_host)
 {
     r2p_image im = strip_image(c->width, c->bpp, rows, dicom);
+    order_before_write(c->out_order, c->stream);        // earlier strips' readers are done
     hchk(hipEventRecord(c->ev[1], c->stream), "filter");
     chk(r2p_filter_rows(&im, c->d_input, rows, c->d_prior[c->cur], c->d_selected, c->d_prior[c->cur ^ 1], 0,
                         c->stream), "gpu_filter_process");
     c->cur ^= 1;
     hchk(hipEventRecord(c->ev[2], c->stream), "filter");
+    order_after_write(c->out_order, c->stream);         // consumers on other streams wait for this
     const size_t out_b = gpu_filter_output_size(c, rows);
     if (want_host) hchk(hipMemcpyAsync(c->h_output, c->d_selected, out_b, hipMemcpyDeviceToHost, c->stream), "filter");
     hchk(hipEventRecord(c->ev[3], c->stream), "filter");
@@ -164,9 +279,14 @@ const uint8_t* gpu_filter_process_from_device(GpuFilterContext* c, const uint8_t
 {
     if (rows <= 0 || rows > c->strip_height) die("gpu_filter_process_from_device: rows");
     hchk(hipEventRecord(c->ev[0], c->stream), "filter");
+    BufferOrder* src = order_before_read(d_input, (size_t)rows * c->width_bytes, c->stream);
     hchk(hipMemcpyAsync(c->d_input, d_input, (size_t)rows * c->width_bytes, hipMemcpyDeviceToDevice, c->stream), "filter");
-    if (d_prior_row)
+    order_after_read(src, c->ev_read, c->stream);
+    if (d_prior_row) {
+        BufferOrder* pr = order_before_read(d_prior_row, c->width_bytes, c->stream);
         hchk(hipMemcpyAsync(c->d_prior[c->cur], d_prior_row, c->width_bytes, hipMemcpyDeviceToDevice, c->stream), "filter");
+        order_after_read(pr, c->ev_read, c->stream);
+    }
     return filter_run(c, rows, t, dicom, c->h_output != nullptr);
 }
 
@@ -207,6 +327,7 @@ struct GpuChecksumCtx {
     uint32_t* d_res = nullptr;
     uint32_t* h_res = nullptr;
     bool pending = false;
+    hipEvent_t ev_read = nullptr;
 };
 struct GpuAdler32Context : GpuChecksumCtx {};
 struct GpuCrc32Context : GpuChecksumCtx {};
@@ -221,10 +342,13 @@ static void cks_init(GpuChecksumCtx* c, size_t max_in)
     hchk(hipMalloc((void**)&c->d_work, r2p_checksum_work_bytes(max_in)), "checksum_create");
     hchk(hipMalloc((void**)&c->d_res, 16), "checksum_create");
     hchk(hipHostMalloc((void**)&c->h_res, 16, hipHostMallocDefault), "checksum_create");
+    c->ev_read = reader_event("checksum_create");
 }
 static void cks_free(GpuChecksumCtx* c)
 {
     (void)hipStreamSynchronize(c->stream);
+    order_forget_reader(c->ev_read);
+    (void)hipEventDestroy(c->ev_read);
     (void)hipFree(c->d_work); (void)hipFree(c->d_res); (void)hipHostFree(c->h_res);
     (void)hipStreamDestroy(c->stream);
 }
@@ -247,7 +371,9 @@ void gpu_adler32_launch(GpuAdler32Context* c, const uint8_t* d_data, size_t len)
         fprintf(stderr, "gpu_adler32_launch: len %zu exceeds max_input_bytes %zu\n", len, c->max_in);
         exit(1);
     }
+    BufferOrder* src = order_before_read(d_data, len, c->stream);
     chk(r2p_adler32_device(d_data, len, c->d_work, c->d_res, c->stream), "gpu_adler32_launch");
+    order_after_read(src, c->ev_read, c->stream);
     hchk(hipMemcpyAsync(c->h_res, c->d_res, 4, hipMemcpyDeviceToHost, c->stream), "gpu_adler32_launch");
     c->pending = true;
 }
@@ -282,7 +408,9 @@ uint32_t gpu_crc32_compute(GpuCrc32Context* c, const uint8_t* d, size_t len)
         fprintf(stderr, "gpu_crc32_compute: len %zu exceeds max_input_bytes %zu\n", len, c->max_in);
         exit(1);
     }
+    BufferOrder* src = order_before_read(d, len, c->stream);
     chk(r2p_crc32_device(d, len, c->d_work, c->d_res, c->stream), "gpu_crc32_compute");
+    order_after_read(src, c->ev_read, c->stream);
     hchk(hipMemcpyAsync(c->h_res, c->d_res, 4, hipMemcpyDeviceToHost, c->stream), "gpu_crc32_compute");
     hchk(hipStreamSynchronize(c->stream), "gpu_crc32_compute");
     return c->h_res[0];
@@ -297,6 +425,8 @@ struct GpuDeflateContext {
     GpuDeflatePhaseStats ph;
     hipEvent_t ev[2] = {};
     bool timed = false;
+    BufferOrder* out_order = nullptr;   // the output stream's buffer, read by IDAT assembly
+    hipEvent_t ev_read = nullptr;       // end of the reads of the filter's output
 };
 
 GpuDeflateContext* gpu_deflate_create(int max_rows, int max_row_bytes, size_t max_total_bits, bool use_lz77)
@@ -306,11 +436,17 @@ GpuDeflateContext* gpu_deflate_create(int max_rows, int max_row_bytes, size_t ma
     const size_t total_in = max_total_bits / 8 + 1;
     chk(r2p_deflate_create(strip, std::max(total_in, strip), use_lz77 ? 1 : 0, 65536, &c->d), "gpu_deflate_create");
     for (auto& e : c->ev) hchk(hipEventCreate(&e), "gpu_deflate_create");
+    c->out_order = order_register(r2p_deflate_output(c->d), r2p_deflate_output_capacity(c->d));
+    c->ev_read = reader_event("gpu_deflate_create");
     return c;
 }
 void gpu_deflate_destroy(GpuDeflateContext* c)
 {
     if (!c) return;
+    (void)r2p_deflate_sync(c->d);
+    order_unregister(c->out_order);
+    order_forget_reader(c->ev_read);
+    (void)hipEventDestroy(c->ev_read);
     r2p_deflate_destroy(c->d);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
     delete c;
@@ -324,10 +460,14 @@ void gpu_deflate_compress_strip(GpuDeflateContext* c, const uint8_t* d_filtered,
                                 bool is_last)
 {
     hipStream_t s = (hipStream_t)r2p_deflate_stream(c->d);
+    const size_t n = (size_t)rows * (size_t)row_bytes;
+    BufferOrder* src = order_before_read(d_filtered, n, s);   // e.g. the filter's last strip
+    order_before_write(c->out_order, s);                      // assembly reads of earlier output
     hchk(hipEventRecord(c->ev[0], s), "compress_strip");
-    chk(r2p_deflate_strip(c->d, d_filtered, (size_t)rows * (size_t)row_bytes, is_last ? 1 : 0, 1, row_bytes),
-        "gpu_deflate_compress_strip");
+    chk(r2p_deflate_strip(c->d, d_filtered, n, is_last ? 1 : 0, 1, row_bytes), "gpu_deflate_compress_strip");
     hchk(hipEventRecord(c->ev[1], s), "compress_strip");
+    order_after_read(src, c->ev_read, s);
+    order_after_write(c->out_order, s);
     c->timed = true;
 }
 void gpu_deflate_stream_sync(GpuDeflateContext* c)
@@ -370,6 +510,7 @@ struct GpuPngAssembleContext {
     uint8_t* d_buf = nullptr;
     uint8_t* d_work = nullptr;
     GpuPngAssembleStats st;
+    hipEvent_t ev_read = nullptr;
 };
 
 GpuPngAssembleContext* gpu_png_assemble_create(size_t max_chunk_data_bytes)
@@ -382,12 +523,15 @@ GpuPngAssembleContext* gpu_png_assemble_create(size_t max_chunk_data_bytes)
     hchk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "gpu_png_assemble_create");
     hchk(hipMalloc((void**)&c->d_buf, c->cap), "gpu_png_assemble_create");
     hchk(hipMalloc((void**)&c->d_work, r2p_checksum_work_bytes(c->cap)), "gpu_png_assemble_create");
+    c->ev_read = reader_event("gpu_png_assemble_create");
     return c;
 }
 void gpu_png_assemble_destroy(GpuPngAssembleContext* c)
 {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
+    order_forget_reader(c->ev_read);
+    (void)hipEventDestroy(c->ev_read);
     (void)hipFree(c->d_buf); (void)hipFree(c->d_work); (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -396,8 +540,10 @@ size_t gpu_png_assemble_idat_chunk(GpuPngAssembleContext* c, const uint8_t* d_sl
 {
     auto t0 = Clock::now();
     size_t len = 0;
+    BufferOrder* src = order_before_read(d_slice, n, c->stream);
     chk(r2p_idat_chunk(d_slice, n, is_first, is_last, adler, level, c->d_buf, c->cap, c->d_work, &len, c->stream),
         "gpu_png_assemble_idat_chunk");
+    order_after_read(src, c->ev_read, c->stream);
     c->st.chunks++;
     c->st.data_d2d_us += us_since(t0);
     return len;

@@ -251,6 +251,7 @@ bool load_dicom(const char* path, int frame, BufferSource& s, int* nframes_out)
     // file meta group (always explicit VR little endian)
     while (pos + 8 <= n && rd16(p + pos) == 0x0002) {
         if (!dcm_next(p, n, pos, true, e)) return false;
+        if (e.len == 0xFFFFFFFFu || e.off + e.len > n) { fprintf(stderr, "DICOM: malformed file meta\n"); return false; }
         if (e.tag == 0x00020010u) ts.assign((const char*)p + e.off, e.len);
         pos = e.off + e.len;
     }
@@ -274,6 +275,10 @@ bool load_dicom(const char* path, int frame, BufferSource& s, int* nframes_out)
         if (e.off + e.len > n) return false;
         const uint8_t* v = p + e.off;
         bool ok = false;
+        // the US attributes below need their two value bytes
+        const bool us_tag = e.tag == 0x00280002u || e.tag == 0x00280010u || e.tag == 0x00280011u ||
+                            (e.tag >= 0x00280100u && e.tag <= 0x00280103u);
+        if (us_tag && e.len < 2) { fprintf(stderr, "DICOM: short value of (0028,%04X)\n", e.tag & 0xFFFFu); return false; }
         switch (e.tag) {
         case 0x00280010u: rows = rd16(v); break;
         case 0x00280011u: cols = rd16(v); break;

@@ -40,7 +40,7 @@ struct LutKey {
     }
 };
 
-constexpr int kLutSlot = 65536;      // bytes per table slot (the largest table)
+constexpr int kLutSlot = 131072;     // bytes per table slot (the largest table: 16 bits stored, u16 out)
 constexpr int kNarrowRow = 4096;     // rows up to this many bytes go to the first filter launch
 
 }  // namespace
@@ -105,6 +105,9 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     FilterLaunch LW = L;                          // the wide rows: bands after the narrow ones
     LW.band_rows = b->band_rows_w; LW.row_cap = b->row_cap_w; LW.counts = b->d_counts + 3;
     R2P_HIP(launch_filter(LW, 4096, filter_lds_bytes(LW.band_rows, LW.row_cap, LW.lut_cap), s));
+    FilterLaunch LH = L;                          // rows too wide for LDS: one row per workgroup, tiled
+    LH.band_rows = 1; LH.row_cap = kTileBytes; LH.counts = b->d_counts + 10;
+    R2P_HIP(launch_filter_tiled(LH, 2048, filter_tiled_lds_bytes(LH.lut_cap), s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
     // 3. deflate over all slices
     DeflateLaunch DL{};
@@ -239,27 +242,31 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         int rc = resolve(&imgs[i], &geo[i]);
         if (rc) { set_error("image %d: %s", i, r2p_last_error()); return rc; }
         if ((rc = plan_window(&imgs[i], &plan[i]))) return rc;
-        if (geo[i].row_bytes > (size_t)kMaxRowBytesLds) { set_error("image %d row too wide", i); return R2P_ERR_UNSUPPORTED; }
         if (plan[i].lut_bytes > kLutSlot) { set_error("image %d table too large", i); return R2P_ERR_UNSUPPORTED; }
         total_rows += imgs[i].height;
         total_f += geo[i].filtered;
         if (!d_src[i]) { set_error("image %d has no source", i); return R2P_ERR_ARG; }
     }
     if (total_f > b->max_filtered) { set_error("batch needs %zu filtered bytes > %zu", total_f, b->max_filtered); return R2P_ERR_SIZE; }
-    // the filter's two classes (narrow rows first in the job table) and
-    // their band heights
+    // the filter's three classes, in this order in the job table: rows of up
+    // to kNarrowRow bytes, rows up to kMaxRowBytesLds, wider rows (tiled);
+    // the first two get their own band heights
+    auto row_class = [&](int i) {
+        const size_t rc = (geo[i].row_bytes + 15) & ~(size_t)15;
+        return rc <= (size_t)kNarrowRow ? 0 : rc <= (size_t)kMaxRowBytesLds ? 1 : 2;
+    };
     std::vector<int> forder;
     {
         int rn = 16, rw = 16, lmax = 0;
         int64_t rows_n = 0, rows_w = 0;
         for (int i = 0; i < n; i++) {
-            const int rc = (int)((geo[i].row_bytes + 15) & ~(size_t)15);
-            if (rc <= kNarrowRow) { rn = std::max(rn, rc); rows_n += imgs[i].height; }
-            else { rw = std::max(rw, rc); rows_w += imgs[i].height; }
-            if (plan[i].lut_kind >= 0) lmax = std::max(lmax, (plan[i].lut_bytes + 15) & ~15);
+            const int rc = (int)std::min<size_t>((geo[i].row_bytes + 15) & ~(size_t)15, (size_t)kMaxRowBytesLds);
+            if (row_class(i) == 0) { rn = std::max(rn, rc); rows_n += imgs[i].height; }
+            else if (row_class(i) == 1) { rw = std::max(rw, rc); rows_w += imgs[i].height; }
+            if (plan[i].lut_kind >= 0) lmax = std::max(lmax, (plan[i].lds_bytes + 15) & ~15);
         }
-        for (int i = 0; i < n; i++) if ((int)((geo[i].row_bytes + 15) & ~(size_t)15) <= kNarrowRow) forder.push_back(i);
-        for (int i = 0; i < n; i++) if ((int)((geo[i].row_bytes + 15) & ~(size_t)15) > kNarrowRow) forder.push_back(i);
+        for (int c = 0; c < 3; c++)
+            for (int i = 0; i < n; i++) if (row_class(i) == c) forder.push_back(i);
         b->row_cap = rn; b->row_cap_w = rw; b->lut_cap = lmax;
         b->band_rows = filter_band_rows(o.band_rows, rows_n, rn, lmax);
         b->band_rows_w = filter_band_rows(o.band_rows, rows_w, rw, lmax);
@@ -296,8 +303,8 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         F.src = (const uint8_t*)d_src[i]; F.dst = b->d_filt + foff; F.prior = nullptr; F.last_row = nullptr;
         F.lut = lut; F.src_pitch = g.pitch; F.row_bytes = (int32_t)g.row_bytes; F.rows = (int32_t)imgs[i].height;
         F.bpp = g.bpp; F.wmode = wp.wmode; F.shift = wp.shift; F.mask = wp.mask; F.q15_a = wp.q15_a;
-        F.q15_b = wp.q15_b; F.pixel_rep = wp.pixel_rep; F.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
-        F.band_rows = (int)((g.row_bytes + 15) & ~(size_t)15) <= kNarrowRow ? b->band_rows : b->band_rows_w;
+        F.q15_b = wp.q15_b; F.pixel_rep = wp.pixel_rep; F.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
+        F.band_rows = row_class(i) == 0 ? b->band_rows : row_class(i) == 1 ? b->band_rows_w : 1;
 
         const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
         const int nseg = sg.nseg;
@@ -340,16 +347,20 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         segs += (size_t)nseg;
         slices += (size_t)sg.nslice;
     }
-    // band prefix in job-table order: the narrow class, then the wide one
-    uint32_t bands = 0, bands_n = 0;
+    // band prefix in job-table order: the narrow class, the wide one, the tiled one
+    uint32_t bands = 0, bands_n = 0, bands_w = 0;
     for (int k = 0; k < n; k++) {
         bs[k] = bands;
         bands += (uint32_t)((fj[k].rows + fj[k].band_rows - 1) / fj[k].band_rows);
-        if (fj[k].row_bytes <= kNarrowRow) bands_n = bands;
+        const int c = row_class(forder[(size_t)k]);
+        if (c == 0) bands_n = bands;
+        if (c <= 1) bands_w = bands;
     }
+    bands_w = std::max(bands_w, bands_n);
     bs[n] = bands; ss[n] = (uint32_t)slices; cs[n] = chunks;
     cnt[0] = (uint32_t)n; cnt[1] = bands_n; cnt[2] = 0;            // filter, narrow rows
-    cnt[3] = (uint32_t)n; cnt[4] = bands; cnt[5] = bands_n;         // filter, wide rows
+    cnt[3] = (uint32_t)n; cnt[4] = bands_w; cnt[5] = bands_n;       // filter, wide rows
+    cnt[10] = (uint32_t)n; cnt[11] = bands; cnt[12] = bands_w;      // filter, tiled rows
     cnt[6] = (uint32_t)n; cnt[7] = (uint32_t)slices;
     cnt[8] = (uint32_t)n; cnt[9] = chunks;
     b->n = n;

@@ -216,7 +216,7 @@ int plan_window(const r2p_image* img, WindowPlan* wp)
         if (rc) return rc;
         if (p.bits_allocated == 8) {
             wp->wmode = WM_LUT8_FROM8;
-            wp->lut_kind = 0; wp->lut_entries = 256; wp->lut_bytes = 256;
+            wp->lut_kind = 0; wp->lut_entries = 256; wp->lut_bytes = 256; wp->lds_bytes = 256;
             return R2P_OK;
         }
         const int sh = p.high_bit - p.bits_stored + 1;
@@ -226,9 +226,12 @@ int plan_window(const r2p_image* img, WindowPlan* wp)
         wp->lut_kind = 0;
         wp->lut_entries = 1 << p.bits_stored;
         wp->lut_bytes = 2 * wp->lut_entries;
+        wp->lds_bytes = wp->lut_bytes;
         if (wp->lut_bytes > 65536) {
-            set_error("16-bit stored DICOM window table exceeds LDS budget");
-            return R2P_ERR_UNSUPPORTED;
+            // 16 bits stored: the 128 KiB table stays in global memory (read
+            // through the caches) instead of taking the LDS the rows need
+            wp->wmode = WM_LUT16G;
+            wp->lds_bytes = 0;
         }
         return R2P_OK;
     }
@@ -246,6 +249,7 @@ int plan_window(const r2p_image* img, WindowPlan* wp)
         wp->lut_entries = 1 << p.bits_stored;
         wp->mask = (uint32_t)(wp->lut_entries - 1);
         wp->lut_bytes = wp->lut_entries;
+        wp->lds_bytes = wp->lut_bytes;
         return R2P_OK;
     }
     default: set_error("bad pix_mode"); return R2P_ERR_ARG;
@@ -295,6 +299,11 @@ uint32_t idat_prefix_full(int chunk_bytes)
 size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap)
 {
     return (size_t)lut_cap + (size_t)(band_rows + 1) * (size_t)(row_cap + 32);
+}
+
+size_t filter_tiled_lds_bytes(int lut_cap)
+{
+    return (size_t)lut_cap + 2 * (size_t)(kTileBytes + 32);
 }
 
 int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap)
@@ -449,8 +458,17 @@ uint32_t r2p_crc32_combine(uint32_t c1, uint32_t c2, uint64_t len2)
 static int launch_filter_job(const FilterJob& J, int band_rows, hipStream_t s)
 {
     if (J.row_bytes > kMaxRowBytesLds) {
-        set_error("row of %d bytes exceeds the %d-byte LDS row buffer", J.row_bytes, kMaxRowBytesLds);
-        return R2P_ERR_UNSUPPORTED;
+        // rows too wide for LDS: one row per workgroup, in tiles
+        FilterLaunch L{};
+        L.njobs = 1;
+        L.band_rows = 1;
+        L.lut_cap = (J.lut_bytes + 15) & ~15;
+        L.row_cap = kTileBytes;
+        L.total_bands = (uint32_t)J.rows;
+        L.single = J;
+        L.single.band_rows = 1;
+        R2P_HIP(launch_filter_tiled(L, (int)std::min<uint32_t>(L.total_bands, 4096), filter_tiled_lds_bytes(L.lut_cap), s));
+        return R2P_OK;
     }
     FilterLaunch L{};
     L.jobs = nullptr;
@@ -488,7 +506,7 @@ int r2p_filter_rows(const r2p_image* img, const void* d_src, int rows, const voi
     J.lut = cache.d_lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes; J.rows = rows;
     J.bpp = g.bpp; J.wmode = wp.wmode; J.shift = wp.shift; J.mask = wp.mask;
     J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
-    J.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
+    J.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
     return launch_filter_job(J, band_rows, s);
 }
 
@@ -659,7 +677,7 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     J.lut = e->lut.d_lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes;
     J.rows = (int32_t)img->height; J.bpp = g.bpp; J.wmode = wp.wmode; J.shift = wp.shift;
     J.mask = wp.mask; J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
-    J.lut_bytes = wp.lut_kind >= 0 ? wp.lut_bytes : 0;
+    J.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
     if ((rc = launch_filter_job(J, o.band_rows, s))) return rc;
     if (e->timing) R2P_HIP(hipEventRecord(e->ev[1], s));
     if (d_filtered_out)

@@ -28,6 +28,8 @@ struct DeflateLaunch;
 struct AssembleLaunch;
 hipError_t launch_build_lut(const LutBuild& lb, void* out, hipStream_t s);
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s);
+// Rows wider than kMaxRowBytesLds: one row per workgroup, in tiles.
+hipError_t launch_filter_tiled(const FilterLaunch& L, int grid, size_t lds, hipStream_t s);
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s);
 hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s);
 // Dynamic-Huffman tables per segment, then the slices' bits.
@@ -66,6 +68,7 @@ struct WindowPlan {
     int lut_kind = -1;      // -1 none, 0 a2 float window, 1 a4 8-bit LUT
     int lut_entries = 0;
     int lut_bytes = 0;      // bytes of the device table
+    int lds_bytes = 0;      // bytes of it the filter stages in LDS (FilterJob::lut_bytes)
 };
 int plan_window(const r2p_image* img, WindowPlan* wp);
 
@@ -80,6 +83,7 @@ uint32_t idat_prefix_full(int chunk_bytes);
 // Filter bands: LDS bytes of a band (the band's rows and the row above it,
 // beside the table), and the band height to use.
 size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap);
+size_t filter_tiled_lds_bytes(int lut_cap);
 int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap);
 void fill_defaults(r2p_options* o);
 

@@ -19,6 +19,8 @@ enum WinMode : int32_t {
     WM_LUT8_FROM8  = 3,  // raw8 -> u8 table                 (8-bit DICOM window)
     WM_Q15         = 4,  // (raw16 * A + B) >> 15, clamp     (Q15 16 -> 8)
     WM_LUT8_FROM16 = 5,  // (raw16 >> shift) & mask -> u8 table (LUT 16 -> 8)
+    WM_LUT16G      = 6,  // as WM_LUT16, the table read from global memory (16 bits
+                         // stored: 65536 entries, 128 KiB, more than the LDS budget)
 };
 
 // One image (or strip) for the window+filter kernel.
@@ -36,7 +38,8 @@ struct FilterJob {
     int32_t        shift;      // right shift of the raw 16-bit sample
     uint32_t       mask;       // mask after the shift (LUT index range - 1)
     int32_t        q15_a, q15_b, pixel_rep;
-    int32_t        lut_bytes;  // size of the table in bytes (0 = none)
+    int32_t        lut_bytes;  // bytes of the table staged in LDS (0 = none; WM_LUT16G reads
+                               // its table from global memory and stages nothing)
     int32_t        band_rows;  // rows per band (workgroup work item) of this job
 };
 
@@ -172,7 +175,9 @@ constexpr int kAssembleThreads = 1024;
 constexpr int kAsmScanSlices = 2048;  // a single job of up to this many slices and
 constexpr int kAsmScanSegs = 1024;    // segments is scanned by k_assemble itself
 constexpr int kCrcShiftLevels = 29;       // CRC shift tables for 64 * 2^k zero bytes, k < 29
-constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row the filter stages in LDS
+constexpr int kMaxRowBytesLds = 32768;   // widest PNG-ready row k_filter stages in LDS whole;
+                                         // wider rows go to k_filter_tiled
+constexpr int kTileBytes = 16384;        // k_filter_tiled: PNG-ready bytes of a row per tile
 constexpr int kFilterLdsMax = 160 * 1024 - 256;  // LDS one filter workgroup may use
 constexpr int kWindow = 32768;           // DEFLATE window
 constexpr int kHuffSyms = 286 + 30;      // literal/length symbols, then distance codes

@@ -343,6 +343,19 @@ __device__ __forceinline__ u32x4 window_raw(const FilterJob& J, const RawItem& R
         }
         break;
     }
+    case WM_LUT16G: {
+        // the 128 KiB table of 16-bit stored samples, gathered through the caches
+        const u32x4 v = R.v0;
+        const R2P_GLOBAL uint16_t* t16 = (const R2P_GLOBAL uint16_t*)J.lut;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t s0 = t16[((v[k] & 0xFFFFu) >> J.shift) & J.mask];
+            const uint32_t s1 = t16[((v[k] >> 16) >> J.shift) & J.mask];
+            const uint32_t w = s0 | (s1 << 16);
+            o[k] = ((w & 0x00FF00FFu) << 8) | ((w >> 8) & 0x00FF00FFu);
+        }
+        break;
+    }
     case WM_LUT8_FROM8: {
         const u32x4 v = R.v0;
 #pragma unroll
@@ -396,10 +409,12 @@ __device__ __forceinline__ uint32_t window_byte(const FilterJob& J, const uint8_
     switch (J.wmode) {
     case WM_COPY: return src_row[j];
     case WM_SWAP16: return src_row[j ^ 1];
-    case WM_LUT16: {
+    case WM_LUT16:
+    case WM_LUT16G: {
         const int s = j >> 1;
         const uint32_t raw = (uint32_t)src_row[2 * s] | ((uint32_t)src_row[2 * s + 1] << 8);
-        const uint32_t w = ((const uint16_t*)lut_lds)[(raw >> J.shift) & J.mask];
+        const uint16_t* t16 = J.wmode == WM_LUT16 ? (const uint16_t*)lut_lds : (const uint16_t*)J.lut;
+        const uint32_t w = t16[(raw >> J.shift) & J.mask];
         return (j & 1) ? (w & 0xFF) : (w >> 8);
     }
     case WM_LUT8_FROM8: return lut_lds[src_row[j]];
@@ -640,6 +655,168 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
             }
             if (J.last_row && r == J.rows - 1)
                 for (int i = lane; i < rb; i += 64) J.last_row[i] = cur8[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_filter_tiled: rows wider than kMaxRowBytesLds (filter_select_kernel
+// filters any width, gpu_filter.cu:206-233, 285-308; the reference sizes its
+// buffers for 17043-pixel RGB16 rows, gpu_deflate_backend.h:7-9).  One row
+// per workgroup at a time, in tiles of kTileBytes PNG-ready bytes: pass 1
+// windows each tile of the row and of the row above into LDS, with the 16
+// bytes before the tile as left neighbours, and accumulates the five scores;
+// the workgroup then picks the filter (lowest index on ties) and pass 2
+// windows the tiles again and writes the chosen filter's bytes.  The jobs of
+// this launch have band_rows == 1, so a band is a row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kFilterThreads) void k_filter_tiled(FilterLaunch L)
+{
+    extern __shared__ u32x4 smem_tiled[];
+    uint8_t* lut_lds = (uint8_t*)smem_tiled;
+    uint8_t* tcur = lut_lds + L.lut_cap;                  // 16 B front pad + tile + 16 B slack
+    uint8_t* tprv = tcur + kTileBytes + 32;
+    constexpr int NW = kFilterThreads / 64;
+    __shared__ uint32_t red[5][NW];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+    int cur_job = -1;
+    const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
+    const uint32_t first = L.counts ? L.counts[2] : 0u;
+    const int njobs = L.counts ? (int)L.counts[0] : L.njobs;
+    for (uint32_t g = first + blockIdx.x; g < total; g += gridDim.x) {
+        int j = 0;
+        if (L.jobs) {
+            int lo = 0, hi = njobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (L.band_start[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+        }
+        const FilterJob J = L.jobs ? L.jobs[j] : L.single;
+        const int r = (int)(g - (L.jobs ? L.band_start[j] : 0u));
+        const int rb = J.row_bytes;
+        const int ntiles = (rb + kTileBytes - 1) / kTileBytes;
+        const uint8_t* src_cur = J.src + (int64_t)r * J.src_pitch;
+        const uint8_t* src_prv = r > 0 ? J.src + (int64_t)(r - 1) * J.src_pitch : nullptr;
+
+        __syncthreads();                                   // the previous row is done with LDS
+        if (j != cur_job) {
+            if (J.lut_bytes > 0) {
+                const int n16 = (J.lut_bytes + 15) >> 4;
+                for (int i = tid; i < n16; i += kFilterThreads) ((u32x4*)lut_lds)[i] = ((const u32x4*)J.lut)[i];
+            }
+            cur_job = j;
+            __syncthreads();
+        }
+        // tile t of the row (and, with_prev, of the row above: windowed from
+        // the source, or the caller's prior row, or zeros), each with the
+        // item before it as its front pad (zeros before the row)
+        auto stage = [&](int t, bool with_prev) {
+            const int base = t * kTileBytes;
+            const int n = min(kTileBytes, rb - base);
+            const int nfull = n >> 4, tail = n & 15;
+            const int per = nfull + 1;
+            const int nrow = with_prev ? 2 : 1;
+            for (int q = tid; q < nrow * per; q += kFilterThreads) {
+                const int which = q >= per ? 1 : 0;
+                const int i = q - which * per - 1;
+                const int item = (base >> 4) + i;
+                u32x4 v = (u32x4){0, 0, 0, 0};
+                if (item >= 0) {
+                    if (which == 0) v = window_raw(J, load_raw(J, src_cur, item), lut_lds);
+                    else if (src_prv) v = window_raw(J, load_raw(J, src_prv, item), lut_lds);
+                    else if (J.prior) v = *(const u32x4_ua*)(J.prior + 16 * (int64_t)item);
+                }
+                *(u32x4*)((which ? tprv : tcur) + 16 + 16 * i) = v;
+            }
+            for (int q = tid; q < nrow * tail; q += kFilterThreads) {
+                const int which = q >= tail ? 1 : 0;
+                const int jj = (nfull << 4) + (q - which * tail);
+                uint32_t b = 0;
+                if (which == 0) b = window_byte(J, src_cur, base + jj, lut_lds);
+                else if (src_prv) b = window_byte(J, src_prv, base + jj, lut_lds);
+                else if (J.prior) b = J.prior[base + jj];
+                (which ? tprv : tcur)[16 + jj] = (uint8_t)b;
+            }
+        };
+        const int near = J.bpp <= 4;
+        const uint32_t sh = (uint32_t)((4 - (J.bpp & 3)) & 3);
+        const uint32_t* cur = (const uint32_t*)(tcur + 16);
+        const uint32_t* prv = (const uint32_t*)(tprv + 16);
+
+        // ---- pass 1: the five scores over all tiles
+        uint32_t s[5] = {0, 0, 0, 0, 0};
+        for (int t = 0; t < ntiles; t++) {
+            if (t) __syncthreads();                        // the previous tile is scored
+            stage(t, true);
+            __syncthreads();
+            const int n = min(kTileBytes, rb - t * kTileBytes);
+            const int nitems = (n + 15) >> 4;
+            for (int i = tid; i < nitems; i += kFilterThreads) {
+                const int nv = min(16, n - 16 * i);
+                uint32_t dc[6], dp[6], a[4], c[4];
+                load_item(cur, i, dc);
+                load_item(prv, i, dp);
+                left4(dc, near, sh, a);
+                left4(dp, near, sh, c);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t m = nv >= 16 ? ~0u : byte_mask(nv - 4 * q);
+                    (void)cost_dword(dc[2 + q], a[q], dp[2 + q], c[q], m, s);
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            const uint32_t v = wave_total(s[f]);
+            if (lane == 0) red[f][wave] = v;
+        }
+        __syncthreads();
+        uint32_t sc[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            sc[f] = 0;
+            for (int w = 0; w < NW; w++) sc[f] += red[f][w];
+        }
+        int best = 0;
+        uint32_t bsc = sc[0];
+#pragma unroll
+        for (int f = 1; f < 5; f++) if (sc[f] < bsc) { bsc = sc[f]; best = f; }
+
+        // ---- pass 2: the chosen filter's bytes
+        uint8_t* dst = J.dst + (int64_t)r * (rb + 1);
+        if (tid == 0) dst[0] = (uint8_t)best;
+        for (int t = 0; t < ntiles; t++) {
+            __syncthreads();                               // red is read / the previous tile is written
+            stage(t, best >= 2);
+            __syncthreads();
+            const int base = t * kTileBytes;
+            const int n = min(kTileBytes, rb - base);
+            const int nitems = (n + 15) >> 4;
+            for (int i = tid; i < nitems; i += kFilterThreads) {
+                const int nv = min(16, n - 16 * i);
+                uint32_t dc[6], dp[6] = {0, 0, 0, 0, 0, 0}, a[4], c[4] = {0, 0, 0, 0};
+                load_item(cur, i, dc);
+                if (best >= 2) load_item(prv, i, dp);
+                left4(dc, near, sh, a);
+                if (best == 4) left4(dp, near, sh, c);
+                uint32_t o[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t x = dc[2 + q], b = dp[2 + q];
+                    o[q] = best == 0 ? x : best == 1 ? sub8(x, a[q]) : best == 2 ? sub8(x, b)
+                         : best == 3 ? sub8(x, avg8(a[q], b))
+                         : join_lanes(lanes_even(x) - paeth_pred(lanes_even(a[q]), lanes_even(b), lanes_even(c[q])),
+                                      lanes_odd(x) - paeth_pred(lanes_odd(a[q]), lanes_odd(b), lanes_odd(c[q])));
+                }
+                uint8_t* p = dst + 1 + base + 16 * i;
+                if (nv == 16) *(u32x4_ua*)p = (u32x4){o[0], o[1], o[2], o[3]};
+                else for (int k = 0; k < nv; k++) p[k] = (uint8_t)(o[k >> 2] >> (8 * (k & 3)));
+            }
+            if (J.last_row && r == J.rows - 1)
+                for (int i = tid; i < n; i += kFilterThreads) J.last_row[base + i] = tcur[16 + i];
         }
     }
 }
@@ -2553,6 +2730,18 @@ hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_
     if (k <= 16) return launch_filter_k<16>(L, grid, lds, s);
     if (k <= 24) return launch_filter_k<24>(L, grid, lds, s);
     return launch_filter_k<32>(L, grid, lds, s);
+}
+
+hipError_t launch_filter_tiled(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
+{
+    (void)hipGetLastError();
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_filter_tiled, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    k_filter_tiled<<<grid, kFilterThreads, lds, s>>>(L);
+    return hipGetLastError();
 }
 
 // Workgroup sizes: 256 (4 K positions per sub-block, three workgroups per

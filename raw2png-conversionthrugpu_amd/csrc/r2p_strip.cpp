@@ -129,10 +129,16 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     if (len == 0 && !is_last) return R2P_OK;
     uint8_t* win = d->d_win[d->cur];
     hipStream_t s = d->stream;
-    if (len) R2P_HIP(hipMemcpyAsync(win + d->hist, d_filtered, len, hipMemcpyDeviceToDevice, s));
-
     const SliceGeom sg = slice_geom((int64_t)len, d->seg_bytes, d->slice_bytes);
     const int nseg = sg.nseg;
+    // the strip's worst case (9 bits per byte, every segment's framing) must
+    // fit after the output delivered so far: the assembly appends unchecked
+    const size_t zmax = len + len / 8 + 16 * (size_t)nseg + 16;
+    if (d->out_bytes + zmax > d->max_total) {
+        set_error("strip output may reach %zu bytes, the stream buffer holds %zu", d->out_bytes + zmax, d->max_total);
+        return R2P_ERR_SIZE;
+    }
+    if (len) R2P_HIP(hipMemcpyAsync(win + d->hist, d_filtered, len, hipMemcpyDeviceToDevice, s));
     DeflateLaunch DL{};
     DeflateJob& D = DL.single;
     D.data = win; D.first = (int64_t)d->hist; D.len = (int64_t)len; D.dict_lo = 0;
@@ -153,7 +159,6 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     A.total_in = (int64_t)len; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
     A.slice_cap_dw = d->slice_cap_dw; A.slice_bytes = d->slice_bytes;
     A.chunk_bytes = 65536; A.mode = 0; A.final_block = is_last ? 1 : 0;
-    const size_t zmax = len + len / 8 + 16 * (size_t)nseg + 16;
     A.max_chunks = (int32_t)((zmax + 65535) / 65536);
     A.running = d->d_state; A.base_slot = d->d_state + 1;
     AL.jobs = nullptr; AL.njobs = 1; AL.total_chunks = (uint32_t)A.max_chunks;
@@ -197,6 +202,8 @@ int r2p_deflate_sync(r2p_deflate* d)
 const void* r2p_deflate_output(const r2p_deflate* d) { return d ? d->d_out : nullptr; }
 
 size_t r2p_deflate_output_bytes(const r2p_deflate* d) { return d ? d->out_bytes : 0; }
+
+size_t r2p_deflate_output_capacity(const r2p_deflate* d) { return d ? d->max_total : 0; }
 
 uint32_t r2p_deflate_adler(const r2p_deflate* d)
 {

@@ -7,8 +7,12 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <random>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -275,6 +279,119 @@ int main(int argc, char** argv)
         std::vector<uint8_t> z = png_idat(read_file(p2));
         CHECK(inflate_all(z.data(), z.size(), 15) == ofilt, "GPU form inflates to other bytes");
         printf("ok encode_source_to_png: CPU-deflate form byte-identical to oracle, GPU form bit-exact after inflate\n");
+    }
+    // ---- 6. run_one_modern's call order, unchanged (pipeline.cpp:581-803): a
+    //         filter thread over a pool of device-output-only contexts, no
+    //         prior-row copy after the last strip, a compress thread whose
+    //         DEFLATE and Adler-32 run on their own streams straight after the
+    //         hand-off, IDAT chunks assembled from the DEFLATE output.  The
+    //         facade orders the streams itself, so the result is exact.
+    for (const int SH : {1024, 96}) {
+        const int W = 2048, H = 700, bpp = 2, pool_n = 3;
+        const size_t qa_cap = (size_t)std::max(1, pool_n - 2);   // as run_one_modern sizes its queue (:928)
+        const int rb = W * bpp;
+        std::vector<uint8_t> raw((size_t)W * H * bpp);
+        for (size_t i = 0; i < raw.size(); i += 2) {
+            const uint16_t v = (uint16_t)((1500 + (i / 2) % 977 + rng() % 300) & 0xFFF);
+            raw[i] = v & 0xFF; raw[i + 1] = v >> 8;
+        }
+        DicomPixelParams dp;
+        dp.bits_allocated = 16; dp.bits_stored = 12; dp.high_bit = 11; dp.pixel_rep = 0;
+        dp.apply_window = 1; dp.window_center = 2000.f; dp.window_width = 1200.f;
+        r2po_dicom_params op;
+        memcpy(&op, &dp, sizeof(op));
+        std::vector<uint8_t> ready(raw.size()), exp((size_t)H * (rb + 1));
+        r2po_dicom_window(raw.data(), ready.data(), (size_t)W * H, &op);
+        r2po_filter_rows(ready.data(), nullptr, exp.data(), H, rb, bpp);
+
+        std::vector<GpuFilterContext*> pool;
+        for (int i = 0; i < pool_n; i++) pool.push_back(gpu_filter_create(W, bpp, SH, true));
+        const int strips = (H + SH - 1) / SH;
+        GpuDeflateContext* gdef = gpu_deflate_create(SH, rb + 1, (size_t)H * (rb + 1) * 9 + 4096, true);
+        GpuAdler32Context* gad = gpu_adler32_create((size_t)SH * (rb + 1));
+        GpuPngAssembleContext* pa = gpu_png_assemble_create((size_t)H * (rb + 1) * 2);
+        struct Job { int s, rows; bool last; };
+        std::deque<Job> q;
+        std::mutex mu;
+        std::condition_variable cv;
+        std::thread filt([&] {
+            std::vector<uint8_t> prev((size_t)rb, 0);
+            uint8_t* strip = gpu_filter_alloc_pinned((size_t)SH * rb);
+            for (int s = 0; s < strips; s++) {
+                const int rows = std::min(SH, H - s * SH);
+                memcpy(strip, raw.data() + (size_t)s * SH * rb, (size_t)rows * rb);
+                GpuFilterContext* ctx = pool[s % pool_n];
+                GpuTimings gt;
+                gpu_filter_process_from_host(ctx, strip, prev.data(), rows, &gt, &dp, true);
+                if (s < strips - 1) gpu_filter_copy_prior_row_to_host(ctx, prev.data());
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return q.size() < qa_cap; });
+                    q.push_back({s, rows, s == strips - 1});
+                }
+                cv.notify_all();
+            }
+            gpu_filter_free_pinned(strip);
+        });
+        std::vector<uint8_t> file;
+        ParallelDeflateState st;
+        size_t flushed = 0;
+        for (int done = 0; done < strips; done++) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !q.empty(); });
+                j = q.front();
+                q.pop_front();
+            }
+            cv.notify_all();
+            GpuFilterContext* ctx = pool[j.s % pool_n];
+            const uint8_t* d_f = gpu_filter_device_output(ctx);
+            const size_t fb = gpu_filter_output_size(ctx, j.rows);
+            gpu_deflate_compress_strip(gdef, d_f, j.rows, rb + 1, j.last);
+            gpu_adler32_launch(gad, d_f, fb);
+            gpu_deflate_stream_sync(gdef);
+            const size_t avail = gpu_deflate_flushable_byte_length(gdef);
+            size_t clen = 0;
+            if (avail > flushed)
+                clen = gpu_png_assemble_idat_chunk(pa, gpu_deflate_output(gdef) + flushed, avail - flushed, flushed == 0,
+                                                   false, 0, 3);
+            DeflateResult dr;
+            dr.strip_adler = gpu_adler32_collect(gad);
+            dr.input_size = fb;
+            accum_adler(st, dr);
+            if (clen) {
+                std::vector<uint8_t> c(clen);
+                gpu_png_assemble_copy_to_host(pa, c.data(), clen);
+                file.insert(file.end(), c.begin(), c.end());
+                flushed = avail;
+            }
+        }
+        filt.join();
+        {
+            const size_t total = gpu_deflate_output_byte_length(gdef);
+            const size_t clen = gpu_png_assemble_idat_chunk(pa, gpu_deflate_output(gdef) + flushed, total - flushed,
+                                                            flushed == 0, true, (uint32_t)st.running_adler, 3);
+            std::vector<uint8_t> c(clen);
+            gpu_png_assemble_copy_to_host(pa, c.data(), clen);
+            file.insert(file.end(), c.begin(), c.end());
+        }
+        std::vector<uint8_t> z;
+        for (size_t pos = 0; pos < file.size();) {
+            const uint32_t n = ((uint32_t)file[pos] << 24) | ((uint32_t)file[pos + 1] << 16) | ((uint32_t)file[pos + 2] << 8) | file[pos + 3];
+            const uint32_t crc = ((uint32_t)file[pos + 8 + n] << 24) | ((uint32_t)file[pos + 9 + n] << 16) |
+                                 ((uint32_t)file[pos + 10 + n] << 8) | file[pos + 11 + n];
+            CHECK(!memcmp(&file[pos + 4], "IDAT", 4) && crc32(0, &file[pos + 4], n + 4) == crc, "modern idat chunk");
+            z.insert(z.end(), file.begin() + pos + 8, file.begin() + pos + 8 + n);
+            pos += 12 + n;
+        }
+        CHECK(inflate_all(z.data(), z.size(), 15) == exp, "run_one_modern order (strip %d): inflated bytes differ", SH);
+        CHECK((uint32_t)st.running_adler == adler32(1, exp.data(), (uInt)exp.size()), "modern adler");
+        for (auto* c : pool) gpu_filter_destroy(c);
+        gpu_deflate_destroy(gdef);
+        gpu_adler32_destroy(gad);
+        gpu_png_assemble_destroy(pa);
+        printf("ok run_one_modern order, %d strip(s) of %d rows, no host sync between the streams\n", strips, SH);
     }
     printf("ALL OK\n");
     return 0;

@@ -99,22 +99,31 @@ def test_batch_redeclare_and_autotune(torch_dev, r2p, synth):
     b.close()
 
 
-def test_batch_wide_and_narrow_rows(torch_dev, r2p):
-    """Rows on both sides of the filter's narrow/wide split (4096 bytes), in
-    interleaved order, with ragged widths: each PNG equals the plain path's."""
+def test_batch_wide_and_narrow_rows(torch_dev, r2p, oracle):
+    """Rows on all sides of the filter's class splits (4096 bytes; 32 KiB,
+    above which rows are filtered in tiles), in interleaved order, with ragged
+    widths and a 16-bit-stored DICOM image (window table in global memory):
+    each PNG equals the plain path's, and the tiled ones inflate to the
+    oracle's filtered bytes."""
     torch = torch_dev
-    log("case batch wide+narrow rows")
+    log("case batch wide+narrow+tiled rows")
     rng = np.random.default_rng(11)
-    shapes = [(1500, 6, 3, 8, r2p.PIX_READY), (300, 9, 1, 8, r2p.PIX_READY), (2100, 5, 1, 16, r2p.PIX_SWAP16),
-              (1366, 4, 3, 8, r2p.PIX_READY), (1365, 3, 3, 8, r2p.PIX_READY), (77, 13, 3, 16, r2p.PIX_SWAP16),
-              (4096, 3, 1, 8, r2p.PIX_READY), (4097, 2, 1, 8, r2p.PIX_READY), (2001, 7, 3, 16, r2p.PIX_SWAP16)]
-    imgs, srcs = [], []
-    for w, h, ch, bits, mode in shapes:
+    bs16 = r2p.DicomParams.make(16, 16, 15, 1, 1, 1.0, 0.0, 1, 0.0, 3000.0)
+    shapes = [(1500, 6, 3, 8, r2p.PIX_READY, None), (300, 9, 1, 8, r2p.PIX_READY, None),
+              (17043, 3, 3, 16, r2p.PIX_SWAP16, None), (2100, 5, 1, 16, r2p.PIX_SWAP16, None),
+              (1366, 4, 3, 8, r2p.PIX_READY, None), (40000, 2, 1, 8, r2p.PIX_READY, None),
+              (1365, 3, 3, 8, r2p.PIX_READY, None), (77, 13, 3, 16, r2p.PIX_SWAP16, None),
+              (4096, 3, 1, 8, r2p.PIX_READY, None), (4097, 2, 1, 8, r2p.PIX_READY, None),
+              (11000, 4, 3, 8, r2p.PIX_READY, None), (2001, 7, 3, 16, r2p.PIX_SWAP16, None),
+              (640, 33, 1, 16, r2p.PIX_DICOM, bs16), (20000, 3, 1, 16, r2p.PIX_DICOM, bs16)]
+    imgs, srcs, hosts = [], [], []
+    for w, h, ch, bits, mode, dp in shapes:
         n = w * h * ch * (bits // 8)
         px = (np.cumsum(rng.integers(-3, 4, n), dtype=np.int64) % 256).astype(np.uint8)
         px[: n // 4] = rng.integers(0, 256, n // 4, dtype=np.uint8)
-        imgs.append(r2p.Image.make(w, h, ch, bits, mode))
+        imgs.append(r2p.Image.make(w, h, ch, bits, mode, dp))
         srcs.append(torch.from_numpy(px).cuda())
+        hosts.append(px)
     opt = r2p.options()
     fsum = sum(r2p.filtered_bytes(im) for im in imgs)
     isum = sum(r2p.input_bytes(im) for im in imgs)
@@ -126,4 +135,17 @@ def test_batch_wide_and_narrow_rows(torch_dev, r2p):
         for i, im in enumerate(imgs):
             got = b.copy_result(i)
             assert got == plain_png(r2p, torch, im, srcs[i], opt), f"image {i} ({im.width}x{im.height}) differs"
+    for i, im in enumerate(imgs):
+        if r2p.row_bytes(im) <= 32768 and im.pix_mode != r2p.PIX_DICOM:
+            continue
+        hdr, z, raw = check_and_inflate(b.copy_result(i))
+        if im.pix_mode == r2p.PIX_DICOM:
+            p = oracle.DicomParams(*[getattr(im.dicom, f) for f, _ in im.dicom._fields_])
+            ready = oracle.dicom_window(hosts[i].view("<u2"), p)
+        elif im.pix_mode == r2p.PIX_SWAP16:
+            ready = hosts[i].view("<u2").byteswap().view(np.uint8)
+        else:
+            ready = hosts[i]
+        rb = r2p.row_bytes(im)
+        assert raw == oracle.filter_rows(ready, im.height, rb, rb // im.width).tobytes(), f"image {i} vs oracle"
     b.close()

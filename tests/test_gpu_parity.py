@@ -61,6 +61,9 @@ CASES = [
     ("gray16be", 45, 9, 1, 16, 0), ("rgb16be", 23, 8, 3, 16, 0), ("rgba16be", 17, 6, 4, 16, 0),
     ("gray16swap", 64, 10, 1, 16, 1), ("rgb16swap", 21, 5, 3, 16, 1),
     ("rgb8_wide", 6000, 3, 3, 8, 0), ("gray16_wide", 8192, 3, 1, 16, 1),
+    # rows over the 32 KiB that k_filter stages in LDS: the row-tiled kernel
+    ("rgb16_huge", 17043, 7, 3, 16, 0), ("gray8_huge", 40000, 5, 1, 8, 0), ("rgb8_huge", 11000, 4, 3, 8, 0),
+    ("rgba16_huge_swap", 9000, 3, 4, 16, 1), ("gray8_tile_edge", 16384 * 3, 2, 1, 8, 0),
 ]
 
 
@@ -82,11 +85,11 @@ def test_filter_ready_modes(torch_dev, r2p, oracle, name, w, h, ch, bits, mode):
     assert np.array_equal(last, ready[(h - 1) * rb:])
 
 
-@pytest.mark.parametrize("band_rows", [1, 3, 16])
-def test_filter_bands_and_prior(torch_dev, r2p, oracle, band_rows):
-    log(f"case filter bands={band_rows}")
+@pytest.mark.parametrize("band_rows,w", [(1, 300), (3, 300), (16, 300), (0, 12000)])
+def test_filter_bands_and_prior(torch_dev, r2p, oracle, band_rows, w):
+    log(f"case filter bands={band_rows} w={w}")
     rng = np.random.default_rng(band_rows)
-    w, h = 300, 41
+    h = 41
     px = (np.cumsum(rng.integers(-4, 5, (h, w * 3)), axis=1) % 256).astype(np.uint8)
     prior = rng.integers(0, 256, w * 3, dtype=np.uint8)
     img = r2p.Image.make(w, h, 3, 8, 0)
@@ -112,6 +115,12 @@ DICOM_CASES = [
     ("q15_nowindow", 16, 12, 11, 0, 0, 1.0, 0.0, 0, 0.0, 0.0, 3),
     ("lut8_16bit", 16, 16, 15, 1, 1, 1.0, 0.0, 1, 0.0, 0.9, 3),
     ("dicom8_float", 8, 8, 7, 1, 1, 1.5, -3.0, 1, 0.0, 100.0, 2),
+    # 16 bits stored: the 128 KiB window table is read from global memory
+    ("bs16_float16", 16, 16, 15, 0, 1, 1.0, -1024.0, 1, 1000.0, 30000.0, 2),
+    ("bs16_signed_float16", 16, 16, 15, 1, 1, 2.0, -100.0, 1, -300.0, 4000.0, 2),
+    ("bs16_nowindow16", 16, 16, 15, 0, 0, 1.0, 0.0, 0, 0.0, 0.0, 2),
+    ("bs16_signed_nowindow16", 16, 16, 15, 1, 1, 0.5, 10.0, 0, 0.0, 0.0, 2),
+    ("bs16_narrow_window16", 16, 16, 15, 1, 0, 1.0, 0.0, 1, -5.0, 1.0, 2),
 ]
 
 
@@ -190,10 +199,11 @@ def _check_png(oracle, r2p, png, px, img, filt_gpu):
     assert np.array_equal(filt_gpu, exp)
     assert hdr["width"] == img.width and hdr["height"] == img.height
     assert hdr["depth"] == r2p.out_bits(img)
-    im = np.array(PILImage.open(io.BytesIO(png)))
     ob = r2p.out_bits(img)
-    expect = ready.view(">u2") if ob == 16 else ready
-    assert np.array_equal(im.reshape(-1).astype(np.int64), expect.astype(np.int64))
+    if not (ob == 16 and img.channels > 1):          # (PIL reads 16-bit colour as 8-bit)
+        im = np.array(PILImage.open(io.BytesIO(png)))
+        expect = ready.view(">u2") if ob == 16 else ready
+        assert np.array_equal(im.reshape(-1).astype(np.int64), expect.astype(np.int64))
     return z
 
 
@@ -211,6 +221,28 @@ def test_encode_png_workloads(torch_dev, r2p, synth, oracle, kind, lz77):
     png, filt, adler, nseg, t = encode_device(torch_dev, r2p, s.pixels, s.img, opt)
     z = _check_png(oracle, r2p, png, s.pixels, s.img, filt)
     assert adler == zlib.adler32(filt.tobytes())
+
+
+@pytest.mark.parametrize("w,h,ch,bits,mode,dicom", [
+    (17043, 9, 3, 16, 0, None),                                               # RGB16, 102 KB rows
+    (40000, 6, 1, 8, 0, None),                                                # gray8, 40 KB rows
+    (20000, 5, 1, 16, 2, (16, 12, 11, 0, 1, 1.0, -1024.0, 1, 40.0, 400.0)),   # float window, 40 KB rows
+    (36000, 4, 1, 16, 3, (16, 12, 11, 0, 1, 1.0, -1024.0, 1, 40.0, 400.0)),   # Q15 16 -> 8, 36 KB rows
+    (18000, 3, 1, 16, 2, (16, 16, 15, 1, 1, 1.0, 0.0, 1, 0.0, 3000.0)),       # 16 bits stored, global table
+])
+def test_encode_huge_rows(torch_dev, r2p, oracle, w, h, ch, bits, mode, dicom):
+    """Rows wider than k_filter's LDS rows through the whole encoder."""
+    log(f"case encode huge rows {w}x{h}x{ch} bits={bits} mode={mode}")
+    rng = np.random.default_rng(w + h)
+    hi = 256 if bits == 8 else (4096 if dicom and dicom[1] == 12 else 65536)
+    px = (np.cumsum(rng.integers(-3, 4, (h, w * ch)), axis=1) % hi).astype(np.uint8 if bits == 8 else "<u2")
+    px[:, : (w * ch) // 4] = rng.integers(0, hi, (h, (w * ch) // 4))
+    p = r2p.DicomParams.make(*dicom) if dicom else None
+    img = r2p.Image.make(w, h, ch, bits, mode, p) if p else r2p.Image.make(w, h, ch, bits, mode)
+    for lz77 in (0, 1):
+        png, filt, adler, nseg, t = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=lz77))
+        _check_png(oracle, r2p, png, px, img, filt)
+        assert adler == zlib.adler32(filt.tobytes())
 
 
 def test_lz77_finds_repeats(torch_dev, r2p, oracle):

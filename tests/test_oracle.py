@@ -91,6 +91,34 @@ def test_float_window_closed_form(oracle, c, w, slope, icpt, signed):
     assert np.all(be[v > c - 0.5 + (w - 1) / 2] == 65535)
 
 
+@pytest.mark.parametrize("signed,rescale,slope,icpt,window,c,w", [
+    (0, 0, 1.0, 0.0, 0, 0.0, 0.0), (1, 0, 1.0, 0.0, 0, 0.0, 0.0), (1, 1, 2.0, -100.0, 0, 0.0, 0.0),
+    (0, 1, 1.0, -1024.0, 1, 1000.0, 30000.0), (1, 1, 2.0, -100.0, 1, -300.0, 4000.0),
+    (1, 0, 1.0, 0.0, 1, -5.0, 1.0)])
+def test_float_window_16_bits_stored(oracle, signed, rescale, slope, icpt, window, c, w):
+    """All 65536 codes of 16-bit stored samples (no shift, no mask) against the
+    PS3.3 closed form: identity without window/rescale (negative values clamp
+    to 0), the window's special cases exactly, the formula to +-1."""
+    p = oracle.DicomParams.make(16, 16, 15, signed, rescale, slope, icpt, window, c, w)
+    codes = np.arange(65536, dtype=np.uint32).astype("<u2")
+    be = oracle.dicom_window(codes, p).view(">u2").astype(np.int64)
+    stored = codes.astype(np.int64)
+    if signed:
+        stored = np.where(stored >= 32768, stored - 65536, stored)
+    v = stored * slope + icpt if rescale else stored.astype(np.float64)
+    if not window:
+        assert np.array_equal(be, np.clip(np.floor(v), 0, 65535).astype(np.int64))
+        return
+    if w - 1.0 <= 0:
+        assert np.array_equal(be, np.where(v >= c, 65535, 0))
+        return
+    exact = _ps33_exact(v, c, w, 65535.0)
+    assert np.all(np.abs(be - np.floor(exact)) <= 1)
+    assert np.mean(be == np.floor(exact)) > 0.97
+    assert np.all(be[v <= c - 0.5 - (w - 1) / 2] == 0)
+    assert np.all(be[v > c - 0.5 + (w - 1) / 2] == 65535)
+
+
 def test_float_window_shift_mask_and_8bit(oracle):
     # left-aligned 12-bit data (high bit 15): stored value = raw >> 4; garbage low bits ignored
     p = oracle.DicomParams.make(16, 12, 15, 0, 0, 1.0, 0.0, 0)
