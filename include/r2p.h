@@ -144,6 +144,23 @@ typedef struct r2p_timings {
 int r2p_encoder_set_timing(r2p_encoder* enc, int on);
 int r2p_encoder_timings(r2p_encoder* enc, r2p_timings* t);
 
+/* No-allocation variant of r2p_encode_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It allocates nothing: every buffer of the encode lives in one
+ * static device array of R2P_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends), e.g. images of up to about 11 MiB of filtered bytes with
+ * the default options (2048x2048 16-bit); a larger image, or dynamic-Huffman
+ * blocks, is refused (R2P_ERR_SIZE / R2P_ERR_UNSUPPORTED).  The caller passes
+ * the input, the PNG buffer (png_cap >= r2p_png_bound) and d_result (device,
+ * 16 bytes: uint64 PNG length, then the uint32 Adler-32 at byte 8), all
+ * written asynchronously on `stream` (a hipStream_t, NULL = the null stream).
+ * opt: NULL = r2p_options_default.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on a GPU (other streams, other threads) race. */
+#define R2P_NOALLOC_SCRATCH_BYTES (24u << 20)
+int r2p_encode_device_noalloc(const r2p_image* img, const void* d_src, void* d_png, size_t png_cap,
+                              void* d_result, const r2p_options* opt, void* stream);
+
 /* ======================================================================== */
 /* Kernel-level entry points (used by the drop-in C++ facade).               */
 /* ======================================================================== */

@@ -110,6 +110,7 @@ def lib():
             "r2p_encoder_create": ([sz, optp, C.POINTER(vp)], ip),
             "r2p_encoder_destroy": ([vp], ip), "r2p_encoder_stream": ([vp], vp),
             "r2p_encode_device": ([vp, imgp, vp, vp, sz, vp], ip),
+            "r2p_encode_device_noalloc": ([imgp, vp, vp, sz, vp, optp, vp], ip),
             "r2p_encoder_sync": ([vp, C.POINTER(sz), C.POINTER(u32), C.POINTER(u32)], ip),
             "r2p_encode_host": ([vp, imgp, vp, vp, sz, C.POINTER(sz)], ip),
             "r2p_encoder_set_timing": ([vp, ip], ip),

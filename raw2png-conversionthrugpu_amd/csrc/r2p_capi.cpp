@@ -112,6 +112,14 @@ static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev
 static std::mutex g_init_mu;
 static bool g_dev_ready[64];
 
+// Scratch of r2p_encode_device_noalloc: every buffer of one plain encode in
+// one static device array (each device holds its own copy until the process
+// ends).  In host code `scratch` names a host-side stand-in, not a device
+// address: r2p_init looks up each device's address once, and only that
+// address is used.
+static __device__ __align__(256) uint8_t scratch[R2P_NOALLOC_SCRATCH_BYTES];
+static void* g_scratch[64];
+
 // ---------------------------------------------------------------------------
 // geometry / planning
 // ---------------------------------------------------------------------------
@@ -368,6 +376,10 @@ int r2p_init(int device)
         shift32[4][256];
     build_tables(lit, len, dist_rev, crc4, x8pow, shift, shift32);
     R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow, &shift[0][0][0], &shift32[0][0]));
+    void* sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sp) return hip_fail(se != hipSuccess ? se : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = sp;
     g_dev_ready[device] = true;
     return R2P_OK;
 }
@@ -585,6 +597,104 @@ static int enc_grow(uint8_t** p, size_t* cap, size_t need)
     return R2P_OK;
 }
 
+// Device buffers of one plain encode.
+struct EncodeWork {
+    uint8_t* filt = nullptr;           // filtered bytes + kTailSlack readable bytes
+    uint32_t* slice_out = nullptr;     // nslice * slice_cap_dw dwords
+    uint32_t* slice_bits = nullptr;    // nslice
+    uint32_t* slice_adler = nullptr;   // nslice
+    uint32_t* slice_boff = nullptr;    // nslice
+    uint32_t* seg_tbits = nullptr;     // nseg + 1
+    uint64_t* seg_off = nullptr;       // nseg + 2
+    uint32_t* tokens = nullptr;        // dynamic-Huffman blocks only
+    uint32_t* tok_count = nullptr;
+    uint32_t* hist = nullptr;
+    uint32_t* codes = nullptr;
+    uint32_t* hdr = nullptr;
+    uint32_t* hdr_bits = nullptr;
+    int slice_cap_dw = 0;
+    const void* lut = nullptr;         // the image's device table, if its window needs one
+    uint64_t* out_len = nullptr;       // device: PNG length
+    uint32_t* adler_out = nullptr;     // device: Adler-32 of the filtered bytes
+    uint64_t* host_res = nullptr;      // optional mapped host copy of both
+    hipEvent_t* ev = nullptr;          // optional: five events around the stages
+};
+
+// The plain encode's launches: window + filter, DEFLATE (and the dynamic
+// blocks' tables and emission), the segment scan when the assembly cannot
+// do it itself, assembly of the complete PNG.
+static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_image* img, const Geometry& g,
+                       const WindowPlan& wp, const void* d_src, void* d_png, size_t png_cap, void* d_filtered_out,
+                       hipStream_t s, uint32_t* nseg_out)
+{
+    int rc;
+    // The deflate stage reads past the data end (kTailSlack), so it always
+    // works on its own buffer; a caller's copy is made afterwards.
+    uint8_t* filt = w.filt;
+    if (w.ev) R2P_HIP(hipEventRecord(w.ev[0], s));
+    FilterJob J{};
+    J.src = (const uint8_t*)d_src; J.dst = filt; J.prior = nullptr; J.last_row = nullptr;
+    J.lut = w.lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes;
+    J.rows = (int32_t)img->height; J.bpp = g.bpp; J.wmode = wp.wmode; J.shift = wp.shift;
+    J.mask = wp.mask; J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
+    J.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
+    if ((rc = launch_filter_job(J, o.band_rows, s))) return rc;
+    if (w.ev) R2P_HIP(hipEventRecord(w.ev[1], s));
+    if (d_filtered_out)
+        R2P_HIP(hipMemcpyAsync(d_filtered_out, filt, g.filtered, hipMemcpyDeviceToDevice, s));
+
+    const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
+    const int nseg = sg.nseg;
+    DeflateLaunch DL{};
+    DeflateJob& D = DL.single;
+    D.data = filt; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
+    D.slice_out = w.slice_out; D.slice_bits = w.slice_bits; D.slice_adler = w.slice_adler;
+    D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes; D.slice_bytes = o.slice_bytes;
+    D.slice_cap_dw = w.slice_cap_dw;
+    D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
+    D.lz77 = o.lz77; D.nslice = sg.nslice;
+    if (o.huffman) { DL.tokens = w.tokens; DL.tok_count = w.tok_count; DL.hist = w.hist; }
+    DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
+    R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
+    if (o.huffman) {                       // the segments' codes, then the slices' bits
+        HuffJob H{};
+        H.tokens = w.tokens; H.tok_count = w.tok_count; H.hist = w.hist;
+        H.codes = w.codes; H.hdr = w.hdr; H.hdr_bits = w.hdr_bits;
+        H.slice_out = w.slice_out; H.slice_bits = w.slice_bits; H.slice_cap_dw = w.slice_cap_dw;
+        H.slice_bytes = o.slice_bytes; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg; H.final_block = 1;
+        R2P_HIP(launch_huffman(H, s));
+    }
+    if (w.ev) R2P_HIP(hipEventRecord(w.ev[2], s));
+
+    AssembleLaunch AL{};
+    AssembleJob& A = AL.single;
+    A.slice_out = w.slice_out; A.slice_bits = w.slice_bits; A.slice_adler = w.slice_adler;
+    A.slice_boff = w.slice_boff; A.seg_tbits = w.seg_tbits;
+    A.seg_off = w.seg_off; A.out = (uint8_t*)d_png; A.out_len = w.out_len;
+    A.adler_out = w.adler_out; A.out_cap = (int64_t)png_cap;
+    A.host_res = w.host_res;
+    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
+    A.slice_cap_dw = w.slice_cap_dw; A.slice_bytes = o.slice_bytes;
+    A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
+    A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
+    A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
+    A.color_type = (uint8_t)g.color_type;
+    const int lvl = o.zlib_level;
+    A.zflg = lvl == 1 ? 0x01 : (lvl >= 2 && lvl <= 5) ? 0x5E : (lvl >= 6 && lvl <= 7) ? 0x9C : 0xDA;
+    const size_t zmax = g.filtered + g.filtered / 8 + 16 * (size_t)nseg + 16;
+    A.max_chunks = (int32_t)((zmax + o.idat_chunk_bytes - 1) / o.idat_chunk_bytes);
+    A.running = nullptr; A.base_slot = nullptr;
+    AL.jobs = nullptr; AL.njobs = 1; AL.total_chunks = (uint32_t)A.max_chunks;
+    const bool self_scan = assemble_scans_itself(A);
+    if (!self_scan) R2P_HIP(launch_scan(AL, 1, s));
+    if (w.ev) R2P_HIP(hipEventRecord(w.ev[3], s));
+    if (self_scan) R2P_HIP(launch_assemble_self_scan(AL, std::min(A.max_chunks, 4096), s));
+    else R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
+    if (w.ev) R2P_HIP(hipEventRecord(w.ev[4], s));
+    if (nseg_out) *nseg_out = (uint32_t)nseg;
+    return R2P_OK;
+}
+
 extern "C" {
 
 int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_encoder** out)
@@ -667,73 +777,82 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     if ((rc = plan_window(img, &wp))) return rc;
     hipStream_t s = e->stream;
     if ((rc = e->lut.ensure(wp, img->dicom, s))) return rc;
-
-    // The deflate stage reads past the data end (kTailSlack), so it always
-    // works on the encoder's own buffer; a caller's copy is made afterwards.
-    uint8_t* filt = e->d_filt;
-    if (e->timing) R2P_HIP(hipEventRecord(e->ev[0], s));
-    FilterJob J{};
-    J.src = (const uint8_t*)d_src; J.dst = filt; J.prior = nullptr; J.last_row = nullptr;
-    J.lut = e->lut.d_lut; J.src_pitch = g.pitch; J.row_bytes = (int32_t)g.row_bytes;
-    J.rows = (int32_t)img->height; J.bpp = g.bpp; J.wmode = wp.wmode; J.shift = wp.shift;
-    J.mask = wp.mask; J.q15_a = wp.q15_a; J.q15_b = wp.q15_b; J.pixel_rep = wp.pixel_rep;
-    J.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
-    if ((rc = launch_filter_job(J, o.band_rows, s))) return rc;
-    if (e->timing) R2P_HIP(hipEventRecord(e->ev[1], s));
-    if (d_filtered_out)
-        R2P_HIP(hipMemcpyAsync(d_filtered_out, filt, g.filtered, hipMemcpyDeviceToDevice, s));
-
-    const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
-    const int nseg = sg.nseg;
-    DeflateLaunch DL{};
-    DeflateJob& D = DL.single;
-    D.data = filt; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
-    D.slice_out = e->d_slice_out; D.slice_bits = e->d_slice_bits; D.slice_adler = e->d_slice_adler;
-    D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes; D.slice_bytes = o.slice_bytes;
-    D.slice_cap_dw = e->slice_cap_dw;
-    D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
-    D.lz77 = o.lz77; D.nslice = sg.nslice;
-    if (o.huffman) { DL.tokens = e->d_tokens; DL.tok_count = e->d_tok_count; DL.hist = e->d_hist; }
-    DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
-    R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
-    if (o.huffman) {                       // the segments' codes, then the slices' bits
-        HuffJob H{};
-        H.tokens = e->d_tokens; H.tok_count = e->d_tok_count; H.hist = e->d_hist;
-        H.codes = e->d_codes; H.hdr = e->d_hdr; H.hdr_bits = e->d_hdr_bits;
-        H.slice_out = e->d_slice_out; H.slice_bits = e->d_slice_bits; H.slice_cap_dw = e->slice_cap_dw;
-        H.slice_bytes = o.slice_bytes; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg; H.final_block = 1;
-        R2P_HIP(launch_huffman(H, s));
-    }
-    if (e->timing) R2P_HIP(hipEventRecord(e->ev[2], s));
-
-    AssembleLaunch AL{};
-    AssembleJob& A = AL.single;
-    A.slice_out = e->d_slice_out; A.slice_bits = e->d_slice_bits; A.slice_adler = e->d_slice_adler;
-    A.slice_boff = e->d_slice_boff; A.seg_tbits = e->d_seg_tbits;
-    A.seg_off = e->d_seg_off; A.out = (uint8_t*)d_png; A.out_len = e->d_misc;
-    A.adler_out = (uint32_t*)(e->d_misc + 1); A.out_cap = (int64_t)png_cap;
-    A.host_res = e->dh_misc;             // the results reach h_misc without a copy
-    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
-    A.slice_cap_dw = e->slice_cap_dw; A.slice_bytes = o.slice_bytes;
-    A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
-    A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
-    A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
-    A.color_type = (uint8_t)g.color_type;
-    const int lvl = o.zlib_level;
-    A.zflg = lvl == 1 ? 0x01 : (lvl >= 2 && lvl <= 5) ? 0x5E : (lvl >= 6 && lvl <= 7) ? 0x9C : 0xDA;
-    const size_t zmax = g.filtered + g.filtered / 8 + 16 * (size_t)nseg + 16;
-    A.max_chunks = (int32_t)((zmax + o.idat_chunk_bytes - 1) / o.idat_chunk_bytes);
-    A.running = nullptr; A.base_slot = nullptr;
-    AL.jobs = nullptr; AL.njobs = 1; AL.total_chunks = (uint32_t)A.max_chunks;
-    const bool self_scan = assemble_scans_itself(A);
-    if (!self_scan) R2P_HIP(launch_scan(AL, 1, s));
-    if (e->timing) R2P_HIP(hipEventRecord(e->ev[3], s));
-    if (self_scan) R2P_HIP(launch_assemble_self_scan(AL, std::min(A.max_chunks, 4096), s));
-    else R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
-    if (e->timing) R2P_HIP(hipEventRecord(e->ev[4], s));
-    e->last_nseg = (uint32_t)nseg;
+    EncodeWork w;
+    w.filt = e->d_filt; w.slice_out = e->d_slice_out; w.slice_bits = e->d_slice_bits;
+    w.slice_adler = e->d_slice_adler; w.slice_boff = e->d_slice_boff; w.seg_tbits = e->d_seg_tbits;
+    w.seg_off = e->d_seg_off; w.tokens = e->d_tokens; w.tok_count = e->d_tok_count; w.hist = e->d_hist;
+    w.codes = e->d_codes; w.hdr = e->d_hdr; w.hdr_bits = e->d_hdr_bits; w.slice_cap_dw = e->slice_cap_dw;
+    w.lut = e->lut.d_lut;
+    w.out_len = e->d_misc; w.adler_out = (uint32_t*)(e->d_misc + 1);
+    w.host_res = e->dh_misc;               // the results reach h_misc without a copy
+    w.ev = e->timing ? e->ev : nullptr;
+    uint32_t nseg = 0;
+    if ((rc = encode_into(o, w, img, g, wp, d_src, d_png, png_cap, d_filtered_out, s, &nseg))) return rc;
+    e->last_nseg = nseg;
     e->timed = e->timing;
     return R2P_OK;
+}
+
+// Carves the plain encode's buffers for F filtered bytes out of `base`;
+// returns the bytes used.
+static size_t noalloc_layout(uint8_t* base, size_t F, const r2p_options& o, const WindowPlan& wp, EncodeWork* w)
+{
+    const SliceGeom sg = slice_geom((int64_t)F, o.segment_bytes, o.slice_bytes);
+    size_t off = 0;
+    auto take = [&](size_t n) { uint8_t* p = base + off; off += (n + 255) & ~(size_t)255; return p; };
+    w->slice_cap_dw = slice_cap_dwords(o.slice_bytes);
+    w->filt = take(F + kTailSlack);
+    w->slice_out = (uint32_t*)take((size_t)sg.nslice * (size_t)w->slice_cap_dw * 4);
+    w->slice_bits = (uint32_t*)take((size_t)sg.nslice * 4);
+    w->slice_adler = (uint32_t*)take((size_t)sg.nslice * 4);
+    w->slice_boff = (uint32_t*)take((size_t)sg.nslice * 4);
+    w->seg_tbits = (uint32_t*)take(((size_t)sg.nseg + 1) * 4);
+    w->seg_off = (uint64_t*)take(((size_t)sg.nseg + 2) * 8);
+    w->lut = wp.lut_kind >= 0 ? take((size_t)wp.lut_bytes) : nullptr;
+    return off;
+}
+
+int r2p_encode_device_noalloc(const r2p_image* img, const void* d_src, void* d_png, size_t png_cap,
+                              void* d_result, const r2p_options* opt, void* stream)
+{
+    int rc = current_init();              // (tables and the scratch address, once per device)
+    if (rc) return rc;
+    if (!img || !d_src || !d_png || !d_result) { set_error("bad encode arguments"); return R2P_ERR_ARG; }
+    int dev = 0;
+    R2P_HIP(hipGetDevice(&dev));
+    uint8_t* base = (uint8_t*)g_scratch[dev];
+    if (!base) { set_error("device %d has no scratch address", dev); return R2P_ERR_STATE; }
+    r2p_options o;
+    if (opt) o = *opt; else r2p_options_default(&o);
+    fill_defaults(&o);
+    if (o.huffman) { set_error("dynamic Huffman blocks need the encoder's own buffers (r2p_encode_device)"); return R2P_ERR_UNSUPPORTED; }
+    Geometry g;
+    if ((rc = resolve(img, &g))) return rc;
+    WindowPlan wp;
+    if ((rc = plan_window(img, &wp))) return rc;
+    const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
+    if (png_cap < bound) { set_error("png capacity %zu < bound %zu", png_cap, bound); return R2P_ERR_SIZE; }
+    EncodeWork w;
+    const size_t need = noalloc_layout(base, g.filtered, o, wp, &w);
+    if (need > (size_t)R2P_NOALLOC_SCRATCH_BYTES) {
+        set_error("image needs %zu bytes of scratch (%zu filtered bytes); the no-allocation encoder holds %zu",
+                  need, g.filtered, (size_t)R2P_NOALLOC_SCRATCH_BYTES);
+        return R2P_ERR_SIZE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // Nothing in the scratch has to start at zero: every kernel writes what
+    // it later reads (tables, filtered bytes, slice bits and offsets).
+    if (wp.lut_kind >= 0) {
+        LutBuild lb;
+        lb.p = img->dicom;
+        lb.entries = wp.lut_entries;
+        lb.kind = wp.lut_kind;
+        R2P_HIP(launch_build_lut(lb, (void*)w.lut, s));
+    }
+    w.out_len = (uint64_t*)d_result;
+    w.adler_out = (uint32_t*)((uint8_t*)d_result + 8);
+    w.host_res = nullptr;
+    return encode_into(o, w, img, g, wp, d_src, d_png, png_cap, nullptr, s, nullptr);
 }
 
 int r2p_encoder_sync(r2p_encoder* e, size_t* png_len, uint32_t* adler, uint32_t* nseg)
