@@ -281,10 +281,14 @@ int r2p_server_stats_get(r2p_server* s, r2p_server_stats* st);
 /* Open-loop Poisson load test driven from native code: request i (of n)
  * arrives at arrival_s[i] seconds after start and is image imgs[kind[i]]
  * with pixels h_src[kind[i]].  Fills latency_s[i] (arrival -> PNG in host
- * memory) and png_len[i]; *wall_s receives the makespan. */
+ * memory) and png_len[i]; *wall_s receives the makespan.  With expect_png
+ * (optional: expected PNG bytes and lengths per image kind), same[i] = 1
+ * when request i's PNG equals its kind's expected bytes, compared by
+ * verifier threads after the latency is taken. */
 int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int* kind,
                         const r2p_image* imgs, const void* const* h_src, double* latency_s,
-                        uint64_t* png_len, double* wall_s);
+                        uint64_t* png_len, double* wall_s, const void* const* expect_png,
+                        const size_t* expect_len, uint8_t* same);
 
 #ifdef __cplusplus
 }

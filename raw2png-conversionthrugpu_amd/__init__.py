@@ -146,7 +146,8 @@ def lib():
             "r2p_server_stats_get": ([vp, C.POINTER(ServerStats)], ip),
             "r2p_server_loadtest": ([vp, ip, C.POINTER(C.c_double), C.POINTER(ip), imgp,
                                      C.POINTER(vp), C.POINTER(C.c_double), C.POINTER(u64),
-                                     C.POINTER(C.c_double)], ip),
+                                     C.POINTER(C.c_double), C.POINTER(vp), C.POINTER(sz),
+                                     C.POINTER(C.c_uint8)], ip),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -377,3 +378,33 @@ class Batch:
         check(lib().r2p_batch_autotune(self.h, C.byref(o), C.byref(ms)))
         self.opt = o
         return o, ms.value
+
+
+def server_loadtest(server, arrival_s, kind, imgs, h_srcs, expect_pngs=None):
+    """Open-loop load test through r2p_server_loadtest.  arrival_s: seconds
+    after start; kind: index into imgs / h_srcs (host pointers) per request;
+    expect_pngs: optional bytes per kind that every request's PNG must equal.
+    Returns (latency_s, png_len, wall_s, same) -- same is None without
+    expected PNGs, else a uint8 array (1 where the bytes matched)."""
+    n = len(arrival_s)
+    arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+    kd = np.ascontiguousarray(kind, dtype=np.int32)
+    im = (Image * len(imgs))(*imgs)
+    src = (C.c_void_p * len(h_srcs))(*[C.c_void_p(p) for p in h_srcs])
+    lat = np.zeros(n)
+    plen = np.zeros(n, np.uint64)
+    wall = C.c_double()
+    same = ex = exl = None
+    keep = None
+    if expect_pngs is not None:
+        keep = [np.frombuffer(b, np.uint8) for b in expect_pngs]
+        ex = (C.c_void_p * len(keep))(*[C.c_void_p(k.ctypes.data) for k in keep])
+        exl = (C.c_size_t * len(keep))(*[k.size for k in keep])
+        same = np.zeros(n, np.uint8)
+    check(lib().r2p_server_loadtest(server, n, arr.ctypes.data_as(C.POINTER(C.c_double)),
+                                    kd.ctypes.data_as(C.POINTER(C.c_int)), im, src,
+                                    lat.ctypes.data_as(C.POINTER(C.c_double)),
+                                    plen.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(wall), ex, exl,
+                                    same.ctypes.data_as(C.POINTER(C.c_uint8)) if same is not None else None))
+    del keep
+    return lat, plen, wall.value, same

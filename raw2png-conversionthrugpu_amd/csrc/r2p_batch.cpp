@@ -885,9 +885,11 @@ int r2p_server_stats_get(r2p_server* s, r2p_server_stats* st)
 }
 
 int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int* kind, const r2p_image* imgs,
-                        const void* const* h_src, double* latency_s, uint64_t* png_len, double* wall_s)
+                        const void* const* h_src, double* latency_s, uint64_t* png_len, double* wall_s,
+                        const void* const* expect_png, const size_t* expect_len, uint8_t* same)
 {
     if (!s || n <= 0 || !arrival_s || !kind || !imgs || !h_src || !latency_s) { set_error("bad loadtest arguments"); return R2P_ERR_ARG; }
+    if (expect_png && (!expect_len || !same)) { set_error("expected PNGs need their lengths and a result array"); return R2P_ERR_ARG; }
     std::vector<int64_t> ids((size_t)n, -1);
     const auto t0 = SClock::now() + std::chrono::milliseconds(5);
     int rc = R2P_OK;
@@ -907,24 +909,69 @@ int r2p_server_loadtest(r2p_server* s, int n, const double* arrival_s, const int
             s->cv_done.notify_all();
         }
     });
-    // results are consumed (and their buffers recycled) as they complete, in
-    // arrival order, while the client keeps submitting
+    // Byte checks: each finished PNG is compared with the expected bytes of
+    // its image by a few verifier threads, after its latency is taken; they
+    // then recycle its buffer.
+    std::mutex vmu;
+    std::condition_variable vcv;
+    std::deque<std::pair<int, Request*>> vq;
+    bool vstop = false;
+    std::vector<std::thread> verifiers;
+    auto release = [s](Request* r) { s->pool.put(r->png, r->png_cap); delete r; };
+    if (expect_png) {
+        for (int i = 0; i < n; i++) same[i] = 0;
+        for (int t = 0; t < 4; t++)
+            verifiers.emplace_back([&] {
+                for (;;) {
+                    std::pair<int, Request*> it;
+                    {
+                        std::unique_lock<std::mutex> lk(vmu);
+                        vcv.wait(lk, [&] { return vstop || !vq.empty(); });
+                        if (vq.empty()) return;
+                        it = vq.front();
+                        vq.pop_front();
+                    }
+                    Request* r = it.second;
+                    const int k = kind[it.first];
+                    same[it.first] = r->status == R2P_OK && r->png_len == expect_len[k] &&
+                                     (r->png_len == 0 || memcmp(r->png, expect_png[k], r->png_len) == 0);
+                    release(r);
+                }
+            });
+    }
+    // results are consumed in arrival order while the client keeps submitting
     SClock::time_point last = t0;
     for (int i = 0; i < n; i++) {
-        std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_done.wait(lk, [&] { return submitted > i || client_failed; });
-        if (submitted <= i) break;
-        Request* r = s->all[ids[(size_t)i]];
-        s->cv_done.wait(lk, [r] { return r->done; });
-        latency_s[i] = std::chrono::duration<double>(r->done_at - r->arrival).count();
-        if (png_len) png_len[i] = r->png_len;
-        if (r->done_at > last) last = r->done_at;
-        if (r->status) rc = r->status;
-        s->all.erase(ids[(size_t)i]);
-        s->pool.put(r->png, r->png_cap);
-        delete r;
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_done.wait(lk, [&] { return submitted > i || client_failed; });
+            if (submitted <= i) break;
+            r = s->all[ids[(size_t)i]];
+            s->cv_done.wait(lk, [r] { return r->done; });
+            latency_s[i] = std::chrono::duration<double>(r->done_at - r->arrival).count();
+            if (png_len) png_len[i] = r->png_len;
+            if (r->done_at > last) last = r->done_at;
+            if (r->status) rc = r->status;
+            s->all.erase(ids[(size_t)i]);
+        }
+        if (expect_png) {
+            {
+                std::lock_guard<std::mutex> lk(vmu);
+                vq.push_back({i, r});
+            }
+            vcv.notify_one();
+        } else {
+            release(r);
+        }
     }
     client.join();
+    {
+        std::lock_guard<std::mutex> lk(vmu);
+        vstop = true;
+    }
+    vcv.notify_all();
+    for (auto& t : verifiers) t.join();
     if (wall_s) *wall_s = std::chrono::duration<double>(last - t0).count();
     return rc;
 }

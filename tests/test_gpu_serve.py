@@ -84,7 +84,7 @@ def test_serve_loadtest_small(torch_dev, r2p, synth):
     r2p.check(L.r2p_server_loadtest(h, n, arr.ctypes.data_as(C.POINTER(C.c_double)),
                                     kind.ctypes.data_as(C.POINTER(C.c_int)), imgs, srcs,
                                     lat.ctypes.data_as(C.POINTER(C.c_double)),
-                                    plen.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(wall)))
+                                    plen.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(wall), None, None, None))
     assert np.all(lat > 0) and np.all(plen > 0) and wall.value > 0
     alone = [len(plain_png(r2p, s.img, p, cfg.opt)) for s, p in zip(ims, pins)]
     assert all(int(plen[i]) == alone[kind[i]] for i in range(n))
