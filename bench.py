@@ -50,30 +50,84 @@ def pct(xs, q):
     return xs[k]
 
 
-def cpu_baseline(oracle, s, level, threads, budget_s=12.0):
-    """Times the oracle's CPU path (window + filter + chunked zlib) on the same image."""
+def cpu_info():
+    """Host cores and CPU model as the OS reports them."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def oracle_encode(oracle, s, level, threads, strip_height=128, want_filtered=False):
     img = s.img
     mode = {0: oracle.MODE_READY, 1: oracle.MODE_SWAP16, 2: oracle.MODE_DICOM, 3: oracle.MODE_DICOM8}[img.pix_mode]
     p = oracle.DicomParams(*[getattr(img.dicom, f) for f, _ in img.dicom._fields_])
+    return oracle.encode_png(s.pixels, img.width, img.height, img.channels, img.in_bits, mode, p,
+                             strip_height=strip_height, level=level, threads=threads, want_filtered=want_filtered)
+
+
+def cpu_baseline(oracle, s, level, threads, budget_s=12.0, strip_height=128):
+    """Times the oracle's CPU path (window + filter + chunked zlib, the
+    reference's deflate_strip scheme) on the same image: 3 warm-ups, then up
+    to 15 timed encodes within the budget (bench_deflate's 3 + 15); median."""
+    img = s.img
     times, png_len = [], 0
     t_end = time.perf_counter() + budget_s
     it = 0
     while it < 3 or (time.perf_counter() < t_end and it < 18):
         t0 = time.perf_counter()
-        e = oracle.encode_png(s.pixels, img.width, img.height, img.channels, img.in_bits, mode, p,
-                              strip_height=128, level=level, threads=threads, want_filtered=False)
+        e = oracle_encode(oracle, s, level, threads, strip_height)
         dt = time.perf_counter() - t0
         png_len = len(e.png)
-        if it >= 3:           # 3 warm-ups, then timed iterations (bench_deflate's 3 + 15)
+        if it >= 3:
             times.append(dt)
         it += 1
     med = statistics.median(times)
     return {"value": round(s.megapixels / med, 2), "unit": "MP/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(times)} timed encodes (after 3 warm-ups) of the same {img.width}x{img.height} "
-                      f"image, zlib {oracle.zlib_version()} level {level}, strip height 128, "
+                      f"image, zlib {oracle.zlib_version()} level {level}, strip height {strip_height}, "
                       f"{threads} threads; median",
             "png_bytes": png_len}
+
+
+def cpu_leg(args, oracle, synth, r, gpu_pngs):
+    """The CPU side: the oracle's filtered bytes of the timed image (the
+    bit-exact check of the GPU's PNG), the timed baselines (all usable cores
+    and the reference's 6 deflate threads at level 3 on the radiograph; level
+    0 on the CT slice, the reference's CPU-runnable case) and zlib's PNG size
+    at levels 0, 1 and 3 for the three single-image workloads."""
+    out = {"host": cpu_info()}
+    # the host's share of cores on a GPU box is 16 whatever nproc says
+    threads = min(16, os.cpu_count() or 1)
+    out["threads_all"] = threads
+    s1 = r["s"]
+    e = oracle_encode(oracle, s1, args.cpu_level, threads, want_filtered=True)
+    out["bit_exact"] = bool(r["inflated"] == e.filtered.tobytes())
+    base = {}
+    base["radiograph_level3_all"] = cpu_baseline(oracle, s1, args.cpu_level, threads)
+    base["radiograph_level3_6threads"] = cpu_baseline(oracle, s1, args.cpu_level, 6, budget_s=8.0)
+    ct = synth.workload("ct_slice")
+    base["ct_slice_level0_strip128"] = cpu_baseline(oracle, ct, 0, threads, budget_s=4.0)
+    out["baselines"] = {k: {kk: v[kk] for kk in ("value", "cores", "sample")} for k, v in base.items()}
+    out["primary"] = base["radiograph_level3_all"]
+    ratios = {}
+    for name, s in (("ct_slice", ct), ("radiograph", s1), ("camera_rgb", r.get("camera"))):
+        if s is None:
+            continue
+        fb = s.img.height * (s.img.width * s.img.channels * (8 if s.img.pix_mode == 3 else s.img.in_bits) // 8 + 1)
+        row = {}
+        for lvl in (0, 1, 3):
+            row[f"zlib_level{lvl}"] = round(len(oracle_encode(oracle, s, lvl, threads).png) / fb, 4)
+        row.update(gpu_pngs.get(name, {}))
+        ratios[name] = row
+    out["ratios"] = ratios
+    return out
 
 
 def measured_traffic(args, kernel):
@@ -88,8 +142,17 @@ def measured_traffic(args, kernel):
     if args.workload != "radiograph" or (t.get("segment_bytes"), t.get("slice_bytes"), t.get("wg_threads"),
                                          t.get("lz77")) != (args.seg, args.slice, args.wg, args.lz77):
         return None
+    # counters taken on other kernel sources say nothing about these
+    if t.get("kernels_sha16") != kernels_sha16():
+        return None
     k = t.get("kernels", {}).get("k_" + kernel)
     return k["bytes"] if k else None
+
+
+def kernels_sha16():
+    import hashlib
+    with open(os.path.join(PKG_DIR, "csrc", "r2p_kernels.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def run_plain(args, r2p, synth, torch):
@@ -124,20 +187,16 @@ def run_plain(args, r2p, synth, torch):
         enc.sync()
         t = enc.timings()
         ft.append(t.filter_ms); dt.append(t.deflate_ms); st.append(t.scan_ms); at.append(t.assemble_ms)
-    # correctness of the timed output: inflate and compare with the filtered
-    # scanlines, which one more (untimed) encode also copies out
+    # the timed output: its inflated bytes (compared with the oracle's in the
+    # CPU leg) and one more encode that also copies the filtered scanlines out
     png = d_png[:n].cpu().numpy().tobytes()
     enc.encode_device(img, d_src.data_ptr(), d_png.data_ptr(), cap, d_filt.data_ptr())
     enc.sync()
     assert d_png[:n].cpu().numpy().tobytes() == png, "encode is not deterministic"
     filt = d_filt.cpu().numpy().tobytes()
-    pos, z = 8, b""
-    while pos < len(png):
-        ln = int.from_bytes(png[pos:pos + 4], "big")
-        if png[pos + 4:pos + 8] == b"IDAT":
-            z += png[pos + 8:pos + 8 + ln]
-        pos += 12 + ln
-    ok = zlib.decompress(z) == filt
+    z = idat_stream(png)
+    inflated = zlib.decompress(z)
+    host = run_plain_host(args, r2p, torch, enc, img, s, cap, png)
     mp = s.megapixels
     raw_in = r2p.input_bytes(img)
     kern = {"filter": statistics.mean(ft), "deflate": statistics.mean(dt),
@@ -146,12 +205,67 @@ def run_plain(args, r2p, synth, torch):
     dom = max(kern, key=kern.get)
     res = {
         "workload": args.workload, "mp": mp, "steps": args.steps, "wall": wall, "lat": lat,
-        "png_bytes": n, "filtered_bytes": fb, "raw_in": raw_in, "zlen": len(z), "ok": ok,
+        "png_bytes": n, "filtered_bytes": fb, "raw_in": raw_in, "zlen": len(z),
+        "inflated": inflated, "self_consistent": inflated == filt, "host_to_host": host,
         "kern_ms": kern, "algo_bytes": algo, "dominant": dom, "nseg": nseg, "s": s,
     }
     enc.close()
     res["dynamic"] = None if args.no_dynamic else run_plain_dynamic(args, r2p, torch, img, opt, d_src, filt, fb)
     return res
+
+
+def idat_stream(png: bytes) -> bytes:
+    pos, z = 8, []
+    while pos < len(png):
+        ln = int.from_bytes(png[pos:pos + 4], "big")
+        if png[pos + 4:pos + 8] == b"IDAT":
+            z.append(png[pos + 8:pos + 8 + ln])
+        pos += 12 + ln
+    return b"".join(z)
+
+
+def run_plain_host(args, r2p, torch, enc, img, s, cap, png_dev):
+    """The plain form from host memory to host memory (r2p_encode_host: the
+    pixels cross PCIe in, the PNG out), pinned buffers on both sides."""
+    import ctypes as C
+    h_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).pin_memory()
+    h_png = torch.empty(cap, dtype=torch.uint8).pin_memory()
+    L = r2p.lib()
+    n = C.c_size_t()
+    for _ in range(3):
+        r2p.check(L.r2p_encode_host(enc.h, C.byref(img), h_src.data_ptr(), h_png.data_ptr(), cap, C.byref(n)))
+    steps = max(5, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r2p.check(L.r2p_encode_host(enc.h, C.byref(img), h_src.data_ptr(), h_png.data_ptr(), cap, C.byref(n)))
+    dt = (time.perf_counter() - t0) / steps
+    same = h_png[:n.value].numpy().tobytes() == png_dev
+    return {"mp_per_s": round(s.megapixels / dt, 1), "ms_per_image": round(dt * 1e3, 4),
+            "same_png_as_device_form": bool(same),
+            "note": "r2p_encode_host: H2D of the pixels, the encode, D2H of the PNG; pinned host buffers"}
+
+
+def run_ratios(args, r2p, synth, torch):
+    """PNG size over filtered bytes of the CT slice and the camera image with
+    fixed-Huffman (the timed form) and dynamic-Huffman blocks."""
+    out, samples = {}, {}
+    for name in ("ct_slice", "camera_rgb"):
+        s = synth.workload(name)
+        samples[name] = s
+        d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
+        fb = r2p.filtered_bytes(s.img)
+        row = {}
+        for label, huff in (("gpu_fixed", 0), ("gpu_dynamic", 1)):
+            o = r2p.options(huffman=huff)
+            enc = r2p.Encoder(fb, o)
+            cap = r2p.png_bound(s.img, o)
+            d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
+            enc.encode_device(s.img, d_src.data_ptr(), d_png.data_ptr(), cap)
+            n, _, _ = enc.sync()
+            enc.close()
+            row[label] = round(n / fb, 4)
+        out[name] = row
+    return out, samples
 
 
 def run_plain_dynamic(args, r2p, torch, img, opt, d_src, filt, fb):
@@ -173,12 +287,7 @@ def run_plain_dynamic(args, r2p, torch, img, opt, d_src, filt, fb):
     ms = (time.perf_counter() - t0) / steps * 1e3
     png = d_png[:n].cpu().numpy().tobytes()
     enc.close()
-    pos, z = 8, b""
-    while pos < len(png):
-        ln = int.from_bytes(png[pos:pos + 4], "big")
-        if png[pos + 4:pos + 8] == b"IDAT":
-            z += png[pos + 8:pos + 8 + ln]
-        pos += 12 + ln
+    z = idat_stream(png)
     return {"png_over_filtered": round(n / fb, 4), "ms_per_image": round(ms, 4),
             "mp_per_s": round(img.width * img.height / 1e6 / (ms * 1e-3), 1),
             "inflates_to_filtered": zlib.decompress(z) == filt}
@@ -261,20 +370,16 @@ def run_batch(args, r2p, synth, torch):
     for im, sp in list(zip(imgs, srcs))[:8]:
         enc.encode_device(im, sp, d_png.data_ptr(), cap); enc.sync()
     lat = []
-    same = True
-    rng = np.random.default_rng(0)
-    check = set(rng.choice(len(imgs), size=min(16, len(imgs)), replace=False).tolist())
+    differ = 0
     torch.cuda.synchronize()
-    t2 = time.perf_counter()
     for i, (im, sp) in enumerate(zip(imgs, srcs)):
         ts = time.perf_counter()
         enc.encode_device(im, sp, d_png.data_ptr(), cap)
         n, _, _ = enc.sync()
         lat.append(time.perf_counter() - ts)
-        if i in check:
-            same &= d_png[:n].cpu().numpy().tobytes() == b.copy_result(i)
-    pwall = sum(lat)          # the check copies between encodes are not timed
-    _ = t2
+        # every image's PNG against the batch's (the copies are not timed)
+        differ += d_png[:n].cpu().numpy().tobytes() != b.copy_result(i)
+    pwall = sum(lat)
     enc.close()
     png_total = sum(b.result(i)[1] for i in range(len(imgs)))
     b.close()
@@ -296,14 +401,16 @@ def run_batch(args, r2p, synth, torch):
                       "wg_threads": tuned.wg_threads,
                       "band_rows": tuned.band_rows, "best_ms": tuned_ms},
         "png_over_filtered": round(png_total / fsum, 4),
-        "byte_identical_to_plain": bool(same), "steps": steps,
+        "byte_identical_to_plain": differ == 0, "images_compared": len(imgs), "steps": steps,
     }
 
 
 def run_serve(args, r2p, synth, torch):
     """Served form: open-loop Poisson arrivals drawn from the mixed batch's
     size/type mix plus 5% 8192x8192 uint16 radiographs, admitted against a
-    64 GB HBM budget; latency is arrival -> PNG bytes in host memory."""
+    64 GB HBM budget; latency is arrival -> PNG bytes in host memory.  Every
+    request's PNG is compared byte for byte with the plain form's PNG of its
+    image (native verifier threads, after the latency is taken)."""
     import ctypes as C
     L = r2p.lib()
     rng = np.random.Generator(np.random.PCG64(4))
@@ -311,7 +418,7 @@ def run_serve(args, r2p, synth, torch):
     big = synth.radiograph_device(torch, 8192, seed=44)
     tmpl = pool + [big]
     torch.cuda.synchronize()
-    # expected PNG length of every template from the plain form
+    # expected PNG of every template from the plain form
     opt = r2p.options()
     expect = []
     for im, t in tmpl:
@@ -321,14 +428,13 @@ def run_serve(args, r2p, synth, torch):
         enc.encode_device(im, t.data_ptr(), d_png.data_ptr(), cap)
         n, _, _ = enc.sync()
         enc.close()
-        expect.append(n)
+        expect.append(d_png[:n].cpu().numpy().tobytes())
     pins = [t.cpu().pin_memory() for _, t in tmpl]
-    imgs = (r2p.Image * len(tmpl))(*[im for im, _ in tmpl])
-    srcs = (C.c_void_p * len(tmpl))(*[C.c_void_p(p.data_ptr()) for p in pins])
-    mp_of = np.array([im.width * im.height / 1e6 for im, _ in tmpl])
-    in_of = np.array([r2p.input_bytes(im) for im, _ in tmpl], dtype=np.float64)
+    imgs = [im for im, _ in tmpl]
+    mp_of = np.array([im.width * im.height / 1e6 for im in imgs])
+    in_of = np.array([r2p.input_bytes(im) for im in imgs], dtype=np.float64)
     out = {"seconds_per_rate": args.serve_seconds, "hbm_budget_gb": 64, "lanes": args.serve_lanes,
-           "max_batch_images": args.serve_batch, "pool": len(pool)}
+           "max_batch_images": args.serve_batch, "pool": len(pool), "big_requests": "5% 8192x8192 16-bit"}
     for lam in args.serve_rates:
         n = int(lam * args.serve_seconds)
         arr = np.cumsum(rng.exponential(1.0 / lam, n))
@@ -341,29 +447,22 @@ def run_serve(args, r2p, synth, torch):
         cfg.opt = opt
         h = C.c_void_p()
         r2p.check(L.r2p_server_create(C.byref(cfg), C.byref(h)))
-        lat = np.zeros(n)
-        plen = np.zeros(n, np.uint64)
-        wall = C.c_double()
         try:
-            r2p.check(L.r2p_server_loadtest(h, n, arr.ctypes.data_as(C.POINTER(C.c_double)),
-                                            kind.ctypes.data_as(C.POINTER(C.c_int)), imgs, srcs,
-                                            lat.ctypes.data_as(C.POINTER(C.c_double)),
-                                            plen.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(wall)))
+            lat, plen, wall, same = r2p.server_loadtest(h, arr, kind, imgs, [p.data_ptr() for p in pins], expect)
             st = r2p.ServerStats()
             r2p.check(L.r2p_server_stats_get(h, C.byref(st)))
         finally:
             L.r2p_server_destroy(h)
         mp = float(mp_of[kind].sum())
-        same = bool(np.all(plen == np.array(expect, np.uint64)[kind]))
         out[str(lam)] = {
             "requests": n, "offered_mp_per_s": round(mp / float(arr[-1]), 1),
-            "mp_per_s": round(mp / wall.value, 1), "wall_s": round(wall.value, 3),
-            "h2d_gb_per_s": round(float(in_of[kind].sum()) / wall.value / 1e9, 2),
-            "d2h_gb_per_s": round(float(plen.astype(np.float64).sum()) / wall.value / 1e9, 2),
+            "mp_per_s": round(mp / wall, 1), "wall_s": round(wall, 3),
+            "h2d_gb_per_s": round(float(in_of[kind].sum()) / wall / 1e9, 2),
+            "d2h_gb_per_s": round(float(plen.astype(np.float64).sum()) / wall / 1e9, 2),
             "latency_ms": {q: round(float(np.percentile(lat, int(q[1:]))) * 1e3, 3) for q in ("p50", "p95", "p99")},
             "batches": int(st.batches), "mean_batch_images": round(st.mean_batch_images, 2),
             "peak_admitted_gb": round(st.peak_admitted_bytes / 1e9, 3),
-            "png_lengths_equal_plain": same,
+            "png_bytes_equal_plain": int(same.sum()) == n, "requests_compared": n,
         }
         log(f"bench: serve lambda={lam}: {json.dumps(out[str(lam)])}")
     return out
@@ -387,8 +486,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic-Huffman encodes of the plain workload")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 800.0, 2000.0])
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
-                    help="arrival window per rate (the served workload names 60 s)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window per rate")
     ap.add_argument("--serve-pool", type=int, default=64, help="distinct request images of the mix")
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=64)
@@ -423,10 +521,11 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"{args.workload} {r['s'].img.width}x{r['s'].img.height}",
                    "form": "plain per-image", "segment_bytes": args.seg, "slice_bytes": args.slice, "wg_threads": args.wg, "lz77": args.lz77,
-                   "bit_exact": r["ok"]},
+                   "bit_exact": None},
         "latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
         "ratio": {"gpu_png_over_filtered": round(r["png_bytes"] / r["filtered_bytes"], 4)},
+        "host_to_host": r["host_to_host"],
         "dynamic_huffman": r["dynamic"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -438,12 +537,24 @@ def main():
     fk = "filter"
     fach = r["algo_bytes"][fk] / (kern[fk] * 1e-3) / 1e9
     out["filter_kernel"] = {"achieved": round(fach, 1), "frac": round(fach / HBM_PEAK_GBS, 4)}
+    gpu_ratios, samples = run_ratios(args, r2p, synth, torch)
+    gpu_ratios["radiograph"] = {"gpu_fixed": out["ratio"]["gpu_png_over_filtered"]}
+    if r["dynamic"]:
+        gpu_ratios["radiograph"]["gpu_dynamic"] = r["dynamic"]["png_over_filtered"]
+    r["camera"] = samples["camera_rgb"]
     if not args.no_cpu:
+        # the oracle is the checker of the timed PNG and the timed CPU baseline
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(O, r["s"], args.cpu_level, threads)
+        c = cpu_leg(args, O, synth, r, gpu_ratios)
+        out["config"]["bit_exact"] = c["bit_exact"]
+        cb = c["primary"]
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        out["ratio"]["zlib_level%d_png_over_filtered" % args.cpu_level] = round(cb["png_bytes"] / r["filtered_bytes"], 4)
+        out["cpu_baseline"]["host"] = c["host"]
+        out["cpu_baselines"] = c["baselines"]
+        out["ratio"]["by_workload"] = c["ratios"]
+        log("bench: cpu " + json.dumps({k: c[k] for k in ("bit_exact", "host", "baselines")}))
+    else:
+        out["ratio"]["by_workload"] = gpu_ratios
     if args.checksum_mib > 0:
         out["checksum_kernels"] = run_checksums(args, r2p, torch)
     if args.mode in ("batch", "all"):
