@@ -111,6 +111,15 @@ static void build_tables(uint32_t lit[256], uint32_t len[259], uint32_t dist_rev
 
 static std::mutex g_init_mu;
 static bool g_dev_ready[64];
+static int g_cu[64];
+
+// Compute units of the current device (256 on MI355X).
+static int cu_count()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_cu[dev] <= 0) return 256;
+    return g_cu[dev];
+}
 
 // Scratch of r2p_encode_device_noalloc: every buffer of one plain encode in
 // one static device array (each device holds its own copy until the process
@@ -376,6 +385,7 @@ int r2p_init(int device)
         shift32[4][256];
     build_tables(lit, len, dist_rev, crc4, x8pow, shift, shift32);
     R2P_HIP(upload_tables(lit, len, dist_rev, &crc4[0][0], x8pow, &shift[0][0][0], &shift32[0][0]));
+    g_cu[device] = prop.multiProcessorCount;
     void* sp = nullptr;
     const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
     if (se != hipSuccess || !sp) return hip_fail(se != hipSuccess ? se : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
@@ -688,8 +698,12 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     const bool self_scan = assemble_scans_itself(A);
     if (!self_scan) R2P_HIP(launch_scan(AL, 1, s));
     if (w.ev) R2P_HIP(hipEventRecord(w.ev[3], s));
-    if (self_scan) R2P_HIP(launch_assemble_self_scan(AL, std::min(A.max_chunks, 4096), s));
-    else R2P_HIP(launch_assemble(AL, std::min(A.max_chunks, 4096), s));
+    // one workgroup per CU is resident (1024 threads, ~110 VGPRs): a grid of
+    // at most that many loops over the chunks in one round, where a grid of
+    // the 9/8 bound would leave a second round of workgroups with no chunk
+    const int grid = std::min(A.max_chunks, cu_count());
+    if (self_scan) R2P_HIP(launch_assemble_self_scan(AL, grid, s));
+    else R2P_HIP(launch_assemble(AL, grid, s));
     if (w.ev) R2P_HIP(hipEventRecord(w.ev[4], s));
     if (nseg_out) *nseg_out = (uint32_t)nseg;
     return R2P_OK;
