@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <filesystem>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -236,9 +237,19 @@ double ds_first(const uint8_t* p, uint32_t len, bool* ok)
     return v;
 }
 
-bool load_dicom(const char* path, int frame, BufferSource& s, int* nframes_out)
+// A parsed Part 10 file: its bytes, the pixel data's place and the image
+// attributes; frames are cut from it without reading the file again.
+struct DicomFile {
+    std::vector<uint8_t> bytes;
+    size_t pix_off = 0, pix_len = 0;
+    ImageInfo inf;
+    DicomPixelParams dp;
+    int nframes = 1;
+};
+
+bool parse_dicom(const char* path, DicomFile& df)
 {
-    std::vector<uint8_t> f;
+    std::vector<uint8_t>& f = df.bytes;
     if (!read_file(path, f) || f.size() < 132 || memcmp(&f[128], "DICM", 4) != 0) {
         fprintf(stderr, "DICOM: cannot load '%s' (not Part 10)\n", path);
         return false;
@@ -302,14 +313,12 @@ bool load_dicom(const char* path, int frame, BufferSource& s, int* nframes_out)
         return false;
     }
     if (!has_hb) hb = bs - 1;                                  // dicom_loader.cpp:329-330
-    if (nframes_out) *nframes_out = nframes;
-    if (frame < 0 || frame >= nframes) { fprintf(stderr, "DICOM: frame %d out of range\n", frame); return false; }
-    s.inf.width = cols; s.inf.height = rows; s.inf.channels = (int)spp; s.inf.bits_per_sample = (int)ba;
-    s.inf.bpp = (int)(spp * ba / 8);
-    const size_t fbytes = (size_t)rows * cols * s.inf.bpp;
-    if ((size_t)(frame + 1) * fbytes > pix_len) { fprintf(stderr, "DICOM: short pixel data\n"); return false; }
-    s.px.assign(p + pix_off + (size_t)frame * fbytes, p + pix_off + (size_t)(frame + 1) * fbytes);
-    DicomPixelParams& d = s.dp;
+    df.nframes = nframes;
+    df.pix_off = pix_off;
+    df.pix_len = pix_len;
+    df.inf.width = cols; df.inf.height = rows; df.inf.channels = (int)spp; df.inf.bits_per_sample = (int)ba;
+    df.inf.bpp = (int)(spp * ba / 8);
+    DicomPixelParams& d = df.dp;
     d = DicomPixelParams{};
     d.bits_allocated = (int)ba; d.bits_stored = (int)bs; d.high_bit = (int)hb; d.pixel_rep = (int)rep;
     if ((has_slope || has_icpt) && (std::fabs(slope - 1.0) > 1e-9 || std::fabs(icpt) > 1e-9)) {   // :380-388
@@ -318,7 +327,21 @@ bool load_dicom(const char* path, int frame, BufferSource& s, int* nframes_out)
     if (has_wc && has_ww && ww > 0.0) {                                                         // :390-397
         d.apply_window = 1; d.window_center = (float)wc; d.window_width = (float)ww;
     }
+    return true;
+}
+
+// Frame `frame` of a parsed file as an in-memory source.
+bool dicom_frame(const DicomFile& df, int frame, BufferSource& s)
+{
+    if (frame < 0 || frame >= df.nframes) { fprintf(stderr, "DICOM: frame %d out of range\n", frame); return false; }
+    s.inf = df.inf;
+    const size_t fbytes = (size_t)df.inf.height * df.inf.width * df.inf.bpp;
+    if ((size_t)(frame + 1) * fbytes > df.pix_len) { fprintf(stderr, "DICOM: short pixel data\n"); return false; }
+    const uint8_t* base = df.bytes.data() + df.pix_off + (size_t)frame * fbytes;
+    s.px.assign(base, base + fbytes);
+    s.dp = df.dp;
     s.has_dicom = true;
+    s.next = 0;
     return true;
 }
 
@@ -591,8 +614,9 @@ bool encode_raw_to_png(const char*, const char*, const PipelineConfig&)
 bool encode_dicom_to_png(const char* in, const char* out, const PipelineConfig& cfg)
 {
     auto t0 = Clock::now();
+    DicomFile df;
     BufferSource s;
-    if (!load_dicom(in, cfg.frame >= 0 ? cfg.frame : 0, s, nullptr)) return false;
+    if (!parse_dicom(in, df) || !dicom_frame(df, cfg.frame >= 0 ? cfg.frame : 0, s)) return false;
     auto t1 = Clock::now();
     const bool ok = run(s, out, cfg);
     if (cfg.use_gpu_deflate)
@@ -601,20 +625,56 @@ bool encode_dicom_to_png(const char* in, const char* out, const PipelineConfig& 
     return ok;
 }
 
+// encode_dicom_all_frames_to_png (pipeline.cpp:1760-1879 of the reference):
+// the file is read and parsed once; K frame workers (cfg.gpu_streams, else
+// 4, at most the frame count) take frames in turn, each with its own device
+// encoder and stream (and its share of the deflate threads on the CPU-deflate
+// path), writing <out_dir>/slice_%04d.png.  The output folder is created.
 bool encode_dicom_all_frames_to_png(const char* in, const char* out_dir, const PipelineConfig& cfg)
 {
-    int nf = 1;
-    BufferSource first;
-    if (!load_dicom(in, 0, first, &nf)) return false;
-    for (int i = 0; i < nf; i++) {
-        BufferSource s;
-        if (i == 0) s = first; else if (!load_dicom(in, i, s, nullptr)) return false;
-        s.next = 0;
-        char path[4096];
-        snprintf(path, sizeof(path), "%s/slice_%04d.png", out_dir, i);
-        if (!run(s, path, cfg, true)) { (void)tl_writer.wait(); return false; }
+    DicomFile df;
+    if (!parse_dicom(in, df)) return false;
+    const int n = df.nframes;
+    std::error_code ec;
+    std::filesystem::create_directories(out_dir, ec);
+    if (ec) {
+        fprintf(stderr, "Cannot create output folder '%s': %s\n", out_dir, ec.message().c_str());
+        return false;
     }
-    return tl_writer.wait();                       // every frame's file written
+    fprintf(stdout, "Detected %d frame%s\n", n, n == 1 ? "" : "s");
+    const int workers = std::max(1, std::min(cfg.gpu_streams > 0 ? cfg.gpu_streams : 4, n));
+    PipelineConfig wcfg = cfg;
+    wcfg.deflate_threads = std::max(1, cfg.deflate_threads / workers);
+    std::atomic<int> next{0};
+    std::atomic<bool> ok{true};
+    std::mutex print_mu;
+    auto worker = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n || !ok.load()) break;
+            BufferSource s;
+            if (!dicom_frame(df, i, s)) { ok = false; break; }
+            char path[4096];
+            snprintf(path, sizeof(path), "%s/slice_%04d.png", out_dir, i);
+            {
+                std::lock_guard<std::mutex> lk(print_mu);
+                fprintf(stdout, "[%d/%d] slice_%04d.png\n", i + 1, n, i);
+            }
+            if (!run(s, path, wcfg, true)) { ok = false; break; }
+        }
+        if (!tl_writer.wait()) ok = false;           // this worker's files written
+    };
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::vector<std::thread> th;
+    for (int w = 0; w < workers; w++)
+        th.emplace_back([&, dev] {
+            (void)hipSetDevice(dev);
+            worker();
+        });
+    for (auto& t : th) t.join();
+    if (!ok.load()) fprintf(stderr, "Export aborted after an error.\n");
+    return ok.load();
 }
 
 void pipeline_reset_gpu_batch_stats() { g_files = 0; g_dicom_ms = 0; g_total_ms = 0; }

@@ -282,18 +282,27 @@ def _ds(v) -> bytes:
 
 
 def write_dicom(path: str, pixels: np.ndarray, p: DicomParams, frames: int = 1,
-                slope_str=None, intercept_str=None, wc_str=None, ww_str=None) -> None:
-    """Explicit VR LE Part 10 file holding `pixels` (uint16 or uint8, (F,)H,W[,C])."""
+                slope_str=None, intercept_str=None, wc_str=None, ww_str=None, implicit: bool = False) -> None:
+    """Uncompressed little-endian Part 10 file holding `pixels` (uint16 or
+    uint8, (F,)H,W[,C]): Explicit VR, or Implicit VR with `implicit` (the file
+    meta group is Explicit VR either way)."""
     import struct
 
-    def el(g, e, vr, val: bytes):
+    def el_explicit(g, e, vr, val: bytes):
         if len(val) % 2:
             val += b"\0" if vr in (b"UI", b"OB") else b" "
         if vr in (b"OB", b"OW", b"SQ", b"UN", b"UT"):
             return struct.pack("<HH2sHI", g, e, vr, 0, len(val)) + val
         return struct.pack("<HH2sH", g, e, vr, len(val)) + val
 
-    ts = b"1.2.840.10008.1.2.1"
+    def el(g, e, vr, val: bytes):
+        if g == 0x0002 or not implicit:
+            return el_explicit(g, e, vr, val)
+        if len(val) % 2:
+            val += b"\0" if vr in (b"UI", b"OB") else b" "
+        return struct.pack("<HHI", g, e, len(val)) + val
+
+    ts = b"1.2.840.10008.1.2" if implicit else b"1.2.840.10008.1.2.1"
     meta_body = el(0x0002, 0x0001, b"OB", b"\0\1") + el(0x0002, 0x0010, b"UI", ts)
     meta = el(0x0002, 0x0000, b"UL", struct.pack("<I", len(meta_body))) + meta_body
     px = np.ascontiguousarray(pixels)

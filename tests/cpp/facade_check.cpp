@@ -125,6 +125,11 @@ int main(int argc, char** argv)
         if (kind == "tiff") ok = encode_tiff_to_png(argv[3], argv[4], cfg);
         else if (kind == "dicom") ok = encode_dicom_to_png(argv[3], argv[4], cfg);
         else if (kind == "dicom-all") ok = encode_dicom_all_frames_to_png(argv[3], argv[4], cfg);
+        else if (kind == "dicom-frame") {      // strip_height slot: the frame to encode
+            cfg.frame = argc > 6 ? atoi(argv[6]) : 0;
+            cfg.strip_height = 128;
+            ok = encode_dicom_to_png(argv[3], argv[4], cfg);
+        }
         printf("%s\n", ok ? "encode ok" : "encode failed");
         return ok ? 0 : 1;
     }

@@ -63,11 +63,46 @@ def test_file_readers_through_pipeline(torch_dev, r2p, synth, oracle, tmp_path, 
     r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "mf"), str(gpu_deflate)],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
-    # a directory that does not exist: the (asynchronous) writes fail and the call says so
-    r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "missing"), str(gpu_deflate)],
-                       capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0
     for k in range(nfr):
         got = np.array(PILImage.open(str(tmp_path / "mf" / f"slice_{k:04d}.png"))).astype(np.int64)
         exp = oracle.dicom_window(frames[k].reshape(-1), p).view(">u2").reshape(64, 64)
         assert np.array_equal(got, exp.astype(np.int64))
+    # the output folder is created, parents included (pipeline.cpp:1768-1774) ...
+    r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "new" / "sub"),
+                        str(gpu_deflate)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert sorted(os.listdir(str(tmp_path / "new" / "sub"))) == [f"slice_{k:04d}.png" for k in range(nfr)]
+    # ... and a folder that cannot be created fails the call
+    open(str(tmp_path / "plainfile"), "w").write("x")
+    r = subprocess.run([exe, "encode", "dicom-all", str(tmp_path / "mf.dcm"), str(tmp_path / "plainfile" / "sub"),
+                        str(gpu_deflate)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    # PipelineConfig::frame selects one frame of the multi-frame file
+    for k in (0, 4):
+        out = str(tmp_path / f"frame{k}.png")
+        r = subprocess.run([exe, "encode", "dicom-frame", str(tmp_path / "mf.dcm"), out, str(gpu_deflate), str(k)],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        exp = oracle.dicom_window(frames[k].reshape(-1), p).view(">u2").reshape(64, 64)
+        assert np.array_equal(np.array(PILImage.open(out)).astype(np.int64), exp.astype(np.int64))
+    r = subprocess.run([exe, "encode", "dicom-frame", str(tmp_path / "mf.dcm"), str(tmp_path / "bad.png"),
+                        str(gpu_deflate), str(nfr)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0                       # frame out of range
+    # Implicit VR little endian, same pixels and attributes
+    synth.write_dicom(str(tmp_path / "ct_ivr.dcm"), ct.pixels, ct.img.dicom, implicit=True)
+    r = subprocess.run([exe, "encode", "dicom", str(tmp_path / "ct_ivr.dcm"), str(tmp_path / "ct_ivr.png"),
+                        str(gpu_deflate)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert open(str(tmp_path / "ct_ivr.png"), "rb").read() == open(str(tmp_path / "ct.png"), "rb").read()
+    # RGB DICOM (8-bit, 3 samples per pixel; the 8-bit branch of the window,
+    # gpu_filter.cu:122-130), with a rescale so the window is not the identity
+    rgb = synth.camera_rgb(5, 97, 61).pixels
+    prgb = r2p.DicomParams.make(8, 8, 7, 0, 1, 0.5, 10.0, 0)
+    synth.write_dicom(str(tmp_path / "rgb.dcm"), rgb, prgb)
+    r = subprocess.run([exe, "encode", "dicom", str(tmp_path / "rgb.dcm"), str(tmp_path / "rgb.png"), str(gpu_deflate)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    op = oracle.DicomParams(*[getattr(prgb, f) for f, _ in prgb._fields_])
+    exp = oracle.dicom_window(rgb.reshape(-1), op).reshape(61, 97, 3)
+    got = np.array(PILImage.open(str(tmp_path / "rgb.png")))
+    assert got.shape == (61, 97, 3) and np.array_equal(got, exp)
