@@ -1296,6 +1296,28 @@ constexpr uint32_t kMinKey = (3u << 16) | (0xFFFFu - 16384u);
 // (lengths 3..10: the reversed 7-bit codes of symbols 257..264; lengths
 // 11..18: those of 265..268 with the extra bit above them); selector bytes
 // 0x0C give zero bytes, so the result is that byte alone.
+// The length part of a match of length L (3..258), its fixed code (7 bits
+// for symbols 257..279, 8 for 280..285) and extra bits, as code | nbits << 26.
+__device__ __forceinline__ uint32_t long_len_tok(uint32_t L)
+{
+    const uint32_t sym = huff_len_sym(L), e = huff_len_extra_bits(L);
+    const uint32_t ex = (L - 3u) & ((1u << e) - 1u);
+    const bool b8 = sym >= 280;
+    const uint32_t code = b8 ? __builtin_bitreverse32(0xC0u + sym - 280u) >> 24 : __builtin_bitreverse32(sym - 256u) >> 25;
+    const uint32_t nb = b8 ? 8u : 7u;
+    return (code | (ex << nb)) | ((nb + e) << 26);
+}
+
+// The distance part of match key k: the 5-bit code and its extra bits.
+__device__ __forceinline__ uint32_t dist_tok(uint32_t k)
+{
+    const uint32_t dm = 0xFFFEu - (k & 0xFFFFu);
+    const uint32_t deb = 30u - __builtin_clz(dm | 2u);
+    const uint32_t dc = (deb << 1) + (dm >> deb);
+    const uint32_t dex = dm & ((1u << deb) - 1u);
+    return ((__builtin_bitreverse32(dc) >> 27) | (dex << 5)) | ((5u + deb) << 26);
+}
+
 __device__ __forceinline__ uint32_t match_tok(uint32_t k)
 {
     const uint32_t L = k >> 16;
@@ -1629,25 +1651,63 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                            0xFFFFu - (uint32_t)(bpp > 1 ? bpp : 1), 0xFFFFu - (uint32_t)(row_ok ? dr : 1),
                                            T, tag1 << kPosBits, tag2 << kPosBits);
             }
-            // greedy parse: each token's code (code | nbits << 26, 0 where no
-            // token starts) and the lane's bit count
+            // greedy parse of the lane: token starts (tmask)
+            uint32_t tmask = 0;
+            {
+                uint32_t nextp = 0;
+#pragma unroll
+                for (int i = 0; i < LANE; i++) {
+                    const uint32_t L = key[i] >> 16;   // 0: literal
+                    const bool tok = ((uint32_t)i == nextp) & (i < le);
+                    tmask |= (tok ? 1u : 0u) << i;
+                    nextp = tok ? (uint32_t)i + max(L, 1u) : nextp;
+                }
+            }
+            // ---- matches across lanes: a lane whose one token is a match of
+            // all its 16 bytes at distance d, after a lane that is the same
+            // (d included), continues that match: the lane heading such a run
+            // absorbs up to 15 following lanes into its token (16 + 15 * 16 =
+            // 256 <= 258 bytes), the next one heads a new run.  Runs stay
+            // inside the wave.  (Flat areas and repeated rows become one token
+            // per 256 bytes instead of one per 16.)
+            const bool whole = tmask == 1u && (key[0] >> 16) == (uint32_t)LANE;
+            const uint32_t kw = whole ? key[0] : 0u;
+            const uint32_t kprev = (uint32_t)__shfl_up((int)kw, 1, 64);
+            const bool cont = lane > 0 && whole && kprev != 0 && ((kprev ^ kw) & 0xFFFFu) == 0;
+            const uint64_t C = __ballot(cont);
+            const uint64_t below = lane ? (~C & ((1ull << lane) - 1ull)) : 0ull;
+            const int head = below ? 63 - __builtin_clzll(below) : 0;       // lane heading my run (if cont)
+            const bool absorbed = cont && ((lane - head) & 15) != 0;
+            // the continuing lanes after a (sub)head that it absorbs: up to 15
+            const uint64_t after = lane < 63 ? ~(C >> (lane + 1)) : 1ull;
+            const uint32_t nabs = (whole && !absorbed) ? min(15u, (uint32_t)__builtin_ctzll(after)) : 0u;
+            const uint32_t Lrun = (uint32_t)LANE + 16u * nabs;
+            if (absorbed) tmask = 0;
+
+            // each token's code (code | nbits << 26, 0 where no token starts;
+            // a run's token as its length part, then its distance part in the
+            // next slot) and the lane's bit count
             uint32_t tk[LANE];
-            uint32_t nextp = 0, nbits = 0;
+            uint32_t nbits = 0;
+            const uint32_t ext_len = DYN ? 0u : long_len_tok(Lrun);
+            const uint32_t ext_dist = DYN ? 0u : dist_tok(kw);
 #pragma unroll
             for (int i = 0; i < LANE; i++) {
                 if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
-                const uint32_t L = key[i] >> 16;       // 0: literal
-                const bool tok = ((uint32_t)i == nextp) & (i < le);
+                const bool tok = (tmask >> i) & 1u;
+                const uint32_t b = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                 if constexpr (DYN) {                   // raw token; nbits counts tokens
-                    const uint32_t b = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                    const uint32_t L = (i == 0 && nabs) ? Lrun : key[i] >> 16;
                     tk[i] = !tok ? kNoToken : L ? 0x80000000u | ((L - 3u) << 16) | (0xFFFEu - (key[i] & 0xFFFFu)) : b;
                     nbits += tok ? 1u : 0u;
                 } else {
-                    const uint32_t t = L ? match_tok(key[i]) : lit_tok((o[i >> 2] >> (8 * (i & 3))) & 0xFFu);
-                    tk[i] = tok ? t : 0u;
-                    nbits += tok ? t >> 26 : 0u;
+                    const uint32_t L = key[i] >> 16;
+                    uint32_t t = tok ? (L ? match_tok(key[i]) : lit_tok(b)) : 0u;
+                    if (i == 0) t = nabs ? ext_len : t;
+                    if (i == 1) t = nabs ? ext_dist : t;
+                    tk[i] = t;
+                    nbits += t >> 26;
                 }
-                nextp = tok ? (uint32_t)i + max(L, 1u) : nextp;
             }
             // ---- block scan of bit counts ----------------------------------------------
             const uint32_t incl = wave_incl_scan(nbits);
@@ -1656,7 +1716,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 uint32_t m_cnt = 0, m_bytes = 0, m_dist = 0, m_max = 0;
 #pragma unroll
                 for (int i = 0; i < LANE; i++) {
-                    const uint32_t Lt = tk[i] ? key[i] >> 16 : 0u;
+                    const uint32_t Lt = (tmask >> i) & 1u ? (nabs && i == 0 ? Lrun : key[i] >> 16) : 0u;
                     m_cnt += Lt ? 1u : 0u; m_bytes += Lt; m_dist += Lt ? 0xFFFFu - (key[i] & 0xFFFFu) : 0u;
                     m_max = max(m_max, Lt);
                 }
@@ -2284,6 +2344,80 @@ __device__ uint32_t seg_bits32(const AssembleJob& A, int s, uint32_t p, int& k)
     return low_bits(v, tb - p);
 }
 
+// NQ * 16 bytes of segment s from its bit p (a multiple of 8), into v: the
+// token bits of slice k (the slice holding p, or the segment's last) and,
+// past its end, of slice k + 1; zero past the segment's token bits (EOB, the
+// stored block's header and the pad are zero bits), then 0xFF for the sync
+// flush's last two bytes.  False (v untouched) when a third slice would be
+// needed.  Every load is issued up front: slice k + 1 is read from ceil(n1
+// / 32) dwords before its start (slice k's buffer), so that dword i of the
+// result takes its high part from loaded word i at one lane-uniform shift.
+template <int NQ>
+__device__ __forceinline__ bool seg_bytes(const AssembleJob& A, int s, uint32_t p, int k, u32x4* v)
+{
+    constexpr int ND = 4 * NQ;
+    const int k1 = min(s * A.spb + A.spb, A.nslice);
+    const uint32_t bk = A.slice_boff[k], ek = bk + A.slice_bits[k];
+    const uint32_t n1 = p < ek ? ek - p : 32u * ND;          // bits from slice k
+    const bool has2 = k + 1 < k1;
+    const uint32_t n2 = has2 ? A.slice_bits[k + 1] : 0u;
+    if (n1 < 32u * ND && has2 && n1 + n2 < 32u * ND && k + 2 < k1) return false;
+    const uint32_t off = p - bk;
+    const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
+    uint32_t t[ND + 1];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const u32x4 x = *(R2P_GLOBAL const u32x4_ua*)(w + 4 * q);
+        t[4 * q] = x[0]; t[4 * q + 1] = x[1]; t[4 * q + 2] = x[2]; t[4 * q + 3] = x[3];
+    }
+    t[ND] = w[ND];
+    uint32_t d[ND];
+    const uint32_t sh = off & 31;
+#pragma unroll
+    for (int i = 0; i < ND; i++) d[i] = __builtin_amdgcn_alignbit(t[i + 1], t[i], sh);
+    if (n1 < 32u * ND) {
+        uint32_t u[ND + 1];
+#pragma unroll
+        for (int i = 0; i <= ND; i++) u[i] = 0;
+        if (has2) {
+            const uint32_t c = (n1 + 31) >> 5;
+            const uint32_t* w2 = A.slice_out + (int64_t)(k + 1) * A.slice_cap_dw - c;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const u32x4 x = *(R2P_GLOBAL const u32x4_ua*)(w2 + 4 * q);
+                u[4 * q] = x[0]; u[4 * q + 1] = x[1]; u[4 * q + 2] = x[2]; u[4 * q + 3] = x[3];
+            }
+            u[ND] = w2[ND];
+        }
+        const uint32_t sh2 = (32u - (n1 & 31u)) & 31u;
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const uint32_t vu = __builtin_amdgcn_alignbit(u[i + 1], u[i], sh2);
+            const int r = (int)n1 - 32 * i;                    // bits of dword i from slice k
+            const uint32_t m = r >= 32 ? ~0u : r <= 0 ? 0u : (1u << r) - 1u;
+            d[i] = (d[i] & m) | (vu & ~m);
+        }
+    }
+    const uint32_t tb = A.seg_tbits[s];
+    if (p + 32u * ND > tb) {
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const int r = (int)tb - (int)p - 32 * i;          // token bits left in dword i
+            d[i] &= r >= 32 ? ~0u : r <= 0 ? 0u : (1u << r) - 1u;
+        }
+    }
+    const uint32_t seglen = (uint32_t)(A.seg_off[s + 1] - A.seg_off[s]);
+    const uint32_t bo = p >> 3;
+    if (!((s == A.nseg - 1) && A.final_block) && bo + 4u * ND > seglen - 2) {
+#pragma unroll
+        for (int i = 0; i < 4 * ND; i++)
+            if (bo + i >= seglen - 2) d[i >> 2] |= 0xFFu << (8 * (i & 3));
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = (u32x4){d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]};
+    return true;
+}
+
 // Payload bytes [q, q + n) (n <= 16) of mode 1: [78 FLG][segments][adler BE]
 // (mode 0: the segments), assembled 4 bytes at a time where they come from
 // one segment; s and k are segment and slice cursors at or before q.
@@ -2472,31 +2606,16 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             // with one cursor lookup and four loads in flight
             whole = e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
                          (int64_t)A.seg_off[s + 1] >= a0 + kAsmBytes;
-            uint32_t pw = 0;
+            u32x4 v[4];
             if (whole) {
-                pw = 8 * (uint32_t)(a0 - (int64_t)A.seg_off[s]);
+                const uint32_t pw = 8 * (uint32_t)(a0 - (int64_t)A.seg_off[s]);
                 const int k1 = min(s * A.spb + A.spb, A.nslice);
                 while (k + 1 < k1 && A.slice_boff[k + 1] <= pw) k++;
-                whole = pw + 8 * kAsmBytes <= A.slice_boff[k] + A.slice_bits[k];
+                whole = seg_bytes<4>(A, s, pw, k, v);
             }
             if (whole) {
-                const uint32_t off = pw - A.slice_boff[k];
-                const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
-                u32x4 t[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) t[q] = *(R2P_GLOBAL const u32x4_ua*)(w + 4 * q);
-                const uint32_t t16 = w[16];
-                const uint32_t sh = off & 31;
-                u32x4 v[4];
-#pragma unroll
-                for (int jg = 0; jg < 4; jg++) {
-                    const uint32_t nx = jg < 3 ? t[jg + 1][0] : t16;
-                    v[jg][0] = __builtin_amdgcn_alignbit(t[jg][1], t[jg][0], sh);
-                    v[jg][1] = __builtin_amdgcn_alignbit(t[jg][2], t[jg][1], sh);
-                    v[jg][2] = __builtin_amdgcn_alignbit(t[jg][3], t[jg][2], sh);
-                    v[jg][3] = __builtin_amdgcn_alignbit(nx, t[jg][3], sh);
-                    stage[wave][stage_slot(lane, jg)] = v[jg];
-                }
+                for (int jg = 0; jg < 4; jg++) stage[wave][stage_slot(lane, jg)] = v[jg];
                 // two independent 32-byte chains, joined by the 32-byte shift
                 uint32_t r0 = 0, r1 = 0;
 #pragma unroll
@@ -2516,29 +2635,20 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 const int64_t q = b + 16 * jg;
                 if (q >= e) break;
                 const int64_t a = q - hdr;
-                bool fast = q + 16 <= e && a >= 0 && a + 16 <= segbytes;
+                // inside one segment's bytes (no zlib framing): from its token bits
+                bool inseg = q + 16 <= e && a >= 0 && a + 16 <= segbytes;
                 uint32_t p = 0;
-                if (fast) {
+                if (inseg) {
                     while ((int64_t)A.seg_off[s + 1] <= a) s++;
+                    inseg = a + 16 <= (int64_t)A.seg_off[s + 1];
                     p = 8 * (uint32_t)(a - (int64_t)A.seg_off[s]);
                     const int k1 = min(s * A.spb + A.spb, A.nslice);
                     k = max(k, s * A.spb);
                     while (k + 1 < k1 && A.slice_boff[k + 1] <= p) k++;
-                    // all 128 bits inside slice k's tokens
-                    fast = p + 128 <= A.slice_boff[k] + A.slice_bits[k];
                 }
-                if (fast) {
-                    const uint32_t off = p - A.slice_boff[k];
-                    const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
-                    const u32x4 lo4 = *(R2P_GLOBAL const u32x4_ua*)w;
-                    const uint32_t w4 = w[4];
-                    const uint32_t sh = off & 31;
-                    u32x4 v;
-                    v[0] = __builtin_amdgcn_alignbit(lo4[1], lo4[0], sh);
-                    v[1] = __builtin_amdgcn_alignbit(lo4[2], lo4[1], sh);
-                    v[2] = __builtin_amdgcn_alignbit(lo4[3], lo4[2], sh);
-                    v[3] = __builtin_amdgcn_alignbit(w4, lo4[3], sh);
-                    stage[wave][stage_slot(lane, jg)] = v;
+                u32x4 vg;
+                if (inseg && seg_bytes<1>(A, s, p, k, &vg)) {
+                    stage[wave][stage_slot(lane, jg)] = vg;
                 } else {
                     if (!slow) { sc = s; kc = k; }
                     slow |= 1u << jg;

@@ -258,6 +258,10 @@ def test_lz77_finds_repeats(torch_dev, r2p, oracle):
     png1, filt1, _, _, _ = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=1))
     z1 = _check_png(oracle, r2p, png1, px, img, filt1)
     assert len(png1) < 0.35 * len(png0), (len(png1), len(png0))
+    # the same through the dynamic-Huffman blocks (long matches in the tokens)
+    png2, filt2, _, _, _ = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=1, huffman=1))
+    _check_png(oracle, r2p, png2, px, img, filt2)
+    assert len(png2) < len(png1)
 
 
 @pytest.mark.parametrize("seg,chunk", [(4096, 1000), (65536, 65536), (1 << 20, 8192)])
@@ -351,6 +355,8 @@ def test_strip_deflate_concatenation(torch_dev, r2p):
     assert zlib.decompress(raw, -15) == data.tobytes()
     assert sd.adler() == zlib.adler32(data.tobytes())
     assert st.matches > 0 and st.matched_bytes > 100000
+    assert st.max_len > 16                        # runs merged across parse lanes (<= 258)
+    assert st.max_len <= 258
     # a stream whose strips are all non-final is a valid sync-flushed prefix
     sd.reset()
     sd.strip(d.data_ptr(), 1000, is_last=False)
