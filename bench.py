@@ -362,17 +362,24 @@ def run_batch(args, r2p, synth, torch):
         b.run(False); b.sync()
         t = b.timings()
         ktimes.append((t.filter_ms, t.deflate_ms, t.scan_ms, t.assemble_ms, t.total_ms))
-    # plain form, one image at a time, same options
+    # plain form, one image at a time, with the options each image got in the
+    # batch (the tuned slice size of its size class): one encoder per slice size
     maxf = max(r2p.filtered_bytes(im) for im in imgs)
-    enc = r2p.Encoder(maxf, tuned)
+    job_opt = [b.job_options(i) for i in range(len(imgs))]
+    encs = {}
+    for jo in job_opt:
+        if jo.slice_bytes not in encs:
+            encs[jo.slice_bytes] = r2p.Encoder(maxf, jo)
     cap = max(r2p.png_bound(im, tuned) for im in imgs)
     d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
-    for im, sp in list(zip(imgs, srcs))[:8]:
+    for i, (im, sp) in list(enumerate(zip(imgs, srcs)))[:8]:
+        enc = encs[job_opt[i].slice_bytes]
         enc.encode_device(im, sp, d_png.data_ptr(), cap); enc.sync()
     lat = []
     differ = 0
     torch.cuda.synchronize()
     for i, (im, sp) in enumerate(zip(imgs, srcs)):
+        enc = encs[job_opt[i].slice_bytes]
         ts = time.perf_counter()
         enc.encode_device(im, sp, d_png.data_ptr(), cap)
         n, _, _ = enc.sync()
@@ -380,7 +387,9 @@ def run_batch(args, r2p, synth, torch):
         # every image's PNG against the batch's (the copies are not timed)
         differ += d_png[:n].cpu().numpy().tobytes() != b.copy_result(i)
     pwall = sum(lat)
-    enc.close()
+    class_slices = b.class_slices()
+    for enc in encs.values():
+        enc.close()
     png_total = sum(b.result(i)[1] for i in range(len(imgs)))
     b.close()
     kt = np.array(ktimes).mean(axis=0)
@@ -397,7 +406,8 @@ def run_batch(args, r2p, synth, torch):
                        "assemble": round(kt[3], 3)},
         "filter_gbs": round((isum + fsum) / (kt[0] * 1e-3) / 1e9, 1),
         "deflate_gbs": round((fsum + png_total) / (kt[1] * 1e-3) / 1e9, 1),
-        "autotuned": {"segment_bytes": tuned.segment_bytes, "slice_bytes": tuned.slice_bytes,
+        "autotuned": {"segment_bytes": tuned.segment_bytes,
+                      "slice_bytes_by_class": dict(zip(("under_2MiB", "under_8MiB", "larger"), class_slices)),
                       "wg_threads": tuned.wg_threads,
                       "band_rows": tuned.band_rows, "best_ms": tuned_ms},
         "png_over_filtered": round(png_total / fsum, 4),
@@ -482,7 +492,8 @@ def main():
     ap.add_argument("--lz77", type=int, default=1)
     ap.add_argument("--wg", type=int, default=256, choices=[256, 512])
     ap.add_argument("--seg", type=int, default=65536, choices=[16384, 32768, 65536])
-    ap.add_argument("--slice", type=int, default=16384, choices=[4096, 8192, 16384, 32768, 65536])
+    ap.add_argument("--slice", type=int, default=0, choices=[0, 4096, 8192, 16384, 32768, 65536],
+                    help="DEFLATE slice bytes (0: by image size; 16 KiB for the 3072x3072 radiograph)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic-Huffman encodes of the plain workload")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 800.0, 2000.0])
@@ -520,7 +531,8 @@ def main():
         "ms_per_step": round(r["wall"] / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"{args.workload} {r['s'].img.width}x{r['s'].img.height}",
-                   "form": "plain per-image", "segment_bytes": args.seg, "slice_bytes": args.slice, "wg_threads": args.wg, "lz77": args.lz77,
+                   "form": "plain per-image", "segment_bytes": args.seg, "slice_bytes": args.slice or "auto (16384 here)",
+                   "wg_threads": args.wg, "lz77": args.lz77,
                    "bit_exact": None},
         "latency_ms": {"p50": round(pct(lat_ms, 50), 4), "p99": round(pct(lat_ms, 99), 4)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},

@@ -101,7 +101,8 @@ typedef struct r2p_options {
     int lane_bytes;        /* LZ77 parse span per lane; 16 is the one supported value */
     int wg_threads;        /* DEFLATE workgroup size (256 or 512); 0 = auto (256) */
     int slice_bytes;       /* DEFLATE work unit: one workgroup per slice of a segment (a power
-                              of two >= 4096 dividing segment_bytes); 0 = auto (16384).
+                              of two >= 4096 dividing segment_bytes); 0 = auto by the image's
+                              filtered size (16 KiB from 8 MiB, 8 KiB from 4 MiB, else 4 KiB).
                               Changes speed only, never the decoded output */
     int huffman;           /* 0: fixed-Huffman blocks (the reference's, gpu_deflate_backend.h:3);
                               1: each segment a dynamic-Huffman block where that is smaller
@@ -246,11 +247,19 @@ int r2p_batch_sync(r2p_batch* b);
 int r2p_batch_result(r2p_batch* b, int i, const void** d_png, size_t* len);
 int r2p_batch_copy_result(r2p_batch* b, int i, void* h_dst, size_t cap, size_t* len);
 int r2p_batch_timings(r2p_batch* b, r2p_timings* t);
-/* Autotuner: times candidate (band_rows, wg_threads, segment_bytes)
- * settings on the declared batch and keeps the fastest.  Writes the choice
- * to *chosen. */
+/* Autotuner: times candidate settings on the declared batch and keeps the
+ * fastest -- DEFLATE workgroup size and filter band height for the launch,
+ * then the slice size per size class of the images (filtered streams under
+ * 2 MiB, under 8 MiB, larger).  Writes the launch settings to *chosen (its
+ * slice_bytes 0: per image, see r2p_batch_job_options). */
 int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms);
+/* Sets the options (and clears the autotuner's per-class slice sizes). */
 int r2p_batch_options(r2p_batch* b, const r2p_options* opt);
+/* The options image i of the declared batch is encoded with, its slice size
+ * resolved: the plain encoder with these options produces the same PNG. */
+int r2p_batch_job_options(r2p_batch* b, int i, r2p_options* opt);
+/* The autotuner's slice size per size class (0: by image size). */
+int r2p_batch_class_slices(r2p_batch* b, int* slices3);
 
 /* ======================================================================== */
 /* Served form: continuous batching under an HBM budget.                      */

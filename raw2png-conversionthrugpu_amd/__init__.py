@@ -139,6 +139,8 @@ def lib():
             "r2p_batch_timings": ([vp, C.POINTER(Timings)], ip),
             "r2p_batch_autotune": ([vp, optp, C.POINTER(C.c_double)], ip),
             "r2p_batch_options": ([vp, optp], ip),
+            "r2p_batch_job_options": ([vp, ip, optp], ip),
+            "r2p_batch_class_slices": ([vp, C.POINTER(C.c_int)], ip),
             "r2p_server_create": ([C.POINTER(ServerConfig), C.POINTER(vp)], ip),
             "r2p_server_destroy": ([vp], ip),
             "r2p_server_submit": ([vp, imgp, vp, C.POINTER(i64)], ip),
@@ -378,6 +380,18 @@ class Batch:
         check(lib().r2p_batch_autotune(self.h, C.byref(o), C.byref(ms)))
         self.opt = o
         return o, ms.value
+
+    def job_options(self, i: int) -> Options:
+        """The options image i is encoded with (its slice size resolved): the
+        plain encoder with them produces the same PNG."""
+        o = Options()
+        check(lib().r2p_batch_job_options(self.h, i, C.byref(o)))
+        return o
+
+    def class_slices(self):
+        v = (C.c_int * 3)()
+        check(lib().r2p_batch_class_slices(self.h, v))
+        return list(v)
 
 
 def server_loadtest(server, arrival_s, kind, imgs, h_srcs, expect_pngs=None):

@@ -88,7 +88,14 @@ struct r2p_batch {
     bool timed_kernels = false;
     std::vector<r2p_image> shadow_img;
     std::vector<const void*> shadow_src;
+    // slice size per size class of the filtered stream (the autotuner's
+    // choice; 0: by resolve_slice_bytes) and the one each image got
+    int class_slice[3] = {0, 0, 0};
+    std::vector<int> job_slice;
 };
+
+// Size classes of the autotuner: filtered streams under 2 MiB, under 8 MiB, larger.
+static int size_class(size_t filtered) { return filtered < ((size_t)2 << 20) ? 0 : filtered < ((size_t)8 << 20) ? 1 : 2; }
 
 // Enqueues one batch run.  Per-kernel events are recorded only for eager
 // runs (timed); a captured graph is timed as a whole around its launch.
@@ -214,6 +221,15 @@ int r2p_batch_options(r2p_batch* b, const r2p_options* opt)
     fill_defaults(&o);
     if (o.huffman) { set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
     b->opt = o;
+    b->class_slice[0] = b->class_slice[1] = b->class_slice[2] = 0;
+    return R2P_OK;
+}
+
+int r2p_batch_job_options(r2p_batch* b, int i, r2p_options* opt)
+{
+    if (!b || !opt || i < 0 || i >= b->n || (size_t)i >= b->job_slice.size()) { set_error("bad batch job index"); return R2P_ERR_ARG; }
+    *opt = b->opt;
+    opt->slice_bytes = b->job_slice[(size_t)i];
     return R2P_OK;
 }
 
@@ -231,7 +247,6 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     uint32_t* ss = (uint32_t*)(ht + b->off_ss);
     uint32_t* cs = (uint32_t*)(ht + b->off_cs);
     uint32_t* cnt = (uint32_t*)(ht + b->off_cnt);
-    const int cap_dw = slice_cap_dwords(o.slice_bytes);
 
     // band height: auto from the batch's total rows unless fixed by the options
     int64_t total_rows = 0;
@@ -277,9 +292,10 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     // device tables, deduplicated by parameter set
     std::map<LutKey, int> lut_slot;
     b->luts.clear();
-    size_t foff = 0, segs = 0, slices = 0, ooff = 0;
+    size_t foff = 0, segs = 0, slices = 0, ooff = 0, slice_dw = 0;
     uint32_t chunks = 0;
     b->out_off.assign((size_t)n, 0);
+    b->job_slice.assign((size_t)n, 0);
     for (int i = 0; i < n; i++) {
         const Geometry& g = geo[i];
         const WindowPlan& wp = plan[i];
@@ -306,18 +322,23 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         F.q15_b = wp.q15_b; F.pixel_rep = wp.pixel_rep; F.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
         F.band_rows = row_class(i) == 0 ? b->band_rows : row_class(i) == 1 ? b->band_rows_w : 1;
 
-        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
+        // the slice size of this image: its size class's tuned one, else by its size
+        int sl = b->class_slice[size_class(g.filtered)];
+        if (sl <= 0 || o.segment_bytes % sl) sl = resolve_slice_bytes(o, g.filtered);
+        b->job_slice[(size_t)i] = sl;
+        const int cap_dw = slice_cap_dwords(sl);
+        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, sl);
         const int nseg = sg.nseg;
-        if ((slices + sg.nslice) * (size_t)cap_dw > b->slice_arena_dw || slices + sg.nslice > b->max_slices ||
+        if (slice_dw + (size_t)sg.nslice * (size_t)cap_dw > b->slice_arena_dw || slices + sg.nslice > b->max_slices ||
             segs + nseg > b->max_segs) {
             set_error("slice arena exhausted"); return R2P_ERR_SIZE;
         }
         DeflateJob& D = dj[i];
         D = DeflateJob{};
         D.data = F.dst; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
-        D.slice_out = b->d_slice_out + slices * (size_t)cap_dw; D.slice_bits = b->d_slice_bits + slices;
+        D.slice_out = b->d_slice_out + slice_dw; D.slice_bits = b->d_slice_bits + slices;
         D.slice_adler = b->d_slice_adler + slices; D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes;
-        D.slice_bytes = o.slice_bytes; D.slice_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp;
+        D.slice_bytes = sl; D.slice_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp;
         D.row_stride = (int32_t)(g.row_bytes + 1); D.lz77 = o.lz77; D.nslice = sg.nslice;
         ss[i] = (uint32_t)slices;
 
@@ -330,7 +351,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         A.seg_off = b->d_seg_off + segs + (size_t)i; A.out = b->d_out + ooff;
         A.out_len = b->d_res + 2 * (size_t)i; A.adler_out = (uint32_t*)(b->d_res + 2 * (size_t)i + 1);
         A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice;
-        A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = o.slice_bytes;
+        A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = sl;
         A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
         A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
         A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
@@ -346,6 +367,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         foff += (g.filtered + 255) & ~(size_t)255;
         segs += (size_t)nseg;
         slices += (size_t)sg.nslice;
+        slice_dw += (size_t)sg.nslice * (size_t)cap_dw;
     }
     // band prefix in job-table order: the narrow class, the wide one, the tiled one
     uint32_t bands = 0, bands_n = 0, bands_w = 0;
@@ -458,47 +480,76 @@ int r2p_batch_timings(r2p_batch* b, r2p_timings* t)
     return R2P_OK;
 }
 
-// Autotuner: times each candidate (slice_bytes x wg_threads x band_rows)
-// on the declared batch -- one warm-up graph replay, then the best of three
-// timed replays -- and keeps the fastest, re-declaring the batch with it.
+// Autotuner: times candidates on the declared batch -- per candidate one
+// warm-up graph replay, then the best of three timed replays -- first the
+// launch-wide knobs (DEFLATE workgroup size x filter band height, slices by
+// image size), then, one size class at a time (filtered streams under 2 MiB,
+// under 8 MiB, larger), the slice size of that class's images, keeping the
+// fastest of each step and re-declaring the batch with it.
+static int time_batch(r2p_batch* b, int n, const std::vector<r2p_image>& imgs, const std::vector<const void*>& srcs,
+                      double* t_out)
+{
+    int rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
+    double t_min = 1e30;
+    for (int rep = 0; rep < 4 && !rc; rep++) {
+        rc = r2p_batch_run(b, 1);
+        if (!rc) rc = r2p_batch_sync(b);
+        r2p_timings t{};
+        if (!rc && rep > 0 && r2p_batch_timings(b, &t) == R2P_OK) t_min = std::min(t_min, (double)t.total_ms);
+    }
+    *t_out = t_min;
+    return rc;
+}
+
 int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
 {
     if (!b || b->n <= 0) { set_error("autotune needs a declared batch"); return R2P_ERR_STATE; }
     const std::vector<r2p_image> imgs = b->shadow_img;
     const std::vector<const void*> srcs = b->shadow_src;
     const int n = b->n;
-    const r2p_options base = b->opt;
-    r2p_options best = base;
+    r2p_options best = b->opt;
+    best.slice_bytes = 0;
     double best_t = 1e30;
-    static const int slices_c[] = {16384, 32768, 65536};
     static const int shapes[] = {256, 512};   // DEFLATE wg_threads
     static const int bands[] = {4, 8, 16};
-    for (int sl : slices_c) {
-        if (sl > base.segment_bytes || base.segment_bytes % sl) continue;
-        for (int sh : shapes)
-            for (int bd : bands) {
-                r2p_options o = base;
-                o.slice_bytes = sl; o.wg_threads = sh; o.band_rows = bd;
-                int rc = r2p_batch_options(b, &o);
-                if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
-                if (rc) continue;
-                double t_min = 1e30;
-                for (int rep = 0; rep < 4 && !rc; rep++) {
-                    rc = r2p_batch_run(b, 1);
-                    if (!rc) rc = r2p_batch_sync(b);
-                    r2p_timings t{};
-                    if (!rc && rep > 0 && r2p_batch_timings(b, &t) == R2P_OK)
-                        t_min = std::min(t_min, (double)t.total_ms);
-                }
-                if (rc) return rc;
-                if (t_min < best_t) { best_t = t_min; best = o; }
-            }
-    }
+    for (int sh : shapes)
+        for (int bd : bands) {
+            r2p_options o = best;
+            o.wg_threads = sh; o.band_rows = bd;
+            int rc = r2p_batch_options(b, &o);
+            double t = 1e30;
+            if (!rc) rc = time_batch(b, n, imgs, srcs, &t);
+            if (rc) return rc;
+            if (t < best_t) { best_t = t; best.wg_threads = sh; best.band_rows = bd; }
+        }
     int rc = r2p_batch_options(b, &best);
-    if (!rc) rc = r2p_batch_set(b, n, imgs.data(), srcs.data());
     if (rc) return rc;
+    // which size classes the batch has
+    bool has[3] = {false, false, false};
+    for (int i = 0; i < n; i++) has[size_class(r2p_image_filtered_bytes(&imgs[(size_t)i]))] = true;
+    int cls[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++) {
+        if (!has[c]) continue;
+        for (int sl = 4096; sl <= 65536; sl <<= 1) {
+            if (best.segment_bytes % sl) continue;
+            for (int q = 0; q < 3; q++) b->class_slice[q] = cls[q];
+            b->class_slice[c] = sl;
+            double t = 1e30;
+            if ((rc = time_batch(b, n, imgs, srcs, &t))) return rc;
+            if (t < best_t) { best_t = t; cls[c] = sl; }
+        }
+    }
+    for (int q = 0; q < 3; q++) b->class_slice[q] = cls[q];
+    if ((rc = r2p_batch_set(b, n, imgs.data(), srcs.data()))) return rc;
     if (chosen) *chosen = best;
     if (best_ms) *best_ms = best_t;
+    return R2P_OK;
+}
+
+int r2p_batch_class_slices(r2p_batch* b, int* slices3)
+{
+    if (!b || !slices3) { set_error("bad arguments"); return R2P_ERR_ARG; }
+    for (int q = 0; q < 3; q++) slices3[q] = b->class_slice[q];
     return R2P_OK;
 }
 

@@ -335,13 +335,44 @@ int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap
     return h;
 }
 
+int resolve_slice_bytes(const r2p_options& o, size_t filtered)
+{
+    if (o.slice_bytes > 0) return o.slice_bytes;
+    int sl = 16384;
+    while (sl > 4096 && ((size_t)sl * 512 > filtered || o.segment_bytes % sl)) sl >>= 1;
+    return sl;
+}
+
+int slice_cap_for(int slice_bytes, int huffman)
+{
+    // a dynamic-Huffman slice may take up to 16 bits per byte (15-bit
+    // literal codes; a match of 3 or more bytes is at most 48 bits) plus the
+    // segment's header
+    return huffman ? (16 * slice_bytes + 32 * kHuffHdrWords + 31) / 32 + 8 : slice_cap_dwords(slice_bytes);
+}
+
+void slice_arena_bound(const r2p_options& o, size_t filtered, size_t* max_slices, size_t* max_dwords)
+{
+    size_t ns = 0, nd = 0;
+    for (int sl = 4096; sl <= 65536; sl <<= 1) {
+        if (o.slice_bytes > 0 && sl != o.slice_bytes) continue;
+        if (o.slice_bytes == 0 && sl > 16384) continue;
+        const size_t n = (filtered + sl - 1) / sl + 1;
+        ns = std::max(ns, n);
+        nd = std::max(nd, n * (size_t)slice_cap_for(sl, o.huffman));
+    }
+    *max_slices = ns;
+    *max_dwords = nd;
+}
+
 void fill_defaults(r2p_options* o)
 {
     if (o->segment_bytes <= 0) o->segment_bytes = 65536;
     o->segment_bytes = std::max(4096, (o->segment_bytes + 4095) & ~4095);
-    // a slice is a power of two from 4 KiB up that divides the segment
-    int sl = o->slice_bytes > 0 ? o->slice_bytes : 16384;
-    if (sl > o->segment_bytes || (sl & (sl - 1)) || sl < 4096 || o->segment_bytes % sl) {
+    // a slice is a power of two from 4 KiB up that divides the segment (0: auto, per stream)
+    if (o->slice_bytes < 0) o->slice_bytes = 0;
+    int sl = o->slice_bytes;
+    if (sl && (sl > o->segment_bytes || (sl & (sl - 1)) || sl < 4096 || o->segment_bytes % sl)) {
         sl = 4096;
         while (sl * 2 <= std::min(o->segment_bytes, 16384) && o->segment_bytes % (sl * 2) == 0) sl *= 2;
     }
@@ -442,7 +473,7 @@ void r2p_options_default(r2p_options* o)
     o->band_rows = 0;
     o->lane_bytes = 16;
     o->wg_threads = 256;
-    o->slice_bytes = 16384;
+    o->slice_bytes = 0;      // auto: by the stream's size (resolve_slice_bytes)
     o->huffman = 0;
 }
 
@@ -569,8 +600,8 @@ struct r2p_encoder {
     hipEvent_t ev[5] = {};
     size_t max_filtered = 0;
     int max_seg = 0;
-    int slice_cap_dw = 0;
-    int max_slice = 0;
+    size_t max_slice = 0;            // slice buffers for any slice size the options allow
+    size_t slice_arena_dw = 0;
     uint8_t* d_filt = nullptr;
     uint32_t* d_slice_out = nullptr;
     uint32_t* d_slice_bits = nullptr;
@@ -721,25 +752,20 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
     fill_defaults(&e->opt);
     e->max_filtered = max_filtered_bytes;
     e->max_seg = (int)((max_filtered_bytes + e->opt.segment_bytes - 1) / e->opt.segment_bytes);
-    e->max_slice = (int)((max_filtered_bytes + e->opt.slice_bytes - 1) / e->opt.slice_bytes) + 1;
-    // a dynamic-Huffman slice may take up to 16 bits per byte (15-bit
-    // literal codes; a match of 3 or more bytes is at most 48 bits) plus the
-    // segment's header
-    e->slice_cap_dw = e->opt.huffman ? (16 * e->opt.slice_bytes + 32 * kHuffHdrWords + 31) / 32 + 8
-                                     : slice_cap_dwords(e->opt.slice_bytes);
+    slice_arena_bound(e->opt, max_filtered_bytes, &e->max_slice, &e->slice_arena_dw);
     hipError_t err;
 #define TRY(x) do { if ((err = (x)) != hipSuccess) { r2p_encoder_destroy(e); return hip_fail(err, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& v : e->ev) TRY(hipEventCreate(&v));
     TRY(hipMalloc((void**)&e->d_filt, max_filtered_bytes + kTailSlack));
-    TRY(hipMalloc((void**)&e->d_slice_out, (size_t)e->max_slice * e->slice_cap_dw * 4));
+    TRY(hipMalloc((void**)&e->d_slice_out, e->slice_arena_dw * 4));
     TRY(hipMalloc((void**)&e->d_slice_bits, (size_t)e->max_slice * 4));
     TRY(hipMalloc((void**)&e->d_slice_adler, (size_t)e->max_slice * 4));
     TRY(hipMalloc((void**)&e->d_slice_boff, (size_t)e->max_slice * 4));
     TRY(hipMalloc((void**)&e->d_seg_tbits, ((size_t)e->max_seg + 1) * 4));
     TRY(hipMalloc((void**)&e->d_seg_off, ((size_t)e->max_seg + 2) * 8));
     if (e->opt.huffman) {
-        TRY(hipMalloc((void**)&e->d_tokens, (size_t)e->max_slice * e->opt.slice_bytes * 4));
+        TRY(hipMalloc((void**)&e->d_tokens, (max_filtered_bytes + 65536) * 4));
         TRY(hipMalloc((void**)&e->d_tok_count, (size_t)e->max_slice * 4));
         TRY(hipMalloc((void**)&e->d_hist, (size_t)e->max_slice * kHuffSyms * 4));
         TRY(hipMalloc((void**)&e->d_codes, ((size_t)e->max_seg + 1) * kHuffSyms * 4));
@@ -784,7 +810,8 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
         set_error("image needs %zu filtered bytes, encoder sized for %zu", g.filtered, e->max_filtered);
         return R2P_ERR_SIZE;
     }
-    const r2p_options& o = e->opt;
+    r2p_options o = e->opt;
+    o.slice_bytes = resolve_slice_bytes(e->opt, g.filtered);
     const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
     if (png_cap < bound) { set_error("png capacity %zu < bound %zu", png_cap, bound); return R2P_ERR_SIZE; }
     WindowPlan wp;
@@ -795,7 +822,8 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     w.filt = e->d_filt; w.slice_out = e->d_slice_out; w.slice_bits = e->d_slice_bits;
     w.slice_adler = e->d_slice_adler; w.slice_boff = e->d_slice_boff; w.seg_tbits = e->d_seg_tbits;
     w.seg_off = e->d_seg_off; w.tokens = e->d_tokens; w.tok_count = e->d_tok_count; w.hist = e->d_hist;
-    w.codes = e->d_codes; w.hdr = e->d_hdr; w.hdr_bits = e->d_hdr_bits; w.slice_cap_dw = e->slice_cap_dw;
+    w.codes = e->d_codes; w.hdr = e->d_hdr; w.hdr_bits = e->d_hdr_bits;
+    w.slice_cap_dw = slice_cap_for(o.slice_bytes, o.huffman);
     w.lut = e->lut.d_lut;
     w.out_len = e->d_misc; w.adler_out = (uint32_t*)(e->d_misc + 1);
     w.host_res = e->dh_misc;               // the results reach h_misc without a copy
@@ -814,7 +842,7 @@ static size_t noalloc_layout(uint8_t* base, size_t F, const r2p_options& o, cons
     const SliceGeom sg = slice_geom((int64_t)F, o.segment_bytes, o.slice_bytes);
     size_t off = 0;
     auto take = [&](size_t n) { uint8_t* p = base + off; off += (n + 255) & ~(size_t)255; return p; };
-    w->slice_cap_dw = slice_cap_dwords(o.slice_bytes);
+    w->slice_cap_dw = slice_cap_for(o.slice_bytes, 0);
     w->filt = take(F + kTailSlack);
     w->slice_out = (uint32_t*)take((size_t)sg.nslice * (size_t)w->slice_cap_dw * 4);
     w->slice_bits = (uint32_t*)take((size_t)sg.nslice * 4);
@@ -846,6 +874,7 @@ int r2p_encode_device_noalloc(const r2p_image* img, const void* d_src, void* d_p
     if ((rc = plan_window(img, &wp))) return rc;
     const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
     if (png_cap < bound) { set_error("png capacity %zu < bound %zu", png_cap, bound); return R2P_ERR_SIZE; }
+    o.slice_bytes = resolve_slice_bytes(o, g.filtered);
     EncodeWork w;
     const size_t need = noalloc_layout(base, g.filtered, o, wp, &w);
     if (need > (size_t)R2P_NOALLOC_SCRATCH_BYTES) {

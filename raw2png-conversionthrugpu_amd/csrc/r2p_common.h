@@ -86,6 +86,16 @@ size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap);
 size_t filter_tiled_lds_bytes(int lut_cap);
 int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap);
 void fill_defaults(r2p_options* o);
+// DEFLATE slice size for a stream of `filtered` bytes: the options' value,
+// or (0, auto) 16, 8 or 4 KiB -- the largest that still gives about 512
+// slices, two per CU: small images get more, shorter workgroups.
+int resolve_slice_bytes(const r2p_options& o, size_t filtered);
+// Slice dwords of one slice of `slice_bytes` (the dynamic-Huffman bound with huffman).
+int slice_cap_for(int slice_bytes, int huffman);
+// Slice buffers any stream of up to `filtered` bytes needs with these
+// options: the most slices, and the most dwords of slice output, over the
+// sizes resolve_slice_bytes can pick.
+void slice_arena_bound(const r2p_options& o, size_t filtered, size_t* max_slices, size_t* max_dwords);
 
 // Cached device tables keyed by DICOM parameter set.
 struct LutCache {

@@ -60,7 +60,7 @@ int r2p_deflate_create(size_t max_strip_bytes, size_t max_total_bytes, int lz77,
     o.segment_bytes = segment_bytes;
     fill_defaults(&o);
     d->seg_bytes = o.segment_bytes;
-    d->slice_bytes = o.slice_bytes;
+    d->slice_bytes = resolve_slice_bytes(o, max_strip_bytes);   // one size for every strip
     d->slice_cap_dw = slice_cap_dwords(d->slice_bytes);
     d->max_seg = (int)((max_strip_bytes + d->seg_bytes - 1) / d->seg_bytes) + 1;
     d->max_slice = (int)((max_strip_bytes + d->slice_bytes - 1) / d->slice_bytes) + 1;

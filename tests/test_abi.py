@@ -55,7 +55,7 @@ def test_geometry_helpers_without_device(r2p):
 def test_options_default(r2p):
     o = r2p.options()
     assert (o.segment_bytes, o.lz77, o.idat_chunk_bytes, o.zlib_level) == (65536, 1, 65536, 3)
-    assert (o.lane_bytes, o.wg_threads, o.slice_bytes) == (16, 256, 16384)
+    assert (o.lane_bytes, o.wg_threads, o.slice_bytes) == (16, 256, 0)    # slices: by image size
 
 
 def test_host_combines_match_zlib(r2p):

@@ -92,10 +92,13 @@ def test_batch_redeclare_and_autotune(torch_dev, r2p, synth):
     b.run(True); b.sync()
     assert [b.copy_result(i) for i in range(5)] == first
     o, ms = b.autotune()
-    assert o.segment_bytes == 65536 and o.slice_bytes in (16384, 32768, 65536) and o.wg_threads in (256, 512) and ms > 0
+    assert o.segment_bytes == 65536 and o.wg_threads in (256, 512) and ms > 0
+    assert all(c in (0, 4096, 8192, 16384, 32768, 65536) for c in b.class_slices())
     b.run(True); b.sync()
     for i, s in enumerate(ims):
-        assert b.copy_result(i) == plain_png(r2p, torch, s.img, srcs[i], o)
+        jo = b.job_options(i)
+        assert jo.slice_bytes in (4096, 8192, 16384, 32768, 65536)
+        assert b.copy_result(i) == plain_png(r2p, torch, s.img, srcs[i], jo)
     b.close()
 
 
