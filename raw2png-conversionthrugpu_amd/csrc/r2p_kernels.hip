@@ -88,15 +88,17 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
-// Inclusive prefix sum across the 64 lanes of a wave.
+// Inclusive prefix sum across the 64 lanes of a wave (DPP: row shifts by
+// 1, 2, 4 and 8 scan each row of 16 lanes, then the row broadcasts of lanes
+// 15 and 31 carry the rows' totals forward).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t u = __shfl_up(v, o);
-        if (lane >= o) v += u;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);   // row_bcast:31
     return v;
 }
 
@@ -1841,130 +1843,195 @@ __device__ uint32_t block_sum(uint32_t v, uint32_t* red)
     return t;
 }
 
-// Code lengths (at most LIMIT bits) of a Huffman code for freq[0..n) into
-// len[0..n), by the workgroup: at least two symbols get a code (a used one alone,
-// or none, gets partners of count 1); the used symbols are ranked by (count,
-// symbol) and merged by the two-queue method on lane 0; the depths come
-// from the parent links by pointer jumping; lengths over the limit are
-// repaired on the length counts (moving the Kraft sum back to 1, one unit
-// of 2^-LIMIT per step), longest codes going to the least frequent symbols.
-template <int LIMIT>
-__device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint16_t* sorted, uint32_t* w, uint32_t* link,
-                             uint32_t* cnt)
+// Number of keys in key[0..n16) greater than ks, with the keys read 16 at a
+// time (every lane reads the same words: broadcasts), so that the loads are
+// in flight together.  key is 16-byte aligned and zero-padded to n16.
+__device__ __forceinline__ uint32_t keys_above(const uint32_t* key, int n16, uint32_t ks)
 {
-    const int tid = threadIdx.x;
+    uint32_t r = 0;
+    for (int t0 = 0; t0 < n16; t0 += 16) {
+        u32x4 k4[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) k4[q] = *(const u32x4*)(key + t0 + 4 * q);
+#pragma unroll
+        for (int j = 0; j < 16; j++) r += k4[j >> 2][j & 3] > ks;
+    }
+    return r;
+}
+
+constexpr int kHuffChunks = 5;           // 64-symbol chunks of the largest alphabet (286)
+constexpr int kHuffPhases = 3;           // shortening passes of the length construction
+
+// Code lengths of at most LIMIT bits for freq[0..n) (n <= 320) into
+// len[0..n), by the workgroup, with no long serial chain.
+//  1. Shannon lengths ceil(log2(T / f)), clamped to [1, LIMIT], against
+//     T = ceil(total * U / (U - m)) (U = 2^LIMIT; m = symbols with
+//     f * U < 2 * total, which bounds the clamped ones), so the Kraft sum
+//     stays <= 1 with the clamp: clamped terms add at most m / U, the others
+//     at most total / T = (U - m) / U.
+//  2. The symbols are ordered once by f * 2^L descending (bits saved per unit
+//     of Kraft sum by a shortening), then f descending, then symbol.
+//  3. Wave 0 walks that order kHuffPhases times, holding the order in
+//     registers: a symbol is shortened by one bit when its term fits in the
+//     remaining slack, and stays in the next pass only if it was.  A pass is
+//     a few rounds of 64-lane prefix sums: the lanes before the first one that
+//     does not fit all move, the slack drops below that lane's term, and the
+//     round repeats from the next lane with the smaller terms only.
+//  4. Any slack left is a multiple of the longest length's term: codes of the
+//     longest length move up one bit (first in the order first) until the sum
+//     is exactly 1 -- inflate takes complete codes only.
+// At least two symbols get a code (a used one alone, or none, gets partners of
+// count 1).  On the images measured this comes within 0.4 % of the optimal
+// (two-queue Huffman) code.  Scratch: key (288 u32, 16-byte aligned),
+// slot (288 u32), red (>= 4 u32).
+template <int LIMIT>
+__device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint32_t* key, uint32_t* slot, uint32_t* red)
+{
+    constexpr uint32_t U = 1u << LIMIT;                       // the Kraft sum in units of 2^-LIMIT
+    const int tid = threadIdx.x, lane = tid & 63;
     __syncthreads();
-    uint32_t used = 0;
-    for (int s = tid; s < n; s += kHuffThreads) used += freq[s] != 0;
-    used = block_sum(used, cnt);
-    if (used < 2) {
+    uint32_t used, total;
+    for (;;) {
+        uint32_t v = 0;
+        for (int s = tid; s < n; s += kHuffThreads) v += freq[s] + (freq[s] != 0 ? 1u << 23 : 0u);
+        v = block_sum(v, red);                                // total < 2^23 for every alphabet here
+        used = v >> 23;
+        total = v & ((1u << 23) - 1);
+        if (used >= 2) break;
         if (tid == 0)
             for (int i = 0; used < 2 && i < n; i++)
                 if (freq[i] == 0) { freq[i] = 1; used++; }
-        used = 2;
-    }
-    __syncthreads();
-    const int m = (int)used;
-    for (int s = tid; s < n; s += kHuffThreads) {
-        len[s] = 0;
-        const uint32_t f = freq[s];
-        if (f == 0) continue;
-        int r = 0;
-        for (int t = 0; t < n; t++) r += freq[t] != 0 && (freq[t] < f || (freq[t] == f && t < s));
-        sorted[r] = (uint16_t)s;
-    }
-    if (tid < 16) cnt[tid] = 0;
-    __syncthreads();
-    for (int i = tid; i < m; i += kHuffThreads) w[i] = freq[sorted[i]];
-    __syncthreads();
-    const int root = 2 * m - 2;
-    if (tid == 0) {                                          // the two-queue merge
-        int li = 0, ii = m, next = m;
-        uint32_t wl = w[0];
-        for (int k = 0; k < m - 1; k++) {
-            int c[2];
-            uint32_t wc[2];
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const uint32_t wi = ii < next ? w[ii] : 0xFFFFFFFFu;
-                if (li < m && wl <= wi) { c[q] = li; wc[q] = wl; li++; wl = li < m ? w[li] : 0xFFFFFFFFu; }
-                else { c[q] = ii; wc[q] = wi; ii++; }
-            }
-            w[next] = wc[0] + wc[1];
-            link[c[0]] = (1u << 16) | (uint32_t)next;        // distance 1 to the parent
-            link[c[1]] = (1u << 16) | (uint32_t)next;
-            next++;
-        }
-        link[root] = (uint32_t)root;                         // the root: distance 0 to itself
-    }
-    __syncthreads();
-    // depths: link = distance << 16 | ancestor, jumped to the root (each entry
-    // read and written whole, so a round may mix old and new entries safely)
-    for (int round = 0; round < 10; round++) {
-        for (int v = tid; v <= root; v += kHuffThreads) {
-            const uint32_t a = link[v], b = link[a & 0xFFFFu];
-            link[v] = ((a >> 16) + (b >> 16)) << 16 | (b & 0xFFFFu);
-        }
         __syncthreads();
     }
-    for (int i = tid; i < m; i += kHuffThreads) atomicAdd(&cnt[min((int)(link[i] >> 16), LIMIT)], 1u);
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t blc[LIMIT + 1];
-#pragma unroll
-        for (int b = 0; b <= LIMIT; b++) blc[b] = cnt[b];
-        uint32_t kraft = 0;                                  // in units of 2^-LIMIT
-#pragma unroll
-        for (int b = 1; b <= LIMIT; b++) kraft += blc[b] << (LIMIT - b);
-        while (kraft > (1u << LIMIT)) {
-            blc[LIMIT]--;                                    // one code at the limit goes
-            bool done = false;
-#pragma unroll
-            for (int b = LIMIT - 1; b > 0; b--)              // one shorter code splits in two
-                if (!done && blc[b]) { blc[b]--; blc[b + 1] += 2; done = true; }
-            kraft--;
+    uint32_t mc = 0;
+    for (int s = tid; s < n; s += kHuffThreads) mc += freq[s] != 0 && (uint64_t)freq[s] * U < 2ull * total;
+    const uint32_t m = block_sum(mc, red);
+    const uint32_t T = (uint32_t)(((uint64_t)total * U + (U - m - 1)) / (U - m));
+    const int n16 = (n + 15) & ~15;
+    for (int s = tid; s < n16; s += kHuffThreads) {
+        const uint32_t f = s < n ? freq[s] : 0u;
+        uint32_t k = 0;
+        if (f) {
+            const uint32_t q = (T + f - 1) / f;
+            const uint32_t L = min(max(q <= 1 ? 1u : 32u - __builtin_clz(q - 1), 1u), (uint32_t)LIMIT);
+            k = (f << L) << 13 | (15u - L) << 9 | (511u - (uint32_t)s);   // f << L < 2T < 2^18
         }
-        // sorted positions [start[b], start[b] + blc[b]) get length b, longest first
-        uint32_t at = 0;
-#pragma unroll
-        for (int b = LIMIT; b >= 1; b--) { cnt[b] = at; at += blc[b]; }
-        cnt[0] = at;
+        key[s] = k;
+        if (s < n) len[s] = 0;
     }
     __syncthreads();
-    for (int i = tid; i < m; i += kHuffThreads) {
-        int b = LIMIT;
-        while (b > 1 && (uint32_t)i >= (b > 1 ? cnt[b - 1] : cnt[0])) b--;
-        len[sorted[i]] = (uint8_t)b;
+    for (int s = tid; s < n; s += kHuffThreads) {
+        const uint32_t k = key[s];
+        if (k) slot[keys_above(key, n16, k)] = (uint32_t)s | (15u - ((k >> 9) & 15u)) << 16;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        uint32_t sy[kHuffChunks], L[kHuffChunks];
+        bool el[kHuffChunks];
+        uint32_t kk = 0;
+#pragma unroll
+        for (int c = 0; c < kHuffChunks; c++) {
+            const int p = c * 64 + lane;
+            const uint32_t v = p < (int)used ? slot[p] : 0u;
+            sy[c] = v & 0xFFFFu;
+            L[c] = v >> 16;                                   // 0: no symbol at this place
+            el[c] = L[c] > 1;
+            kk += L[c] ? U >> L[c] : 0u;
+        }
+        uint32_t S = U - wave_total(kk);                      // the slack
+        for (int ph = 0; ph < kHuffPhases && S; ph++) {
+            uint32_t bound = S;                               // the largest term that can still fit
+#pragma unroll
+            for (int c = 0; c < kHuffChunks; c++) {
+                bool moved = false;
+                int start = 0;
+                while (S && start < 64 && c * 64 < (int)used) {
+                    const bool adm = el[c] && lane >= start && (U >> L[c]) <= bound;
+                    const uint32_t incl = wave_incl_scan(adm ? U >> L[c] : 0u);
+                    const uint64_t fail = __ballot(adm && incl > S);
+                    const int f = fail ? (int)__builtin_ctzll(fail) : 64;
+                    const bool take = adm && lane < f;
+                    L[c] -= take;
+                    moved |= take;
+                    S -= f ? (uint32_t)__builtin_amdgcn_readlane((int)incl, f - 1) : 0u;
+                    bound = S;
+                    start = f + 1;
+                }
+                el[c] = moved && L[c] > 1;
+            }
+        }
+        // complete the sum with the longest codes
+        for (int l = LIMIT; l > 1 && S; l--) {
+            const uint32_t unit = U >> l;
+            uint32_t before = 0;
+            const uint32_t want = S / unit;
+#pragma unroll
+            for (int c = 0; c < kHuffChunks; c++) {
+                const uint64_t b = __ballot(L[c] == (uint32_t)l);
+                const uint32_t idx = before + (uint32_t)__builtin_popcountll(b & ((1ull << lane) - 1ull));
+                if (L[c] == (uint32_t)l && idx < want) L[c] -= 1;
+                before += (uint32_t)__builtin_popcountll(b);
+            }
+            S -= min(before, want) * unit;
+        }
+#pragma unroll
+        for (int c = 0; c < kHuffChunks; c++)
+            if (L[c]) len[sy[c]] = (uint8_t)L[c];
     }
     __syncthreads();
 }
 
-// Canonical codes (RFC 1951 3.2.2) for len[0..n), bit-reversed for the
-// LSB-first stream: out[s] = code | length << 16 (0 where unused); by the
-// workgroup, each symbol's code is its length's first code plus the number of
-// earlier symbols of that length.
-__device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* first)
+// Canonical codes (RFC 1951 3.2.2) for len[0..n) (n <= 320), bit-reversed
+// for the LSB-first stream: out[s] = code | length << 16 (0 where unused); by
+// the workgroup.  A symbol's code is its length's first code plus the number
+// of lower symbols of that length, counted with one ballot per length over
+// each 64-symbol chunk.  Scratch: cnt (kHuffChunks * 16 u32), first (16 u32).
+__device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* cnt, uint32_t* first)
 {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = kHuffThreads / 64;
     __syncthreads();
-    if (tid < 16) first[tid] = 0;
-    __syncthreads();
-    for (int s = tid; s < n; s += kHuffThreads) atomicAdd(&first[len[s]], 1u);   // codes per length
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t blc[16];
+    uint32_t myl[2] = {0, 0}, below[2] = {0, 0};
 #pragma unroll
-        for (int b = 0; b < 16; b++) blc[b] = first[b];
-        blc[0] = 0;
+    for (int i = 0; i < 2; i++) {
+        const int ch = wave + NW * i;
+        if (ch >= kHuffChunks || ch * 64 >= n) continue;
+        const int s = ch * 64 + lane;
+        const uint32_t l = s < n ? len[s] : 0u;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int b = 1; b < 16; b++) {
+            const uint64_t m = __ballot(l == (uint32_t)b);
+            if (lane == b) cnt[ch * 16 + b] = (uint32_t)__builtin_popcountll(m);
+            mine = l == (uint32_t)b ? m : mine;
+        }
+        myl[i] = l;
+        below[i] = (uint32_t)__builtin_popcountll(mine & ((1ull << lane) - 1ull));
+    }
+    const int nch = min(kHuffChunks, (n + 63) >> 6);
+    __syncthreads();
+    if (tid < 64) {
+        // codes per length, then first[b] = sum over k < b of count[k] << (b - k)
+        uint32_t bc = 0;
+        if (lane >= 1 && lane < 16)
+            for (int ch = 0; ch < nch; ch++) bc += cnt[ch * 16 + lane];
         uint32_t code = 0;
 #pragma unroll
-        for (int b = 1; b < 16; b++) { code = (code + blc[b - 1]) << 1; first[b] = code; }
+        for (int k = 1; k < 15; k++) {
+            const uint32_t ck = (uint32_t)__builtin_amdgcn_readlane((int)bc, k);
+            code += lane > k && lane < 16 ? ck << (lane - k) : 0u;
+        }
+        if (lane < 16) first[lane] = code;
     }
     __syncthreads();
-    for (int s = tid; s < n; s += kHuffThreads) {
-        const uint32_t l = len[s];
-        uint32_t r = 0;
-        for (int t = 0; t < s; t++) r += len[t] == l;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int ch = wave + NW * i;
+        const int s = ch * 64 + lane;
+        if (ch >= kHuffChunks || s >= n) continue;
+        const uint32_t l = myl[i];
+        uint32_t r = below[i];
+        for (int c2 = 0; c2 < ch; c2++) r += l ? cnt[c2 * 16 + l] : 0u;
         out[s] = l ? (__builtin_bitreverse32(first[l] + r) >> (32 - l)) | (l << 16) : 0u;
     }
     __syncthreads();
@@ -1977,8 +2044,8 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
 {
     __shared__ uint32_t freq[kHuffSyms];
     __shared__ uint8_t len[kHuffSyms];
-    __shared__ uint16_t sorted[286];
-    __shared__ uint32_t w[2 * 286], link[2 * 286];
+    __shared__ __attribute__((aligned(16))) uint32_t w[288];
+    __shared__ uint32_t slot[288];
     __shared__ uint32_t cnt[16], first[16];
     __shared__ uint32_t cfreq[19], ccode[19], table[kHuffSyms];
     __shared__ uint8_t clen[19];
@@ -1996,8 +2063,8 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     for (int i = tid; i < 19; i += kHuffThreads) cfreq[i] = 0;
     for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) hdr[i] = 0;
     if (tid == 0) { s_hlit = 257; s_hdist = 1; }
-    huff_lengths<15>(freq, 286, len, sorted, w, link, cnt);
-    huff_lengths<15>(freq + 286, 30, len + 286, sorted, w, link, cnt);
+    huff_lengths<15>(freq, 286, len, w, slot, cnt);
+    huff_lengths<15>(freq + 286, 30, len + 286, w, slot, cnt);
     // code sizes of this code and of the fixed one (extra bits are the same
     // for both); the sent alphabet sizes; the code-length histogram
     uint32_t dyn = 0, fix = 0;
@@ -2013,8 +2080,8 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     const int hlit = (int)s_hlit, hdist = (int)s_hdist, N = hlit + hdist;
     // the code lengths are sent one symbol each (no run-length codes)
     for (int j = tid; j < N; j += kHuffThreads) atomicAdd(&cfreq[j < hlit ? len[j] : len[286 + j - hlit]], 1u);
-    huff_lengths<7>(cfreq, 19, clen, sorted, w, link, cnt);
-    huff_codes(clen, 19, ccode, first);
+    huff_lengths<7>(cfreq, 19, clen, w, slot, cnt);
+    huff_codes(clen, 19, ccode, w, first);
     const uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
     int hclen = 19;
     while (hclen > 4 && clen[order[hclen - 1]] == 0) hclen--;
@@ -2053,8 +2120,8 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
         }
         if (tid == 0) H.hdr_bits[seg] = base;
         __syncthreads();
-        huff_codes(len, 286, table, first);
-        huff_codes(len + 286, 30, table + 286, first);
+        huff_codes(len, 286, table, w, first);
+        huff_codes(len + 286, 30, table + 286, w, first);
     } else {
         // the fixed code (RFC 1951 3.2.6): its canonical codes directly
         if (tid == 0) { hdr[0] = (bfinal ? 1u : 0u) | (1u << 1); H.hdr_bits[seg] = 3; }
