@@ -147,7 +147,7 @@ struct HuffJob {
     const uint32_t* tokens;      // as DeflateJob
     const uint32_t* tok_count;
     const uint32_t* hist;
-    uint32_t*       codes;       // nseg * kHuffSyms: bit-reversed code | length << 16 (0: unused)
+    uint32_t*       codes;       // nseg * kHuffTab: bit-reversed code (extra bits in place) | bit count << 24
     uint32_t*       hdr;         // nseg * kHuffHdrWords: the block header, BFINAL/BTYPE first
     uint32_t*       hdr_bits;    // nseg
     uint32_t*       slice_out;   // as DeflateJob: the slices' bit strings
@@ -181,6 +181,7 @@ constexpr int kTileBytes = 16384;        // k_filter_tiled: PNG-ready bytes of a
 constexpr int kFilterLdsMax = 160 * 1024 - 256;  // LDS one filter workgroup may use
 constexpr int kWindow = 32768;           // DEFLATE window
 constexpr int kHuffSyms = 286 + 30;      // literal/length symbols, then distance codes
+constexpr int kHuffTab = kHuffSyms + 256; // the emit table: the symbols, then match lengths 3..258
 constexpr int kHuffHdrWords = 96;        // a dynamic block header: at most 3 + 14 + 57 + 316 * 7 bits
 // Every buffer k_deflate reads holds at least this many readable bytes after
 // the last data byte, so its 16-byte loads need no tail guard.

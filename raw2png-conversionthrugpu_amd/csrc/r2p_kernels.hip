@@ -2134,34 +2134,61 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     }
     __syncthreads();
     for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) H.hdr[(int64_t)seg * kHuffHdrWords + i] = hdr[i];
-    for (int s = tid; s < kHuffSyms; s += kHuffThreads) H.codes[(int64_t)seg * kHuffSyms + s] = table[s];
+    // the emit table: value | bit count << 24 per symbol, then per match
+    // length 3..258 its code with the extra bits already in place
+    uint32_t* C = H.codes + (int64_t)seg * kHuffTab;
+    for (int s = tid; s < kHuffSyms; s += kHuffThreads) C[s] = (table[s] & 0xFFFFu) | (table[s] >> 16) << 24;
+    for (int k = tid; k < 256; k += kHuffThreads) {
+        const uint32_t L = (uint32_t)k + 3u, lc = table[huff_len_sym(L)], le = huff_len_extra_bits(L), ln = lc >> 16;
+        C[kHuffSyms + k] = ((lc & 0xFFFFu) | ((L - 3u) & ((1u << le) - 1u)) << ln) | (ln + le) << 24;
+    }
 }
 
-// The bits of token v with table T: up to two (value, count) parts of at
-// most 28 bits each.  v == kNoToken stands for the end-of-block code.
-__device__ __forceinline__ uint32_t token_bits(const uint32_t* T, uint32_t v, uint32_t& v1, uint32_t& n1, uint32_t& n0)
+// The bits of token v (a literal, 256 for the end-of-block code, or a match)
+// with the emit table T: up to two (value, count) parts of at most 28 bits --
+// the literal/length code with its extra bits, then for a match the distance
+// code with its extra bits.
+__device__ __forceinline__ void token_code(const uint32_t* T, uint32_t v, uint32_t& a, uint32_t& na, uint32_t& b,
+                                           uint32_t& nb)
 {
-    if (v == kNoToken) { const uint32_t c = T[256]; n0 = c >> 16; n1 = 0; v1 = 0; return c & 0xFFFFu; }
-    if (!(v >> 31)) { const uint32_t c = T[v]; n0 = c >> 16; n1 = 0; v1 = 0; return c & 0xFFFFu; }
-    const uint32_t L = ((v >> 16) & 0xFFu) + 3u, d = (v & 0xFFFFu) + 1u;
-    const uint32_t lc = T[huff_len_sym(L)], le = huff_len_extra_bits(L);
-    const uint32_t lx = (L - 3u) & ((1u << le) - 1u);
-    const uint32_t dc = T[286 + huff_dist_sym(d)], de = 30u - __builtin_clz((d - 1u) | 2u);
-    const uint32_t dx = (d - 1u) & ((1u << de) - 1u);
-    n0 = (lc >> 16) + le;
-    n1 = (dc >> 16) + de;
-    v1 = (dc & 0xFFFFu) | (dx << (dc >> 16));
-    return (lc & 0xFFFFu) | (lx << (lc >> 16));
+    const bool m = v >> 31;
+    const uint32_t e1 = T[m ? (uint32_t)kHuffSyms + ((v >> 16) & 0xFFu) : v];
+    a = e1 & 0xFFFFFFu;
+    na = e1 >> 24;
+    b = 0;
+    nb = 0;
+    if (m) {
+        const uint32_t dm = v & 0xFFFFu;                       // distance - 1
+        const uint32_t e = 30u - __builtin_clz(dm | 2u);        // extra bits
+        const uint32_t dc = T[286u + (e << 1) + (dm >> e)];
+        const uint32_t dn = dc >> 24;
+        b = (dc & 0xFFFFFFu) | (dm & ((1u << e) - 1u)) << dn;
+        nb = dn + e;
+    }
+}
+
+// Ors the n (<= 28) bits of v into LDS words at bit offset off.
+__device__ __forceinline__ void put_bits(uint32_t* obuf, uint32_t v, uint32_t n, uint32_t off)
+{
+    const uint32_t w = off >> 5, sh = off & 31;
+    atomicOr(&obuf[w], v << sh);
+    if (sh + n > 32) atomicOr(&obuf[w + 1], v >> (32 - sh));
 }
 
 constexpr int kEmitThreads = 256;
 constexpr int kEmitPer = 8;                                    // tokens per thread and round
 constexpr int kEmitWords = kEmitThreads * kEmitPer * 48 / 32 + kHuffHdrWords + 8;
 
+// One workgroup per slice: rounds of kEmitThreads * kEmitPer tokens, each
+// thread's 8 tokens read as two 16-byte loads issued one round ahead (the
+// token buffer has room past every slice's last token: loads start below the
+// token count and run at most 7 tokens beyond it), their codes looked up, a
+// block scan of the bit counts, every code ored into LDS words at its offset,
+// flush of the whole words.
 __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
 {
     constexpr int NT = kEmitThreads, NW = NT / 64;
-    __shared__ uint32_t T[kHuffSyms];
+    __shared__ uint32_t T[kHuffTab];
     __shared__ uint32_t obuf[kEmitWords];
     __shared__ uint32_t red[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -2170,8 +2197,19 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
         const int seg = sl / H.spb;
         const bool first = sl % H.spb == 0;
         const bool lastsl = sl % H.spb == H.spb - 1 || sl == H.nslice - 1;
+        const uint32_t nreal = H.tok_count[sl];
+        const uint32_t ntok = nreal + (lastsl ? 1u : 0u);   // + the end-of-block code
+        const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes;
+        u32x4 nx[kEmitPer / 4];
+        auto fetch = [&](uint32_t r) {
+            const uint32_t t0 = r * NT * kEmitPer + tid * kEmitPer;
+#pragma unroll
+            for (int q = 0; q < kEmitPer / 4; q++)
+                nx[q] = t0 + 4 * q < nreal ? *(const u32x4*)(tk + t0 + 4 * q) : u32x4{0u, 0u, 0u, 0u};
+        };
+        fetch(0);
         __syncthreads();                                       // the previous slice is done with T
-        for (int i = tid; i < kHuffSyms; i += NT) T[i] = H.codes[(int64_t)seg * kHuffSyms + i];
+        for (int i = tid; i < kHuffTab; i += NT) T[i] = H.codes[(int64_t)seg * kHuffTab + i];
         // bits already in obuf: the segment's header before its first slice
         uint32_t pre = 0;
         if (first) {
@@ -2179,51 +2217,42 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
             for (uint32_t i = tid; i < (pre + 31) / 32; i += NT) obuf[i] = H.hdr[(int64_t)seg * kHuffHdrWords + i];
         }
         __syncthreads();
-        const uint32_t nreal = H.tok_count[sl];
-        const uint32_t ntok = nreal + (lastsl ? 1u : 0u);   // + the end-of-block code
-        const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes;
         uint32_t* out = H.slice_out + (int64_t)sl * H.slice_cap_dw;
         int64_t out_dw = 0;
         uint32_t carry_bits = pre;
         const uint32_t rounds = max(1u, (ntok + NT * kEmitPer - 1) / (NT * kEmitPer));
         for (uint32_t r = 0; r < rounds; r++) {
             const uint32_t t0 = r * NT * kEmitPer + tid * kEmitPer;
-            uint32_t a[kEmitPer], na[kEmitPer], b[kEmitPer], nb[kEmitPer];
+            uint32_t cur[kEmitPer];
+#pragma unroll
+            for (int q = 0; q < kEmitPer / 4; q++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) cur[4 * q + j] = nx[q][j];
+            if (r + 1 < rounds) fetch(r + 1);
+            uint32_t a[kEmitPer], b[kEmitPer], nn[kEmitPer];
             uint32_t nbits = 0;
 #pragma unroll
             for (int i = 0; i < kEmitPer; i++) {
                 const uint32_t t = t0 + i;
-                na[i] = nb[i] = 0; a[i] = b[i] = 0;
-                if (t < ntok) {
-                    const uint32_t v = t < nreal ? tk[t] : kNoToken;
-                    a[i] = token_bits(T, v, b[i], nb[i], na[i]);
-                }
-                nbits += na[i] + nb[i];
+                uint32_t na = 0, nb = 0;
+                a[i] = b[i] = 0;
+                if (t < ntok) token_code(T, t < nreal ? cur[i] : 256u, a[i], na, b[i], nb);
+                nn[i] = na | nb << 8;
+                nbits += na + nb;
             }
             const uint32_t incl = wave_incl_scan(nbits);
             if (lane == 63) red[wave] = incl;
             __syncthreads();
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
-            const uint32_t off = carry_bits + wbase + incl - nbits;
-            {   // as k_deflate: the lane's first and last words are shared (atomicOr)
-                const uint32_t w0 = off >> 5;
-                uint32_t wi = w0, nacc = off & 31;
-                uint64_t acc = 0;
-                auto push = [&](uint32_t v, uint32_t n) {
-                    acc |= (uint64_t)v << nacc;
-                    nacc += n;
-                    if (nacc >= 32) {
-                        if (wi == w0) atomicOr(&obuf[wi], (uint32_t)acc);
-                        else obuf[wi] = (uint32_t)acc;
-                        wi++;
-                        acc >>= 32;
-                        nacc -= 32;
-                    }
-                };
+            uint32_t off = carry_bits + wbase + incl - nbits;
 #pragma unroll
-                for (int i = 0; i < kEmitPer; i++) { push(a[i], na[i]); push(b[i], nb[i]); }
-                if (nacc > 0) atomicOr(&obuf[wi], (uint32_t)acc);
+            for (int i = 0; i < kEmitPer; i++) {
+                const uint32_t na = nn[i] & 0xFFu, nb = nn[i] >> 8;
+                if (na) put_bits(obuf, a[i], na, off);
+                off += na;
+                if (nb) put_bits(obuf, b[i], nb, off);
+                off += nb;
             }
             __syncthreads();
             const uint32_t tb = carry_bits + tot;
