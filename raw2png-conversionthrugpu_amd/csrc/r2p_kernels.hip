@@ -1843,18 +1843,52 @@ __device__ uint32_t block_sum(uint32_t v, uint32_t* red)
     return t;
 }
 
-// Number of keys in key[0..n16) greater than ks, with the keys read 16 at a
-// time (every lane reads the same words: broadcasts), so that the loads are
-// in flight together.  key is 16-byte aligned and zero-padded to n16.
-__device__ __forceinline__ uint32_t keys_above(const uint32_t* key, int n16, uint32_t ks)
+// Ors the n (<= 28) bits of v into LDS words at bit offset off.
+__device__ __forceinline__ void put_bits(uint32_t* obuf, uint32_t v, uint32_t n, uint32_t off)
+{
+    const uint32_t w = off >> 5, sh = off & 31;
+    atomicOr(&obuf[w], v << sh);
+    if (sh + n > 32) atomicOr(&obuf[w + 1], v >> (32 - sh));
+}
+
+// Maximum of v over the 64 lanes of the wave (the DPP pattern of wave_total), uniform.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// The order key of symbol s with frequency f > 0: its Shannon length
+// L = ceil(log2(T / f)) clamped to [1, LIMIT], packed as f << L (< 2T < 2^18),
+// then 15 - L, then 511 - s, so that the larger key is the larger f * 2^L
+// (bits saved per unit of Kraft sum by a shortening), then the larger f, then
+// the lower symbol.  L comes back as 15 - ((key >> 9) & 15).
+template <int LIMIT>
+__device__ __forceinline__ uint32_t shannon_key(uint32_t f, uint32_t T, int s)
+{
+    const uint32_t q = (T + f - 1) / f;
+    const uint32_t L = min(max(q <= 1 ? 1u : 32u - __builtin_clz(q - 1), 1u), (uint32_t)LIMIT);
+    return (f << L) << 13 | (15u - L) << 9 | (511u - (uint32_t)s);
+}
+__device__ __forceinline__ uint32_t key_len(uint32_t k) { return 15u - ((k >> 9) & 15u); }
+
+// Number of keys in key[b..e) greater than ks (e - b a multiple of 24, b of
+// 4), read 8 at a time (every lane reads the same words: broadcasts), three
+// steps unrolled so that the loads are in flight together.
+__device__ __forceinline__ uint32_t keys_above(const uint32_t* key, int b, int e, uint32_t ks)
 {
     uint32_t r = 0;
-    for (int t0 = 0; t0 < n16; t0 += 16) {
-        u32x4 k4[4];
+    for (int t0 = b; t0 < e; t0 += 24) {
+        u32x4 k4[6];
 #pragma unroll
-        for (int q = 0; q < 4; q++) k4[q] = *(const u32x4*)(key + t0 + 4 * q);
+        for (int q = 0; q < 6; q++) k4[q] = *(const u32x4*)(key + t0 + 4 * q);
 #pragma unroll
-        for (int j = 0; j < 16; j++) r += k4[j >> 2][j & 3] > ks;
+        for (int j = 0; j < 24; j++) r += k4[j >> 2][j & 3] > ks;
     }
     return r;
 }
@@ -1862,142 +1896,256 @@ __device__ __forceinline__ uint32_t keys_above(const uint32_t* key, int n16, uin
 constexpr int kHuffChunks = 5;           // 64-symbol chunks of the largest alphabet (286)
 constexpr int kHuffPhases = 3;           // shortening passes of the length construction
 
-// Code lengths of at most LIMIT bits for freq[0..n) (n <= 320) into
-// len[0..n), by the workgroup, with no long serial chain.
-//  1. Shannon lengths ceil(log2(T / f)), clamped to [1, LIMIT], against
-//     T = ceil(total * U / (U - m)) (U = 2^LIMIT; m = symbols with
-//     f * U < 2 * total, which bounds the clamped ones), so the Kraft sum
-//     stays <= 1 with the clamp: clamped terms add at most m / U, the others
-//     at most total / T = (U - m) / U.
-//  2. The symbols are ordered once by f * 2^L descending (bits saved per unit
-//     of Kraft sum by a shortening), then f descending, then symbol.
-//  3. Wave 0 walks that order kHuffPhases times, holding the order in
-//     registers: a symbol is shortened by one bit when its term fits in the
-//     remaining slack, and stays in the next pass only if it was.  A pass is
-//     a few rounds of 64-lane prefix sums: the lanes before the first one that
-//     does not fit all move, the slack drops below that lane's term, and the
-//     round repeats from the next lane with the smaller terms only.
-//  4. Any slack left is a multiple of the longest length's term: codes of the
-//     longest length move up one bit (first in the order first) until the sum
-//     is exactly 1 -- inflate takes complete codes only.
-// At least two symbols get a code (a used one alone, or none, gets partners of
-// count 1).  On the images measured this comes within 0.4 % of the optimal
-// (two-queue Huffman) code.  Scratch: key (288 u32, 16-byte aligned),
-// slot (288 u32), red (>= 4 u32).
-template <int LIMIT>
-__device__ void huff_lengths(uint32_t* freq, int n, uint8_t* len, uint32_t* key, uint32_t* slot, uint32_t* red)
+// The length construction's second half, by one wave, on the symbols in
+// registers in the order of their keys: lane p of chunk c holds the length
+// L[c] of the symbol of order 64c + p (0: none), used symbols in all.  The
+// lengths start at the Shannon lengths against a T that keeps the Kraft sum
+// <= 1 with the clamp.
+//  1. kHuffPhases walks of the order: a symbol is shortened by one bit when
+//     its term fits in the remaining slack, and stays in the next walk only
+//     if it was.  A walk is a few rounds of 64-lane prefix sums: the lanes
+//     before the first one that does not fit all move, the slack drops below
+//     that lane's term, and the round repeats from the next lane with the
+//     smaller terms only.
+//  2. Any slack left is a multiple of the longest length's term: codes of the
+//     longest length move up one bit (first in the order first) until the
+//     sum is exactly 1 -- inflate takes complete codes only.  (Each step
+//     leaves a sum that is a multiple of the next shorter length's term, and
+//     all codes at length 1 would sum to >= 1.)
+// On the images measured the result is within 0.4 % of the optimal
+// (two-queue Huffman) code.
+template <int LIMIT, int NC>
+__device__ __forceinline__ void walk_lengths(uint32_t (&L)[NC], uint32_t used)
 {
     constexpr uint32_t U = 1u << LIMIT;                       // the Kraft sum in units of 2^-LIMIT
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int lane = threadIdx.x & 63;
+    bool el[NC];
+    uint32_t kk = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        el[c] = L[c] > 1;
+        kk += L[c] ? U >> L[c] : 0u;
+    }
+    uint32_t S = U - wave_total(kk);                          // the slack
+    for (int ph = 0; ph < kHuffPhases && S; ph++) {
+        uint32_t bound = S;                                   // the largest term that can still fit
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            bool moved = false;
+            int start = 0;
+            while (S && start < 64 && c * 64 < (int)used) {
+                const bool adm = el[c] && lane >= start && (U >> L[c]) <= bound;
+                const uint32_t incl = wave_incl_scan(adm ? U >> L[c] : 0u);
+                const uint64_t fail = __ballot(adm && incl > S);
+                const int f = fail ? (int)__builtin_ctzll(fail) : 64;
+                const bool take = adm && lane < f;
+                L[c] -= take;
+                moved |= take;
+                S -= f ? (uint32_t)__builtin_amdgcn_readlane((int)incl, f - 1) : 0u;
+                bound = S;
+                start = f + 1;
+            }
+            el[c] = moved && L[c] > 1;
+        }
+    }
+    for (int l = LIMIT; l > 1 && S; l--) {
+        const uint32_t want = S >> (LIMIT - l);
+        uint32_t before = 0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if (c * 64 >= (int)used) break;
+            const uint64_t bm = __ballot(L[c] == (uint32_t)l);
+            const uint32_t idx = before + (uint32_t)__builtin_popcountll(bm & ((1ull << lane) - 1ull));
+            if (L[c] == (uint32_t)l && idx < want) L[c] -= 1;
+            before += (uint32_t)__builtin_popcountll(bm);
+        }
+        S -= min(before, want) << (LIMIT - l);
+    }
+}
+
+// Code lengths of at most LIMIT bits for an alphabet of N <= 63 symbols held
+// by one wave (symbol = lane, f its frequency; 0 past N); returns the lane's
+// length.  At least two symbols get a code (a used one alone, or none, gets
+// partners of count 1).  The Shannon lengths are taken against
+// T = ceil(total * U / (U - m)) (U = 2^LIMIT, m = symbols with
+// f * U < 2 * total, which bounds the clamped ones), so the Kraft sum stays
+// <= 1 with the clamp: clamped terms add at most m / U, the others at most
+// total / T = (U - m) / U.  The order is a rank among the wave's keys; lane p
+// takes the symbol of rank p (a forward permute) for walk_lengths, and each
+// symbol reads its length back from the lane of its rank.
+template <int LIMIT, int N>
+__device__ uint32_t wave_lengths(uint32_t f)
+{
+    constexpr uint32_t U = 1u << LIMIT;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t used = (uint32_t)__builtin_popcountll(__ballot(f != 0));
+    if (used < 2) {
+        const uint64_t z = __ballot(f == 0 && lane < N);
+        if (f == 0 && lane < N && (uint32_t)__builtin_popcountll(z & lt) < 2u - used) f = 1;
+        used = 2;
+    }
+    const uint32_t total = wave_total(f);
+    const uint32_t m = (uint32_t)__builtin_popcountll(__ballot(f != 0 && (uint64_t)f * U < 2ull * total));
+    const uint32_t T = (uint32_t)(((uint64_t)total * U + (U - m - 1)) / (U - m));
+    const uint32_t key = f ? shannon_key<LIMIT>(f, T, lane) : 0u;
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < N; j++) r += (uint32_t)__builtin_amdgcn_readlane((int)key, j) > key;
+    // unused symbols all have rank `used` (< 64): that lane is not read
+    const uint32_t v = (uint32_t)__builtin_amdgcn_ds_permute((int)(r * 4), (int)(f ? key_len(key) : 0u));
+    uint32_t L[1] = {(uint32_t)lane < used ? v : 0u};
+    walk_lengths<LIMIT, 1>(L, used);
+    const uint32_t mine = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r * 4), (int)L[0]);
+    return f ? mine : 0u;
+}
+
+// Canonical code (RFC 1951 3.2.2) of a wave-held alphabet (symbol = lane,
+// l its length, 0: unused), bit-reversed for the LSB-first stream:
+// code | l << 16.  A symbol's code is its length's first code plus the
+// number of lower symbols of that length (one ballot per length).
+__device__ uint32_t wave_codes(uint32_t l)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t mine = 0;
+    uint32_t bc = 0;                                           // lane b: codes of length b
+#pragma unroll
+    for (int b = 1; b < 16; b++) {
+        const uint64_t m = __ballot(l == (uint32_t)b);
+        if (lane == b) bc = (uint32_t)__builtin_popcountll(m);
+        mine = l == (uint32_t)b ? m : mine;
+    }
+    uint32_t code = 0;                                         // lane b: the first code of length b
+#pragma unroll
+    for (int k = 1; k < 15; k++) {
+        const uint32_t ck = (uint32_t)__builtin_amdgcn_readlane((int)bc, k);
+        code += lane > k && lane < 16 ? ck << (lane - k) : 0u;
+    }
+    const uint32_t fl = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(l * 4), (int)code);
+    const uint32_t c = fl + (uint32_t)__builtin_popcountll(mine & ((1ull << lane) - 1ull));
+    return l ? (__builtin_bitreverse32(c) >> (32 - l)) | (l << 16) : 0u;
+}
+
+// Fixed-code length of literal/length symbol s (RFC 1951 3.2.6).
+__device__ __forceinline__ uint32_t fixed_len(int s) { return s < 144 ? 8u : s < 256 ? 9u : s < 280 ? 7u : 8u; }
+
+// The order in which a dynamic header sends the code-length code's lengths (RFC 1951 3.2.7).
+__device__ const uint8_t kClOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
+{
+    constexpr int NW = kHuffThreads / 64;
+    __shared__ uint32_t freq[kHuffSyms];
+    __shared__ uint8_t len[kHuffSyms];
+    __shared__ __attribute__((aligned(16))) uint32_t key[288];
+    __shared__ uint32_t slot[288];
+    __shared__ uint32_t red[16], first[16], cnt[kHuffChunks * 16], rank4[32];
+    __shared__ uint32_t cfreq[19], table[kHuffSyms];
+    __shared__ uint32_t hdr[kHuffHdrWords];
+    __shared__ uint32_t s_hlit, s_hdist, s_dynamic;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int seg = blockIdx.x;
+    const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
+    const bool bfinal = seg == H.nseg - 1 && H.final_block;
+    // the segment's histogram: its slices' summed, 8 loads in flight
+    for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
+        const uint32_t* hp = H.hist + (int64_t)k0 * kHuffSyms + s;
+        uint32_t f = 0;
+        int k = 0;
+        for (; k + 8 <= k1 - k0; k += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = hp[(int64_t)(k + u) * kHuffSyms];
+#pragma unroll
+            for (int u = 0; u < 8; u++) f += v[u];
+        }
+        for (; k < k1 - k0; k++) f += hp[(int64_t)k * kHuffSyms];
+        freq[s] = f + (s == 256 ? 1u : 0u);                  // the end-of-block code
+    }
+    for (int i = tid; i < 19; i += kHuffThreads) cfreq[i] = 0;
+    for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) hdr[i] = 0;
+    if (tid < 32) rank4[tid] = 0;
     __syncthreads();
+    // literal/length code: counts, keys and ranks by the workgroup (at least
+    // two symbols: the end-of-block code always has one); T as in
+    // wave_lengths with m bounded by the used symbols
     uint32_t used, total;
     for (;;) {
         uint32_t v = 0;
-        for (int s = tid; s < n; s += kHuffThreads) v += freq[s] + (freq[s] != 0 ? 1u << 23 : 0u);
-        v = block_sum(v, red);                                // total < 2^23 for every alphabet here
+        for (int s = tid; s < 286; s += kHuffThreads) v += freq[s] + (freq[s] ? 1u << 23 : 0u);
+        v = block_sum(v, red);                                // total < 2^23
         used = v >> 23;
         total = v & ((1u << 23) - 1);
         if (used >= 2) break;
         if (tid == 0)
-            for (int i = 0; used < 2 && i < n; i++)
+            for (int i = 0; used < 2 && i < 286; i++)
                 if (freq[i] == 0) { freq[i] = 1; used++; }
         __syncthreads();
     }
-    uint32_t mc = 0;
-    for (int s = tid; s < n; s += kHuffThreads) mc += freq[s] != 0 && (uint64_t)freq[s] * U < 2ull * total;
-    const uint32_t m = block_sum(mc, red);
-    const uint32_t T = (uint32_t)(((uint64_t)total * U + (U - m - 1)) / (U - m));
-    const int n16 = (n + 15) & ~15;
-    for (int s = tid; s < n16; s += kHuffThreads) {
-        const uint32_t f = s < n ? freq[s] : 0u;
-        uint32_t k = 0;
-        if (f) {
-            const uint32_t q = (T + f - 1) / f;
-            const uint32_t L = min(max(q <= 1 ? 1u : 32u - __builtin_clz(q - 1), 1u), (uint32_t)LIMIT);
-            k = (f << L) << 13 | (15u - L) << 9 | (511u - (uint32_t)s);   // f << L < 2T < 2^18
-        }
-        key[s] = k;
-        if (s < n) len[s] = 0;
+    const uint32_t T = (uint32_t)(((uint64_t)total * 32768u + (32768u - used - 1)) / (32768u - used));
+    for (int s = tid; s < 288; s += kHuffThreads) {
+        const uint32_t f = s < 286 ? freq[s] : 0u;
+        key[s] = f ? shannon_key<15>(f, T, s) : 0u;
+        if (s < 286) len[s] = 0;
     }
     __syncthreads();
-    for (int s = tid; s < n; s += kHuffThreads) {
-        const uint32_t k = key[s];
-        if (k) slot[keys_above(key, n16, k)] = (uint32_t)s | (15u - ((k >> 9) & 15u)) << 16;
+    {   // symbol tid against all keys; symbols 256..285 against a quarter each
+        const uint32_t k = key[tid];
+        if (k) slot[keys_above(key, 0, 288, k)] = (uint32_t)tid | key_len(k) << 16;
+        const uint32_t k4 = lane < 30 ? key[256 + lane] : 0u;
+        if (k4) atomicAdd(&rank4[lane], keys_above(key, 72 * wave, 72 * wave + 72, k4));
     }
     __syncthreads();
-    if (tid < 64) {
+    if (wave == 0) {
+        // the literal/length lengths: walk in rank order, lengths out, HLIT
+        if (lane < 30 && key[256 + lane]) slot[rank4[lane]] = (256u + lane) | key_len(key[256 + lane]) << 16;
         uint32_t sy[kHuffChunks], L[kHuffChunks];
-        bool el[kHuffChunks];
-        uint32_t kk = 0;
 #pragma unroll
         for (int c = 0; c < kHuffChunks; c++) {
             const int p = c * 64 + lane;
             const uint32_t v = p < (int)used ? slot[p] : 0u;
             sy[c] = v & 0xFFFFu;
-            L[c] = v >> 16;                                   // 0: no symbol at this place
-            el[c] = L[c] > 1;
-            kk += L[c] ? U >> L[c] : 0u;
+            L[c] = v >> 16;
         }
-        uint32_t S = U - wave_total(kk);                      // the slack
-        for (int ph = 0; ph < kHuffPhases && S; ph++) {
-            uint32_t bound = S;                               // the largest term that can still fit
-#pragma unroll
-            for (int c = 0; c < kHuffChunks; c++) {
-                bool moved = false;
-                int start = 0;
-                while (S && start < 64 && c * 64 < (int)used) {
-                    const bool adm = el[c] && lane >= start && (U >> L[c]) <= bound;
-                    const uint32_t incl = wave_incl_scan(adm ? U >> L[c] : 0u);
-                    const uint64_t fail = __ballot(adm && incl > S);
-                    const int f = fail ? (int)__builtin_ctzll(fail) : 64;
-                    const bool take = adm && lane < f;
-                    L[c] -= take;
-                    moved |= take;
-                    S -= f ? (uint32_t)__builtin_amdgcn_readlane((int)incl, f - 1) : 0u;
-                    bound = S;
-                    start = f + 1;
-                }
-                el[c] = moved && L[c] > 1;
-            }
-        }
-        // complete the sum with the longest codes
-        for (int l = LIMIT; l > 1 && S; l--) {
-            const uint32_t unit = U >> l;
-            uint32_t before = 0;
-            const uint32_t want = S / unit;
-#pragma unroll
-            for (int c = 0; c < kHuffChunks; c++) {
-                const uint64_t b = __ballot(L[c] == (uint32_t)l);
-                const uint32_t idx = before + (uint32_t)__builtin_popcountll(b & ((1ull << lane) - 1ull));
-                if (L[c] == (uint32_t)l && idx < want) L[c] -= 1;
-                before += (uint32_t)__builtin_popcountll(b);
-            }
-            S -= min(before, want) * unit;
-        }
+        walk_lengths<15, kHuffChunks>(L, used);
+        uint32_t mx = 0;
 #pragma unroll
         for (int c = 0; c < kHuffChunks; c++)
-            if (L[c]) len[sy[c]] = (uint8_t)L[c];
+            if (L[c]) { len[sy[c]] = (uint8_t)L[c]; mx = max(mx, sy[c] + 1); }
+        mx = wave_max(mx);
+        if (lane == 0) s_hlit = max(257u, mx);
+    } else if (wave == 1) {
+        // the distance code, by this wave alone: lengths, codes, HDIST
+        const uint32_t l = wave_lengths<15, 30>(lane < 30 ? freq[286 + lane] : 0u);
+        const uint32_t c = wave_codes(l);
+        if (lane < 30) { len[286 + lane] = (uint8_t)l; table[286 + lane] = c; }
+        const uint32_t mx = wave_max(l ? (uint32_t)lane + 1 : 0u);
+        if (lane == 0) s_hdist = max(1u, mx);
     }
     __syncthreads();
-}
-
-// Canonical codes (RFC 1951 3.2.2) for len[0..n) (n <= 320), bit-reversed
-// for the LSB-first stream: out[s] = code | length << 16 (0 where unused); by
-// the workgroup.  A symbol's code is its length's first code plus the number
-// of lower symbols of that length, counted with one ballot per length over
-// each 64-symbol chunk.  Scratch: cnt (kHuffChunks * 16 u32), first (16 u32).
-__device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* cnt, uint32_t* first)
-{
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int NW = kHuffThreads / 64;
-    __syncthreads();
+    const int hlit = (int)s_hlit, hdist = (int)s_hdist, N = hlit + hdist;
+    // code sizes of this code and of the fixed one (extra bits are the same
+    // for both); the code-length histogram (the lengths are sent one symbol
+    // each, no run-length codes); the literal/length code's counts per length
+    {
+        uint32_t dyn = 0, fix = 0;
+        for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
+            const uint32_t f = freq[s], l = len[s];
+            dyn += f * l;
+            fix += f * (s < 286 ? fixed_len(s) : 5u);
+        }
+        for (int j = tid; j < N; j += kHuffThreads) atomicAdd(&cfreq[j < hlit ? len[j] : len[286 + j - hlit]], 1u);
+        dyn = wave_total(dyn);
+        fix = wave_total(fix);
+        if (lane == 0) { red[4 + wave] = dyn; red[8 + wave] = fix; }
+    }
     uint32_t myl[2] = {0, 0}, below[2] = {0, 0};
 #pragma unroll
     for (int i = 0; i < 2; i++) {
         const int ch = wave + NW * i;
-        if (ch >= kHuffChunks || ch * 64 >= n) continue;
+        if (ch >= kHuffChunks) continue;
         const int s = ch * 64 + lane;
-        const uint32_t l = s < n ? len[s] : 0u;
+        const uint32_t l = s < 286 ? len[s] : 0u;
         uint64_t mine = 0;
 #pragma unroll
         for (int b = 1; b < 16; b++) {
@@ -2008,123 +2156,75 @@ __device__ void huff_codes(const uint8_t* len, int n, uint32_t* out, uint32_t* c
         myl[i] = l;
         below[i] = (uint32_t)__builtin_popcountll(mine & ((1ull << lane) - 1ull));
     }
-    const int nch = min(kHuffChunks, (n + 63) >> 6);
     __syncthreads();
-    if (tid < 64) {
-        // codes per length, then first[b] = sum over k < b of count[k] << (b - k)
-        uint32_t bc = 0;
-        if (lane >= 1 && lane < 16)
-            for (int ch = 0; ch < nch; ch++) bc += cnt[ch * 16 + lane];
-        uint32_t code = 0;
+    if (wave == 0) {
+        {   // the literal/length code's first codes per length
+            uint32_t bc = 0;
+            if (lane >= 1 && lane < 16)
+                for (int ch = 0; ch < kHuffChunks; ch++) bc += cnt[ch * 16 + lane];
+            uint32_t code = 0;
 #pragma unroll
-        for (int k = 1; k < 15; k++) {
-            const uint32_t ck = (uint32_t)__builtin_amdgcn_readlane((int)bc, k);
-            code += lane > k && lane < 16 ? ck << (lane - k) : 0u;
+            for (int k = 1; k < 15; k++) {
+                const uint32_t ck = (uint32_t)__builtin_amdgcn_readlane((int)bc, k);
+                code += lane > k && lane < 16 ? ck << (lane - k) : 0u;
+            }
+            if (lane < 16) first[lane] = code;
         }
-        if (lane < 16) first[lane] = code;
-    }
-    __syncthreads();
+        // the code-length code, its order of lengths, HCLEN
+        const uint32_t cl = wave_lengths<7, 19>(lane < 19 ? cfreq[lane] : 0u);
+        const uint32_t cc = wave_codes(cl);
+        const uint32_t clo = (uint32_t)__builtin_amdgcn_ds_bpermute(lane < 19 ? 4 * (int)kClOrder[lane] : 0, (int)cl);
+        const uint64_t nz = __ballot(lane < 19 && clo != 0);
+        const int hclen = max(4, nz ? 64 - (int)__builtin_clzll(nz) : 0);
+        // the code lengths' codes and bits
+        uint32_t lc[5], hb = 0;
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-        const int ch = wave + NW * i;
-        const int s = ch * 64 + lane;
-        if (ch >= kHuffChunks || s >= n) continue;
-        const uint32_t l = myl[i];
-        uint32_t r = below[i];
-        for (int c2 = 0; c2 < ch; c2++) r += l ? cnt[c2 * 16 + l] : 0u;
-        out[s] = l ? (__builtin_bitreverse32(first[l] + r) >> (32 - l)) | (l << 16) : 0u;
+        for (int i = 0; i < 5; i++) {
+            const int j = 64 * i + lane;
+            const uint32_t l = j < N ? len[j < hlit ? j : 286 + j - hlit] : 0u;
+            const uint32_t c = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(4 * l), (int)cc);
+            lc[i] = j < N ? c : 0u;
+            hb += lc[i] >> 16;
+        }
+        hb = wave_total(hb);
+        const uint32_t dyn = red[4] + red[5] + red[6] + red[7];
+        const uint32_t fix = red[8] + red[9] + red[10] + red[11] + 3;
+        const uint32_t fixed_part = 3 + 14 + 3 * (uint32_t)hclen;
+        const bool dynamic = dyn + fixed_part + hb < fix;
+        if (dynamic) {
+            if (lane == 0)
+                atomicOr(&hdr[0], (bfinal ? 1u : 0u) | (2u << 1) | (uint32_t)(hlit - 257) << 3 |
+                                      (uint32_t)(hdist - 1) << 8 | (uint32_t)(hclen - 4) << 13);
+            if (lane < hclen) put_bits(hdr, clo, 3, 17 + 3 * (uint32_t)lane);
+            uint32_t base = fixed_part;
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const uint32_t nb = lc[i] >> 16;
+                const uint32_t incl = wave_incl_scan(nb);
+                if (nb) put_bits(hdr, lc[i] & 0xFFFFu, nb, base + incl - nb);
+                base += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            }
+            if (lane == 0) H.hdr_bits[seg] = base;
+        } else if (lane == 0) {
+            hdr[0] = (bfinal ? 1u : 0u) | (1u << 1);
+            H.hdr_bits[seg] = 3;
+        }
+        if (lane == 0) s_dynamic = dynamic;
     }
     __syncthreads();
-}
-
-// Fixed-code length of literal/length symbol s (RFC 1951 3.2.6).
-__device__ __forceinline__ uint32_t fixed_len(int s) { return s < 144 ? 8u : s < 256 ? 9u : s < 280 ? 7u : 8u; }
-
-__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
-{
-    __shared__ uint32_t freq[kHuffSyms];
-    __shared__ uint8_t len[kHuffSyms];
-    __shared__ __attribute__((aligned(16))) uint32_t w[288];
-    __shared__ uint32_t slot[288];
-    __shared__ uint32_t cnt[16], first[16];
-    __shared__ uint32_t cfreq[19], ccode[19], table[kHuffSyms];
-    __shared__ uint8_t clen[19];
-    __shared__ uint32_t hdr[kHuffHdrWords];
-    __shared__ uint32_t s_hlit, s_hdist;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int seg = blockIdx.x;
-    const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
-    const bool bfinal = seg == H.nseg - 1 && H.final_block;
-    for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
-        uint32_t f = 0;
-        for (int k = k0; k < k1; k++) f += H.hist[(int64_t)k * kHuffSyms + s];
-        freq[s] = f + (s == 256 ? 1u : 0u);                  // the end-of-block code
-    }
-    for (int i = tid; i < 19; i += kHuffThreads) cfreq[i] = 0;
-    for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) hdr[i] = 0;
-    if (tid == 0) { s_hlit = 257; s_hdist = 1; }
-    huff_lengths<15>(freq, 286, len, w, slot, cnt);
-    huff_lengths<15>(freq + 286, 30, len + 286, w, slot, cnt);
-    // code sizes of this code and of the fixed one (extra bits are the same
-    // for both); the sent alphabet sizes; the code-length histogram
-    uint32_t dyn = 0, fix = 0;
-    for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
-        const uint32_t f = freq[s], l = len[s];
-        dyn += f * l;
-        fix += f * (s < 286 ? fixed_len(s) : 5u);
-        if (l) atomicMax(s < 286 ? &s_hlit : &s_hdist, s < 286 ? (uint32_t)s + 1 : (uint32_t)(s - 286) + 1);
-    }
-    dyn = block_sum(dyn, cnt);
-    fix = block_sum(fix, cnt) + 3;
-    __syncthreads();
-    const int hlit = (int)s_hlit, hdist = (int)s_hdist, N = hlit + hdist;
-    // the code lengths are sent one symbol each (no run-length codes)
-    for (int j = tid; j < N; j += kHuffThreads) atomicAdd(&cfreq[j < hlit ? len[j] : len[286 + j - hlit]], 1u);
-    huff_lengths<7>(cfreq, 19, clen, w, slot, cnt);
-    huff_codes(clen, 19, ccode, w, first);
-    const uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
-    int hclen = 19;
-    while (hclen > 4 && clen[order[hclen - 1]] == 0) hclen--;
-    uint32_t hbits = 0;                                      // the code lengths' bits
-    for (int j = tid; j < N; j += kHuffThreads) hbits += ccode[j < hlit ? len[j] : len[286 + j - hlit]] >> 16;
-    const uint32_t fixed_part = 3 + 14 + 3 * (uint32_t)hclen;
-    const bool dynamic = dyn + fixed_part + block_sum(hbits, cnt) < fix;
-    if (dynamic) {
-        if (tid == 0) {
-            uint64_t acc = (bfinal ? 1u : 0u) | (2u << 1) | (uint64_t)(hlit - 257) << 3 | (uint64_t)(hdist - 1) << 8 |
-                           (uint64_t)(hclen - 4) << 13;
-            uint32_t na = 17;
-            int nw = 0;
-            for (int k = 0; k < hclen; k++) {
-                acc |= (uint64_t)clen[order[k]] << na;
-                na += 3;
-                if (na >= 32) { hdr[nw++] = (uint32_t)acc; acc >>= 32; na -= 32; }
-            }
-            if (na) hdr[nw] = (uint32_t)acc;
+    if (s_dynamic) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int ch = wave + NW * i;
+            const int s = ch * 64 + lane;
+            if (ch >= kHuffChunks || s >= 286) continue;
+            const uint32_t l = myl[i];
+            uint32_t r = below[i];
+            for (int c2 = 0; c2 < ch; c2++) r += l ? cnt[c2 * 16 + l] : 0u;
+            table[s] = l ? (__builtin_bitreverse32(first[l] + r) >> (32 - l)) | (l << 16) : 0u;
         }
-        __syncthreads();
-        // the N code lengths, packed by wave 0 at prefix-summed offsets
-        uint32_t base = fixed_part;
-        for (int j0 = 0; j0 < N && tid < 64; j0 += 64) {
-            const int j = j0 + lane;
-            const uint32_t c = j < N ? ccode[j < hlit ? len[j] : len[286 + j - hlit]] : 0u;
-            const uint32_t nb = c >> 16;
-            const uint32_t incl = wave_incl_scan(nb);
-            const uint32_t off = base + incl - nb;
-            if (nb) {
-                const uint64_t v = (uint64_t)(c & 0xFFFFu) << (off & 31);
-                atomicOr(&hdr[off >> 5], (uint32_t)v);
-                if ((off & 31) + nb > 32) atomicOr(&hdr[(off >> 5) + 1], (uint32_t)(v >> 32));
-            }
-            base += __shfl(incl, 63);
-        }
-        if (tid == 0) H.hdr_bits[seg] = base;
-        __syncthreads();
-        huff_codes(len, 286, table, w, first);
-        huff_codes(len + 286, 30, table + 286, w, first);
     } else {
         // the fixed code (RFC 1951 3.2.6): its canonical codes directly
-        if (tid == 0) { hdr[0] = (bfinal ? 1u : 0u) | (1u << 1); H.hdr_bits[seg] = 3; }
         for (int s = tid; s < 286; s += kHuffThreads) {
             const uint32_t l = fixed_len(s);
             const uint32_t c = s < 144 ? 0x30u + s : s < 256 ? 0x190u + (s - 144) : s < 280 ? (uint32_t)(s - 256) : 0xC0u + (s - 280);
@@ -2165,14 +2265,6 @@ __device__ __forceinline__ void token_code(const uint32_t* T, uint32_t v, uint32
         b = (dc & 0xFFFFFFu) | (dm & ((1u << e) - 1u)) << dn;
         nb = dn + e;
     }
-}
-
-// Ors the n (<= 28) bits of v into LDS words at bit offset off.
-__device__ __forceinline__ void put_bits(uint32_t* obuf, uint32_t v, uint32_t n, uint32_t off)
-{
-    const uint32_t w = off >> 5, sh = off & 31;
-    atomicOr(&obuf[w], v << sh);
-    if (sh + n > 32) atomicOr(&obuf[w + 1], v >> (32 - sh));
 }
 
 constexpr int kEmitThreads = 256;
