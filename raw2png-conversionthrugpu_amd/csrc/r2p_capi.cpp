@@ -765,7 +765,7 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
     TRY(hipMalloc((void**)&e->d_seg_tbits, ((size_t)e->max_seg + 1) * 4));
     TRY(hipMalloc((void**)&e->d_seg_off, ((size_t)e->max_seg + 2) * 8));
     if (e->opt.huffman) {
-        TRY(hipMalloc((void**)&e->d_tokens, (max_filtered_bytes + 65536) * 4));
+        TRY(hipMalloc((void**)&e->d_tokens, (max_filtered_bytes + 65536) * 2));
         TRY(hipMalloc((void**)&e->d_tok_count, (size_t)e->max_slice * 4));
         TRY(hipMalloc((void**)&e->d_hist, (size_t)e->max_slice * kHuffSyms * 4));
         TRY(hipMalloc((void**)&e->d_codes, ((size_t)e->max_seg + 1) * kHuffTab * 4));

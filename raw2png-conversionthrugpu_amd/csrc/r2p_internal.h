@@ -132,10 +132,12 @@ struct DeflateLaunch {
     const uint32_t*   counts;      // optional device [njobs, total]
     DeflateJob        single;
     // dynamic-Huffman mode (single job; k_deflate<.., true>): instead of
-    // bits, each slice writes its tokens (slice_bytes u32 per slice:
-    // literal b, or 1 << 31 | (length - 3) << 16 | (distance - 1)), their
-    // count, and a histogram of kHuffSyms u32 (literal/length symbols
-    // 0..285, then distance codes 0..29)
+    // bits, each slice writes a 16-bit slot per position (slice_bytes u16 per
+    // slice: a literal, 0x8000 | length - 3 where a match starts and its
+    // distance - 1 in the next slot, kTokNone elsewhere; in units of 4096
+    // positions, the slot pair at positions 16l + 8q + 2j, + 1 in dword
+    // 4(l + 256q) + j), the slice's length in tok_count, and a histogram of
+    // kHuffSyms u32 (literal/length symbols 0..285, then distance codes 0..29)
     uint32_t*         tokens;
     uint32_t*         tok_count;
     uint32_t*         hist;

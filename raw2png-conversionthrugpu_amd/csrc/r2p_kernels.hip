@@ -1280,6 +1280,9 @@ __device__ __forceinline__ uint32_t huff_dist_sym(uint32_t d)
     return (e << 1) + (dm >> e);
 }
 constexpr uint32_t kNoToken = 0x7FFFFFFFu;   // (a parse position where no token starts)
+// In the dynamic-Huffman token slots, a position where no token starts: an
+// index of the emit table whose entry is zero (no bits).
+constexpr uint32_t kTokNone = (uint32_t)kHuffTab;
 
 // A match as a key: len << 16 | (0xFFFF - dist).  The larger key is the
 // longer match, and of equal lengths the shorter distance.
@@ -1557,7 +1560,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         uint32_t carry = opens ? (last ? 3u : 2u) : 0u;
         uint32_t carry_bits = opens ? 3u : 0u;
         int64_t out_dw = 0;
-        uint32_t ntok_slice = 0;                   // DYN: tokens written so far
         uint32_t* hist = obuf;                     // DYN: the slice's histogram (obuf is zero)
         // The flush of a sub-block's bits waits for the next sub-block's first
         // barrier (or the slice end's), which then also orders it before the
@@ -1699,8 +1701,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const bool tok = (tmask >> i) & 1u;
                 const uint32_t b = (o[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                 if constexpr (DYN) {                   // raw token; nbits counts tokens
+                    // (a match: 0x8000 | length - 3, distance - 1 in the high half)
                     const uint32_t L = (i == 0 && nabs) ? Lrun : key[i] >> 16;
-                    tk[i] = !tok ? kNoToken : L ? 0x80000000u | ((L - 3u) << 16) | (0xFFFEu - (key[i] & 0xFFFFu)) : b;
+                    tk[i] = !tok ? kTokNone : L ? 0x8000u | (L - 3u) | (0xFFFEu - (key[i] & 0xFFFFu)) << 16 : b;
                     nbits += tok ? 1u : 0u;
                 } else {
                     const uint32_t L = key[i] >> 16;
@@ -1743,20 +1746,39 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             }
             if constexpr (DYN) {
-                // ---- tokens in parse order, and their symbols' histogram --------
-                uint32_t* tp = DL.tokens + (int64_t)sl * J.slice_bytes + ntok_slice + (wbase + incl - nbits);
+                // ---- a 16-bit slot per position, and the symbols' histogram ----
+                // A slot holds a literal, 0x8000 | length - 3 where a match
+                // starts with its distance - 1 in the next slot (a match never
+                // starts at a lane's last position), kTokNone elsewhere.  The
+                // slot pairs (16l + 2k, 16l + 2k + 1) of each 4096-position unit
+                // of the slice go to dword 4(l + 256q) + j, k = 4q + j: every
+                // 16-byte store instruction covers contiguous lanes, and so
+                // does every load of k_huff_emit.
+                static_assert(LANE == 16, "the token layout assumes 16 positions per lane");
+                if (le > 0) {
+                    uint32_t d[LANE / 2];
+#pragma unroll
+                    for (int k = 0; k < LANE / 2; k++) {
+                        const uint32_t lo = 2 * k > 0 && (tk[2 * k - 1] & 0x8000u) ? tk[2 * k - 1] >> 16 : tk[2 * k] & 0xFFFFu;
+                        const uint32_t hi = (tk[2 * k] & 0x8000u) ? tk[2 * k] >> 16 : tk[2 * k + 1] & 0xFFFFu;
+                        d[k] = lo | hi << 16;
+                    }
+                    const int p0 = lb - s0_rel;
+                    uint32_t* tp = DL.tokens + ((int64_t)sl * J.slice_bytes + (p0 & ~4095)) / 2 + 4 * ((p0 >> 4) & 255);
+#pragma unroll
+                    for (int q = 0; q < 2; q++)
+                        *(u32x4*)(tp + 1024 * q) = u32x4{d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]};
+                }
 #pragma unroll
                 for (int i = 0; i < LANE; i++) {
-                    if (tk[i] == kNoToken) continue;
-                    *tp++ = tk[i];
-                    if (tk[i] >> 31) {
-                        atomicAdd(&hist[huff_len_sym(((tk[i] >> 16) & 0xFFu) + 3u)], 1u);
-                        atomicAdd(&hist[286 + huff_dist_sym((tk[i] & 0xFFFFu) + 1u)], 1u);
+                    if (tk[i] == kTokNone) continue;
+                    if (tk[i] & 0x8000u) {
+                        atomicAdd(&hist[huff_len_sym((tk[i] & 0xFFu) + 3u)], 1u);
+                        atomicAdd(&hist[286 + huff_dist_sym((tk[i] >> 16) + 1u)], 1u);
                     } else {
                         atomicAdd(&hist[tk[i]], 1u);
                     }
                 }
-                ntok_slice += tot;
                 __syncthreads();                   // the sub-block's T1/T2 and red uses done
                 continue;
             }
@@ -1797,7 +1819,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 DL.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
                 hist[i] = 0;
             }
-            if (tid == 0) DL.tok_count[sl] = ntok_slice;
+            if (tid == 0) DL.tok_count[sl] = (uint32_t)(s1 - s0);
         }
         if (!DYN && tid == 0) {
             if (carry_bits) out[out_dw] = carry;
@@ -1843,6 +1865,15 @@ __device__ uint32_t block_sum(uint32_t v, uint32_t* red)
     return t;
 }
 
+// Ors the n (<= 28) bits of v into LDS words at bit offset off: one 64-bit
+// or, a second one when the bits cross into the next 8-byte word.
+__device__ __forceinline__ void put_bits64(uint32_t* obuf, uint32_t v, uint32_t n, uint32_t off)
+{
+    const uint32_t w = off >> 6, sh = off & 63;
+    unsigned long long* o64 = (unsigned long long*)obuf;
+    atomicOr(&o64[w], (unsigned long long)v << sh);
+    if (sh + n > 64) atomicOr(&o64[w + 1], (unsigned long long)(v >> (64 - sh)));
+}
 // Ors the n (<= 28) bits of v into LDS words at bit offset off.
 __device__ __forceinline__ void put_bits(uint32_t* obuf, uint32_t v, uint32_t n, uint32_t off)
 {
@@ -2244,60 +2275,43 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     }
 }
 
-// The bits of token v (a literal, 256 for the end-of-block code, or a match)
-// with the emit table T: up to two (value, count) parts of at most 28 bits --
-// the literal/length code with its extra bits, then for a match the distance
-// code with its extra bits.
-__device__ __forceinline__ void token_code(const uint32_t* T, uint32_t v, uint32_t& a, uint32_t& na, uint32_t& b,
-                                           uint32_t& nb)
-{
-    const bool m = v >> 31;
-    const uint32_t e1 = T[m ? (uint32_t)kHuffSyms + ((v >> 16) & 0xFFu) : v];
-    a = e1 & 0xFFFFFFu;
-    na = e1 >> 24;
-    b = 0;
-    nb = 0;
-    if (m) {
-        const uint32_t dm = v & 0xFFFFu;                       // distance - 1
-        const uint32_t e = 30u - __builtin_clz(dm | 2u);        // extra bits
-        const uint32_t dc = T[286u + (e << 1) + (dm >> e)];
-        const uint32_t dn = dc >> 24;
-        b = (dc & 0xFFFFFFu) | (dm & ((1u << e) - 1u)) << dn;
-        nb = dn + e;
-    }
-}
-
 constexpr int kEmitThreads = 256;
-constexpr int kEmitPer = 8;                                    // tokens per thread and round
-constexpr int kEmitWords = kEmitThreads * kEmitPer * 48 / 32 + kHuffHdrWords + 8;
+constexpr int kEmitPer = 16;                                   // positions per thread and round
+// a round's bits: at most 16 per position (a literal's 15, a match's 48 over
+// >= 3 positions; one match may run past the round), the carry and the header
+constexpr int kEmitWords = kEmitThreads * kEmitPer * 16 / 32 + kHuffHdrWords + 8;
+static_assert(kEmitThreads * kEmitPer == 4096, "a round is one unit of k_deflate's token layout");
 
-// One workgroup per slice: rounds of kEmitThreads * kEmitPer tokens, each
-// thread's 8 tokens read as two 16-byte loads issued one round ahead (the
-// token buffer has room past every slice's last token: loads start below the
-// token count and run at most 7 tokens beyond it), their codes looked up, a
-// block scan of the bit counts, every code ored into LDS words at its offset,
-// flush of the whole words.
+// One workgroup per slice: rounds of 4096 positions (one unit of the slot
+// layout k_deflate writes), thread e's 16 slots (positions 16e..) read as two
+// contiguous-lane 16-byte loads issued one round ahead, each slot's code
+// looked up (a literal's, a match length's with its extra bits, or in the slot
+// after a match start the distance's with its extra bits), a block scan of the
+// bit counts, every code ored into LDS at its offset, flush of the whole
+// words.  Slot npos of the segment's last slice is the end-of-block code.
 __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
 {
     constexpr int NT = kEmitThreads, NW = NT / 64;
-    __shared__ uint32_t T[kHuffTab];
-    __shared__ uint32_t obuf[kEmitWords];
+    __shared__ uint32_t T[kHuffTab + 1];                      // + kTokNone's zero entry
+    __shared__ __attribute__((aligned(8))) uint32_t obuf[kEmitWords];
     __shared__ uint32_t red[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < kEmitWords; i += NT) obuf[i] = 0;
+    if (tid == 0) T[kTokNone] = 0;
     for (int sl = blockIdx.x; sl < H.nslice; sl += gridDim.x) {
         const int seg = sl / H.spb;
         const bool first = sl % H.spb == 0;
         const bool lastsl = sl % H.spb == H.spb - 1 || sl == H.nslice - 1;
-        const uint32_t nreal = H.tok_count[sl];
-        const uint32_t ntok = nreal + (lastsl ? 1u : 0u);   // + the end-of-block code
-        const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes;
-        u32x4 nx[kEmitPer / 4];
+        const uint32_t npos = H.tok_count[sl];
+        const uint32_t ntok = npos + (lastsl ? 1u : 0u);     // + the end-of-block code
+        const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes / 2;
+        u32x4 nx[2];
         auto fetch = [&](uint32_t r) {
-            const uint32_t t0 = r * NT * kEmitPer + tid * kEmitPer;
+            const bool any = r * NT * kEmitPer + (uint32_t)tid * kEmitPer < npos;
 #pragma unroll
-            for (int q = 0; q < kEmitPer / 4; q++)
-                nx[q] = t0 + 4 * q < nreal ? *(const u32x4*)(tk + t0 + 4 * q) : u32x4{0u, 0u, 0u, 0u};
+            for (int q = 0; q < 2; q++)
+                nx[q] = any ? *(const u32x4*)(tk + r * NT * kEmitPer / 2 + 4 * (tid + NT * q))
+                            : u32x4{kTokNone * 0x10001u, kTokNone * 0x10001u, kTokNone * 0x10001u, kTokNone * 0x10001u};
         };
         fetch(0);
         __syncthreads();                                       // the previous slice is done with T
@@ -2315,22 +2329,27 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
         const uint32_t rounds = max(1u, (ntok + NT * kEmitPer - 1) / (NT * kEmitPer));
         for (uint32_t r = 0; r < rounds; r++) {
             const uint32_t t0 = r * NT * kEmitPer + tid * kEmitPer;
-            uint32_t cur[kEmitPer];
+            uint32_t v[kEmitPer];
 #pragma unroll
-            for (int q = 0; q < kEmitPer / 4; q++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) cur[4 * q + j] = nx[q][j];
+            for (int i = 0; i < kEmitPer; i++) v[i] = (nx[i >> 3][(i >> 1) & 3] >> (16 * (i & 1))) & 0xFFFFu;
             if (r + 1 < rounds) fetch(r + 1);
-            uint32_t a[kEmitPer], b[kEmitPer], nn[kEmitPer];
+            if (t0 + kEmitPer > npos && t0 <= npos) {         // the lane holding the slice's end
+#pragma unroll
+                for (int i = 0; i < kEmitPer; i++)
+                    if (t0 + i >= npos) v[i] = t0 + i == npos && lastsl ? 256u : kTokNone;
+            }
+            uint32_t a[kEmitPer], n[kEmitPer];
             uint32_t nbits = 0;
 #pragma unroll
             for (int i = 0; i < kEmitPer; i++) {
-                const uint32_t t = t0 + i;
-                uint32_t na = 0, nb = 0;
-                a[i] = b[i] = 0;
-                if (t < ntok) token_code(T, t < nreal ? cur[i] : 256u, a[i], na, b[i], nb);
-                nn[i] = na | nb << 8;
-                nbits += na + nb;
+                const bool isd = i > 0 && (v[i - 1] & 0x8000u);   // a distance - 1
+                const uint32_t e = 30u - __builtin_clz(v[i] | 2u);
+                const uint32_t idx = isd ? 286u + (e << 1) + (v[i] >> e)
+                                         : (v[i] & 0x8000u) ? (uint32_t)kHuffSyms + (v[i] & 0xFFu) : v[i];
+                const uint32_t c = T[idx], cn = c >> 24;
+                a[i] = (c & 0xFFFFFFu) | (isd ? (v[i] & ((1u << e) - 1u)) << cn : 0u);
+                n[i] = cn + (isd ? e : 0u);
+                nbits += n[i];
             }
             const uint32_t incl = wave_incl_scan(nbits);
             if (lane == 63) red[wave] = incl;
@@ -2340,11 +2359,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
             uint32_t off = carry_bits + wbase + incl - nbits;
 #pragma unroll
             for (int i = 0; i < kEmitPer; i++) {
-                const uint32_t na = nn[i] & 0xFFu, nb = nn[i] >> 8;
-                if (na) put_bits(obuf, a[i], na, off);
-                off += na;
-                if (nb) put_bits(obuf, b[i], nb, off);
-                off += nb;
+                if (n[i]) put_bits64(obuf, a[i], n[i], off);
+                off += n[i];
             }
             __syncthreads();
             const uint32_t tb = carry_bits + tot;
