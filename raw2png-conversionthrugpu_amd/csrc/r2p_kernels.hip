@@ -1908,24 +1908,8 @@ __device__ __forceinline__ uint32_t shannon_key(uint32_t f, uint32_t T, int s)
 }
 __device__ __forceinline__ uint32_t key_len(uint32_t k) { return 15u - ((k >> 9) & 15u); }
 
-// Number of keys in key[b..e) greater than ks (e - b a multiple of 24, b of
-// 4), read 8 at a time (every lane reads the same words: broadcasts), three
-// steps unrolled so that the loads are in flight together.
-__device__ __forceinline__ uint32_t keys_above(const uint32_t* key, int b, int e, uint32_t ks)
-{
-    uint32_t r = 0;
-    for (int t0 = b; t0 < e; t0 += 24) {
-        u32x4 k4[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) k4[q] = *(const u32x4*)(key + t0 + 4 * q);
-#pragma unroll
-        for (int j = 0; j < 24; j++) r += k4[j >> 2][j & 3] > ks;
-    }
-    return r;
-}
-
 constexpr int kHuffChunks = 5;           // 64-symbol chunks of the largest alphabet (286)
-constexpr int kHuffPhases = 3;           // shortening passes of the length construction
+constexpr int kHuffPhases = 2;           // shortening passes of the length construction
 
 // The length construction's second half, by one wave, on the symbols in
 // registers in the order of their keys: lane p of chunk c holds the length
@@ -1979,7 +1963,11 @@ __device__ __forceinline__ void walk_lengths(uint32_t (&L)[NC], uint32_t used)
             el[c] = moved && L[c] > 1;
         }
     }
-    for (int l = LIMIT; l > 1 && S; l--) {
+    uint32_t lmax = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) lmax = max(lmax, L[c]);
+    lmax = S ? wave_max(lmax) : 0u;
+    for (int l = (int)lmax; l > 1 && S; l--) {
         const uint32_t want = S >> (LIMIT - l);
         uint32_t before = 0;
 #pragma unroll
@@ -2068,9 +2056,9 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     constexpr int NW = kHuffThreads / 64;
     __shared__ uint32_t freq[kHuffSyms];
     __shared__ uint8_t len[kHuffSyms];
-    __shared__ __attribute__((aligned(16))) uint32_t key[288];
     __shared__ uint32_t slot[288];
-    __shared__ uint32_t red[16], first[16], cnt[kHuffChunks * 16], rank4[32];
+    __shared__ uint32_t red[16], first[16], cnt[kHuffChunks * 16];
+    __shared__ __attribute__((aligned(16))) uint32_t bcnt[256], bstart[256];
     __shared__ uint32_t cfreq[19], table[kHuffSyms];
     __shared__ uint32_t hdr[kHuffHdrWords];
     __shared__ uint32_t s_hlit, s_hdist, s_dynamic;
@@ -2078,24 +2066,34 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     const int seg = blockIdx.x;
     const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
     const bool bfinal = seg == H.nseg - 1 && H.final_block;
-    // the segment's histogram: its slices' summed, 8 loads in flight
-    for (int s = tid; s < kHuffSyms; s += kHuffThreads) {
-        const uint32_t* hp = H.hist + (int64_t)k0 * kHuffSyms + s;
-        uint32_t f = 0;
+    // the segment's histogram: its slices' summed, for symbols tid and
+    // tid + 256 together, 16 loads in flight
+    static_assert(kHuffSyms <= 2 * kHuffThreads, "two symbols per thread");
+    {
+        const uint32_t* hp = H.hist + (int64_t)k0 * kHuffSyms + tid;
+        const bool two = tid + kHuffThreads < kHuffSyms;
+        uint32_t f0 = 0, f1 = 0;
         int k = 0;
         for (; k + 8 <= k1 - k0; k += 8) {
-            uint32_t v[8];
+            uint32_t v[8], w2[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = hp[(int64_t)(k + u) * kHuffSyms];
+            for (int u = 0; u < 8; u++) {
+                v[u] = hp[(int64_t)(k + u) * kHuffSyms];
+                w2[u] = two ? hp[(int64_t)(k + u) * kHuffSyms + kHuffThreads] : 0u;
+            }
 #pragma unroll
-            for (int u = 0; u < 8; u++) f += v[u];
+            for (int u = 0; u < 8; u++) { f0 += v[u]; f1 += w2[u]; }
         }
-        for (; k < k1 - k0; k++) f += hp[(int64_t)k * kHuffSyms];
-        freq[s] = f + (s == 256 ? 1u : 0u);                  // the end-of-block code
+        for (; k < k1 - k0; k++) {
+            f0 += hp[(int64_t)k * kHuffSyms];
+            if (two) f1 += hp[(int64_t)k * kHuffSyms + kHuffThreads];
+        }
+        freq[tid] = f0;
+        if (two) freq[tid + kHuffThreads] = f1 + (tid + kHuffThreads == 256 ? 1u : 0u);   // the end-of-block code
     }
+    for (int i = tid; i < 256; i += kHuffThreads) bcnt[i] = 0;
     for (int i = tid; i < 19; i += kHuffThreads) cfreq[i] = 0;
     for (int i = tid; i < kHuffHdrWords; i += kHuffThreads) hdr[i] = 0;
-    if (tid < 32) rank4[tid] = 0;
     __syncthreads();
     // literal/length code: counts, keys and ranks by the workgroup (at least
     // two symbols: the end-of-block code always has one); T as in
@@ -2114,22 +2112,44 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
         __syncthreads();
     }
     const uint32_t T = (uint32_t)(((uint64_t)total * 32768u + (32768u - used - 1)) / (32768u - used));
-    for (int s = tid; s < 288; s += kHuffThreads) {
-        const uint32_t f = s < 286 ? freq[s] : 0u;
-        key[s] = f ? shannon_key<15>(f, T, s) : 0u;
-        if (s < 286) len[s] = 0;
+    // The order of the keys by a bucket sort: 256 buckets by f << L
+    // (< 2T), descending; a bucket's start from a scan of the counts; a
+    // symbol's rank is its bucket's start plus the keys of its bucket above
+    // its own.  Symbols tid and tid + 256.
+    uint32_t kk[2] = {0, 0}, bk[2] = {0, 0}, bi[2] = {0, 0};
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int s = tid + kHuffThreads * i;
+        if (s >= 286) continue;
+        const uint32_t f = freq[s];
+        len[s] = 0;
+        if (!f) continue;
+        kk[i] = shannon_key<15>(f, T, s);
+        bk[i] = 255u - min(255u, ((kk[i] >> 13) * 128u) / T);
+        bi[i] = atomicAdd(&bcnt[bk[i]], 1u);
     }
     __syncthreads();
-    {   // symbol tid against all keys; symbols 256..285 against a quarter each
-        const uint32_t k = key[tid];
-        if (k) slot[keys_above(key, 0, 288, k)] = (uint32_t)tid | key_len(k) << 16;
-        const uint32_t k4 = lane < 30 ? key[256 + lane] : 0u;
-        if (k4) atomicAdd(&rank4[lane], keys_above(key, 72 * wave, 72 * wave + 72, k4));
+    if (wave == 0) {
+        const uint4 c = *(const uint4*)&bcnt[4 * lane];
+        const uint32_t t4 = c.x + c.y + c.z + c.w, ex = wave_incl_scan(t4) - t4;
+        *(uint4*)&bstart[4 * lane] = uint4{ex, ex + c.x, ex + c.x + c.y, ex + c.x + c.y + c.z};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+        if (kk[i]) table[bstart[bk[i]] + bi[i]] = kk[i];        // (table is free until the codes)
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        if (!kk[i]) continue;
+        const uint32_t b0 = bstart[bk[i]], nb = bcnt[bk[i]];
+        uint32_t r = b0;
+        for (uint32_t j = b0; j < b0 + nb; j++) r += table[j] > kk[i];
+        slot[r] = (uint32_t)(tid + kHuffThreads * i) | key_len(kk[i]) << 16;
     }
     __syncthreads();
     if (wave == 0) {
         // the literal/length lengths: walk in rank order, lengths out, HLIT
-        if (lane < 30 && key[256 + lane]) slot[rank4[lane]] = (256u + lane) | key_len(key[256 + lane]) << 16;
         uint32_t sy[kHuffChunks], L[kHuffChunks];
 #pragma unroll
         for (int c = 0; c < kHuffChunks; c++) {
