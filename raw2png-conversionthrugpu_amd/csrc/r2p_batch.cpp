@@ -66,12 +66,23 @@ struct r2p_batch {
     void* d_tab = nullptr;
     void* h_tab = nullptr;
     size_t tab_bytes = 0;
-    FilterJob* d_fj = nullptr; DeflateJob* d_dj = nullptr; AssembleJob* d_aj = nullptr;
+    FilterJob* d_fj = nullptr; DeflateJob* d_dj = nullptr; AssembleJob* d_aj = nullptr; HuffJob* d_hj = nullptr;
     uint32_t* d_band_start = nullptr; uint32_t* d_slice_start = nullptr; uint32_t* d_chunk_start = nullptr;
-    uint32_t* d_counts = nullptr;      // [0..1] filter, [2..3] deflate, [4..5] assemble
+    uint32_t* d_seg_start = nullptr;
+    uint32_t* d_counts = nullptr;      // launch sizes: per launch [jobs, total, first] (see r2p_batch_set)
+    // dynamic-Huffman blocks (allocated while the options ask for them):
+    // token slots, counts and histograms per slice, codes and block headers
+    // per segment, and the slice arena at the larger per-slice capacity
+    uint32_t* d_tokens = nullptr;
+    uint32_t* d_tok_count = nullptr;
+    uint32_t* d_hist = nullptr;
+    uint32_t* d_codes = nullptr;
+    uint32_t* d_hdr = nullptr;
+    uint32_t* d_hdr_bits = nullptr;
+    size_t dyn_tok_dw = 0, dyn_segs = 0, dyn_slices = 0;
     uint64_t* d_res = nullptr;         // per image: png length, adler
     uint64_t* h_res = nullptr;         // pinned
-    size_t off_fj, off_dj, off_aj, off_bs, off_ss, off_cs, off_cnt;
+    size_t off_fj, off_dj, off_aj, off_hj, off_bs, off_ss, off_cs, off_gs, off_cnt;
     // current batch
     int n = 0;
     std::vector<size_t> out_off;
@@ -83,7 +94,8 @@ struct r2p_batch {
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1, g_wg = -1, g_row_cap_w = -1, g_band_w = -1;
+    int g_row_cap = -1, g_lut_cap = -1, g_band = -1, g_lane = -1, g_wg = -1, g_row_cap_w = -1, g_band_w = -1,
+        g_huff = -1;
     bool ran = false;
     bool timed_kernels = false;
     std::vector<r2p_image> shadow_img;
@@ -96,6 +108,51 @@ struct r2p_batch {
 
 // Size classes of the autotuner: filtered streams under 2 MiB, under 8 MiB, larger.
 static int size_class(size_t filtered) { return filtered < ((size_t)2 << 20) ? 0 : filtered < ((size_t)8 << 20) ? 1 : 2; }
+
+static void free_dynamic(r2p_batch* b)
+{
+    (void)hipFree(b->d_tokens); (void)hipFree(b->d_tok_count); (void)hipFree(b->d_hist);
+    (void)hipFree(b->d_codes); (void)hipFree(b->d_hdr); (void)hipFree(b->d_hdr_bits);
+    b->d_tokens = b->d_tok_count = b->d_hist = b->d_codes = b->d_hdr = b->d_hdr_bits = nullptr;
+    b->dyn_tok_dw = b->dyn_segs = b->dyn_slices = 0;
+}
+
+// The dynamic-Huffman buffers for the current options (none without them):
+// per slice up to slice_bytes 16-bit token slots (a job's slices cover its
+// stream rounded up to a slice: filtered + 64 KiB per image bounds them), a
+// count and a histogram; per segment (of the options' size) the codes and
+// the header; and a slice arena at slice_cap_for(.., 1) dwords per slice.
+static int ensure_dynamic(r2p_batch* b)
+{
+    if (!b->opt.huffman) return R2P_OK;
+    const size_t n = (size_t)b->max_images;
+    const size_t tok_dw = (b->max_filtered + n * 65536) / 2 + 1024;
+    const size_t segs = b->max_filtered / (size_t)b->opt.segment_bytes + n + 1;
+    const size_t slices = b->max_slices;
+    const size_t arena = (b->max_filtered + n * 65536) / 2 + (size_t)(slice_cap_for(4096, 1) - 2048) * slices + 1024;
+    if (b->d_tokens && b->dyn_tok_dw >= tok_dw && b->dyn_segs >= segs && b->dyn_slices >= slices &&
+        b->slice_arena_dw >= arena)
+        return R2P_OK;
+    free_dynamic(b);
+    hipError_t e;
+#define TRY(x) do { if ((e = (x)) != hipSuccess) { free_dynamic(b); return hip_fail(e, #x); } } while (0)
+    TRY(hipMalloc((void**)&b->d_tokens, tok_dw * 4));
+    TRY(hipMalloc((void**)&b->d_tok_count, slices * 4));
+    TRY(hipMalloc((void**)&b->d_hist, slices * kHuffSyms * 4));
+    TRY(hipMalloc((void**)&b->d_codes, segs * kHuffTab * 4));
+    TRY(hipMalloc((void**)&b->d_hdr, segs * kHuffHdrWords * 4));
+    TRY(hipMalloc((void**)&b->d_hdr_bits, segs * 4));
+    if (b->slice_arena_dw < arena) {
+        (void)hipFree(b->d_slice_out);
+        b->d_slice_out = nullptr;
+        b->slice_arena_dw = 0;
+        TRY(hipMalloc((void**)&b->d_slice_out, arena * 4));
+        b->slice_arena_dw = arena;
+    }
+#undef TRY
+    b->dyn_tok_dw = tok_dw; b->dyn_segs = segs; b->dyn_slices = slices;
+    return R2P_OK;
+}
 
 // Enqueues one batch run.  Per-kernel events are recorded only for eager
 // runs (timed); a captured graph is timed as a whole around its launch.
@@ -118,8 +175,14 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
     // 3. deflate over all slices
     DeflateLaunch DL{};
-    DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 6;
+    DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 6; DL.dyn = b->opt.huffman;
     R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, b->opt.wg_threads, 1024, s));
+    if (b->opt.huffman) {                 // every segment's codes, then every slice's bits
+        HuffLaunch HL{};
+        HL.jobs = b->d_hj; HL.seg_start = b->d_seg_start; HL.slice_start = b->d_slice_start;
+        HL.counts = b->d_counts + 13;
+        R2P_HIP(launch_huffman(HL, 2048, 2048, s));
+    }
     if (timed) R2P_HIP(hipEventRecord(b->ev[2], s));
     // 4. per-image scan, 5. assemble all chunks
     AssembleLaunch AL{};
@@ -146,7 +209,6 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     r2p_batch* b = new r2p_batch();
     if (opt) b->opt = *opt; else r2p_options_default(&b->opt);
     fill_defaults(&b->opt);
-    if (b->opt.huffman) { delete b; set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
     b->max_images = max_images;
     b->max_input = max_input_bytes;
     b->max_filtered = max_filtered_bytes;
@@ -160,10 +222,12 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     b->off_fj = 0;
     b->off_dj = b->off_fj + sizeof(FilterJob) * (size_t)max_images;
     b->off_aj = b->off_dj + sizeof(DeflateJob) * (size_t)max_images;
-    b->off_bs = b->off_aj + sizeof(AssembleJob) * (size_t)max_images;
+    b->off_hj = b->off_aj + sizeof(AssembleJob) * (size_t)max_images;
+    b->off_bs = b->off_hj + sizeof(HuffJob) * (size_t)max_images;
     b->off_ss = b->off_bs + 4 * ((size_t)max_images + 1);
     b->off_cs = b->off_ss + 4 * ((size_t)max_images + 1);
-    b->off_cnt = b->off_cs + 4 * ((size_t)max_images + 1);
+    b->off_gs = b->off_cs + 4 * ((size_t)max_images + 1);
+    b->off_cnt = b->off_gs + 4 * ((size_t)max_images + 1);
     b->tab_bytes = b->off_cnt + 64;
     hipError_t e;
 #define TRY(x) do { if ((e = (x)) != hipSuccess) { r2p_batch_destroy(b); return hip_fail(e, #x); } } while (0)
@@ -188,10 +252,13 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     b->d_fj = (FilterJob*)(dt + b->off_fj);
     b->d_dj = (DeflateJob*)(dt + b->off_dj);
     b->d_aj = (AssembleJob*)(dt + b->off_aj);
+    b->d_hj = (HuffJob*)(dt + b->off_hj);
     b->d_band_start = (uint32_t*)(dt + b->off_bs);
     b->d_slice_start = (uint32_t*)(dt + b->off_ss);
     b->d_chunk_start = (uint32_t*)(dt + b->off_cs);
+    b->d_seg_start = (uint32_t*)(dt + b->off_gs);
     b->d_counts = (uint32_t*)(dt + b->off_cnt);
+    if ((rc = ensure_dynamic(b))) { r2p_batch_destroy(b); return rc; }
     *out = b;
     return R2P_OK;
 }
@@ -206,6 +273,7 @@ int r2p_batch_destroy(r2p_batch* b)
     (void)hipFree(b->d_slice_adler); (void)hipFree(b->d_slice_boff); (void)hipFree(b->d_seg_tbits);
     (void)hipFree(b->d_seg_off); (void)hipFree(b->d_out);
     (void)hipFree(b->d_lut); (void)hipFree(b->d_tab); (void)hipFree(b->d_res);
+    free_dynamic(b);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
     if (b->h_res) (void)hipHostFree(b->h_res);
     for (auto& v : b->ev) if (v) (void)hipEventDestroy(v);
@@ -219,10 +287,10 @@ int r2p_batch_options(r2p_batch* b, const r2p_options* opt)
     if (!b || !opt) { set_error("bad options"); return R2P_ERR_ARG; }
     r2p_options o = *opt;
     fill_defaults(&o);
-    if (o.huffman) { set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
+    R2P_HIP(hipStreamSynchronize(b->stream));   // a run in flight may use the buffers
     b->opt = o;
     b->class_slice[0] = b->class_slice[1] = b->class_slice[2] = 0;
-    return R2P_OK;
+    return ensure_dynamic(b);
 }
 
 int r2p_batch_job_options(r2p_batch* b, int i, r2p_options* opt)
@@ -243,6 +311,8 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     FilterJob* fj = (FilterJob*)(ht + b->off_fj);
     DeflateJob* dj = (DeflateJob*)(ht + b->off_dj);
     AssembleJob* aj = (AssembleJob*)(ht + b->off_aj);
+    HuffJob* hj = (HuffJob*)(ht + b->off_hj);
+    uint32_t* gs = (uint32_t*)(ht + b->off_gs);
     uint32_t* bs = (uint32_t*)(ht + b->off_bs);
     uint32_t* ss = (uint32_t*)(ht + b->off_ss);
     uint32_t* cs = (uint32_t*)(ht + b->off_cs);
@@ -292,7 +362,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     // device tables, deduplicated by parameter set
     std::map<LutKey, int> lut_slot;
     b->luts.clear();
-    size_t foff = 0, segs = 0, slices = 0, ooff = 0, slice_dw = 0;
+    size_t foff = 0, segs = 0, slices = 0, ooff = 0, slice_dw = 0, tok_dw = 0;
     uint32_t chunks = 0;
     b->out_off.assign((size_t)n, 0);
     b->job_slice.assign((size_t)n, 0);
@@ -326,11 +396,12 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         int sl = b->class_slice[size_class(g.filtered)];
         if (sl <= 0 || o.segment_bytes % sl) sl = resolve_slice_bytes(o, g.filtered);
         b->job_slice[(size_t)i] = sl;
-        const int cap_dw = slice_cap_dwords(sl);
+        const int cap_dw = slice_cap_for(sl, o.huffman);
         const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, sl);
         const int nseg = sg.nseg;
         if (slice_dw + (size_t)sg.nslice * (size_t)cap_dw > b->slice_arena_dw || slices + sg.nslice > b->max_slices ||
-            segs + nseg > b->max_segs) {
+            segs + nseg > b->max_segs ||
+            (o.huffman && (tok_dw + (size_t)sg.nslice * (size_t)sl / 2 > b->dyn_tok_dw || segs + nseg > b->dyn_segs))) {
             set_error("slice arena exhausted"); return R2P_ERR_SIZE;
         }
         DeflateJob& D = dj[i];
@@ -341,6 +412,19 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         D.slice_bytes = sl; D.slice_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp;
         D.row_stride = (int32_t)(g.row_bytes + 1); D.lz77 = o.lz77; D.nslice = sg.nslice;
         ss[i] = (uint32_t)slices;
+        gs[i] = (uint32_t)segs;
+        if (o.huffman) {
+            D.tokens = b->d_tokens + tok_dw; D.tok_count = b->d_tok_count + slices;
+            D.hist = b->d_hist + slices * kHuffSyms;
+            HuffJob& H = hj[i];
+            H = HuffJob{};
+            H.tokens = D.tokens; H.tok_count = D.tok_count; H.hist = D.hist;
+            H.codes = b->d_codes + segs * kHuffTab; H.hdr = b->d_hdr + segs * kHuffHdrWords;
+            H.hdr_bits = b->d_hdr_bits + segs; H.slice_out = D.slice_out; H.slice_bits = D.slice_bits;
+            H.slice_cap_dw = cap_dw; H.slice_bytes = sl; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg;
+            H.final_block = 1;
+            tok_dw += (size_t)sg.nslice * (size_t)sl / 2;
+        }
 
         const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
         if (ooff + bound > b->max_out) { set_error("output arena exhausted"); return R2P_ERR_SIZE; }
@@ -352,7 +436,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         A.out_len = b->d_res + 2 * (size_t)i; A.adler_out = (uint32_t*)(b->d_res + 2 * (size_t)i + 1);
         A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice;
         A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = sl;
-        A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1;
+        A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
         A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
         A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
         A.color_type = (uint8_t)g.color_type;
@@ -379,12 +463,13 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         if (c <= 1) bands_w = bands;
     }
     bands_w = std::max(bands_w, bands_n);
-    bs[n] = bands; ss[n] = (uint32_t)slices; cs[n] = chunks;
+    bs[n] = bands; ss[n] = (uint32_t)slices; cs[n] = chunks; gs[n] = (uint32_t)segs;
     cnt[0] = (uint32_t)n; cnt[1] = bands_n; cnt[2] = 0;            // filter, narrow rows
     cnt[3] = (uint32_t)n; cnt[4] = bands_w; cnt[5] = bands_n;       // filter, wide rows
     cnt[10] = (uint32_t)n; cnt[11] = bands; cnt[12] = bands_w;      // filter, tiled rows
     cnt[6] = (uint32_t)n; cnt[7] = (uint32_t)slices;
     cnt[8] = (uint32_t)n; cnt[9] = chunks;
+    cnt[13] = (uint32_t)n; cnt[14] = (uint32_t)segs; cnt[15] = (uint32_t)slices;   // dynamic-Huffman tables, emit
     b->n = n;
     // build the distinct tables now (outside the graph)
     for (auto& kv : b->luts) {
@@ -415,7 +500,7 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
     const int lane = b->opt.lane_bytes;
     if (!b->gexec || b->g_row_cap != b->row_cap || b->g_lut_cap != b->lut_cap ||
         b->g_band != b->band_rows || b->g_lane != lane || b->g_wg != b->opt.wg_threads ||
-        b->g_row_cap_w != b->row_cap_w || b->g_band_w != b->band_rows_w) {
+        b->g_row_cap_w != b->row_cap_w || b->g_band_w != b->band_rows_w || b->g_huff != b->opt.huffman) {
         if (b->gexec) { R2P_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
         if (b->graph) { R2P_HIP(hipGraphDestroy(b->graph)); b->graph = nullptr; }
         R2P_HIP(hipStreamSynchronize(s));
@@ -428,7 +513,7 @@ int r2p_batch_run(r2p_batch* b, int use_graph)
         b->graph = g;
         R2P_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
         b->g_row_cap = b->row_cap; b->g_lut_cap = b->lut_cap; b->g_band = b->band_rows; b->g_lane = lane; b->g_wg = b->opt.wg_threads;
-        b->g_row_cap_w = b->row_cap_w; b->g_band_w = b->band_rows_w;
+        b->g_row_cap_w = b->row_cap_w; b->g_band_w = b->band_rows_w; b->g_huff = b->opt.huffman;
     }
     R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(hipGraphLaunch(b->gexec, s));
@@ -815,7 +900,6 @@ int r2p_server_create(const r2p_server_config* cfg, r2p_server** out)
     s->cfg = *cfg;
     s->dev = dev;
     fill_defaults(&s->cfg.opt);
-    if (s->cfg.opt.huffman) { delete s; set_error("dynamic Huffman blocks: plain encoder only"); return R2P_ERR_UNSUPPORTED; }
     const int nl = cfg->inflight_batches;
     const size_t lane_budget = cfg->hbm_budget_bytes / (size_t)nl;
     // per lane: input arena 1/4, filtered 1/4 of what remains after the input

@@ -694,16 +694,19 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     D.slice_cap_dw = w.slice_cap_dw;
     D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
     D.lz77 = o.lz77; D.nslice = sg.nslice;
-    if (o.huffman) { DL.tokens = w.tokens; DL.tok_count = w.tok_count; DL.hist = w.hist; }
+    if (o.huffman) { D.tokens = w.tokens; D.tok_count = w.tok_count; D.hist = w.hist; }
+    DL.dyn = o.huffman;
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
     if (o.huffman) {                       // the segments' codes, then the slices' bits
-        HuffJob H{};
+        HuffLaunch HL{};
+        HuffJob& H = HL.single;
         H.tokens = w.tokens; H.tok_count = w.tok_count; H.hist = w.hist;
         H.codes = w.codes; H.hdr = w.hdr; H.hdr_bits = w.hdr_bits;
         H.slice_out = w.slice_out; H.slice_bits = w.slice_bits; H.slice_cap_dw = w.slice_cap_dw;
         H.slice_bytes = o.slice_bytes; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg; H.final_block = 1;
-        R2P_HIP(launch_huffman(H, s));
+        HL.jobs = nullptr; HL.njobs = 1; HL.total_segs = (uint32_t)nseg; HL.total_slices = (uint32_t)sg.nslice;
+        R2P_HIP(launch_huffman(HL, nseg, std::min(sg.nslice, 8192), s));
     }
     if (w.ev) R2P_HIP(hipEventRecord(w.ev[2], s));
 

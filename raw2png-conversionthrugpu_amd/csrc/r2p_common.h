@@ -33,7 +33,7 @@ hipError_t launch_filter_tiled(const FilterLaunch& L, int grid, size_t lds, hipS
 hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_threads, int grid, hipStream_t s);
 hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s);
 // Dynamic-Huffman tables per segment, then the slices' bits.
-hipError_t launch_huffman(const HuffJob& H, hipStream_t s);
+hipError_t launch_huffman(const HuffLaunch& HL, int grid_tables, int grid_emit, hipStream_t s);
 hipError_t launch_assemble(const AssembleLaunch& AL, int grid, hipStream_t s);
 // A single job small enough for k_assemble to scan it itself (no k_scan
 // launch; the scan arrays stay in LDS): launch_assemble_self_scan.

@@ -76,6 +76,16 @@ struct DeflateJob {
     int32_t        row_stride; // filtered row size (match candidate one row up), 0 = none
     int32_t        lz77;
     int32_t        nslice;     // max(1, ceil(len / slice_bytes))
+    // dynamic-Huffman blocks (k_deflate<.., true>): instead of bits, each
+    // slice writes a 16-bit slot per position (slice_bytes u16 per slice: a
+    // literal, 0x8000 | length - 3 where a match starts and its distance - 1
+    // in the next slot, kTokNone elsewhere; in units of 4096 positions, the
+    // slot pair at positions 16l + 8q + 2j, + 1 in dword 4(l + 256q) + j),
+    // the slice's length in tok_count, and a histogram of kHuffSyms u32
+    // (literal/length symbols 0..285, then distance codes 0..29)
+    uint32_t*      tokens;
+    uint32_t*      tok_count;
+    uint32_t*      hist;
 };
 
 // Where a job's segments go once their sizes are known.
@@ -131,16 +141,7 @@ struct DeflateLaunch {
     uint32_t          total_slices;
     const uint32_t*   counts;      // optional device [njobs, total]
     DeflateJob        single;
-    // dynamic-Huffman mode (single job; k_deflate<.., true>): instead of
-    // bits, each slice writes a 16-bit slot per position (slice_bytes u16 per
-    // slice: a literal, 0x8000 | length - 3 where a match starts and its
-    // distance - 1 in the next slot, kTokNone elsewhere; in units of 4096
-    // positions, the slot pair at positions 16l + 8q + 2j, + 1 in dword
-    // 4(l + 256q) + j), the slice's length in tok_count, and a histogram of
-    // kHuffSyms u32 (literal/length symbols 0..285, then distance codes 0..29)
-    uint32_t*         tokens;
-    uint32_t*         tok_count;
-    uint32_t*         hist;
+    int32_t           dyn;         // the jobs' dynamic-Huffman form (k_deflate<.., true>)
 };
 
 // Dynamic-Huffman blocks of one job (after k_deflate<.., true>): per
@@ -160,6 +161,18 @@ struct HuffJob {
     int32_t         nslice;
     int32_t         nseg;
     int32_t         final_block;
+};
+
+// The dynamic-Huffman launches over one job or a table of them: workgroups
+// take global segments (k_huff_tables) or slices (k_huff_emit) in order.
+struct HuffLaunch {
+    const HuffJob*  jobs;        // null: the single job
+    const uint32_t* seg_start;   // prefix of segment counts per job (njobs + 1)
+    const uint32_t* slice_start; // prefix of slice counts per job (njobs + 1)
+    const uint32_t* counts;      // optional device [njobs, total segments, total slices]
+    int32_t         njobs;
+    uint32_t        total_segs, total_slices;
+    HuffJob         single;
 };
 
 struct AssembleLaunch {

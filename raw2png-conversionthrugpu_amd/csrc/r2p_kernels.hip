@@ -1764,7 +1764,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         d[k] = lo | hi << 16;
                     }
                     const int p0 = lb - s0_rel;
-                    uint32_t* tp = DL.tokens + ((int64_t)sl * J.slice_bytes + (p0 & ~4095)) / 2 + 4 * ((p0 >> 4) & 255);
+                    uint32_t* tp = J.tokens + ((int64_t)sl * J.slice_bytes + (p0 & ~4095)) / 2 + 4 * ((p0 >> 4) & 255);
 #pragma unroll
                     for (int q = 0; q < 2; q++)
                         *(u32x4*)(tp + 1024 * q) = u32x4{d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]};
@@ -1816,10 +1816,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         if constexpr (DYN) {
             for (int i = tid; i < kHuffSyms; i += NT) {    // the histogram out, obuf zero again
-                DL.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
+                J.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
                 hist[i] = 0;
             }
-            if (tid == 0) DL.tok_count[sl] = (uint32_t)(s1 - s0);
+            if (tid == 0) J.tok_count[sl] = (uint32_t)(s1 - s0);
         }
         if (!DYN && tid == 0) {
             if (carry_bits) out[out_dw] = carry;
@@ -2045,13 +2045,26 @@ __device__ uint32_t wave_codes(uint32_t l)
     return l ? (__builtin_bitreverse32(c) >> (32 - l)) | (l << 16) : 0u;
 }
 
+// The job of global item g (a segment or slice) and g's index inside it:
+// the last job whose start is <= g.
+__device__ __forceinline__ int huff_job_of(const HuffLaunch& HL, const uint32_t* start, uint32_t g, int njobs)
+{
+    if (!HL.jobs) return 0;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Fixed-code length of literal/length symbol s (RFC 1951 3.2.6).
 __device__ __forceinline__ uint32_t fixed_len(int s) { return s < 144 ? 8u : s < 256 ? 9u : s < 280 ? 7u : 8u; }
 
 // The order in which a dynamic header sends the code-length code's lengths (RFC 1951 3.2.7).
 __device__ const uint8_t kClOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
 
-__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
+__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffLaunch HL)
 {
     constexpr int NW = kHuffThreads / 64;
     __shared__ uint32_t freq[kHuffSyms];
@@ -2063,9 +2076,15 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
     __shared__ uint32_t hdr[kHuffHdrWords];
     __shared__ uint32_t s_hlit, s_hdist, s_dynamic;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int seg = blockIdx.x;
+    const uint32_t total = HL.counts ? HL.counts[1] : HL.total_segs;
+    const int njobs = HL.counts ? (int)HL.counts[0] : HL.njobs;
+    for (uint32_t gs = blockIdx.x; gs < total; gs += gridDim.x) {
+    const int j = huff_job_of(HL, HL.seg_start, gs, njobs);
+    const HuffJob H = HL.jobs ? HL.jobs[j] : HL.single;
+    const int seg = (int)(gs - (HL.jobs ? HL.seg_start[j] : 0u));
     const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
     const bool bfinal = seg == H.nseg - 1 && H.final_block;
+    __syncthreads();                                          // the previous segment is done with the LDS
     // the segment's histogram: its slices' summed, for symbols tid and
     // tid + 256 together, 16 loads in flight
     static_assert(kHuffSyms <= 2 * kHuffThreads, "two symbols per thread");
@@ -2293,6 +2312,7 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffJob H)
         const uint32_t L = (uint32_t)k + 3u, lc = table[huff_len_sym(L)], le = huff_len_extra_bits(L), ln = lc >> 16;
         C[kHuffSyms + k] = ((lc & 0xFFFFu) | ((L - 3u) & ((1u << le) - 1u)) << ln) | (ln + le) << 24;
     }
+    }
 }
 
 constexpr int kEmitThreads = 256;
@@ -2309,7 +2329,7 @@ static_assert(kEmitThreads * kEmitPer == 4096, "a round is one unit of k_deflate
 // after a match start the distance's with its extra bits), a block scan of the
 // bit counts, every code ored into LDS at its offset, flush of the whole
 // words.  Slot npos of the segment's last slice is the end-of-block code.
-__global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
+__global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
 {
     constexpr int NT = kEmitThreads, NW = NT / 64;
     __shared__ uint32_t T[kHuffTab + 1];                      // + kTokNone's zero entry
@@ -2318,7 +2338,12 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffJob H)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < kEmitWords; i += NT) obuf[i] = 0;
     if (tid == 0) T[kTokNone] = 0;
-    for (int sl = blockIdx.x; sl < H.nslice; sl += gridDim.x) {
+    const uint32_t total = HL.counts ? HL.counts[2] : HL.total_slices;
+    const int njobs = HL.counts ? (int)HL.counts[0] : HL.njobs;
+    for (uint32_t gsl = blockIdx.x; gsl < total; gsl += gridDim.x) {
+        const int j = huff_job_of(HL, HL.slice_start, gsl, njobs);
+        const HuffJob H = HL.jobs ? HL.jobs[j] : HL.single;
+        const int sl = (int)(gsl - (HL.jobs ? HL.slice_start[j] : 0u));
         const int seg = sl / H.spb;
         const bool first = sl % H.spb == 0;
         const bool lastsl = sl % H.spb == H.spb - 1 || sl == H.nslice - 1;
@@ -3086,7 +3111,7 @@ hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_thread
     (void)lane_bytes;
     (void)hipGetLastError();
     const bool stats = !DL.jobs && DL.single.slice_stats;
-    if (!DL.jobs && DL.tokens) k_deflate<256, false, true><<<grid, 256, 0, s>>>(DL);
+    if (DL.dyn)                 k_deflate<256, false, true><<<grid, 256, 0, s>>>(DL);
     else if (stats)             k_deflate<256, true><<<grid, 256, 0, s>>>(DL);
     else if (wg_threads == 256) k_deflate<256, false><<<grid, 256, 0, s>>>(DL);
     else                        k_deflate<512, false><<<grid, 512, 0, s>>>(DL);
@@ -3100,11 +3125,11 @@ hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_huffman(const HuffJob& H, hipStream_t s)
+hipError_t launch_huffman(const HuffLaunch& HL, int grid_tables, int grid_emit, hipStream_t s)
 {
     (void)hipGetLastError();
-    k_huff_tables<<<H.nseg, kHuffThreads, 0, s>>>(H);
-    k_huff_emit<<<std::min(H.nslice, 8192), kEmitThreads, 0, s>>>(H);
+    k_huff_tables<<<grid_tables, kHuffThreads, 0, s>>>(HL);
+    k_huff_emit<<<grid_emit, kEmitThreads, 0, s>>>(HL);
     return hipGetLastError();
 }
 

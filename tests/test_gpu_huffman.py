@@ -1,12 +1,15 @@
 """Dynamic-Huffman blocks (r2p_options.huffman = 1): every PNG inflates under
 stock zlib to the oracle's filtered bytes, no segment grows past its
-fixed-Huffman size, and the batched and served forms reject the option."""
+fixed-Huffman size, and the batched form gives each image the plain path's
+bytes."""
 import sys
 import zlib
 
 import numpy as np
 import pytest
 
+from pngutil import check_and_inflate
+from test_gpu_batch import plain_png
 from test_gpu_parity import _check_png, encode_device
 
 pytestmark = pytest.mark.gpu
@@ -63,11 +66,67 @@ def test_dynamic_geometry_and_content(torch_dev, r2p, oracle, seg, slc, w, h, co
         assert len(png) <= len(fixed)
 
 
-def test_dynamic_rejected_outside_plain(torch_dev, r2p):
-    log("case dynamic rejected by batch and server")
-    with pytest.raises(r2p.R2PError):
-        r2p.Batch(4, 1 << 20, 1 << 20, r2p.options(huffman=1))
-    b = r2p.Batch(4, 1 << 20, 1 << 20, r2p.options())
-    with pytest.raises(r2p.R2PError):
-        r2p.check(r2p.lib().r2p_batch_options(b.h, r2p.C.byref(r2p.options(huffman=1))))
+def _oracle_filtered(oracle, r2p, s):
+    p = oracle.DicomParams(*[getattr(s.img.dicom, f) for f, _ in s.img.dicom._fields_])
+    flat = s.pixels.reshape(-1)
+    if s.img.pix_mode == r2p.PIX_DICOM8:
+        ready = oracle.window8(flat, p)
+    elif s.img.pix_mode == r2p.PIX_DICOM:
+        ready = oracle.dicom_window(flat, p)
+    else:
+        ready = flat.view(np.uint8)
+    rb = r2p.row_bytes(s.img)
+    return oracle.filter_rows(ready, s.img.height, rb, rb // s.img.width)
+
+
+def _first_btype(png):
+    """BTYPE of the first DEFLATE block of the PNG's zlib stream."""
+    _, z, _ = check_and_inflate(png)
+    return (z[2] >> 1) & 3
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_dynamic_batch_matches_plain(torch_dev, r2p, synth, oracle, graph):
+    """Dynamic blocks in the batched form: each PNG equals the plain path's
+    with the image's own options (its slice size included) and inflates to
+    the oracle's filtered bytes; a batch switches to fixed blocks and back."""
+    torch = torch_dev
+    log(f"case dynamic batch graph={graph}")
+    ims = synth.mixed_batch(16, seed=13, max_side=900, min_side=16)
+    srcs = [torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda() for s in ims]
+    fsum = sum(r2p.filtered_bytes(s.img) for s in ims)
+    isum = sum(r2p.input_bytes(s.img) for s in ims)
+    opt = r2p.options(huffman=1)
+    b = r2p.Batch(32, isum, fsum, opt)
+    b.set([s.img for s in ims], [t.data_ptr() for t in srcs])
+    first = None
+    for rep in range(2):
+        b.run(graph=graph)
+        b.sync()
+        got = [b.copy_result(i) for i in range(len(ims))]
+        if first is None:
+            first = got
+            for i, s in enumerate(ims):
+                jo = b.job_options(i)
+                assert jo.huffman == 1
+                assert got[i] == plain_png(r2p, torch, s.img, srcs[i], jo), f"image {i} differs from the plain path"
+        else:
+            assert got == first
+    assert any(_first_btype(png) == 2 for png in first)
+    for i in (0, 5, 11):
+        s = ims[i]
+        _, _, raw = check_and_inflate(first[i])
+        assert raw == _oracle_filtered(oracle, r2p, s).tobytes()
+    # the same batch with fixed blocks, then dynamic ones again
+    b.set_options(r2p.options())
+    b.set([s.img for s in ims], [t.data_ptr() for t in srcs])
+    b.run(graph=graph)
+    b.sync()
+    for i in (1, 6):
+        assert b.copy_result(i) == plain_png(r2p, torch, ims[i].img, srcs[i], b.job_options(i))
+    b.set_options(opt)
+    b.set([s.img for s in ims], [t.data_ptr() for t in srcs])
+    b.run(graph=graph)
+    b.sync()
+    assert [b.copy_result(i) for i in range(len(ims))] == first
     b.close()

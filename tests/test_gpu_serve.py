@@ -1,5 +1,6 @@
 """Served form: requests admitted into overlapping device batches produce
-the same PNG bytes as each request encoded alone by the plain path."""
+the same PNG bytes as each request encoded alone by the plain path, with
+fixed- and with dynamic-Huffman blocks."""
 import ctypes as C
 import sys
 
@@ -21,21 +22,22 @@ def plain_png(r2p, img, pixels, opt):
     return out
 
 
-def make_server(r2p, budget=1 << 30, max_batch=8, lanes=3):
+def make_server(r2p, budget=1 << 30, max_batch=8, lanes=3, opt=None):
     cfg = r2p.ServerConfig()
     cfg.hbm_budget_bytes = budget
     cfg.max_batch_images = max_batch
     cfg.inflight_batches = lanes
-    cfg.opt = r2p.options()
+    cfg.opt = opt or r2p.options()
     h = C.c_void_p()
     r2p.check(r2p.lib().r2p_server_create(C.byref(cfg), C.byref(h)))
     return h, cfg
 
 
-def test_serve_same_alone_and_in_mix(torch_dev, r2p, synth):
-    log("case serve mix")
+@pytest.mark.parametrize("huffman", [0, 1])
+def test_serve_same_alone_and_in_mix(torch_dev, r2p, synth, huffman):
+    log(f"case serve mix huffman={huffman}")
     ims = synth.mixed_batch(20, seed=21, max_side=600, min_side=32)
-    h, cfg = make_server(r2p)
+    h, cfg = make_server(r2p, opt=r2p.options(huffman=huffman))
     L = r2p.lib()
     pins = [np.ascontiguousarray(s.pixels) for s in ims]
     ids = []
