@@ -119,9 +119,10 @@ static void free_dynamic(r2p_batch* b)
 
 // The dynamic-Huffman buffers for the current options (none without them):
 // per slice up to slice_bytes 16-bit token slots (a job's slices cover its
-// stream rounded up to a slice: filtered + 64 KiB per image bounds them), a
-// count and a histogram; per segment (of the options' size) the codes and
-// the header; and a slice arena at slice_cap_for(.., 1) dwords per slice.
+// stream rounded up to a slice: filtered + 64 KiB per image bounds them) and
+// a count; per segment (of the options' size) a histogram (zeroed here, kept
+// zero by the kernels), the codes and the header; and a slice arena at
+// slice_cap_for(.., 1) dwords per slice.
 static int ensure_dynamic(r2p_batch* b)
 {
     if (!b->opt.huffman) return R2P_OK;
@@ -138,7 +139,8 @@ static int ensure_dynamic(r2p_batch* b)
 #define TRY(x) do { if ((e = (x)) != hipSuccess) { free_dynamic(b); return hip_fail(e, #x); } } while (0)
     TRY(hipMalloc((void**)&b->d_tokens, tok_dw * 4));
     TRY(hipMalloc((void**)&b->d_tok_count, slices * 4));
-    TRY(hipMalloc((void**)&b->d_hist, slices * kHuffSyms * 4));
+    TRY(hipMalloc((void**)&b->d_hist, segs * kHuffSyms * 4));
+    TRY(hipMemset(b->d_hist, 0, segs * kHuffSyms * 4));     // kept zero by the kernels
     TRY(hipMalloc((void**)&b->d_codes, segs * kHuffTab * 4));
     TRY(hipMalloc((void**)&b->d_hdr, segs * kHuffHdrWords * 4));
     TRY(hipMalloc((void**)&b->d_hdr_bits, segs * 4));
@@ -415,7 +417,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         gs[i] = (uint32_t)segs;
         if (o.huffman) {
             D.tokens = b->d_tokens + tok_dw; D.tok_count = b->d_tok_count + slices;
-            D.hist = b->d_hist + slices * kHuffSyms;
+            D.hist = b->d_hist + segs * kHuffSyms;
             HuffJob& H = hj[i];
             H = HuffJob{};
             H.tokens = D.tokens; H.tok_count = D.tok_count; H.hist = D.hist;
@@ -903,8 +905,10 @@ int r2p_server_create(const r2p_server_config* cfg, r2p_server** out)
     const int nl = cfg->inflight_batches;
     const size_t lane_budget = cfg->hbm_budget_bytes / (size_t)nl;
     // per lane: input arena 1/4, filtered 1/4 of what remains after the input
+    // (1/8 with dynamic-Huffman blocks, whose token slots, histograms and
+    // larger slice arena take about twice the filtered size more)
     const size_t in_cap = lane_budget / 4;
-    s->lane_filtered = std::max<size_t>((lane_budget - in_cap) / 4, (size_t)16 << 20);
+    s->lane_filtered = std::max<size_t>((lane_budget - in_cap) / (s->cfg.opt.huffman ? 8 : 4), (size_t)16 << 20);
     s->lanes.resize((size_t)nl);
     for (Lane& L : s->lanes) {
         r2p_options o = s->cfg.opt;

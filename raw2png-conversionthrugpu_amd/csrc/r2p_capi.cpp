@@ -770,7 +770,8 @@ int r2p_encoder_create(size_t max_filtered_bytes, const r2p_options* opt, r2p_en
     if (e->opt.huffman) {
         TRY(hipMalloc((void**)&e->d_tokens, (max_filtered_bytes + 65536) * 2));
         TRY(hipMalloc((void**)&e->d_tok_count, (size_t)e->max_slice * 4));
-        TRY(hipMalloc((void**)&e->d_hist, (size_t)e->max_slice * kHuffSyms * 4));
+        TRY(hipMalloc((void**)&e->d_hist, ((size_t)e->max_seg + 1) * kHuffSyms * 4));
+        TRY(hipMemset(e->d_hist, 0, ((size_t)e->max_seg + 1) * kHuffSyms * 4));   // kept zero by the kernels
         TRY(hipMalloc((void**)&e->d_codes, ((size_t)e->max_seg + 1) * kHuffTab * 4));
         TRY(hipMalloc((void**)&e->d_hdr, ((size_t)e->max_seg + 1) * kHuffHdrWords * 4));
         TRY(hipMalloc((void**)&e->d_hdr_bits, ((size_t)e->max_seg + 1) * 4));

@@ -81,8 +81,10 @@ struct DeflateJob {
     // literal, 0x8000 | length - 3 where a match starts and its distance - 1
     // in the next slot, kTokNone elsewhere; in units of 4096 positions, the
     // slot pair at positions 16l + 8q + 2j, + 1 in dword 4(l + 256q) + j),
-    // the slice's length in tok_count, and a histogram of kHuffSyms u32
-    // (literal/length symbols 0..285, then distance codes 0..29)
+    // the slice's length in tok_count, and adds its symbol counts to its
+    // segment's histogram of kHuffSyms u32 (literal/length symbols 0..285,
+    // then distance codes 0..29; zero before the launch, and k_huff_tables
+    // leaves it zero again)
     uint32_t*      tokens;
     uint32_t*      tok_count;
     uint32_t*      hist;
@@ -149,7 +151,7 @@ struct DeflateLaunch {
 struct HuffJob {
     const uint32_t* tokens;      // as DeflateJob
     const uint32_t* tok_count;
-    const uint32_t* hist;
+    uint32_t*       hist;        // per segment (read, then cleared)
     uint32_t*       codes;       // nseg * kHuffTab: bit-reversed code (extra bits in place) | bit count << 24
     uint32_t*       hdr;         // nseg * kHuffHdrWords: the block header, BFINAL/BTYPE first
     uint32_t*       hdr_bits;    // nseg

@@ -1815,8 +1815,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (pending) flush();
         }
         if constexpr (DYN) {
-            for (int i = tid; i < kHuffSyms; i += NT) {    // the histogram out, obuf zero again
-                J.hist[(int64_t)sl * kHuffSyms + i] = hist[i];
+            for (int i = tid; i < kHuffSyms; i += NT) {    // into the segment's histogram, obuf zero again
+                if (hist[i]) atomicAdd(&J.hist[(int64_t)seg * kHuffSyms + i], hist[i]);
                 hist[i] = 0;
             }
             if (tid == 0) J.tok_count[sl] = (uint32_t)(s1 - s0);
@@ -2064,7 +2064,8 @@ __device__ __forceinline__ uint32_t fixed_len(int s) { return s < 144 ? 8u : s <
 // The order in which a dynamic header sends the code-length code's lengths (RFC 1951 3.2.7).
 __device__ const uint8_t kClOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
 
-__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffLaunch HL)
+// The code tables and block header of segment seg of job H, by the workgroup.
+__device__ __forceinline__ void huff_tables_segment(const HuffJob& H, int seg)
 {
     constexpr int NW = kHuffThreads / 64;
     __shared__ uint32_t freq[kHuffSyms];
@@ -2076,37 +2077,16 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffLaunch HL)
     __shared__ uint32_t hdr[kHuffHdrWords];
     __shared__ uint32_t s_hlit, s_hdist, s_dynamic;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t total = HL.counts ? HL.counts[1] : HL.total_segs;
-    const int njobs = HL.counts ? (int)HL.counts[0] : HL.njobs;
-    for (uint32_t gs = blockIdx.x; gs < total; gs += gridDim.x) {
-    const int j = huff_job_of(HL, HL.seg_start, gs, njobs);
-    const HuffJob H = HL.jobs ? HL.jobs[j] : HL.single;
-    const int seg = (int)(gs - (HL.jobs ? HL.seg_start[j] : 0u));
-    const int k0 = seg * H.spb, k1 = min(k0 + H.spb, H.nslice);
     const bool bfinal = seg == H.nseg - 1 && H.final_block;
-    __syncthreads();                                          // the previous segment is done with the LDS
-    // the segment's histogram: its slices' summed, for symbols tid and
-    // tid + 256 together, 16 loads in flight
+    // the segment's histogram (its slices' atomic sums), left zero for the
+    // next run
     static_assert(kHuffSyms <= 2 * kHuffThreads, "two symbols per thread");
     {
-        const uint32_t* hp = H.hist + (int64_t)k0 * kHuffSyms + tid;
+        uint32_t* hp = H.hist + (int64_t)seg * kHuffSyms + tid;
         const bool two = tid + kHuffThreads < kHuffSyms;
-        uint32_t f0 = 0, f1 = 0;
-        int k = 0;
-        for (; k + 8 <= k1 - k0; k += 8) {
-            uint32_t v[8], w2[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                v[u] = hp[(int64_t)(k + u) * kHuffSyms];
-                w2[u] = two ? hp[(int64_t)(k + u) * kHuffSyms + kHuffThreads] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) { f0 += v[u]; f1 += w2[u]; }
-        }
-        for (; k < k1 - k0; k++) {
-            f0 += hp[(int64_t)k * kHuffSyms];
-            if (two) f1 += hp[(int64_t)k * kHuffSyms + kHuffThreads];
-        }
+        const uint32_t f0 = hp[0], f1 = two ? hp[kHuffThreads] : 0u;
+        hp[0] = 0;
+        if (two) hp[kHuffThreads] = 0;
         freq[tid] = f0;
         if (two) freq[tid + kHuffThreads] = f1 + (tid + kHuffThreads == 256 ? 1u : 0u);   // the end-of-block code
     }
@@ -2312,6 +2292,23 @@ __global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffLaunch HL)
         const uint32_t L = (uint32_t)k + 3u, lc = table[huff_len_sym(L)], le = huff_len_extra_bits(L), ln = lc >> 16;
         C[kHuffSyms + k] = ((lc & 0xFFFFu) | ((L - 3u) & ((1u << le) - 1u)) << ln) | (ln + le) << 24;
     }
+}
+
+// One workgroup per segment (the single job: the grid is its segments), or
+// workgroups taking the job table's segments in turn.
+template <bool JOBS>
+__global__ __launch_bounds__(kHuffThreads) void k_huff_tables(HuffLaunch HL)
+{
+    if (!JOBS) {
+        huff_tables_segment(HL.single, blockIdx.x);
+        return;
+    }
+    const uint32_t total = HL.counts ? HL.counts[1] : HL.total_segs;
+    const int njobs = HL.counts ? (int)HL.counts[0] : HL.njobs;
+    for (uint32_t gs = blockIdx.x; gs < total; gs += gridDim.x) {
+        const int j = huff_job_of(HL, HL.seg_start, gs, njobs);
+        __syncthreads();                                      // the previous segment is done with the LDS
+        huff_tables_segment(HL.jobs[j], (int)(gs - HL.seg_start[j]));
     }
 }
 
@@ -2329,6 +2326,7 @@ static_assert(kEmitThreads * kEmitPer == 4096, "a round is one unit of k_deflate
 // after a match start the distance's with its extra bits), a block scan of the
 // bit counts, every code ored into LDS at its offset, flush of the whole
 // words.  Slot npos of the segment's last slice is the end-of-block code.
+template <bool JOBS>
 __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
 {
     constexpr int NT = kEmitThreads, NW = NT / 64;
@@ -2341,9 +2339,9 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
     const uint32_t total = HL.counts ? HL.counts[2] : HL.total_slices;
     const int njobs = HL.counts ? (int)HL.counts[0] : HL.njobs;
     for (uint32_t gsl = blockIdx.x; gsl < total; gsl += gridDim.x) {
-        const int j = huff_job_of(HL, HL.slice_start, gsl, njobs);
-        const HuffJob H = HL.jobs ? HL.jobs[j] : HL.single;
-        const int sl = (int)(gsl - (HL.jobs ? HL.slice_start[j] : 0u));
+        const int j = JOBS ? huff_job_of(HL, HL.slice_start, gsl, njobs) : 0;
+        const HuffJob H = JOBS ? HL.jobs[j] : HL.single;
+        const int sl = (int)(gsl - (JOBS ? HL.slice_start[j] : 0u));
         const int seg = sl / H.spb;
         const bool first = sl % H.spb == 0;
         const bool lastsl = sl % H.spb == H.spb - 1 || sl == H.nslice - 1;
@@ -2351,8 +2349,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
         const uint32_t ntok = npos + (lastsl ? 1u : 0u);     // + the end-of-block code
         const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes / 2;
         u32x4 nx[2];
+        // (loads bounded by the slice's slots, not its length: the first
+        // round's loads do not wait for the count; slots past the length are
+        // replaced below)
         auto fetch = [&](uint32_t r) {
-            const bool any = r * NT * kEmitPer + (uint32_t)tid * kEmitPer < npos;
+            const bool any = r * NT * kEmitPer + (uint32_t)tid * kEmitPer < (uint32_t)H.slice_bytes;
 #pragma unroll
             for (int q = 0; q < 2; q++)
                 nx[q] = any ? *(const u32x4*)(tk + r * NT * kEmitPer / 2 + 4 * (tid + NT * q))
@@ -2378,7 +2379,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
 #pragma unroll
             for (int i = 0; i < kEmitPer; i++) v[i] = (nx[i >> 3][(i >> 1) & 3] >> (16 * (i & 1))) & 0xFFFFu;
             if (r + 1 < rounds) fetch(r + 1);
-            if (t0 + kEmitPer > npos && t0 <= npos) {         // the lane holding the slice's end
+            if (t0 + kEmitPer > npos) {                       // the slice's end, and past it
 #pragma unroll
                 for (int i = 0; i < kEmitPer; i++)
                     if (t0 + i >= npos) v[i] = t0 + i == npos && lastsl ? 256u : kTokNone;
@@ -3128,8 +3129,13 @@ hipError_t launch_scan(const AssembleLaunch& AL, int njobs, hipStream_t s)
 hipError_t launch_huffman(const HuffLaunch& HL, int grid_tables, int grid_emit, hipStream_t s)
 {
     (void)hipGetLastError();
-    k_huff_tables<<<grid_tables, kHuffThreads, 0, s>>>(HL);
-    k_huff_emit<<<grid_emit, kEmitThreads, 0, s>>>(HL);
+    if (HL.jobs) {
+        k_huff_tables<true><<<grid_tables, kHuffThreads, 0, s>>>(HL);
+        k_huff_emit<true><<<grid_emit, kEmitThreads, 0, s>>>(HL);
+    } else {
+        k_huff_tables<false><<<grid_tables, kHuffThreads, 0, s>>>(HL);
+        k_huff_emit<false><<<grid_emit, kEmitThreads, 0, s>>>(HL);
+    }
     return hipGetLastError();
 }
 
