@@ -2141,9 +2141,18 @@ __device__ __forceinline__ void huff_tables_segment(const HuffJob& H, int seg)
 #pragma unroll
     for (int i = 0; i < 2; i++) {
         if (!kk[i]) continue;
-        const uint32_t b0 = bstart[bk[i]], nb = bcnt[bk[i]];
-        uint32_t r = b0;
-        for (uint32_t j = b0; j < b0 + nb; j++) r += table[j] > kk[i];
+        // (equal counts share a bucket: up to a few hundred keys, read 8 at a
+        // time so that the loads are in flight together)
+        const uint32_t b0 = bstart[bk[i]], b1 = b0 + bcnt[bk[i]];
+        uint32_t r = b0, j = b0;
+        for (; j + 8 <= b1; j += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = table[j + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) r += v[u] > kk[i];
+        }
+        for (; j < b1; j++) r += table[j] > kk[i];
         slot[r] = (uint32_t)(tid + kHuffThreads * i) | key_len(kk[i]) << 16;
     }
     __syncthreads();
