@@ -246,25 +246,37 @@ def run_plain_host(args, r2p, torch, enc, img, s, cap, png_dev):
 
 
 def run_ratios(args, r2p, synth, torch):
-    """PNG size over filtered bytes of the CT slice and the camera image with
-    fixed-Huffman (the timed form) and dynamic-Huffman blocks."""
+    """PNG size over filtered bytes of the three single-image workloads with
+    fixed-Huffman (the timed form) and dynamic-Huffman blocks, and the time
+    per image of each (device-resident input, encode + sync, median of 20
+    after 3 warm-ups): dynamic over fixed time."""
     out, samples = {}, {}
-    for name in ("ct_slice", "camera_rgb"):
+    for name in ("ct_slice", "radiograph", "camera_rgb"):
         s = synth.workload(name)
         samples[name] = s
         d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
         fb = r2p.filtered_bytes(s.img)
-        row = {}
+        row, ms = {}, {}
         for label, huff in (("gpu_fixed", 0), ("gpu_dynamic", 1)):
             o = r2p.options(huffman=huff)
             enc = r2p.Encoder(fb, o)
             cap = r2p.png_bound(s.img, o)
             d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
-            enc.encode_device(s.img, d_src.data_ptr(), d_png.data_ptr(), cap)
-            n, _, _ = enc.sync()
+            ts = []
+            for k in range(23):
+                t0 = time.perf_counter()
+                enc.encode_device(s.img, d_src.data_ptr(), d_png.data_ptr(), cap)
+                n, _, _ = enc.sync()
+                if k >= 3:
+                    ts.append(time.perf_counter() - t0)
             enc.close()
             row[label] = round(n / fb, 4)
+            ms[label] = float(np.median(ts)) * 1e3
+        row["gpu_ms_fixed"] = round(ms["gpu_fixed"], 4)
+        row["gpu_ms_dynamic"] = round(ms["gpu_dynamic"], 4)
+        row["dynamic_over_fixed_time"] = round(ms["gpu_dynamic"] / ms["gpu_fixed"], 3)
         out[name] = row
+        del d_src
     return out, samples
 
 
@@ -392,6 +404,7 @@ def run_batch(args, r2p, synth, torch):
         enc.close()
     png_total = sum(b.result(i)[1] for i in range(len(imgs)))
     b.close()
+    dyn = run_batch_dynamic(args, r2p, torch, imgs, srcs, isum, fsum, mp, tuned)
     kt = np.array(ktimes).mean(axis=0)
     lat_ms = [x * 1e3 for x in lat]
     return {
@@ -412,7 +425,46 @@ def run_batch(args, r2p, synth, torch):
                       "band_rows": tuned.band_rows, "best_ms": tuned_ms},
         "png_over_filtered": round(png_total / fsum, 4),
         "byte_identical_to_plain": differ == 0, "images_compared": len(imgs), "steps": steps,
+        "dynamic_huffman": dyn,
     }
+
+
+def run_batch_dynamic(args, r2p, torch, imgs, srcs, isum, fsum, mp, tuned):
+    """The same batch with dynamic-Huffman blocks (graph replays, slices by
+    image size), and every image's PNG compared with the plain form's."""
+    o = r2p.options(huffman=1, band_rows=tuned.band_rows)
+    b = r2p.Batch(len(imgs), isum, fsum, o)
+    b.set(imgs, srcs)
+    for _ in range(2):
+        b.run(True); b.sync()
+    steps = max(3, args.batch_steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        b.run(True); b.sync()
+    torch.cuda.synchronize()
+    bwall = (time.perf_counter() - t1) / steps
+    job_opt = [b.job_options(i) for i in range(len(imgs))]
+    maxf = max(r2p.filtered_bytes(im) for im in imgs)
+    encs = {}
+    for jo in job_opt:
+        if jo.slice_bytes not in encs:
+            encs[jo.slice_bytes] = r2p.Encoder(maxf, jo)
+    cap = max(r2p.png_bound(im, o) for im in imgs)
+    d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    differ = 0
+    for i, (im, sp) in enumerate(zip(imgs, srcs)):
+        enc = encs[job_opt[i].slice_bytes]
+        enc.encode_device(im, sp, d_png.data_ptr(), cap)
+        n, _, _ = enc.sync()
+        differ += d_png[:n].cpu().numpy().tobytes() != b.copy_result(i)
+    for enc in encs.values():
+        enc.close()
+    png_total = sum(b.result(i)[1] for i in range(len(imgs)))
+    b.close()
+    return {"batched_mp_per_s": round(mp / bwall, 1), "batched_ms": round(bwall * 1e3, 3),
+            "png_over_filtered": round(png_total / fsum, 4), "byte_identical_to_plain": differ == 0,
+            "images_compared": len(imgs), "steps": steps}
 
 
 def run_serve(args, r2p, synth, torch):
@@ -428,25 +480,26 @@ def run_serve(args, r2p, synth, torch):
     big = synth.radiograph_device(torch, 8192, seed=44)
     tmpl = pool + [big]
     torch.cuda.synchronize()
-    # expected PNG of every template from the plain form
-    opt = r2p.options()
-    expect = []
-    for im, t in tmpl:
-        enc = r2p.Encoder(r2p.filtered_bytes(im), opt)
-        cap = r2p.png_bound(im, opt)
-        d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
-        enc.encode_device(im, t.data_ptr(), d_png.data_ptr(), cap)
-        n, _, _ = enc.sync()
-        enc.close()
-        expect.append(d_png[:n].cpu().numpy().tobytes())
     pins = [t.cpu().pin_memory() for _, t in tmpl]
     imgs = [im for im, _ in tmpl]
     mp_of = np.array([im.width * im.height / 1e6 for im in imgs])
     in_of = np.array([r2p.input_bytes(im) for im in imgs], dtype=np.float64)
-    out = {"seconds_per_rate": args.serve_seconds, "hbm_budget_gb": 64, "lanes": args.serve_lanes,
-           "max_batch_images": args.serve_batch, "pool": len(pool), "big_requests": "5% 8192x8192 16-bit"}
-    for lam in args.serve_rates:
-        n = int(lam * args.serve_seconds)
+
+    def expected(opt):
+        """The plain form's PNG of every template."""
+        exp = []
+        for im, t in tmpl:
+            enc = r2p.Encoder(r2p.filtered_bytes(im), opt)
+            cap = r2p.png_bound(im, opt)
+            d_png = torch.empty(cap, dtype=torch.uint8, device="cuda")
+            enc.encode_device(im, t.data_ptr(), d_png.data_ptr(), cap)
+            n, _, _ = enc.sync()
+            enc.close()
+            exp.append(d_png[:n].cpu().numpy().tobytes())
+        return exp
+
+    def one_rate(lam, seconds, opt, expect):
+        n = int(lam * seconds)
         arr = np.cumsum(rng.exponential(1.0 / lam, n))
         kind = rng.integers(0, len(pool), n).astype(np.int32)
         kind[rng.uniform(size=n) < 0.05] = len(tmpl) - 1
@@ -464,7 +517,7 @@ def run_serve(args, r2p, synth, torch):
         finally:
             L.r2p_server_destroy(h)
         mp = float(mp_of[kind].sum())
-        out[str(lam)] = {
+        return {
             "requests": n, "offered_mp_per_s": round(mp / float(arr[-1]), 1),
             "mp_per_s": round(mp / wall, 1), "wall_s": round(wall, 3),
             "h2d_gb_per_s": round(float(in_of[kind].sum()) / wall / 1e9, 2),
@@ -472,9 +525,24 @@ def run_serve(args, r2p, synth, torch):
             "latency_ms": {q: round(float(np.percentile(lat, int(q[1:]))) * 1e3, 3) for q in ("p50", "p95", "p99")},
             "batches": int(st.batches), "mean_batch_images": round(st.mean_batch_images, 2),
             "peak_admitted_gb": round(st.peak_admitted_bytes / 1e9, 3),
+            "png_over_input": round(float(plen.astype(np.float64).sum()) / float(in_of[kind].sum()), 4),
             "png_bytes_equal_plain": int(same.sum()) == n, "requests_compared": n,
         }
+
+    out = {"seconds_per_rate": args.serve_seconds, "hbm_budget_gb": 64, "lanes": args.serve_lanes,
+           "max_batch_images": args.serve_batch, "pool": len(pool), "big_requests": "5% 8192x8192 16-bit"}
+    opt = r2p.options()
+    expect = expected(opt)
+    for lam in args.serve_rates:
+        out[str(lam)] = one_rate(lam, args.serve_seconds, opt, expect)
         log(f"bench: serve lambda={lam}: {json.dumps(out[str(lam)])}")
+    if args.serve_dynamic_seconds > 0:
+        # the same mix with dynamic-Huffman blocks, at the middle rate
+        dopt = r2p.options(huffman=1)
+        lam = args.serve_rates[len(args.serve_rates) // 2]
+        out["dynamic_huffman"] = {str(lam): one_rate(lam, args.serve_dynamic_seconds, dopt, expected(dopt)),
+                                  "seconds": args.serve_dynamic_seconds}
+        log(f"bench: serve dynamic lambda={lam}: {json.dumps(out['dynamic_huffman'])}")
     return out
 
 
@@ -496,8 +564,12 @@ def main():
                     help="DEFLATE slice bytes (0: by image size; 16 KiB for the 3072x3072 radiograph)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic-Huffman encodes of the plain workload")
+    ap.add_argument("--no-ratios", action="store_true",
+                    help="skip the fixed/dynamic size and time table of the three single-image workloads")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 800.0, 2000.0])
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window per rate")
+    ap.add_argument("--serve-dynamic-seconds", type=float, default=20.0,
+                    help="arrival window of the dynamic-Huffman served run (0: skip)")
     ap.add_argument("--serve-pool", type=int, default=64, help="distinct request images of the mix")
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=64)
@@ -549,11 +621,11 @@ def main():
     fk = "filter"
     fach = r["algo_bytes"][fk] / (kern[fk] * 1e-3) / 1e9
     out["filter_kernel"] = {"achieved": round(fach, 1), "frac": round(fach / HBM_PEAK_GBS, 4)}
-    gpu_ratios, samples = run_ratios(args, r2p, synth, torch)
-    gpu_ratios["radiograph"] = {"gpu_fixed": out["ratio"]["gpu_png_over_filtered"]}
-    if r["dynamic"]:
-        gpu_ratios["radiograph"]["gpu_dynamic"] = r["dynamic"]["png_over_filtered"]
-    r["camera"] = samples["camera_rgb"]
+    if args.no_ratios:
+        gpu_ratios, r["camera"] = {}, None
+    else:
+        gpu_ratios, samples = run_ratios(args, r2p, synth, torch)
+        r["camera"] = samples["camera_rgb"]
     if not args.no_cpu:
         # the oracle is the checker of the timed PNG and the timed CPU baseline
         import oracle as O
