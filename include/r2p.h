@@ -106,7 +106,8 @@ typedef struct r2p_options {
                               Changes speed only, never the decoded output */
     int huffman;           /* 0: fixed-Huffman blocks (the reference's, gpu_deflate_backend.h:3);
                               1: each segment a dynamic-Huffman block where that is smaller
-                              (plain encoder only; the batched and served forms reject it) */
+                              (plain, batched and served forms; not the no-allocation entry
+                              point) */
 } r2p_options;
 
 void r2p_options_default(r2p_options* o);

@@ -1179,7 +1179,9 @@ __global__ __launch_bounds__(kFinalThreads) void k_adler_final(const uint32_t* p
 //      position, else T1's (most recent position before the sub-block)
 //      within the window.  The longest match wins, ties to the shorter
 //      distance (a max over length << 16 | (0xFFFF - distance) keys);
-//      matches stop at the lane end, so each lane parses greedily on its own;
+//      matches stop at the lane end, so each lane parses greedily on its own,
+//      after which lanes that are one 16-byte match at the distance of the
+//      lane before them join that lane's token (runs of up to 256 bytes);
 //   C. per-lane bit counts, block exclusive scan, then each lane packs its
 //      tokens into 32-bit words at its bit offset (the two words shared
 //      with neighbouring lanes by LDS atomicOr) and full words stream out;
@@ -1837,17 +1839,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // ---------------------------------------------------------------------------
 // Dynamic-Huffman blocks (RFC 1951 3.2.7), one per segment
 //
-// k_huff_tables: one workgroup per segment sums its slices' histograms, builds
-// length-limited Huffman codes for the literal/length and distance
-// alphabets and for the code-length alphabet (the lengths are sent one
-// symbol each, without run-length codes: a little more header for a
-// header written in parallel), and writes the code tables and the block
+// k_huff_tables: one workgroup per segment reads its histogram (summed by the
+// slices' atomics), builds length-limited codes for the literal/length and
+// distance alphabets and for the code-length alphabet (the lengths are sent
+// one symbol each, without run-length codes: a little more header for a
+// header written in parallel), and writes the emit table and the block
 // header -- or the fixed code and its 3-bit header when that is no longer
 // (so a segment never grows past its fixed-Huffman size).
-// k_huff_emit: one workgroup per slice turns its tokens into bits with the
-// segment's table (the segment's first slice starts with the header, its
-// last ends with the end-of-block code), in rounds of 16 tokens per thread:
-// per-thread bit counts, a block scan, packing into LDS words, flush.
+// k_huff_emit: one workgroup per slice turns its token slots into bits with
+// the segment's table (the segment's first slice starts with the header, its
+// last ends with the end-of-block code), in rounds of 16 slots per thread:
+// per-slot code lookups, a block scan of the bit counts, ors into LDS words,
+// flush.
 // ---------------------------------------------------------------------------
 
 constexpr int kHuffThreads = 256;        // k_huff_tables workgroup
