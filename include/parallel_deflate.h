@@ -34,3 +34,12 @@ void zlib_header(int level, uint8_t out[2]);
 void zlib_trailer(const ParallelDeflateState& state, uint8_t out[4]);
 unsigned long adler32_of(const uint8_t* data, size_t len);
 unsigned long crc32_of(const uint8_t* data, size_t len);
+
+// Standalone CPU DEFLATE throughput table (replaces include/parallel_deflate.h:82
+// and src/parallel_deflate.cpp:261-335 of the reference, the CLI's
+// --bench-deflate): deflate_strip over strip_bytes of a high-entropy synthetic
+// pattern (byte i = the top byte of i * 0x9E3779B9, as the reference's) with
+// num_threads chunks at levels 0, 1 and 3; 3 warm-ups and 15 timed strips per
+// level; prints ms per strip, input MB/s, compressed size and the summed
+// z_stream set-up time to stdout.
+void bench_deflate(size_t strip_bytes, int num_threads);

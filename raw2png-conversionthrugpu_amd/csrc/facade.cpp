@@ -643,6 +643,37 @@ DeflateResult deflate_strip(const uint8_t* input, size_t n, const ParallelDeflat
     return r;
 }
 
+void bench_deflate(size_t strip_bytes, int num_threads)
+{
+    constexpr int kWarm = 3, kIters = 15;
+    std::vector<uint8_t> strip(strip_bytes);
+    for (size_t i = 0; i < strip_bytes; i++) strip[i] = (uint8_t)(((uint32_t)i * 0x9E3779B9u) >> 24);
+    ThreadPool pool(std::max(1, num_threads));
+    const double mb = (double)strip_bytes / (1024.0 * 1024.0);
+    printf("\nDeflate benchmark (zlib %s): %zu-byte strip (%.2f MB), %d chunks per strip, %d timed (+%d warm-up)\n",
+           zlibVersion(), strip_bytes, mb, num_threads, kIters, kWarm);
+    printf("  level   ms/strip   MB/s in   out/in   set-up sum\n");
+    for (int level : {0, 1, 3}) {
+        ParallelDeflateConfig cfg;
+        cfg.num_threads = num_threads;
+        cfg.zlib_level = level;
+        for (int w = 0; w < kWarm; w++) (void)deflate_strip(strip.data(), strip_bytes, cfg, pool, false);
+        double ms = 0, init = 0;
+        size_t out = 0;
+        for (int it = 0; it < kIters; it++) {
+            const auto t0 = Clock::now();
+            const DeflateResult r = deflate_strip(strip.data(), strip_bytes, cfg, pool, false);
+            ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+            init += (double)r.init_us;
+            out = r.data.size();
+        }
+        ms /= kIters;
+        printf("  %5d   %8.2f   %7.1f   %6.4f   %8.0f us\n", level, ms, mb / (ms * 1e-3),
+               strip_bytes ? (double)out / (double)strip_bytes : 0.0, init / kIters);
+    }
+    printf("  (set-up sum: z_stream acquire/reset time summed over the parallel chunks)\n");
+}
+
 void accum_adler(ParallelDeflateState& st, const DeflateResult& dr)
 {
     if (dr.input_size == 0) return;

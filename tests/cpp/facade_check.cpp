@@ -115,6 +115,10 @@ struct VecSource : ImageSource {
 int main(int argc, char** argv)
 {
     // facade_check encode <tiff|dicom|dicom-all> <in> <out> <gpu_deflate 0|1> [strip_height]
+    if (argc >= 4 && std::string(argv[1]) == "bench-deflate") {
+        bench_deflate((size_t)atoll(argv[2]), atoi(argv[3]));
+        return 0;
+    }
     if (argc >= 6 && std::string(argv[1]) == "encode") {
         PipelineConfig cfg;
         cfg.use_gpu_deflate = atoi(argv[5]) != 0;
