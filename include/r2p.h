@@ -94,7 +94,10 @@ typedef struct r2p_options {
     int segment_bytes;     /* DEFLATE segment (independent fixed-Huffman block, closed by a
                               sync flush), a multiple of 4096; default 65536 */
     int lz77;              /* 0: literal-only fixed Huffman (the reference default,
-                              pipeline.h:40); 1: LZ77 over a 32 KiB primed window */
+                              pipeline.h:40); 1: LZ77 over a 32 KiB primed window, hashed
+                              candidates searched at every 4th position (default); 2: at
+                              every 2nd position (slower; with fixed-Huffman blocks about
+                              1-4 % smaller output) */
     int idat_chunk_bytes;  /* IDAT chunk payload size; default 65536 */
     int zlib_level;        /* only sets the zlib header FLEVEL (parallel_deflate.cpp:214-228) */
     int band_rows;         /* rows per filter workgroup; 0 = auto */

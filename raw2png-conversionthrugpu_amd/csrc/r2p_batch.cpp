@@ -178,6 +178,7 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     // 3. deflate over all slices
     DeflateLaunch DL{};
     DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 6; DL.dyn = b->opt.huffman;
+    DL.dense = b->opt.lz77 == 2;
     R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, b->opt.wg_threads, 1024, s));
     if (b->opt.huffman) {                 // every segment's codes, then every slice's bits
         HuffLaunch HL{};

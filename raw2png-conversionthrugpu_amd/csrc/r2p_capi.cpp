@@ -382,6 +382,7 @@ void fill_defaults(r2p_options* o)
     o->lane_bytes = 16;
     if (o->wg_threads != 256 && o->wg_threads != 512) o->wg_threads = 256;
     if (o->lz77 < 0) o->lz77 = 1;
+    if (o->lz77 > 2) o->lz77 = 2;
     o->huffman = o->huffman ? 1 : 0;
 }
 
@@ -696,6 +697,7 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     D.lz77 = o.lz77; D.nslice = sg.nslice;
     if (o.huffman) { D.tokens = w.tokens; D.tok_count = w.tok_count; D.hist = w.hist; }
     DL.dyn = o.huffman;
+    DL.dense = o.lz77 == 2;
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
     if (o.huffman) {                       // the segments' codes, then the slices' bits

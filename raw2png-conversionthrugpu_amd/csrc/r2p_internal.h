@@ -143,7 +143,8 @@ struct DeflateLaunch {
     uint32_t          total_slices;
     const uint32_t*   counts;      // optional device [njobs, total]
     DeflateJob        single;
-    int32_t           dyn;         // the jobs' dynamic-Huffman form (k_deflate<.., true>)
+    int32_t           dyn;         // the jobs' dynamic-Huffman form (k_deflate<.., true, ..>)
+    int32_t           dense;       // hashed search at every 2nd position (r2p_options.lz77 = 2), else every 4th
 };
 
 // Dynamic-Huffman blocks of one job (after k_deflate<.., true>): per

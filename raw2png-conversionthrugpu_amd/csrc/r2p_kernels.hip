@@ -1171,7 +1171,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_adler_final(const uint32_t* p
 // owns the 16 consecutive positions of its parse lane and keeps them in
 // registers:
 //   A. the positions' 4-byte hashes (kept, two per register); every
-//      kInsertStride-th position enters T2 (first position of the sub-block
+//      SS-th position enters T2 (first position of the sub-block
 //      with that hash, atomicMin);
 //   B. candidates per position: distance 1, distance bpp and one row up
 //      (equality bitmasks from the lane's own bytes, no extra loads), and
@@ -1185,7 +1185,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_adler_final(const uint32_t* p
 //   C. per-lane bit counts, block exclusive scan, then each lane packs its
 //      tokens into 32-bit words at its bit offset (the two words shared
 //      with neighbouring lanes by LDS atomicOr) and full words stream out;
-//      every kInsertStride-th position of the sub-block then enters T1.
+//      every SS-th position of the sub-block then enters T1.
 // Table entries carry a tag (the slice for T1, the sub-block for T2), so
 // stale entries never match and the tables are not cleared per slice.
 // The result depends only on the input bytes: every table update is an
@@ -1207,8 +1207,9 @@ constexpr int kPosBits = 17;
 constexpr uint32_t kPosMask = (1u << kPosBits) - 1;
 constexpr uint32_t kTagMax = (1u << (32 - kPosBits)) - 1;
 constexpr int kPrimeStride = 8;    // window positions entered into T1 at this stride
-constexpr int kInsertStride = 2;   // the slice's own positions entered into T1 and T2, and searched
-                                   // for a hashed candidate, at this stride
+// The slice's own positions enter T1 and T2, and are searched for a hashed
+// candidate, at a stride SS of 4 (r2p_options.lz77 = 1) or 2 (lz77 = 2): a
+// template parameter of k_deflate.
 constexpr int kHashCmpDwords = 2;  // a hashed candidate is compared over at most 8 bytes
 
 __device__ __forceinline__ uint32_t hash4(uint32_t w) { return (w * 2654435761u) >> (32 - kHashBits); }
@@ -1366,20 +1367,21 @@ __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t* c, int KW
     else c[0] = g32u(p);
 }
 
-// The hash of position i (a multiple of kInsertStride: only those are hashed).
-__device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return hh[i / kInsertStride]; }
+// The hash of position i (a multiple of SS: only those are hashed).
+template <int SS>
+__device__ __forceinline__ uint32_t hash_at(const uint32_t* hh, int i) { return hh[i / SS]; }
 
 // Best match key of every position of the lane (0 for a literal).
 // Candidates: distance 1, bpp and one row up, whose runs come from the
 // equality masks n* (bit j set where byte j differs from the byte that far
-// back, or j is past the lane's end), and at every kInsertStride-th position
+// back, or j is past the lane's end), and at every SS-th position
 // one hashed earlier position -- T2's entry when it precedes the position,
 // else T1's within the window.
 // Positions go in groups of G so that a group's table reads, then its
 // candidate loads, are in flight together; a scheduling barrier between
 // groups bounds the registers.  Branch-free: an unusable hashed candidate
 // loads from the window base and is then discarded.
-template <int G>
+template <int G, int SS>
 __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, const uint32_t* o, const uint32_t* hh,
                                           int lb, int le, uint32_t hmask, uint32_t n1, uint32_t nb, uint32_t nr,
                                           uint32_t kb, uint32_t kr, const uint32_t* T,
@@ -1397,8 +1399,8 @@ __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, cons
             key[i] = max(max((r1 << 16) | (0xFFFFu - 1u), (rb << 16) | kb), (rr << 16) | kr);
             hv[u] = false;
             C[u] = 0;
-            if (i % kInsertStride) continue;        // (i is a constant: no code for these positions)
-            const uint32_t h = hash_at(hh, i);
+            if (i % SS) continue;                   // (i is a constant: no code for these positions)
+            const uint32_t h = hash_at<SS>(hh, i);
             const u32x2 e = *(const u32x2*)(T + 2 * h);                       // T1 and T2 entries, one read
             const uint32_t x2 = e[1] ^ base2;                                // position if the tag matches
             const uint32_t x1 = e[0] ^ base1;
@@ -1411,11 +1413,11 @@ __device__ __forceinline__ void lane_keys(uint32_t* key, const uint8_t* wb, cons
         uint32_t c[G][4];
 #pragma unroll
         for (int u = 0; u < G; u++)
-            if ((g0 + u) % kInsertStride == 0) load_words(wb + C[u], c[u], min(kHashCmpDwords, (kLane - (g0 + u) + 3) / 4));
+            if ((g0 + u) % SS == 0) load_words(wb + C[u], c[u], min(kHashCmpDwords, (kLane - (g0 + u) + 3) / 4));
 #pragma unroll
         for (int u = 0; u < G; u++) {
             const int i = g0 + u;
-            if (i % kInsertStride) { key[i] = key[i] >= kMinKey ? key[i] : 0u; continue; }
+            if (i % SS) { key[i] = key[i] >= kMinKey ? key[i] : 0u; continue; }
             const uint32_t lh = min(cmp_run(o, c[u], i, min(kHashCmpDwords, (kLane - i + 3) / 4)), (uint32_t)max(le - i, 0));
             const uint32_t k = max(key[i], hv[u] ? mkey(lh, (uint32_t)(lb + i) - C[u]) : 0u);
             key[i] = k >= kMinKey ? k : 0u;
@@ -1447,7 +1449,7 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
 // not carry their registers.  DYN: the parse's tokens and their symbol
 // histogram go to memory (DeflateLaunch::tokens / tok_count / hist) for a
 // dynamic-Huffman block per segment; no bits are emitted here.
-template <int NT, bool STATS, bool DYN = false>
+template <int NT, bool STATS, bool DYN, int SS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_deflate(DeflateLaunch DL)
 {
     constexpr int LANE = kLane;
@@ -1599,26 +1601,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 ad_s += s16;                           // < 16 * 4080 per slice
                 ad_t = (ad_t + (uint32_t)(lb - s0_rel) * s16 + w16) % 65521u;   // < 2^29 before the mod
             }
-            // ---- A. hashes of every kInsertStride-th position (kept), the
+            // ---- A. hashes of every SS-th position (kept), the
             // sub-block's first of those positions per hash ------------------------
             // hashable positions: inside the lane, with 4 bytes before the end
             const int hl = min(le, end_rel - 3 - lb);
             const uint32_t hmask = hl <= 0 ? 0u : hl >= LANE ? 0xFFFFu : (1u << hl) - 1u;
-            uint32_t hh[LANE / kInsertStride];
+            uint32_t hh[LANE / SS];
 #pragma unroll
-            for (int i = 0; i < LANE / kInsertStride; i++) hh[i] = 0;
+            for (int i = 0; i < LANE / SS; i++) hh[i] = 0;
             if (lz && le > 0) {
 #pragma unroll
-                for (int i = 0; i < LANE; i += kInsertStride) hh[i / kInsertStride] = hash4(bytes_at(o, i));
+                for (int i = 0; i < LANE; i += SS) hh[i / SS] = hash4(bytes_at(o, i));
                 if (hmask == 0xFFFFu) {            // every position hashable: no test each
 #pragma unroll
-                    for (int i = 0; i < LANE; i += kInsertStride)
-                        atomicMin(&T[2 * hash_at(hh, i) + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
+                    for (int i = 0; i < LANE; i += SS)
+                        atomicMin(&T[2 * hash_at<SS>(hh, i) + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 } else {
 #pragma unroll
-                    for (int i = 0; i < LANE; i += kInsertStride)
+                    for (int i = 0; i < LANE; i += SS)
                         if ((hmask >> i) & 1u)
-                            atomicMin(&T[2 * hash_at(hh, i) + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
+                            atomicMin(&T[2 * hash_at<SS>(hh, i) + 1], (tag2 << kPosBits) | (uint32_t)(lb + i));
                 }
             }
             __syncthreads();
@@ -1653,7 +1655,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 // runs stop at the lane's end: past it every position "differs"
                 const uint32_t cap = le >= LANE ? 0u : ~0u << le;
-                lane_keys<R2P_MATCH_GROUP>(key, wb, o, hh, lb, le, hmask, ~m1 | cap, ~mb | cap, ~mr | cap,
+                lane_keys<R2P_MATCH_GROUP, SS>(key, wb, o, hh, lb, le, hmask, ~m1 | cap, ~mb | cap, ~mr | cap,
                                            0xFFFFu - (uint32_t)(bpp > 1 ? bpp : 1), 0xFFFFu - (uint32_t)(row_ok ? dr : 1),
                                            T, tag1 << kPosBits, tag2 << kPosBits);
             }
@@ -1734,17 +1736,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             const uint32_t off = carry_bits + wbase + incl - nbits;
-            // the sub-block's positions (every kInsertStride-th) enter T1; T2's
+            // the sub-block's positions (every SS-th) enter T1; T2's
             // entries expire with its tag
             if (lz && le > 0) {
                 if (hmask == 0xFFFFu) {
 #pragma unroll
-                    for (int i = 0; i < LANE; i += kInsertStride)
-                        atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                    for (int i = 0; i < LANE; i += SS)
+                        atomicMax(&T[2 * hash_at<SS>(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
                 } else {
 #pragma unroll
-                    for (int i = 0; i < LANE; i += kInsertStride)
-                        if ((hmask >> i) & 1u) atomicMax(&T[2 * hash_at(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
+                    for (int i = 0; i < LANE; i += SS)
+                        if ((hmask >> i) & 1u) atomicMax(&T[2 * hash_at<SS>(hh, i)], (tag1 << kPosBits) | (uint32_t)(lb + i));
                 }
             }
             if constexpr (DYN) {
@@ -3124,10 +3126,17 @@ hipError_t launch_deflate(const DeflateLaunch& DL, int lane_bytes, int wg_thread
     (void)lane_bytes;
     (void)hipGetLastError();
     const bool stats = !DL.jobs && DL.single.slice_stats;
-    if (DL.dyn)                 k_deflate<256, false, true><<<grid, 256, 0, s>>>(DL);
-    else if (stats)             k_deflate<256, true><<<grid, 256, 0, s>>>(DL);
-    else if (wg_threads == 256) k_deflate<256, false><<<grid, 256, 0, s>>>(DL);
-    else                        k_deflate<512, false><<<grid, 512, 0, s>>>(DL);
+    if (DL.dense) {             // hashed search at every 2nd position
+        if (DL.dyn)                 k_deflate<256, false, true, 2><<<grid, 256, 0, s>>>(DL);
+        else if (stats)             k_deflate<256, true, false, 2><<<grid, 256, 0, s>>>(DL);
+        else if (wg_threads == 256) k_deflate<256, false, false, 2><<<grid, 256, 0, s>>>(DL);
+        else                        k_deflate<512, false, false, 2><<<grid, 512, 0, s>>>(DL);
+    } else {                    // at every 4th
+        if (DL.dyn)                 k_deflate<256, false, true, 4><<<grid, 256, 0, s>>>(DL);
+        else if (stats)             k_deflate<256, true, false, 4><<<grid, 256, 0, s>>>(DL);
+        else if (wg_threads == 256) k_deflate<256, false, false, 4><<<grid, 256, 0, s>>>(DL);
+        else                        k_deflate<512, false, false, 4><<<grid, 512, 0, s>>>(DL);
+    }
     return hipGetLastError();
 }
 

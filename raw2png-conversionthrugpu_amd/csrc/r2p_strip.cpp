@@ -55,7 +55,7 @@ int r2p_deflate_create(size_t max_strip_bytes, size_t max_total_bytes, int lz77,
     if (!out || max_strip_bytes == 0) { set_error("bad deflate arguments"); return R2P_ERR_ARG; }
     r2p_deflate* d = new r2p_deflate();
     d->max_strip = max_strip_bytes;
-    d->lz77 = lz77 ? 1 : 0;
+    d->lz77 = lz77 < 0 ? 1 : std::min(lz77, 2);
     r2p_options o{};
     o.segment_bytes = segment_bytes;
     fill_defaults(&o);
@@ -147,6 +147,7 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     D.slice_cap_dw = d->slice_cap_dw;
     D.final_block = is_last ? 1 : 0; D.bpp = bpp > 0 ? bpp : 1; D.row_stride = row_stride;
     D.lz77 = d->lz77; D.nslice = sg.nslice;
+    DL.dense = d->lz77 == 2;
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, 16, 256, sg.nslice, s));
 

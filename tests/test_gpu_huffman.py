@@ -20,7 +20,7 @@ def log(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("lz77", [0, 1])
+@pytest.mark.parametrize("lz77", [0, 1, 2])
 @pytest.mark.parametrize("kind", ["ct_slice", "radiograph_small", "camera_small"])
 def test_dynamic_workloads(torch_dev, r2p, synth, oracle, kind, lz77):
     log(f"case dynamic {kind} lz77={lz77}")

@@ -207,7 +207,7 @@ def _check_png(oracle, r2p, png, px, img, filt_gpu):
     return z
 
 
-@pytest.mark.parametrize("lz77", [0, 1])
+@pytest.mark.parametrize("lz77", [0, 1, 2])
 @pytest.mark.parametrize("kind", ["ct_slice", "radiograph_small", "camera_small"])
 def test_encode_png_workloads(torch_dev, r2p, synth, oracle, kind, lz77):
     log(f"case encode {kind} lz77={lz77}")
@@ -239,7 +239,7 @@ def test_encode_huge_rows(torch_dev, r2p, oracle, w, h, ch, bits, mode, dicom):
     px[:, : (w * ch) // 4] = rng.integers(0, hi, (h, (w * ch) // 4))
     p = r2p.DicomParams.make(*dicom) if dicom else None
     img = r2p.Image.make(w, h, ch, bits, mode, p) if p else r2p.Image.make(w, h, ch, bits, mode)
-    for lz77 in (0, 1):
+    for lz77 in (0, 1, 2):
         png, filt, adler, nseg, t = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=lz77))
         _check_png(oracle, r2p, png, px, img, filt)
         assert adler == zlib.adler32(filt.tobytes())
@@ -262,6 +262,10 @@ def test_lz77_finds_repeats(torch_dev, r2p, oracle):
     png2, filt2, _, _, _ = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=1, huffman=1))
     _check_png(oracle, r2p, png2, px, img, filt2)
     assert len(png2) < len(png1)
+    # the denser search (every 2nd position) finds at least as much here
+    png3, filt3, _, _, _ = encode_device(torch_dev, r2p, px, img, r2p.options(lz77=2))
+    _check_png(oracle, r2p, png3, px, img, filt3)
+    assert len(png3) <= len(png1) * 1.01
 
 
 @pytest.mark.parametrize("seg,chunk", [(4096, 1000), (65536, 65536), (1 << 20, 8192)])
@@ -315,7 +319,7 @@ def test_encode_slices(torch_dev, r2p, oracle, seg, slc, w, h):
     px = (np.cumsum(rng.integers(-3, 4, (h, w)), axis=1) % 256).astype(np.uint8)
     px[h // 3: h // 2, :] = 200
     img = r2p.Image.make(w, h, 1, 8, 0)
-    for lz77 in (0, 1):
+    for lz77 in (0, 1, 2):
         opt = r2p.options(segment_bytes=seg, slice_bytes=slc, lz77=lz77)
         assert (opt.segment_bytes, opt.slice_bytes) == (seg, slc)
         png, filt, adler, nseg, t = encode_device(torch_dev, r2p, px, img, opt)
