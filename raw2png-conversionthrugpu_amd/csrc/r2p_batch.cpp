@@ -440,9 +440,9 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice;
         A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = sl;
         A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
-        A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
         A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
         A.color_type = (uint8_t)g.color_type;
+        A.ihdr_crc = ihdr_crc(A.width, A.height, A.bit_depth, A.color_type);
         const int lvl = o.zlib_level;
         A.zflg = lvl == 1 ? 0x01 : (lvl >= 2 && lvl <= 5) ? 0x5E : (lvl >= 6 && lvl <= 7) ? 0x9C : 0xDA;
         const size_t zmax = g.filtered + g.filtered / 8 + 16 * (size_t)nseg + 16;

@@ -307,10 +307,13 @@ size_t png_bound_bytes(size_t filtered, int segment_bytes, int chunk_bytes)
     return 8 + 25 + 12 * nch + z + 12 + 64;
 }
 
-uint32_t idat_prefix_full(int chunk_bytes)
+uint32_t ihdr_crc(uint32_t width, uint32_t height, uint8_t bit_depth, uint8_t color_type)
 {
-    const uint32_t g = ~(uint32_t)crc32(0L, (const Bytef*)"IDAT", 4);
-    return r2p_crc32_combine(g, 0u, (uint64_t)chunk_bytes);
+    const uint8_t ih[17] = {'I', 'H', 'D', 'R',
+                            (uint8_t)(width >> 24), (uint8_t)(width >> 16), (uint8_t)(width >> 8), (uint8_t)width,
+                            (uint8_t)(height >> 24), (uint8_t)(height >> 16), (uint8_t)(height >> 8), (uint8_t)height,
+                            bit_depth, color_type, 0, 0, 0};
+    return (uint32_t)crc32(0L, ih, 17);
 }
 
 size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap)
@@ -722,9 +725,9 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
     A.slice_cap_dw = w.slice_cap_dw; A.slice_bytes = o.slice_bytes;
     A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
-    A.idat_full = idat_prefix_full(o.idat_chunk_bytes);
     A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
     A.color_type = (uint8_t)g.color_type;
+    A.ihdr_crc = ihdr_crc(A.width, A.height, A.bit_depth, A.color_type);
     const int lvl = o.zlib_level;
     A.zflg = lvl == 1 ? 0x01 : (lvl >= 2 && lvl <= 5) ? 0x5E : (lvl >= 6 && lvl <= 7) ? 0x9C : 0xDA;
     const size_t zmax = g.filtered + g.filtered / 8 + 16 * (size_t)nseg + 16;

@@ -77,9 +77,8 @@ int plan_window(const r2p_image* img, WindowPlan* wp);
 bool q15_coeffs(const r2p_dicom_params& p, int16_t* a, int32_t* b);
 
 size_t png_bound_bytes(size_t filtered, int segment_bytes, int chunk_bytes);
-// The CRC-32 register after the chunk type "IDAT", advanced over
-// chunk_bytes zero bytes (the type's share of a full IDAT chunk's CRC).
-uint32_t idat_prefix_full(int chunk_bytes);
+// CRC-32 of a PNG's IHDR chunk (its type and 13 data bytes).
+uint32_t ihdr_crc(uint32_t width, uint32_t height, uint8_t bit_depth, uint8_t color_type);
 // Filter bands: LDS bytes of a band (the band's rows and the row above it,
 // beside the table), and the band height to use.
 size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap);

@@ -115,7 +115,7 @@ struct AssembleJob {
     uint32_t        width, height;
     uint8_t         bit_depth, color_type, zflg, pad_;
     int32_t         max_chunks;  // chunks this job may use (grid sizing bound)
-    uint32_t        idat_full;   // CRC register after "IDAT" times x^(8 * chunk_bytes): a full chunk's prefix term
+    uint32_t        ihdr_crc;    // CRC-32 of the IHDR chunk (type and data), written with chunk 0
     int32_t         eob_in_bits; // the segments' bit strings end with their end-of-block code (dynamic blocks)
     uint64_t*       running;     // optional: running output offset (raw mode, strip appends)
     uint64_t*       base_slot;   // optional: where k_scan records this job's base offset

@@ -2566,10 +2566,16 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
 // leading zeros do not change it, so every piece acts as exactly 64 bytes
 // and the pieces combine in a tree R(A|B) = R(A) * x^(8|B|) + R(B) whose
 // shift at level k is one fixed multiplication by x^(8*64*2^k), done as four
-// table lookups.  The chunk CRC is crc32("IDAT"|D) = ~(g * x^(8|D|) + R(D))
-// with g the register after "IDAT".
+// lookups in the workgroup's LDS copy of the table.  The chunk CRC is
+// crc32("IDAT"|D) = ~(g * x^(8|D|) + R(D)) with g the register after
+// "IDAT": the piece holding the chunk's first byte starts its chain from g
+// instead of 0 (its real bytes run from there), which puts that term in
+// place for any chunk length.
 // ---------------------------------------------------------------------------
 constexpr int kAsmBytes = 64;                  // payload bytes per thread
+constexpr int kAsmShiftLevels = 10;            // tree levels: 64 B to 32 KiB
+constexpr uint32_t kIdatRegister = 0xCA50F9E1u;    // CRC-32 register after "IDAT": ~crc32("IDAT")
+static_assert((64 << (kAsmShiftLevels - 1)) * 2 == 65536, "the tree spans one 64 KiB chunk");
 static_assert(kAssembleThreads * kAsmBytes == 65536, "one 64 KiB chunk per workgroup");
 
 __device__ __forceinline__ void put_be32(uint8_t* p, uint32_t v)
@@ -2757,18 +2763,26 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
 {
     __shared__ uint32_t tab[4][256];
     __shared__ uint32_t tab32[4][256];                     // the 32-byte shift
+    __shared__ uint32_t shf[kAsmShiftLevels][4][256];      // the tree's shifts, 64 B .. 32 KiB
     __shared__ uint32_t red[kAssembleThreads / 64];
     __shared__ u32x4 stage[kAssembleThreads / 64][256];    // 4 KiB per wave, by stage_slot
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t (*shf)[4][256] = c_crcshift;     // read through the caches: ~4 lookups per level
     // the CRC tables (one entry of each per thread): SELF stores them in its
     // scan, before the scan's first barrier, so that their loads and the
     // scan's are in flight together; otherwise they are stored here
     static_assert(kAssembleThreads == 1024, "one entry of each table per thread");
     const uint32_t t_crc = (&c_crc[0][0])[tid], t_s32 = (&c_crcshift32[0][0])[tid];
-    if constexpr (!SELF) {
+    uint32_t t_shf[kAsmShiftLevels];
+#pragma unroll
+    for (int k = 0; k < kAsmShiftLevels; k++) t_shf[k] = (&c_crcshift[k][0][0])[tid];
+    auto store_tables = [&]() {
         (&tab[0][0])[tid] = t_crc;
         (&tab32[0][0])[tid] = t_s32;
+#pragma unroll
+        for (int k = 0; k < kAsmShiftLevels; k++) (&shf[k][0][0])[tid] = t_shf[k];
+    };
+    if constexpr (!SELF) {
+        store_tables();
         __syncthreads();
     }
 
@@ -2796,8 +2810,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             v = seg_len_bytes(tb, (i == S.nseg - 1) && S.final_block, S.eob_in_bits);
         }
         for (int k = tid; k < S.nslice; k += kAssembleThreads) adler_terms(S, k, sa, sb);
-        (&tab[0][0])[tid] = t_crc;
-        (&tab32[0][0])[tid] = t_s32;
+        store_tables();
         const uint32_t incl = wave_incl_scan(v);
         sa = wave_total(sa) % M;
         sb = wave_total(sb) % M;
@@ -2856,7 +2869,23 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         // the segment of the wave's first byte, then each lane's by a short walk
         const int64_t wfirst = max64(z0, z1 - (int64_t)(kAssembleThreads - 64 * wave) * kAsmBytes);
         const int sw = wave_find_seg(A.seg_off, A.nseg, min64(max64(wfirst - hdr, 0), max64(segbytes - 1, 0)));
-        uint32_t r = 0;
+        // the chain of the piece holding the chunk's first byte starts at
+        // the register after "IDAT"
+        const uint32_t r_in = (A.mode == 1 && b == z0 && b < e) ? kIdatRegister : 0u;
+        uint32_t r = r_in;
+        // a full 64-byte piece: two independent 32-byte chains, joined by the
+        // 32-byte shift
+        auto crc_piece = [&](const u32x4* v) {
+            uint32_t r0 = r_in, r1 = 0;
+#pragma unroll
+            for (int jg = 0; jg < 2; jg++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    r0 = crc_step4(r0, v[jg][u], tab);
+                    r1 = crc_step4(r1, v[jg + 2][u], tab);
+                }
+            return crc_shift(r0, tab32) ^ r1;
+        };
         bool whole = false;
         uint32_t slow = 0;                         // this piece's groups that the wave gathers
         int sc = 0, kc = 0;                        // segment and slice cursors at the first of them
@@ -2880,87 +2909,57 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             if (whole) {
 #pragma unroll
                 for (int jg = 0; jg < 4; jg++) stage[wave][stage_slot(lane, jg)] = v[jg];
-                // two independent 32-byte chains, joined by the 32-byte shift
-                uint32_t r0 = 0, r1 = 0;
-#pragma unroll
-                for (int jg = 0; jg < 2; jg++)
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        r0 = crc_step4(r0, v[jg][u], tab);
-                        r1 = crc_step4(r1, v[jg + 2][u], tab);
-                    }
-                r = crc_shift(r0, tab32) ^ r1;
+                r = crc_piece(v);
             }
-            // otherwise 16-byte groups, each from one slice by one read where
-            // it can be; the others (across a slice or segment boundary, or
-            // with framing bytes) are gathered by the wave below
-#pragma unroll 1
-            for (int jg = 0; jg < 4 && !whole; jg++) {
-                const int64_t q = b + 16 * jg;
-                if (q >= e) break;
-                const int64_t a = q - hdr;
-                // inside one segment's bytes (no zlib framing): from its token bits
-                bool inseg = q + 16 <= e && a >= 0 && a + 16 <= segbytes;
-                uint32_t p = 0;
-                if (inseg) {
-                    while ((int64_t)A.seg_off[s + 1] <= a) s++;
-                    inseg = a + 16 <= (int64_t)A.seg_off[s + 1];
-                    p = 8 * (uint32_t)(a - (int64_t)A.seg_off[s]);
-                    const int k1 = min(s * A.spb + A.spb, A.nslice);
-                    k = max(k, s * A.spb);
-                    while (k + 1 < k1 && A.slice_boff[k + 1] <= p) k++;
-                }
-                u32x4 vg;
-                if (inseg && seg_bytes<1>(A, s, p, k, &vg)) {
-                    stage[wave][stage_slot(lane, jg)] = vg;
-                } else {
-                    if (!slow) { sc = s; kc = k; }
-                    slow |= 1u << jg;
-                }
+            // otherwise (a piece across a segment boundary, with framing
+            // bytes, short, or over three slices) its 16-byte groups are
+            // gathered by the wave below, all loads of a round in flight
+            // together
+            if (!whole) {
+                sc = s; kc = k;
+                slow = (1u << (int)((e - b + 15) / 16)) - 1u;
             }
         }
-        // The gathered groups, up to four at a time for the whole wave: lanes
-        // 16j .. 16j + 15 take the j-th pending group (in lane, then group
-        // order), lane 16j + i looking up payload byte i of it (the lanes'
-        // loads in flight together, where one lane would wait for each 4
-        // bytes in turn); the bytes are packed by shuffles and lanes 16j + 0,
-        // 4, 8, 12 store the group's dwords into its owner's stage slot.
+        // The gathered groups, up to 16 at a time for the whole wave: lanes
+        // 4j .. 4j + 3 take the j-th pending group (in lane, then group
+        // order), lane 4j + i assembling dword i of it (the lanes' loads in
+        // flight together, where one lane would wait for each group in turn)
+        // and storing it into the group's stage slot.
         for (uint32_t pend = slow;;) {
             uint64_t ml = __ballot(pend != 0);
             if (ml == 0) break;
-            int srcs[4] = {0, 0, 0, 0}, jgs[4] = {0, 0, 0, 0}, cnt = 0;
+            const int js = lane >> 2, di = lane & 3;
+            int src = 0, jg = 0, cnt = 0, cl = 0;
             uint32_t cur = 0;
-            int cl = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {          // wave-uniform picks
-                if (cur == 0 && ml) {
+            for (int j = 0; j < 16; j++) {         // wave-uniform picks
+                if (cur == 0) {
+                    if (ml == 0) break;
                     cl = __builtin_ctzll(ml);
                     ml &= ml - 1;
                     cur = (uint32_t)__builtin_amdgcn_readlane((int)pend, cl);
                 }
-                if (cur == 0) break;
-                srcs[j] = cl; jgs[j] = __builtin_ctz(cur); cur &= cur - 1; cnt++;
+                const int gj = __builtin_ctz(cur);
+                cur &= cur - 1;
+                if (js == j) { src = cl; jg = gj; }
+                if (lane == cl) pend &= ~(1u << gj);
+                cnt++;
             }
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (j < cnt && lane == srcs[j]) pend &= ~(1u << jgs[j]);
-            const int js = lane >> 4, bi = lane & 15;
-            const int src = js == 0 ? srcs[0] : js == 1 ? srcs[1] : js == 2 ? srcs[2] : srcs[3];
-            const int jg = js == 0 ? jgs[0] : js == 1 ? jgs[1] : js == 2 ? jgs[2] : jgs[3];
             // (every shuffle runs on all lanes: ds_bpermute reads zero from a
             // source lane that is off in EXEC)
             const int64_t q = (int64_t)__shfl((long long)b, src, 64) + 16 * jg;
             const int64_t e_src = (int64_t)__shfl((long long)e, src, 64);
             int s2 = __shfl(sc, src, 64), k2 = __shfl(kc, src, 64);
-            const int nn = js < cnt ? (int)min64(16, e_src - q) : 0;
-            uint32_t x = 0;
-            if (bi < nn) x = (payload16(A, hdr, segbytes, q + bi, 1, s2, k2)[0] & 0xFFu) << (8 * (bi & 3));
-            x |= (uint32_t)__shfl_down((int)x, 1, 64);
-            x |= (uint32_t)__shfl_down((int)x, 2, 64);
-            if (bi < nn && (bi & 3) == 0) ((uint32_t*)&stage[wave][stage_slot(src, jg)])[bi >> 2] = x;
+            const int nn = js < cnt ? (int)min64(4, e_src - q - 4 * di) : 0;
+            if (nn > 0)
+                ((uint32_t*)&stage[wave][stage_slot(src, jg)])[di] = payload16(A, hdr, segbytes, q + 4 * di, nn, s2, k2)[0];
         }
         // the CRC of a piece that is not whole, over its staged groups in order
-        if (b < e && !whole) {
+        if (b < e && !whole && e - b == kAsmBytes) {
+            u32x4 v[4];
+#pragma unroll
+            for (int jg = 0; jg < 4; jg++) v[jg] = stage[wave][stage_slot(lane, jg)];
+            r = crc_piece(v);
+        } else if (b < e && !whole) {
 #pragma unroll 1
             for (int jg = 0; jg < 4; jg++) {
                 const int64_t q = b + 16 * jg;
@@ -3028,22 +3027,11 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 ih[12] = A.bit_depth; ih[13] = A.color_type; ih[14] = 0; ih[15] = 0; ih[16] = 0;
                 put_be32(A.out + 8, 13);
                 for (int i = 0; i < 17; i++) A.out[12 + i] = ih[i];
-                put_be32(A.out + 29, crc_bytes(0, ih, 17, tab));
-            }
-            // g * x^(8n), g the register after "IDAT": given for a full chunk,
-            // else n = 64q + rem as rem zero-byte steps and the set bits of q
-            uint32_t gsh = A.idat_full;
-            if (n != A.chunk_bytes) {
-                gsh = ~crc_bytes(0, (const uint8_t*)"IDAT", 4, tab);
-                int q = n / kAsmBytes, rem = n % kAsmBytes;
-                if (q >= (1 << kCrcShiftLevels)) { q -= 1; rem += kAsmBytes; }
-                for (int i = 0; i < rem; i++) gsh = tab[0][gsh & 0xFF] ^ (gsh >> 8);
-                for (int k = 0; k < kCrcShiftLevels; k++)
-                    if (q & (1 << k)) gsh = crc_shift(gsh, shf[k]);
+                put_be32(A.out + 29, A.ihdr_crc);
             }
             put_be32(o, (uint32_t)n);
             o[4] = 'I'; o[5] = 'D'; o[6] = 'A'; o[7] = 'T';
-            put_be32(o + 8 + n, ~(gsh ^ r));
+            put_be32(o + 8 + n, ~r);
             if (c == nch - 1) {
                 uint8_t* ie = o + 12 + n;
                 put_be32(ie, 0);
