@@ -189,6 +189,7 @@ struct AssembleLaunch {
 
 constexpr int kFilterThreads = 256;
 constexpr int kDeflateThreads = 256;
+constexpr int kXcdRun = 8;              // consecutive DEFLATE slices dealt to one XCD
 constexpr int kAssembleThreads = 1024;
 constexpr int kAsmScanSlices = 2048;  // a single job of up to this many slices and
 constexpr int kAsmScanSegs = 1024;    // segments is scanned by k_assemble itself

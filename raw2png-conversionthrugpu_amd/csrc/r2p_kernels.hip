@@ -1449,6 +1449,21 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
 // not carry their registers.  DYN: the parse's tokens and their symbol
 // histogram go to memory (DeflateLaunch::tokens / tok_count / hist) for a
 // dynamic-Huffman block per segment; no bits are emitted here.
+// Slice of workgroup b: workgroups are dealt to the 8 XCDs in turn (b on XCD
+// b % 8), so consecutive b run on different L2s.  Mapping b to slices in
+// runs of kXcdRun consecutive slices per XCD puts most of a slice's 32 KiB
+// window (the slices just before it) in the L2 of its own XCD, where the
+// neighbouring workgroups have read it, while the runs stay interleaved
+// across the XCDs so that their loads remain even.  A bijection on
+// [0, total): the slices past the last whole 8 * kXcdRun keep their order.
+__device__ __forceinline__ uint32_t xcd_slice(uint32_t b, uint32_t total)
+{
+    constexpr uint32_t X = 8, R = kXcdRun;
+    if (R <= 1 || b >= total / (X * R) * (X * R)) return b;
+    const uint32_t x = b % X, i = b / X;
+    return ((i / R) * X + x) * R + i % R;
+}
+
 template <int NT, bool STATS, bool DYN, int SS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_deflate(DeflateLaunch DL)
 {
@@ -1472,11 +1487,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // so T1's tag counts up per slice and T2's counts down per sub-block.
     uint32_t tag1 = kTagMax, tag2 = 0;
     for (int i = tid; i < OBUF; i += NT) obuf[i] = 0;   // then kept zero by every flush
-    // Slices are dealt to workgroups in order.  (Giving each of the 8 XCDs a
-    // contiguous run of slices, so that the 32 KiB windows hit its L2, cut
-    // the HBM reads to the input size but ran 35% slower on a radiograph:
-    // neighbouring slices cost alike, so the XCDs ended unevenly loaded.)
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+    // Slices are dealt to workgroups in runs per XCD (xcd_slice).  (Giving
+    // each of the 8 XCDs one contiguous eighth of the slices cut the HBM reads
+    // to the input size but ran 35% slower on a radiograph: neighbouring
+    // slices cost alike, so the XCDs ended unevenly loaded.)
+    for (uint32_t g0 = blockIdx.x; g0 < total; g0 += gridDim.x) {
+        const uint32_t g = xcd_slice(g0, total);
         int j = 0;
         if (DL.jobs) {
             int lo = 0, hi = njobs - 1;
