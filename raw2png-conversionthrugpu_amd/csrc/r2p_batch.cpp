@@ -167,10 +167,10 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     L.jobs = b->d_fj; L.band_start = b->d_band_start; L.njobs = 0; L.band_rows = b->band_rows;
     L.total_bands = 0; L.row_cap = b->row_cap; L.lut_cap = b->lut_cap; L.counts = b->d_counts;
     if (timed) R2P_HIP(hipEventRecord(b->ev[0], s));
-    R2P_HIP(launch_filter(L, 4096, filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap), s));
+    R2P_HIP(launch_filter(L, filter_grid(L), filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap), s));
     FilterLaunch LW = L;                          // the wide rows: bands after the narrow ones
     LW.band_rows = b->band_rows_w; LW.row_cap = b->row_cap_w; LW.counts = b->d_counts + 3;
-    R2P_HIP(launch_filter(LW, 4096, filter_lds_bytes(LW.band_rows, LW.row_cap, LW.lut_cap), s));
+    R2P_HIP(launch_filter(LW, filter_grid(LW), filter_lds_bytes(LW.band_rows, LW.row_cap, LW.lut_cap), s));
     FilterLaunch LH = L;                          // rows too wide for LDS: one row per workgroup, tiled
     LH.band_rows = 1; LH.row_cap = kTileBytes; LH.counts = b->d_counts + 10;
     R2P_HIP(launch_filter_tiled(LH, 2048, filter_tiled_lds_bytes(LH.lut_cap), s));
@@ -599,7 +599,7 @@ int r2p_batch_autotune(r2p_batch* b, r2p_options* chosen, double* best_ms)
     best.slice_bytes = 0;
     double best_t = 1e30;
     static const int shapes[] = {256, 512};   // DEFLATE wg_threads
-    static const int bands[] = {4, 8, 16};
+    static const int bands[] = {0, 2, 8};     // filter band rows (0: by row width)
     for (int sh : shapes)
         for (int bd : bands) {
             r2p_options o = best;

@@ -114,7 +114,7 @@ static bool g_dev_ready[64];
 static int g_cu[64];
 
 // Compute units of the current device (256 on MI355X).
-static int cu_count()
+int cu_count()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_cu[dev] <= 0) return 256;
@@ -328,14 +328,49 @@ size_t filter_tiled_lds_bytes(int lut_cap)
 
 int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap)
 {
-    // default: one row per wave of the workgroup -- the band's rows are
-    // filtered in one pass, and the small LDS footprint keeps several
-    // workgroups per CU (on the mixed batch, 8 and 16 rows measured slower)
     (void)total_rows;
-    int h = requested > 0 ? requested : kFilterThreads / 64;
-    // the band and the row above it must fit the LDS beside the table
-    while (h > 1 && filter_lds_bytes(h, row_cap, lut_cap) > (size_t)kFilterLdsMax) h--;
-    return h;
+    auto fits = [&](int br) { return filter_lds_bytes(br, row_cap, lut_cap) <= (size_t)kFilterLdsMax; };
+    if (!filter_split(row_cap)) {
+        // k_filter: one row per wave of the workgroup by default -- the
+        // band's rows are filtered in one pass, and the small LDS footprint
+        // keeps several workgroups per CU (on the mixed batch, 8 and 16 rows
+        // measured slower)
+        int h = requested > 0 ? requested : kFilterThreads / 64;
+        while (h > 1 && !fits(h)) h--;
+        return h;
+    }
+    // k_filter_split: a band of BR rows is filtered by 8 / BR waves per row,
+    // K items per lane: the height that leaves the fewest idle lanes (then
+    // the fewest items per lane), with its BR + 1 rows in LDS beside the table
+    const int items = (row_cap + 15) / 16;
+    if (requested > 0) {
+        int br = 1;
+        while (br * 2 <= std::min(requested, kFilterRowThreads / 64)) br *= 2;
+        while (br > 1 && (!fits(br) || filter_items_per_lane(items, br) > kFilterMaxK)) br >>= 1;
+        return br;
+    }
+    int best = 1, best_k = 1 << 30;
+    double best_u = -1.0;
+    for (int br = kFilterRowThreads / 64; br >= 1; br >>= 1) {
+        const int k = filter_items_per_lane(items, br);
+        if (!fits(br) || k > kFilterMaxK) continue;
+        const double u = (double)items / ((double)k * 64.0 * (kFilterRowThreads / 64 / br));
+        if (u > best_u + 1e-9 || (u > best_u - 1e-9 && k < best_k)) { best = br; best_u = u; best_k = k; }
+    }
+    return best;
+}
+
+
+int filter_grid(const FilterLaunch& L)
+{
+    // k_filter_split is persistent (kFilterSplitWgPerCu workgroups per CU,
+    // each a run of bands); k_filter takes one band per workgroup, or loops
+    // over the bands of a job table whose count is read on the device
+    if (filter_split(L.row_cap)) {
+        const uint32_t g = (uint32_t)(kFilterSplitWgPerCu * cu_count());
+        return (int)(L.counts ? g : std::min<uint32_t>(L.total_bands, g));
+    }
+    return (int)(L.counts ? 4096u : std::min<uint32_t>(L.total_bands, 8192u));
 }
 
 int resolve_slice_bytes(const r2p_options& o, size_t filtered)
@@ -539,8 +574,7 @@ static int launch_filter_job(const FilterJob& J, int band_rows, hipStream_t s)
     L.single = J;
     L.single.band_rows = L.band_rows;
     const size_t lds = filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap);
-    const int grid = (int)std::min<uint32_t>(bands, 8192);
-    R2P_HIP(launch_filter(L, grid, lds, s));
+    R2P_HIP(launch_filter(L, filter_grid(L), lds, s));
     return R2P_OK;
 }
 

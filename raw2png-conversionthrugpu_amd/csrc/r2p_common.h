@@ -84,6 +84,10 @@ uint32_t ihdr_crc(uint32_t width, uint32_t height, uint8_t bit_depth, uint8_t co
 size_t filter_lds_bytes(int band_rows, int row_cap, int lut_cap);
 size_t filter_tiled_lds_bytes(int lut_cap);
 int filter_band_rows(int requested, int64_t total_rows, int row_cap, int lut_cap);
+// Compute units of the current device (after r2p_init; 256 when unknown).
+int cu_count();
+// Workgroups of a filter launch (k_filter or k_filter_split by the row width).
+int filter_grid(const FilterLaunch& L);
 void fill_defaults(r2p_options* o);
 // DEFLATE slice size for a stream of `filtered` bytes: the options' value,
 // or (0, auto) 16, 8 or 4 KiB -- the largest that still gives about 512

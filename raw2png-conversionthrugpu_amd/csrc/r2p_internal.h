@@ -187,7 +187,21 @@ struct AssembleLaunch {
     AssembleJob        single;
 };
 
-constexpr int kFilterThreads = 256;
+constexpr int kFilterThreads = 256;      // k_filter (one row per wave) and k_filter_tiled
+constexpr int kFilterBandRowBytes = 6144; // rows up to this go to k_filter (at most 6 items per lane)
+constexpr int kFilterRowThreads = 512;   // k_filter_split: 8 waves; band rows x waves per row = 8
+constexpr int kFilterMaxK = 8;           // k_filter_split: most 16-byte items of a row per lane
+constexpr int kFilterPrefetch = 2;       // k_filter_split: raw 16-byte vectors of the next band per thread
+constexpr int kFilterSplitWgPerCu = 2;   // k_filter_split: persistent workgroups per CU
+// Wider rows than k_filter takes (one row per wave, K <= 6): k_filter_split.
+inline bool filter_split(int row_cap) { return row_cap > kFilterBandRowBytes; }
+// k_filter_split's items per lane for rows of `items` 16-byte items in bands
+// of band_rows rows (each row spread over 8 / band_rows waves).
+inline int filter_items_per_lane(int items, int band_rows)
+{
+    const int lanes = 64 * (kFilterRowThreads / 64 / band_rows);
+    return (items + lanes - 1) / lanes;
+}
 constexpr int kDeflateThreads = 256;
 constexpr int kXcdRun = 8;              // consecutive DEFLATE slices dealt to one XCD
 constexpr int kAssembleThreads = 1024;

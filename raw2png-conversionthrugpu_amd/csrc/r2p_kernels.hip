@@ -294,15 +294,18 @@ __device__ __forceinline__ uint32_t cost_dword(uint32_t x, uint32_t a, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// k_filter: fused window + adaptive filter
+// k_filter: fused window + adaptive filter (dicom_preprocess_kernel and
+// filter_select_kernel, gpu_filter.cu:71-309), for rows of up to 6 KiB.
 //
-// A workgroup owns a band of consecutive rows.  For each row it windows the
-// raw input into a PNG-ready row in LDS (the previous row stays resident in
-// the other LDS buffer), scores the five PNG filters on 16-byte items (the
-// even and odd bytes of each dword as packed 16-bit lanes, v_sad_u8 for the
-// |int8| cost), reduces the scores
-// across the workgroup, and writes [filter byte | chosen row].  Ties go to
-// the lowest filter index, as filter_select_kernel (gpu_filter.cu:267-273).
+// A workgroup of 4 waves owns a band of consecutive rows (one per wave by
+// default).  The band and the row above it are windowed from the raw input
+// into PNG-ready rows in LDS (raw loads two or four items in flight), then
+// each wave filters whole rows alone: lane l scores items l, l + 64, ... (K
+// of them) for the five PNG filters (the even and odd bytes of each dword as
+// packed 16-bit lanes, v_sad_u8 for the |int8| cost), five DPP wave
+// reductions pick the filter (ties to the lowest index, as
+// gpu_filter.cu:267-273), and the wave writes [filter byte | chosen row],
+// the Paeth results kept in registers from the scoring pass.
 // ---------------------------------------------------------------------------
 
 
@@ -468,7 +471,7 @@ __device__ __forceinline__ uint32_t byte_mask(int nvalid_in_dword)
 }
 
 template <int K>   // items per lane: ceil(16-byte items of the widest row / 64)
-__global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(K <= 6 ? 4 : 1))) void k_filter(FilterLaunch L)
+__global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_filter(FilterLaunch L)
 {
     constexpr bool KEEP = K <= 8;                         // Paeth results stay in registers
     extern __shared__ u32x4 smem_filter[];
@@ -658,6 +661,268 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
             if (J.last_row && r == J.rows - 1)
                 for (int i = lane; i < rb; i += 64) J.last_row[i] = cur8[i];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_filter_split: the same window + filter for rows wider than k_filter
+// takes (6 KiB to 32 KiB), where one row per wave would hold too many items
+// per lane (and one workgroup per CU, by its registers).
+//
+// Persistent workgroups of 8 waves, kFilterSplitWgPerCu per CU.  A band is BR = band_rows
+// rows (1, 2, 4 or 8, chosen by the row width); each row is filtered by
+// WPR = 8 / BR waves, thread p = (wave % WPR) * 64 + lane of the row taking
+// items p, p + 64 WPR, ... (at most K).  LDS holds the window table and
+// BR + 1 PNG-ready rows: row r of a job in slot (r + 1) mod (BR + 1), each
+// slot a 16-byte zero front pad (the first item's left neighbours), the row
+// and 16 bytes of slack.  A workgroup takes a contiguous run of a single
+// job's bands (a band's row above is then the previous band's last row,
+// already in its slot) or every gridDim.x-th band of a job table.  The raw
+// input of the next band is loaded into registers before the current band
+// is scored, so its latency hides behind the scoring and the write pass; it
+// is windowed into the slots the band no longer needs once the write pass
+// is done (three barriers per band).
+// ---------------------------------------------------------------------------
+template <int K>   // items per lane of a row: ceil(16-byte items / (64 * waves per row))
+__global__ __launch_bounds__(kFilterRowThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_filter_split(FilterLaunch L)
+{
+    constexpr int NT = kFilterRowThreads, NW = NT / 64, PF = kFilterPrefetch, PH = PF / 2;
+    static_assert(NW == 8, "band rows times waves per row is 8");
+    static_assert(PF % 2 == 0, "the narrowing modes prefetch two halves per item");
+    extern __shared__ u32x4 smem_filter[];
+    uint8_t* lut_lds = (uint8_t*)smem_filter;
+    uint8_t* bufs = lut_lds + L.lut_cap;
+    const int rbuf = L.row_cap + 32;                      // 16 B front pad + row + 16 B slack
+    __shared__ uint32_t red[NW][5];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+    const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
+    const uint32_t first = L.counts ? L.counts[2] : 0u;
+    const int njobs = L.counts ? (int)L.counts[0] : L.njobs;
+    if (total <= first) return;
+    // this workgroup's bands: a contiguous run of a single job's bands, or
+    // every gridDim.x-th band of a job table
+    uint32_t g, g_hi, g_step;
+    if (!L.jobs) {
+        const uint64_t nb = total - first;
+        g = first + (uint32_t)(nb * blockIdx.x / gridDim.x);
+        g_hi = first + (uint32_t)(nb * (blockIdx.x + 1) / gridDim.x);
+        g_step = 1;
+    } else {
+        g = first + blockIdx.x; g_hi = total; g_step = gridDim.x;
+    }
+    if (g >= g_hi) return;                                 // uniform per workgroup
+    auto job_of = [&](uint32_t gb) {
+        int lo = 0;
+        if (L.jobs) {
+            int hi = njobs - 1;
+            while (lo < hi) {                              // last job with band_start <= gb
+                const int mid = (lo + hi + 1) >> 1;
+                if (L.band_start[mid] <= gb) lo = mid; else hi = mid - 1;
+            }
+        }
+        return lo;
+    };
+    // the slots' front pads stay zero: row data never reaches them
+    for (int b = tid; b <= L.band_rows; b += NT) *(u32x4*)(bufs + b * rbuf) = (u32x4){0, 0, 0, 0};
+
+    // Raw input of rows [ra, re) of job JB, items flattened over the rows:
+    // thread tid holds items tid + NT u in pre[u] (u < PF), or for the modes
+    // that narrow 16-bit samples both 16-byte halves in pre[u] and pre[u + PF
+    // / 2] (u < PF / 2); items past those are loaded when windowed.
+    u32x4 pre[PF];
+    auto prefetch = [&](const FilterJob& JB, int ra, int re) {
+        const int nf = JB.row_bytes >> 4;
+        const int tot = (re - ra) * nf;
+        const bool nw = narrows(JB.wmode);
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int t = tid + NT * (nw ? u % PH : u);
+            if (t < tot) {
+                const int rr = t / nf, i = t - rr * nf;
+                const uint8_t* srow = JB.src + (int64_t)(ra + rr) * JB.src_pitch;
+                pre[u] = g128u(nw ? srow + 32 * i + 16 * (u / PH) : srow + 16 * i);
+            }
+        }
+    };
+    // PNG-ready bytes of rows [ra, re) of JB into their slots, row r in slot
+    // (r + 1) mod (band rows + 1)
+    auto window_rows = [&](const FilterJob& JB, int ra, int re) {
+        const int nf = JB.row_bytes >> 4, tl = JB.row_bytes & 15, S = JB.band_rows + 1;
+        const int tot = (re - ra) * nf;
+        const bool nw = narrows(JB.wmode);
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int t = tid + NT * u;
+            if (t < tot && (!nw || u < PH)) {
+                const int rr = t / nf, i = t - rr * nf;
+                const RawItem R = nw ? RawItem{pre[u], pre[(u + PH) % PF]} : RawItem{pre[u], pre[u]};
+                *(u32x4*)(bufs + ((ra + rr + 1) % S) * rbuf + 16 + 16 * i) = window_raw(JB, R, lut_lds);
+            }
+        }
+        for (int t = tid + NT * (nw ? PH : PF); t < tot; t += NT) {
+            const int rr = t / nf, i = t - rr * nf;
+            const RawItem R = load_raw(JB, JB.src + (int64_t)(ra + rr) * JB.src_pitch, i);
+            *(u32x4*)(bufs + ((ra + rr + 1) % S) * rbuf + 16 + 16 * i) = window_raw(JB, R, lut_lds);
+        }
+        for (int t = tid; t < (re - ra) * tl; t += NT) {
+            const int rr = t / tl, jj = (nf << 4) + (t - rr * tl);
+            bufs[((ra + rr + 1) % S) * rbuf + 16 + jj] =
+                (uint8_t)window_byte(JB, JB.src + (int64_t)(ra + rr) * JB.src_pitch, jj, lut_lds);
+        }
+    };
+    // the row above a job's first row (slot 0): the caller's prior row or zeros
+    auto above_first = [&](const FilterJob& JB) {
+        uint8_t* prev = bufs + 16;
+        if (JB.prior) for (int i = tid; i < JB.row_bytes; i += NT) prev[i] = JB.prior[i];
+        else for (int i = tid; i < ((JB.row_bytes + 3) >> 2); i += NT) ((uint32_t*)prev)[i] = 0;
+    };
+    auto load_lut = [&](const FilterJob& JB) {
+        const int n16 = (JB.lut_bytes + 15) >> 4;
+        for (int i = tid; i < n16; i += NT) ((u32x4*)lut_lds)[i] = ((const u32x4*)JB.lut)[i];
+    };
+
+    // the first band: its rows and the row above, loaded and windowed
+    int j = job_of(g);
+    FilterJob J = L.jobs ? L.jobs[j] : L.single;
+    int r0 = (int)(g - (L.jobs ? L.band_start[j] : 0u)) * J.band_rows;
+    int nr = min(J.rows, r0 + J.band_rows) - r0;
+    int lut_job = -1;
+    {
+        const int ra = r0 > 0 ? r0 - 1 : r0;
+        prefetch(J, ra, r0 + nr);                          // in flight while the table is copied
+        if (J.lut_bytes > 0) { load_lut(J); lut_job = j; }
+        __syncthreads();
+        if (r0 == 0) above_first(J);
+        window_rows(J, ra, r0 + nr);
+        __syncthreads();
+    }
+
+    for (;;) {
+        // ---- the next band's raw input, in flight during this band's work
+        const uint32_t gn = g + g_step;
+        const bool has_next = gn < g_hi;
+        int jn = j, r0n = 0, nrn = 0, ran = 0;
+        FilterJob JN = J;
+        if (has_next) {
+            jn = job_of(gn);
+            if (jn != j) JN = L.jobs[jn];
+            r0n = (int)(gn - (L.jobs ? L.band_start[jn] : 0u)) * JN.band_rows;
+            nrn = min(JN.rows, r0n + JN.band_rows) - r0n;
+            // the row above is this band's last row, in its slot, when the
+            // next band follows it in the same job
+            const bool reuse = jn == j && r0n == r0 + nr;
+            ran = reuse || r0n == 0 ? r0n : r0n - 1;
+            prefetch(JN, ran, r0n + nrn);
+        }
+
+        // ---- score this band: row jr of the band by waves jr * WPR .. + WPR - 1
+        const int rb = J.row_bytes, nfull = rb >> 4, tail = rb & 15, S = J.band_rows + 1;
+        const int WPR = NW / J.band_rows;
+        const int jr = wave / WPR, part = wave - jr * WPR;
+        const bool active = jr < nr;
+        const int near = J.bpp <= 4;                      // neighbour within 1 dword back
+        const uint32_t sh = (uint32_t)((4 - (J.bpp & 3)) & 3);
+        const uint8_t* cur8 = bufs + ((r0 + jr + 1) % S) * rbuf + 16;
+        const uint32_t* cur = (const uint32_t*)cur8;
+        const uint32_t* prv = (const uint32_t*)(bufs + ((r0 + jr) % S) * rbuf + 16);
+        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+        uint32_t f4c[K][4];
+        // scores of the item at i; its first nv bytes count
+        auto score = [&](int i, int nv, uint32_t* f4o) {
+            uint32_t dc[6], dp[6], a[4], c[4];
+            load_item(cur, i, dc);
+            load_item(prv, i, dp);
+            left4(dc, near, sh, a);
+            left4(dp, near, sh, c);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t m = nv >= 16 ? ~0u : byte_mask(nv - 4 * q);
+                uint32_t sv[5] = {s0, s1, s2, s3, s4};
+                f4o[q] = cost_dword(dc[2 + q], a[q], dp[2 + q], c[q], m, sv);
+                s0 = sv[0]; s1 = sv[1]; s2 = sv[2]; s3 = sv[3]; s4 = sv[4];
+            }
+        };
+        const int p0 = part * 64 + lane, pstep = 64 * WPR;
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = p0 + pstep * k;
+                if (i < nfull) score(i, 16, f4c[k]);
+                else if (i == nfull && tail) score(i, tail, f4c[k]);
+            }
+        }
+        {
+            const uint32_t sc[5] = {wave_total(s0), wave_total(s1), wave_total(s2), wave_total(s3), wave_total(s4)};
+            if (lane == 0)
+#pragma unroll
+                for (int f = 0; f < 5; f++) red[wave][f] = sc[f];
+        }
+        __syncthreads();
+
+        // ---- the row's filter (lowest index on ties) and its bytes
+        if (active) {
+            uint32_t sc[5] = {0, 0, 0, 0, 0};
+            for (int w = jr * WPR; w < jr * WPR + WPR; w++)
+#pragma unroll
+                for (int f = 0; f < 5; f++) sc[f] += red[w][f];
+            int best = 0; uint32_t bsc = sc[0];
+#pragma unroll
+            for (int f = 1; f < 5; f++) if (sc[f] < bsc) { bsc = sc[f]; best = f; }
+            const int r = r0 + jr;
+            uint8_t* dst = J.dst + (int64_t)r * (rb + 1);
+            if (part == 0 && lane == 0) dst[0] = (uint8_t)best;
+            // filtered bytes of the item at i (f4: its Paeth result)
+            auto filtered = [&](int i, const uint32_t* f4, uint32_t* o) {
+                if (best == 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) o[q] = f4[q];
+                    return;
+                }
+                uint32_t dc[6], dp[6], a[4];
+                load_item(cur, i, dc);
+                load_item(prv, i, dp);
+                left4(dc, near, sh, a);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t x = dc[2 + q], b = dp[2 + q];
+                    o[q] = best == 0 ? x : best == 1 ? sub8(x, a[q]) : best == 2 ? sub8(x, b) : sub8(x, avg8(a[q], b));
+                }
+            };
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = p0 + pstep * k;
+                if (i < nfull) {
+                    uint32_t o[4];
+                    filtered(i, f4c[k], o);
+                    *(u32x4_ua*)(dst + 1 + 16 * i) = (u32x4){o[0], o[1], o[2], o[3]};
+                } else if (i == nfull && tail) {
+                    uint32_t o[4];
+                    filtered(i, f4c[k], o);
+                    uint8_t* p = dst + 1 + 16 * nfull;
+                    for (int t = 0; t < tail; t++) p[t] = (uint8_t)(o[t >> 2] >> (8 * (t & 3)));
+                }
+            }
+            if (J.last_row && r == J.rows - 1)
+                for (int i = p0; i < rb; i += pstep) J.last_row[i] = cur8[i];
+        }
+        if (!has_next) break;
+        __syncthreads();                                   // every read of this band's slots done
+
+        // ---- the next band's rows into their slots
+        if (jn != j) {
+            J = JN;
+            if (J.lut_bytes > 0 && lut_job != jn) {
+                load_lut(J);
+                lut_job = jn;
+                __syncthreads();
+            }
+            j = jn;
+        }
+        if (r0n == 0) above_first(J);
+        window_rows(J, ran, r0n + nrn);
+        g = gn; r0 = r0n; nr = nrn;
+        __syncthreads();
     }
 }
 
@@ -3091,23 +3356,43 @@ static hipError_t launch_filter_k(const FilterLaunch& L, int grid, size_t lds, h
     return hipGetLastError();
 }
 
-// Instantiation by the widest row: lane l of a wave takes 16-byte items
-// l, l + 64, ... of a row, K of them for rows of up to K * 1024 bytes.
+template <int K>
+static hipError_t launch_filter_split_k(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
+{
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_filter_split<K>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    k_filter_split<K><<<grid, kFilterRowThreads, lds, s>>>(L);
+    return hipGetLastError();
+}
+
+// Instantiation by the widest row: k_filter's lane l of a wave takes items
+// l, l + 64, ... of a row (K of them, rows up to K KiB); k_filter_split
+// spreads a row over the lanes of 8 / band_rows waves.
 hipError_t launch_filter(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
 {
     (void)hipGetLastError();
     const int items = (L.row_cap + 15) / 16;
-    const int k = (items + 63) / 64;
-    if (k <= 1) return launch_filter_k<1>(L, grid, lds, s);
-    if (k <= 2) return launch_filter_k<2>(L, grid, lds, s);
-    if (k <= 3) return launch_filter_k<3>(L, grid, lds, s);
-    if (k <= 4) return launch_filter_k<4>(L, grid, lds, s);
-    if (k <= 6) return launch_filter_k<6>(L, grid, lds, s);
-    if (k <= 8) return launch_filter_k<8>(L, grid, lds, s);
-    if (k <= 12) return launch_filter_k<12>(L, grid, lds, s);
-    if (k <= 16) return launch_filter_k<16>(L, grid, lds, s);
-    if (k <= 24) return launch_filter_k<24>(L, grid, lds, s);
-    return launch_filter_k<32>(L, grid, lds, s);
+    if (!filter_split(L.row_cap)) {
+        const int k = (items + 63) / 64;
+        if (k <= 1) return launch_filter_k<1>(L, grid, lds, s);
+        if (k <= 2) return launch_filter_k<2>(L, grid, lds, s);
+        if (k <= 3) return launch_filter_k<3>(L, grid, lds, s);
+        if (k <= 4) return launch_filter_k<4>(L, grid, lds, s);
+        return launch_filter_k<6>(L, grid, lds, s);
+    }
+    const int k = filter_items_per_lane(items, L.band_rows);
+    if (L.band_rows < 1 || L.band_rows > kFilterRowThreads / 64 || (L.band_rows & (L.band_rows - 1)) ||
+        k > kFilterMaxK)
+        return hipErrorInvalidValue;
+    if (k <= 2) return launch_filter_split_k<2>(L, grid, lds, s);
+    if (k <= 3) return launch_filter_split_k<3>(L, grid, lds, s);
+    if (k <= 4) return launch_filter_split_k<4>(L, grid, lds, s);
+    if (k <= 5) return launch_filter_split_k<5>(L, grid, lds, s);
+    if (k <= 6) return launch_filter_split_k<6>(L, grid, lds, s);
+    return launch_filter_split_k<8>(L, grid, lds, s);
 }
 
 hipError_t launch_filter_tiled(const FilterLaunch& L, int grid, size_t lds, hipStream_t s)
