@@ -85,7 +85,10 @@ def test_filter_ready_modes(torch_dev, r2p, oracle, name, w, h, ch, bits, mode):
     assert np.array_equal(last, ready[(h - 1) * rb:])
 
 
-@pytest.mark.parametrize("band_rows,w", [(1, 300), (3, 300), (16, 300), (0, 12000)])
+# rows up to 6 KiB: one row per wave; 6-32 KiB: rows spread over waves in
+# persistent workgroups (band heights 1, 2, 4, 8); wider: tiled
+@pytest.mark.parametrize("band_rows,w", [(1, 300), (3, 300), (16, 300), (0, 12000), (1, 3000), (3, 2500),
+                                         (16, 4000), (0, 2200), (2, 10000)])
 def test_filter_bands_and_prior(torch_dev, r2p, oracle, band_rows, w):
     log(f"case filter bands={band_rows} w={w}")
     rng = np.random.default_rng(band_rows)
@@ -124,11 +127,14 @@ DICOM_CASES = [
 ]
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["rows512", "rows8k"])
 @pytest.mark.parametrize("case", DICOM_CASES, ids=[c[0] for c in DICOM_CASES])
-def test_window_exhaustive(torch_dev, r2p, oracle, case):
-    """Every stored code (plus garbage high bits) through the fused kernel."""
+def test_window_exhaustive(torch_dev, r2p, oracle, case, wide):
+    """Every stored code (plus garbage high bits) through the fused kernel,
+    in rows of 512 samples and in 8 KiB rows (the kernel that spreads a row
+    over several waves)."""
     name, ba, bs, hb, rep, ar, sl, ic, aw, wc, ww, mode = case
-    log(f"case window {name}")
+    log(f"case window {name} wide={wide}")
     p = _dicom(r2p, ba, bs, hb, rep, ar, sl, ic, aw, wc, ww)
     if ba == 8:
         codes = np.arange(256, dtype=np.uint8)
@@ -140,6 +146,8 @@ def test_window_exhaustive(torch_dev, r2p, oracle, case):
             codes = codes | (np.arange(n, dtype=np.uint32) % 7 << 13) * (bs <= 12)  # garbage high bits
         codes = codes.astype("<u2")
         w = 512
+    if wide:
+        w = 8192 if mode == 3 or ba == 8 else 4096      # 8 KiB PNG-ready rows
     h = (codes.size + w - 1) // w
     px = np.resize(codes, h * w).reshape(h, w)
     img = r2p.Image.make(w, h, 1, ba, mode, p)
