@@ -735,6 +735,7 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     if (o.huffman) { D.tokens = w.tokens; D.tok_count = w.tok_count; D.hist = w.hist; }
     DL.dyn = o.huffman;
     DL.dense = o.lz77 == 2;
+    DL.prio_cus = cu_count();
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
     if (o.huffman) {                       // the segments' codes, then the slices' bits

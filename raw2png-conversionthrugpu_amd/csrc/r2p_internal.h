@@ -145,6 +145,8 @@ struct DeflateLaunch {
     DeflateJob        single;
     int32_t           dyn;         // the jobs' dynamic-Huffman form (k_deflate<.., true, ..>)
     int32_t           dense;       // hashed search at every 2nd position (r2p_options.lz77 = 2), else every 4th
+    int32_t           prio_cus;    // one workgroup per slice: the device's CUs (wave priorities by
+                                   // dispatch round, below); 0: none (persistent workgroups)
 };
 
 // Dynamic-Huffman blocks of one job (after k_deflate<.., true>): per

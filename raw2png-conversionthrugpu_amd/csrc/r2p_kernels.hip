@@ -1745,6 +1745,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // With one workgroup per slice, the workgroups of dispatch round r
+    // (blockIdx / CUs: the r-th resident on a CU, or, past the resident
+    // ones, a late one taking a freed slot) run at wave priority min(r, 3).
+    // The SIMDs otherwise favour the oldest waves: a CU's first workgroup
+    // would finish long before its last, and the slices dispatched after the
+    // resident ones would start behind all of them.
+    if (DL.prio_cus > 0) {
+        switch (min(3u, blockIdx.x / (uint32_t)DL.prio_cus)) {
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        case 3: __builtin_amdgcn_s_setprio(3); break;
+        default: break;
+        }
+    }
 
     const uint32_t total = DL.counts ? DL.counts[1] : DL.total_slices;
     const int njobs = DL.counts ? (int)DL.counts[0] : DL.njobs;

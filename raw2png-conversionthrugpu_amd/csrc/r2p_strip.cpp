@@ -148,6 +148,7 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     D.final_block = is_last ? 1 : 0; D.bpp = bpp > 0 ? bpp : 1; D.row_stride = row_stride;
     D.lz77 = d->lz77; D.nslice = sg.nslice;
     DL.dense = d->lz77 == 2;
+    DL.prio_cus = cu_count();
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, 16, 256, sg.nslice, s));
 
