@@ -179,6 +179,7 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     DeflateLaunch DL{};
     DL.jobs = b->d_dj; DL.slice_start = b->d_slice_start; DL.counts = b->d_counts + 6; DL.dyn = b->opt.huffman;
     DL.dense = b->opt.lz77 == 2;
+    DL.queue = b->d_counts + 16;          // zero: uploaded with the tables
     R2P_HIP(launch_deflate(DL, b->opt.lane_bytes, b->opt.wg_threads, 1024, s));
     if (b->opt.huffman) {                 // every segment's codes, then every slice's bits
         HuffLaunch HL{};
@@ -231,7 +232,7 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     b->off_cs = b->off_ss + 4 * ((size_t)max_images + 1);
     b->off_gs = b->off_cs + 4 * ((size_t)max_images + 1);
     b->off_cnt = b->off_gs + 4 * ((size_t)max_images + 1);
-    b->tab_bytes = b->off_cnt + 64;
+    b->tab_bytes = b->off_cnt + 128;
     hipError_t e;
 #define TRY(x) do { if ((e = (x)) != hipSuccess) { r2p_batch_destroy(b); return hip_fail(e, #x); } } while (0)
     TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -473,6 +474,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     cnt[6] = (uint32_t)n; cnt[7] = (uint32_t)slices;
     cnt[8] = (uint32_t)n; cnt[9] = chunks;
     cnt[13] = (uint32_t)n; cnt[14] = (uint32_t)segs; cnt[15] = (uint32_t)slices;   // dynamic-Huffman tables, emit
+    cnt[16] = 0;                                                    // deflate slice queue
     b->n = n;
     // build the distinct tables now (outside the graph)
     for (auto& kv : b->luts) {

@@ -147,6 +147,8 @@ struct DeflateLaunch {
     int32_t           dense;       // hashed search at every 2nd position (r2p_options.lz77 = 2), else every 4th
     int32_t           prio_cus;    // one workgroup per slice: the device's CUs (wave priorities by
                                    // dispatch round, below); 0: none (persistent workgroups)
+    uint32_t*         queue;       // persistent workgroups: a counter, zero at launch, that hands out
+                                   // the slices after each workgroup's first; null: every grid-th slice
 };
 
 // Dynamic-Huffman blocks of one job (after k_deflate<.., true>): per

@@ -1744,6 +1744,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ uint32_t obuf[OBUF];
     __shared__ uint32_t red[3 * NW + 64];
     __shared__ uint32_t st[4];
+    __shared__ uint32_t next_slice;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // With one workgroup per slice, the workgroups of dispatch round r
     // (blockIdx / CUs: the r-th resident on a CU, or, past the resident
@@ -1769,8 +1770,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // Slices are dealt to workgroups in runs per XCD (xcd_slice).  (Giving
     // each of the 8 XCDs one contiguous eighth of the slices cut the HBM reads
     // to the input size but ran 35% slower on a radiograph: neighbouring
-    // slices cost alike, so the XCDs ended unevenly loaded.)
-    for (uint32_t g0 = blockIdx.x; g0 < total; g0 += gridDim.x) {
+    // slices cost alike, so the XCDs ended unevenly loaded.)  Persistent
+    // workgroups with a queue take their first slice by blockIdx and the
+    // rest from the queue's counter, so that the faster ones take more.
+    for (uint32_t g0 = blockIdx.x; g0 < total;) {
         const uint32_t g = xcd_slice(g0, total);
         int j = 0;
         if (DL.jobs) {
@@ -2130,6 +2133,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         const uint32_t ad = adler_finish(ad_s, ad_t, s1 - s0, red);
         if (tid == 0) J.slice_adler[sl] = ad;
+        if (DL.queue) {
+            // (the loop's first barrier keeps this write after every read of
+            // the previous value)
+            if (tid == 0) next_slice = gridDim.x + atomicAdd(DL.queue, 1u);
+            __syncthreads();
+            g0 = next_slice;
+        } else {
+            g0 += gridDim.x;
+        }
     }
 }
 
