@@ -166,13 +166,15 @@ static int batch_enqueue(r2p_batch* b, hipStream_t s, bool timed)
     FilterLaunch L{};
     L.jobs = b->d_fj; L.band_start = b->d_band_start; L.njobs = 0; L.band_rows = b->band_rows;
     L.total_bands = 0; L.row_cap = b->row_cap; L.lut_cap = b->lut_cap; L.counts = b->d_counts;
+    L.queue = b->d_counts + 17;           // zero: uploaded with the tables
     if (timed) R2P_HIP(hipEventRecord(b->ev[0], s));
     R2P_HIP(launch_filter(L, filter_grid(L), filter_lds_bytes(L.band_rows, L.row_cap, L.lut_cap), s));
     FilterLaunch LW = L;                          // the wide rows: bands after the narrow ones
     LW.band_rows = b->band_rows_w; LW.row_cap = b->row_cap_w; LW.counts = b->d_counts + 3;
+    LW.queue = filter_split(LW.row_cap) ? nullptr : b->d_counts + 18;
     R2P_HIP(launch_filter(LW, filter_grid(LW), filter_lds_bytes(LW.band_rows, LW.row_cap, LW.lut_cap), s));
     FilterLaunch LH = L;                          // rows too wide for LDS: one row per workgroup, tiled
-    LH.band_rows = 1; LH.row_cap = kTileBytes; LH.counts = b->d_counts + 10;
+    LH.band_rows = 1; LH.row_cap = kTileBytes; LH.counts = b->d_counts + 10; LH.queue = nullptr;
     R2P_HIP(launch_filter_tiled(LH, 2048, filter_tiled_lds_bytes(LH.lut_cap), s));
     if (timed) R2P_HIP(hipEventRecord(b->ev[1], s));
     // 3. deflate over all slices
@@ -475,6 +477,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
     cnt[8] = (uint32_t)n; cnt[9] = chunks;
     cnt[13] = (uint32_t)n; cnt[14] = (uint32_t)segs; cnt[15] = (uint32_t)slices;   // dynamic-Huffman tables, emit
     cnt[16] = 0;                                                    // deflate slice queue
+    cnt[17] = 0; cnt[18] = 0;                                       // filter band queues (narrow, wide)
     b->n = n;
     // build the distinct tables now (outside the graph)
     for (auto& kv : b->luts) {

@@ -133,6 +133,8 @@ struct FilterLaunch {
     int32_t           lut_cap;     // LDS table capacity (bytes, multiple of 16)
     const uint32_t*   counts;      // optional device [njobs, band_end, band_begin]: overrides njobs and
                                    // total_bands, and the launch takes the bands [band_begin, band_end)
+    uint32_t*         queue;       // k_filter over a job table: a counter, zero at launch, that hands out
+                                   // the bands after each workgroup's first; null: every grid-th band
     FilterJob         single;
 };
 

@@ -481,11 +481,13 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     constexpr int NW = kFilterThreads / 64;
 
+    __shared__ uint32_t next_band;
     int cur_job = -1;
     const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
     const uint32_t first = L.counts ? L.counts[2] : 0u;
     const int njobs = L.counts ? (int)L.counts[0] : L.njobs;
-    for (uint32_t g = first + blockIdx.x; g < total; g += gridDim.x) {
+    // bands after a workgroup's first: every grid-th, or from the queue
+    for (uint32_t g = first + blockIdx.x; g < total;) {
         int j = 0;
         if (L.jobs) {
             int lo = 0, hi = njobs - 1;
@@ -660,6 +662,15 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
             }
             if (J.last_row && r == J.rows - 1)
                 for (int i = lane; i < rb; i += 64) J.last_row[i] = cur8[i];
+        }
+        if (L.queue) {
+            // (the loop's first barrier keeps this write after every read of
+            // the previous value)
+            if (tid == 0) next_band = first + gridDim.x + atomicAdd(L.queue, 1u);
+            __syncthreads();
+            g = next_band;
+        } else {
+            g += gridDim.x;
         }
     }
 }
