@@ -222,8 +222,9 @@ int r2p_batch_create(int max_images, size_t max_input_bytes, size_t max_filtered
     // ceil(len / slice) buffers of slice_cap_dwords(slice) dwords
     b->max_segs = max_filtered_bytes / 4096 + (size_t)max_images + 1;
     b->max_slices = b->max_segs;
+    // (+ 20 dwords per slice: two half slices of a full one, slice_geom)
     b->slice_arena_dw = max_filtered_bytes * 9 / 32 + 9 * (max_filtered_bytes / 4096) +
-                        (size_t)max_images * (size_t)(slice_cap_dwords(65536) + 16) + 1024;
+                        (size_t)max_images * (size_t)(slice_cap_dwords(65536) + 16) + 20 * b->max_slices + 1024;
     b->max_out = max_filtered_bytes + max_filtered_bytes / 8 + 32 * b->max_segs + 4096 * (size_t)max_images;
     b->off_fj = 0;
     b->off_dj = b->off_fj + sizeof(FilterJob) * (size_t)max_images;
@@ -398,16 +399,24 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         F.q15_b = wp.q15_b; F.pixel_rep = wp.pixel_rep; F.lut_bytes = wp.lut_kind >= 0 ? wp.lds_bytes : 0;
         F.band_rows = row_class(i) == 0 ? b->band_rows : row_class(i) == 1 ? b->band_rows_w : 1;
 
-        // the slice size of this image: its size class's tuned one, else by its size
+        // the slice size of this image: its size class's tuned one, else by
+        // its size -- then, as in the plain form with the slice size left to
+        // it, the stream's last partial round of slices is cut in half, and
+        // the job's options say "auto" (0)
         int sl = b->class_slice[size_class(g.filtered)];
+        const bool auto_sl = (sl <= 0 || o.segment_bytes % sl) && o.slice_bytes == 0;
         if (sl <= 0 || o.segment_bytes % sl) sl = resolve_slice_bytes(o, g.filtered);
-        b->job_slice[(size_t)i] = sl;
-        const int cap_dw = slice_cap_for(sl, o.huffman);
-        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, sl);
+        b->job_slice[(size_t)i] = auto_sl ? 0 : sl;
+        const int cap_dw = slice_cap_for(sl, o.huffman), cap_dw2 = slice_cap_for(sl / 2, o.huffman);
+        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, sl, auto_sl);
         const int nseg = sg.nseg;
-        if (slice_dw + (size_t)sg.nslice * (size_t)cap_dw > b->slice_arena_dw || slices + sg.nslice > b->max_slices ||
+        const size_t dw_i = (size_t)lay_buf(sg.nslice, sg.spb, sg.half_from, cap_dw, cap_dw2);
+        // token slots: the slices' positions, each slice's region rounded up to its size
+        const size_t tok_i = (size_t)(lay_start(sg.nslice - 1, sg.spb, sg.half_from, sl) +
+                                      lay_bytes(sg.nslice - 1, sg.spb, sg.half_from, sl) + 1) / 2;
+        if (slice_dw + dw_i > b->slice_arena_dw || slices + sg.nslice > b->max_slices ||
             segs + nseg > b->max_segs ||
-            (o.huffman && (tok_dw + (size_t)sg.nslice * (size_t)sl / 2 > b->dyn_tok_dw || segs + nseg > b->dyn_segs))) {
+            (o.huffman && (tok_dw + tok_i > b->dyn_tok_dw || segs + nseg > b->dyn_segs))) {
             set_error("slice arena exhausted"); return R2P_ERR_SIZE;
         }
         DeflateJob& D = dj[i];
@@ -415,8 +424,8 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         D.data = F.dst; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
         D.slice_out = b->d_slice_out + slice_dw; D.slice_bits = b->d_slice_bits + slices;
         D.slice_adler = b->d_slice_adler + slices; D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes;
-        D.slice_bytes = sl; D.slice_cap_dw = cap_dw; D.final_block = 1; D.bpp = g.bpp;
-        D.row_stride = (int32_t)(g.row_bytes + 1); D.lz77 = o.lz77; D.nslice = sg.nslice;
+        D.slice_bytes = sl; D.slice_cap_dw = cap_dw; D.slice_cap_dw2 = cap_dw2; D.final_block = 1; D.bpp = g.bpp;
+        D.row_stride = (int32_t)(g.row_bytes + 1); D.lz77 = o.lz77; D.nslice = sg.nslice; D.half_from = sg.half_from;
         ss[i] = (uint32_t)slices;
         gs[i] = (uint32_t)segs;
         if (o.huffman) {
@@ -427,9 +436,11 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
             H.tokens = D.tokens; H.tok_count = D.tok_count; H.hist = D.hist;
             H.codes = b->d_codes + segs * kHuffTab; H.hdr = b->d_hdr + segs * kHuffHdrWords;
             H.hdr_bits = b->d_hdr_bits + segs; H.slice_out = D.slice_out; H.slice_bits = D.slice_bits;
-            H.slice_cap_dw = cap_dw; H.slice_bytes = sl; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg;
+            H.slice_cap_dw = cap_dw; H.slice_cap_dw2 = cap_dw2; H.slice_bytes = sl; H.spb = sg.spb;
+            H.half_from = sg.half_from;
+            H.nslice = sg.nslice; H.nseg = nseg;
             H.final_block = 1;
-            tok_dw += (size_t)sg.nslice * (size_t)sl / 2;
+            tok_dw += tok_i;
         }
 
         const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
@@ -441,7 +452,8 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         A.seg_off = b->d_seg_off + segs + (size_t)i; A.out = b->d_out + ooff;
         A.out_len = b->d_res + 2 * (size_t)i; A.adler_out = (uint32_t*)(b->d_res + 2 * (size_t)i + 1);
         A.out_cap = (int64_t)bound; A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice;
-        A.spb = sg.spb; A.slice_cap_dw = cap_dw; A.slice_bytes = sl;
+        A.spb = sg.spb; A.half_from = sg.half_from; A.slice_cap_dw = cap_dw; A.slice_cap_dw2 = cap_dw2;
+        A.slice_bytes = sl;
         A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
         A.width = imgs[i].width; A.height = imgs[i].height; A.bit_depth = (uint8_t)g.out_bits;
         A.color_type = (uint8_t)g.color_type;
@@ -457,7 +469,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         foff += (g.filtered + 255) & ~(size_t)255;
         segs += (size_t)nseg;
         slices += (size_t)sg.nslice;
-        slice_dw += (size_t)sg.nslice * (size_t)cap_dw;
+        slice_dw += dw_i;
     }
     // band prefix in job-table order: the narrow class, the wide one, the tiled one
     uint32_t bands = 0, bands_n = 0, bands_w = 0;

@@ -395,9 +395,8 @@ void slice_arena_bound(const r2p_options& o, size_t filtered, size_t* max_slices
     for (int sl = 4096; sl <= 65536; sl <<= 1) {
         if (o.slice_bytes > 0 && sl != o.slice_bytes) continue;
         if (o.slice_bytes == 0 && sl > 16384) continue;
-        const size_t n = (filtered + sl - 1) / sl + 1;
-        ns = std::max(ns, n);
-        nd = std::max(nd, n * (size_t)slice_cap_for(sl, o.huffman));
+        ns = std::max(ns, slice_count_bound(filtered, sl));
+        nd = std::max(nd, slice_dword_bound(filtered, sl, slice_cap_for(sl, o.huffman)));
     }
     *max_slices = ns;
     *max_dwords = nd;
@@ -692,11 +691,13 @@ struct EncodeWork {
     uint32_t* hdr = nullptr;
     uint32_t* hdr_bits = nullptr;
     int slice_cap_dw = 0;
+    int slice_cap_dw2 = 0;             // a half slice's (slice_geom)
     const void* lut = nullptr;         // the image's device table, if its window needs one
     uint64_t* out_len = nullptr;       // device: PNG length
     uint32_t* adler_out = nullptr;     // device: Adler-32 of the filtered bytes
     uint64_t* host_res = nullptr;      // optional mapped host copy of both
     hipEvent_t* ev = nullptr;          // optional: five events around the stages
+    bool halve_tail = false;           // the slice size was left to the encoder (slice_geom)
 };
 
 // The plain encode's launches: window + filter, DEFLATE (and the dynamic
@@ -722,20 +723,19 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     if (d_filtered_out)
         R2P_HIP(hipMemcpyAsync(d_filtered_out, filt, g.filtered, hipMemcpyDeviceToDevice, s));
 
-    const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes);
+    const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, o.slice_bytes, w.halve_tail);
     const int nseg = sg.nseg;
     DeflateLaunch DL{};
     DeflateJob& D = DL.single;
     D.data = filt; D.first = 0; D.len = (int64_t)g.filtered; D.dict_lo = 0;
     D.slice_out = w.slice_out; D.slice_bits = w.slice_bits; D.slice_adler = w.slice_adler;
     D.slice_stats = nullptr; D.seg_bytes = o.segment_bytes; D.slice_bytes = o.slice_bytes;
-    D.slice_cap_dw = w.slice_cap_dw;
+    D.slice_cap_dw = w.slice_cap_dw; D.slice_cap_dw2 = w.slice_cap_dw2;
     D.final_block = 1; D.bpp = g.bpp; D.row_stride = (int32_t)(g.row_bytes + 1);
-    D.lz77 = o.lz77; D.nslice = sg.nslice;
+    D.lz77 = o.lz77; D.nslice = sg.nslice; D.half_from = sg.half_from;
     if (o.huffman) { D.tokens = w.tokens; D.tok_count = w.tok_count; D.hist = w.hist; }
     DL.dyn = o.huffman;
     DL.dense = o.lz77 == 2;
-    DL.prio_cus = cu_count();
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, o.lane_bytes, o.wg_threads, sg.nslice, s));
     if (o.huffman) {                       // the segments' codes, then the slices' bits
@@ -744,7 +744,9 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
         H.tokens = w.tokens; H.tok_count = w.tok_count; H.hist = w.hist;
         H.codes = w.codes; H.hdr = w.hdr; H.hdr_bits = w.hdr_bits;
         H.slice_out = w.slice_out; H.slice_bits = w.slice_bits; H.slice_cap_dw = w.slice_cap_dw;
-        H.slice_bytes = o.slice_bytes; H.spb = sg.spb; H.nslice = sg.nslice; H.nseg = nseg; H.final_block = 1;
+        H.slice_cap_dw2 = w.slice_cap_dw2;
+        H.slice_bytes = o.slice_bytes; H.spb = sg.spb; H.half_from = sg.half_from; H.nslice = sg.nslice; H.nseg = nseg;
+        H.final_block = 1;
         HL.jobs = nullptr; HL.njobs = 1; HL.total_segs = (uint32_t)nseg; HL.total_slices = (uint32_t)sg.nslice;
         R2P_HIP(launch_huffman(HL, nseg, std::min(sg.nslice, 8192), s));
     }
@@ -757,8 +759,8 @@ static int encode_into(const r2p_options& o, const EncodeWork& w, const r2p_imag
     A.seg_off = w.seg_off; A.out = (uint8_t*)d_png; A.out_len = w.out_len;
     A.adler_out = w.adler_out; A.out_cap = (int64_t)png_cap;
     A.host_res = w.host_res;
-    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
-    A.slice_cap_dw = w.slice_cap_dw; A.slice_bytes = o.slice_bytes;
+    A.total_in = (int64_t)g.filtered; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb; A.half_from = sg.half_from;
+    A.slice_cap_dw = w.slice_cap_dw; A.slice_cap_dw2 = w.slice_cap_dw2; A.slice_bytes = o.slice_bytes;
     A.chunk_bytes = o.idat_chunk_bytes; A.mode = 1; A.final_block = 1; A.eob_in_bits = o.huffman;
     A.width = img->width; A.height = img->height; A.bit_depth = (uint8_t)g.out_bits;
     A.color_type = (uint8_t)g.color_type;
@@ -868,10 +870,12 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     w.seg_off = e->d_seg_off; w.tokens = e->d_tokens; w.tok_count = e->d_tok_count; w.hist = e->d_hist;
     w.codes = e->d_codes; w.hdr = e->d_hdr; w.hdr_bits = e->d_hdr_bits;
     w.slice_cap_dw = slice_cap_for(o.slice_bytes, o.huffman);
+    w.slice_cap_dw2 = slice_cap_for(o.slice_bytes / 2, o.huffman);
     w.lut = e->lut.d_lut;
     w.out_len = e->d_misc; w.adler_out = (uint32_t*)(e->d_misc + 1);
     w.host_res = e->dh_misc;               // the results reach h_misc without a copy
     w.ev = e->timing ? e->ev : nullptr;
+    w.halve_tail = e->opt.slice_bytes == 0;
     uint32_t nseg = 0;
     if ((rc = encode_into(o, w, img, g, wp, d_src, d_png, png_cap, d_filtered_out, s, &nseg))) return rc;
     e->last_nseg = nseg;
@@ -883,12 +887,13 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
 // returns the bytes used.
 static size_t noalloc_layout(uint8_t* base, size_t F, const r2p_options& o, const WindowPlan& wp, EncodeWork* w)
 {
-    const SliceGeom sg = slice_geom((int64_t)F, o.segment_bytes, o.slice_bytes);
+    const SliceGeom sg = slice_geom((int64_t)F, o.segment_bytes, o.slice_bytes, w->halve_tail);
     size_t off = 0;
     auto take = [&](size_t n) { uint8_t* p = base + off; off += (n + 255) & ~(size_t)255; return p; };
     w->slice_cap_dw = slice_cap_for(o.slice_bytes, 0);
+    w->slice_cap_dw2 = slice_cap_for(o.slice_bytes / 2, 0);
     w->filt = take(F + kTailSlack);
-    w->slice_out = (uint32_t*)take((size_t)sg.nslice * (size_t)w->slice_cap_dw * 4);
+    w->slice_out = (uint32_t*)take((size_t)lay_buf(sg.nslice, sg.spb, sg.half_from, w->slice_cap_dw, w->slice_cap_dw2) * 4);
     w->slice_bits = (uint32_t*)take((size_t)sg.nslice * 4);
     w->slice_adler = (uint32_t*)take((size_t)sg.nslice * 4);
     w->slice_boff = (uint32_t*)take((size_t)sg.nslice * 4);
@@ -918,8 +923,9 @@ int r2p_encode_device_noalloc(const r2p_image* img, const void* d_src, void* d_p
     if ((rc = plan_window(img, &wp))) return rc;
     const size_t bound = png_bound_bytes(g.filtered, o.segment_bytes, o.idat_chunk_bytes);
     if (png_cap < bound) { set_error("png capacity %zu < bound %zu", png_cap, bound); return R2P_ERR_SIZE; }
-    o.slice_bytes = resolve_slice_bytes(o, g.filtered);
     EncodeWork w;
+    w.halve_tail = o.slice_bytes == 0;
+    o.slice_bytes = resolve_slice_bytes(o, g.filtered);
     const size_t need = noalloc_layout(base, g.filtered, o, wp, &w);
     if (need > (size_t)R2P_NOALLOC_SCRATCH_BYTES) {
         set_error("image needs %zu bytes of scratch (%zu filtered bytes); the no-allocation encoder holds %zu",

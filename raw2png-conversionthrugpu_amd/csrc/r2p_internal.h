@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "r2p.h"
 
 namespace r2p {
@@ -71,11 +73,13 @@ struct DeflateJob {
     int32_t        seg_bytes;
     int32_t        slice_bytes;// divides seg_bytes
     int32_t        slice_cap_dw;
+    int32_t        slice_cap_dw2; // a half slice's buffer (from segment half_from on)
     int32_t        final_block; // the job's last segment sets BFINAL (else sync flush)
     int32_t        bpp;
     int32_t        row_stride; // filtered row size (match candidate one row up), 0 = none
     int32_t        lz77;
-    int32_t        nslice;     // max(1, ceil(len / slice_bytes))
+    int32_t        nslice;     // slice_geom's count
+    int32_t        half_from;  // first segment in half-size slices (slice_geom)
     // dynamic-Huffman blocks (k_deflate<.., true>): instead of bits, each
     // slice writes a 16-bit slot per position (slice_bytes u16 per slice: a
     // literal, 0x8000 | length - 3 where a match starts and its distance - 1
@@ -106,8 +110,10 @@ struct AssembleJob {
     int64_t         total_in;    // uncompressed bytes (for the adler combine)
     int32_t         nseg;
     int32_t         nslice;
-    int32_t         spb;         // slices per segment (seg_bytes / slice_bytes)
+    int32_t         spb;         // slices per segment (seg_bytes / slice_bytes), twice as many from half_from on
+    int32_t         half_from;   // first segment in half-size slices (slice_geom)
     int32_t         slice_cap_dw;
+    int32_t         slice_cap_dw2;
     int32_t         slice_bytes;
     int32_t         chunk_bytes; // IDAT payload bytes per chunk
     int32_t         mode;        // 0 = raw deflate concatenation, 1 = complete PNG
@@ -147,8 +153,6 @@ struct DeflateLaunch {
     DeflateJob        single;
     int32_t           dyn;         // the jobs' dynamic-Huffman form (k_deflate<.., true, ..>)
     int32_t           dense;       // hashed search at every 2nd position (r2p_options.lz77 = 2), else every 4th
-    int32_t           prio_cus;    // one workgroup per slice: the device's CUs (wave priorities by
-                                   // dispatch round, below); 0: none (persistent workgroups)
     uint32_t*         queue;       // persistent workgroups: a counter, zero at launch, that hands out
                                    // the slices after each workgroup's first; null: every grid-th slice
 };
@@ -165,8 +169,10 @@ struct HuffJob {
     uint32_t*       slice_out;   // as DeflateJob: the slices' bit strings
     uint32_t*       slice_bits;
     int32_t         slice_cap_dw;
+    int32_t         slice_cap_dw2;
     int32_t         slice_bytes;
-    int32_t         spb;         // slices per segment
+    int32_t         spb;         // slices per segment (twice as many from half_from on)
+    int32_t         half_from;
     int32_t         nslice;
     int32_t         nseg;
     int32_t         final_block;
@@ -233,16 +239,69 @@ constexpr int kCrcBlock = 32768;         // bytes per CRC-32 partial
 // reads may touch past the last written word.
 inline int slice_cap_dwords(int slice_bytes) { return (int)(((int64_t)slice_bytes * 9 + 3 + 31) / 32) + 8; }
 
+// Slice layout of a DEFLATE stream: segments of seg_bytes, each cut into
+// spb slices of slice_bytes, except the segments from half_from on, each cut
+// into 2 spb slices of slice_bytes / 2.  One workgroup deflates a slice;
+// slices stored back to back, every one at a full slice's buffer stride.
+__host__ __device__ inline int lay_first(int s, int spb, int half_from)
+{
+    return s <= half_from ? s * spb : half_from * spb + (s - half_from) * 2 * spb;
+}
+__host__ __device__ inline int lay_seg(int k, int spb, int half_from)
+{
+    const int k0 = half_from * spb;
+    return k < k0 ? k / spb : half_from + (k - k0) / (2 * spb);
+}
+__host__ __device__ inline int64_t lay_start(int k, int spb, int half_from, int slice_bytes)
+{
+    const int k0 = half_from * spb;
+    return k < k0 ? (int64_t)k * slice_bytes : (int64_t)k0 * slice_bytes + (int64_t)(k - k0) * (slice_bytes / 2);
+}
+__host__ __device__ inline int lay_bytes(int k, int spb, int half_from, int slice_bytes)
+{
+    return k < half_from * spb ? slice_bytes : slice_bytes / 2;
+}
+// Dword offset of slice k's bit buffer: cap dwords per full slice, cap2 per
+// half slice.
+__host__ __device__ inline int64_t lay_buf(int k, int spb, int half_from, int cap, int cap2)
+{
+    const int k0 = half_from * spb;
+    return k < k0 ? (int64_t)k * cap : (int64_t)k0 * cap + (int64_t)(k - k0) * cap2;
+}
+
 // Segment / slice counts of a stream of `len` bytes (an empty stream still
-// has one segment and one slice: the block header and its end).
-struct SliceGeom { int nseg, nslice, spb; };
-inline SliceGeom slice_geom(int64_t len, int seg_bytes, int slice_bytes)
+// has one segment and one slice: the block header and its end).  With
+// halve_tail (the plain encoder with the slice size left to it), slices of
+// 8 KiB or more fill whole rounds of kSliceRound workgroups (MI355X's
+// resident DEFLATE workgroups, 4 on each of 256 CUs) at full size, and the
+// segments past those rounds are cut into half-size slices: the last,
+// partial round is made of shorter work items that spread over the CUs as
+// they free up.  (A function of the stream's size and the options only.)
+constexpr int kSliceRound = 1024;
+struct SliceGeom { int nseg, nslice, spb, half_from; };
+inline SliceGeom slice_geom(int64_t len, int seg_bytes, int slice_bytes, bool halve_tail = false)
 {
     SliceGeom g;
     g.nseg = len > 0 ? (int)((len + seg_bytes - 1) / seg_bytes) : 1;
-    g.nslice = len > 0 ? (int)((len + slice_bytes - 1) / slice_bytes) : 1;
     g.spb = seg_bytes / slice_bytes;
+    const int64_t nfull = len > 0 ? (len + slice_bytes - 1) / slice_bytes : 1;
+    const int64_t rounds = nfull / kSliceRound;
+    g.half_from = g.nseg;
+    if (halve_tail && slice_bytes >= 8192 && rounds >= 1 && nfull > rounds * kSliceRound)
+        g.half_from = std::min<int64_t>(g.nseg, rounds * kSliceRound / g.spb);
+    const int64_t full_len = std::min<int64_t>(len, (int64_t)g.half_from * seg_bytes);
+    const int64_t rest = len - full_len;
+    g.nslice = len > 0 ? (int)((full_len + slice_bytes - 1) / slice_bytes + (rest + slice_bytes / 2 - 1) / (slice_bytes / 2))
+                       : 1;
     return g;
+}
+// Most slices slice_geom gives a stream of len bytes, and a bound on the
+// bit buffers' dwords with full slices of cap dwords (two half slices of a
+// full one take at most cap + 20 dwords).
+inline size_t slice_count_bound(size_t len, int slice_bytes) { return 2 * ((len + slice_bytes - 1) / slice_bytes) + 1; }
+inline size_t slice_dword_bound(size_t len, int slice_bytes, int cap)
+{
+    return ((len + slice_bytes - 1) / slice_bytes + 1) * ((size_t)cap + 20);
 }
 
 }  // namespace r2p

@@ -1757,20 +1757,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ uint32_t st[4];
     __shared__ uint32_t next_slice;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // With one workgroup per slice, the workgroups of dispatch round r
-    // (blockIdx / CUs: the r-th resident on a CU, or, past the resident
-    // ones, a late one taking a freed slot) run at wave priority min(r, 3).
-    // The SIMDs otherwise favour the oldest waves: a CU's first workgroup
-    // would finish long before its last, and the slices dispatched after the
-    // resident ones would start behind all of them.
-    if (DL.prio_cus > 0) {
-        switch (min(3u, blockIdx.x / (uint32_t)DL.prio_cus)) {
-        case 1: __builtin_amdgcn_s_setprio(1); break;
-        case 2: __builtin_amdgcn_s_setprio(2); break;
-        case 3: __builtin_amdgcn_s_setprio(3); break;
-        default: break;
-        }
-    }
 
     const uint32_t total = DL.counts ? DL.counts[1] : DL.total_slices;
     const int njobs = DL.counts ? (int)DL.counts[0] : DL.njobs;
@@ -1798,15 +1784,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const DeflateJob J = DL.jobs ? DL.jobs[j] : DL.single;
         const int sl = (int)(g - (DL.jobs ? DL.slice_start[j] : 0u));
         const int spb = J.seg_bytes / J.slice_bytes;
-        const int seg = sl / spb;
+        const int seg = lay_seg(sl, spb, J.half_from);
         const int nseg = J.len > 0 ? (int)((J.len + J.seg_bytes - 1) / J.seg_bytes) : 1;
         const uint8_t* d = J.data;
-        const int64_t s0 = J.first + (int64_t)sl * J.slice_bytes;
-        const int64_t s1 = min64(s0 + J.slice_bytes, J.first + J.len);
+        const int64_t s0 = J.first + lay_start(sl, spb, J.half_from, J.slice_bytes);
+        const int64_t s1 = min64(s0 + lay_bytes(sl, spb, J.half_from, J.slice_bytes), J.first + J.len);
         const int64_t dend = J.first + J.len;
         const bool last = (seg == nseg - 1) && J.final_block;
         const int64_t wlo = max64(J.dict_lo, s0 - (int64_t)kWindow);
-        uint32_t* out = J.slice_out + (int64_t)sl * J.slice_cap_dw;
+        uint32_t* out = J.slice_out + lay_buf(sl, spb, J.half_from, J.slice_cap_dw, J.slice_cap_dw2);
         // Positions below are relative to wb = d + wlo (window base).
         const uint8_t* wb = d + wlo;
         const int s0_rel = (int)(s0 - wlo);
@@ -1869,7 +1855,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         uint32_t ad_s = 0, ad_t = 0;
 
         // the segment's first slice opens its block: BFINAL | BTYPE=01 << 1
-        const bool opens = sl % spb == 0;
+        const bool opens = sl == lay_first(seg, spb, J.half_from);
         uint32_t carry = opens ? (last ? 3u : 2u) : 0u;
         uint32_t carry_bits = opens ? 3u : 0u;
         int64_t out_dw = 0;
@@ -2077,7 +2063,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         d[k] = lo | hi << 16;
                     }
                     const int p0 = lb - s0_rel;
-                    uint32_t* tp = J.tokens + ((int64_t)sl * J.slice_bytes + (p0 & ~4095)) / 2 + 4 * ((p0 >> 4) & 255);
+                    uint32_t* tp = J.tokens + (s0 - J.first + (p0 & ~4095)) / 2 + 4 * ((p0 >> 4) & 255);
 #pragma unroll
                     for (int q = 0; q < 2; q++)
                         *(u32x4*)(tp + 1024 * q) = u32x4{d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]};
@@ -2674,18 +2660,19 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
         const int j = JOBS ? huff_job_of(HL, HL.slice_start, gsl, njobs) : 0;
         const HuffJob H = JOBS ? HL.jobs[j] : HL.single;
         const int sl = (int)(gsl - (JOBS ? HL.slice_start[j] : 0u));
-        const int seg = sl / H.spb;
-        const bool first = sl % H.spb == 0;
-        const bool lastsl = sl % H.spb == H.spb - 1 || sl == H.nslice - 1;
+        const int seg = lay_seg(sl, H.spb, H.half_from);
+        const bool first = sl == lay_first(seg, H.spb, H.half_from);
+        const bool lastsl = sl == lay_first(seg + 1, H.spb, H.half_from) - 1 || sl == H.nslice - 1;
         const uint32_t npos = H.tok_count[sl];
         const uint32_t ntok = npos + (lastsl ? 1u : 0u);     // + the end-of-block code
-        const uint32_t* tk = H.tokens + (int64_t)sl * H.slice_bytes / 2;
+        const uint32_t* tk = H.tokens + lay_start(sl, H.spb, H.half_from, H.slice_bytes) / 2;
+        const uint32_t slots = (uint32_t)lay_bytes(sl, H.spb, H.half_from, H.slice_bytes);
         u32x4 nx[2];
         // (loads bounded by the slice's slots, not its length: the first
         // round's loads do not wait for the count; slots past the length are
         // replaced below)
         auto fetch = [&](uint32_t r) {
-            const bool any = r * NT * kEmitPer + (uint32_t)tid * kEmitPer < (uint32_t)H.slice_bytes;
+            const bool any = r * NT * kEmitPer + (uint32_t)tid * kEmitPer < slots;
 #pragma unroll
             for (int q = 0; q < 2; q++)
                 nx[q] = any ? *(const u32x4*)(tk + r * NT * kEmitPer / 2 + 4 * (tid + NT * q))
@@ -2701,7 +2688,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
             for (uint32_t i = tid; i < (pre + 31) / 32; i += NT) obuf[i] = H.hdr[(int64_t)seg * kHuffHdrWords + i];
         }
         __syncthreads();
-        uint32_t* out = H.slice_out + (int64_t)sl * H.slice_cap_dw;
+        uint32_t* out = H.slice_out + lay_buf(sl, H.spb, H.half_from, H.slice_cap_dw, H.slice_cap_dw2);
         int64_t out_dw = 0;
         uint32_t carry_bits = pre;
         const uint32_t rounds = max(1u, (ntok + NT * kEmitPer - 1) / (NT * kEmitPer));
@@ -2792,7 +2779,8 @@ __device__ __forceinline__ uint32_t seg_len_bytes(uint32_t tb, bool last, bool e
 __device__ __forceinline__ void adler_terms(const AssembleJob& A, int i, uint32_t& sa, uint32_t& sb)
 {
     constexpr uint32_t M = 65521;
-    const int64_t end = min64((int64_t)(i + 1) * A.slice_bytes, A.total_in);
+    const int64_t end = min64(lay_start(i, A.spb, A.half_from, A.slice_bytes) + lay_bytes(i, A.spb, A.half_from, A.slice_bytes),
+                              A.total_in);
     const uint32_t R = (uint32_t)((uint64_t)(A.total_in - end) % M);
     const uint32_t a = A.slice_adler[i];
     const uint32_t am = (a & 0xFFFFu) ? (a & 0xFFFFu) - 1u : M - 1u;
@@ -2816,7 +2804,7 @@ __global__ __launch_bounds__(256) void k_scan(AssembleLaunch AL)
         const int i = c0 + tid;
         uint32_t v = 0;
         if (i < A.nseg) {
-            const int k0 = i * A.spb, k1 = min(k0 + A.spb, A.nslice);
+            const int k0 = lay_first(i, A.spb, A.half_from), k1 = min(lay_first(i + 1, A.spb, A.half_from), A.nslice);
             uint32_t tb = 0;
             for (int kb = k0; kb < k1; kb += 8) {          // eight loads in flight
                 uint32_t bits[8];
@@ -2908,7 +2896,7 @@ __device__ __forceinline__ void st128u(uint8_t* p, u32x4 v) { *(gu32x4_ua_w*)p =
 // the slice buffers have slack words for the second read).
 __device__ __forceinline__ uint32_t slice_read32(const AssembleJob& A, int k, uint32_t q)
 {
-    const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (q >> 5);
+    const uint32_t* w = A.slice_out + lay_buf(k, A.spb, A.half_from, A.slice_cap_dw, A.slice_cap_dw2) + (q >> 5);
     return __builtin_amdgcn_alignbit(w[1], w[0], q & 31);
 }
 
@@ -2922,8 +2910,8 @@ __device__ uint32_t seg_bits32(const AssembleJob& A, int s, uint32_t p, int& k)
 {
     const uint32_t tb = A.seg_tbits[s];
     if (p >= tb) return 0u;
-    const int k1 = min(s * A.spb + A.spb, A.nslice);
-    k = max(k, s * A.spb);
+    const int k1 = min(lay_first(s + 1, A.spb, A.half_from), A.nslice);
+    k = max(k, lay_first(s, A.spb, A.half_from));
     while (k + 1 < k1 && A.slice_boff[k + 1] <= p) k++;
     const uint32_t off = p - A.slice_boff[k];
     const uint32_t avail = A.slice_bits[k] - off;
@@ -2944,14 +2932,14 @@ template <int NQ>
 __device__ __forceinline__ bool seg_bytes(const AssembleJob& A, int s, uint32_t p, int k, u32x4* v)
 {
     constexpr int ND = 4 * NQ;
-    const int k1 = min(s * A.spb + A.spb, A.nslice);
+    const int k1 = min(lay_first(s + 1, A.spb, A.half_from), A.nslice);
     const uint32_t bk = A.slice_boff[k], ek = bk + A.slice_bits[k];
     const uint32_t n1 = p < ek ? ek - p : 32u * ND;          // bits from slice k
     const bool has2 = k + 1 < k1;
     const uint32_t n2 = has2 ? A.slice_bits[k + 1] : 0u;
     if (n1 < 32u * ND && has2 && n1 + n2 < 32u * ND && k + 2 < k1) return false;
     const uint32_t off = p - bk;
-    const uint32_t* w = A.slice_out + (int64_t)k * A.slice_cap_dw + (off >> 5);
+    const uint32_t* w = A.slice_out + lay_buf(k, A.spb, A.half_from, A.slice_cap_dw, A.slice_cap_dw2) + (off >> 5);
     uint32_t t[ND + 1];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
@@ -2969,7 +2957,7 @@ __device__ __forceinline__ bool seg_bytes(const AssembleJob& A, int s, uint32_t 
         for (int i = 0; i <= ND; i++) u[i] = 0;
         if (has2) {
             const uint32_t c = (n1 + 31) >> 5;
-            const uint32_t* w2 = A.slice_out + (int64_t)(k + 1) * A.slice_cap_dw - c;
+            const uint32_t* w2 = A.slice_out + lay_buf(k + 1, A.spb, A.half_from, A.slice_cap_dw, A.slice_cap_dw2) - c;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
                 const u32x4 x = *(R2P_GLOBAL const u32x4_ua*)(w2 + 4 * q);
@@ -3114,7 +3102,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         const int i = tid;                         // segment i (nseg <= kAsmScanSegs == threads)
         uint32_t v = 0, sa = 0, sb = 0;
         if (i < S.nseg) {
-            const int k0 = i * S.spb, k1 = min(k0 + S.spb, S.nslice);
+            const int k0 = lay_first(i, S.spb, S.half_from), k1 = min(lay_first(i + 1, S.spb, S.half_from), S.nslice);
             uint32_t tb = 0;
             for (int kb = k0; kb < k1; kb += 8) {          // eight loads in flight
                 uint32_t bits[8];
@@ -3212,7 +3200,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             const int64_t a0 = b - hdr;
             if (a0 > 0 && a0 < segbytes)
                 while ((int64_t)A.seg_off[s + 1] <= a0) s++;
-            int k = s * A.spb;                         // slice cursor
+            int k = lay_first(s, A.spb, A.half_from);     // slice cursor
             // the usual case: all 64 bytes from one slice's token bits, read
             // with one cursor lookup and four loads in flight
             whole = e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
@@ -3220,7 +3208,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             u32x4 v[4];
             if (whole) {
                 const uint32_t pw = 8 * (uint32_t)(a0 - (int64_t)A.seg_off[s]);
-                const int k1 = min(s * A.spb + A.spb, A.nslice);
+                const int k1 = min(lay_first(s + 1, A.spb, A.half_from), A.nslice);
                 while (k + 1 < k1 && A.slice_boff[k + 1] <= pw) k++;
                 whole = seg_bytes<4>(A, s, pw, k, v);
             }

@@ -63,7 +63,7 @@ int r2p_deflate_create(size_t max_strip_bytes, size_t max_total_bytes, int lz77,
     d->slice_bytes = resolve_slice_bytes(o, max_strip_bytes);   // one size for every strip
     d->slice_cap_dw = slice_cap_dwords(d->slice_bytes);
     d->max_seg = (int)((max_strip_bytes + d->seg_bytes - 1) / d->seg_bytes) + 1;
-    d->max_slice = (int)((max_strip_bytes + d->slice_bytes - 1) / d->slice_bytes) + 1;
+    d->max_slice = (int)((max_strip_bytes + d->slice_bytes - 1) / d->slice_bytes) + 1;   // (strips keep whole slices)
     // output bound: 9/8 of the input plus per-segment framing
     const size_t total_in = std::max(max_total_bytes, max_strip_bytes);
     d->max_total = total_in + total_in / 8 + 16 * (total_in / d->seg_bytes + 2) + 64;
@@ -144,11 +144,10 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     D.data = win; D.first = (int64_t)d->hist; D.len = (int64_t)len; D.dict_lo = 0;
     D.slice_out = d->d_slice_out; D.slice_bits = d->d_slice_bits; D.slice_adler = d->d_slice_adler;
     D.slice_stats = d->d_slice_stats; D.seg_bytes = d->seg_bytes; D.slice_bytes = d->slice_bytes;
-    D.slice_cap_dw = d->slice_cap_dw;
+    D.slice_cap_dw = d->slice_cap_dw; D.slice_cap_dw2 = d->slice_cap_dw;
     D.final_block = is_last ? 1 : 0; D.bpp = bpp > 0 ? bpp : 1; D.row_stride = row_stride;
-    D.lz77 = d->lz77; D.nslice = sg.nslice;
+    D.lz77 = d->lz77; D.nslice = sg.nslice; D.half_from = sg.half_from;
     DL.dense = d->lz77 == 2;
-    DL.prio_cus = cu_count();
     DL.jobs = nullptr; DL.njobs = 1; DL.total_slices = (uint32_t)sg.nslice;
     R2P_HIP(launch_deflate(DL, 16, 256, sg.nslice, s));
 
@@ -158,8 +157,8 @@ int r2p_deflate_strip(r2p_deflate* d, const void* d_filtered, size_t len, int is
     A.slice_boff = d->d_slice_boff; A.seg_tbits = d->d_seg_tbits;
     A.seg_off = d->d_seg_off; A.out = d->d_out; A.out_len = d->d_state + 2;
     A.adler_out = (uint32_t*)(d->d_state + 3); A.out_cap = (int64_t)d->max_total;
-    A.total_in = (int64_t)len; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb;
-    A.slice_cap_dw = d->slice_cap_dw; A.slice_bytes = d->slice_bytes;
+    A.total_in = (int64_t)len; A.nseg = nseg; A.nslice = sg.nslice; A.spb = sg.spb; A.half_from = sg.half_from;
+    A.slice_cap_dw = d->slice_cap_dw; A.slice_cap_dw2 = d->slice_cap_dw; A.slice_bytes = d->slice_bytes;
     A.chunk_bytes = 65536; A.mode = 0; A.final_block = is_last ? 1 : 0;
     A.max_chunks = (int32_t)((zmax + 65535) / 65536);
     A.running = d->d_state; A.base_slot = d->d_state + 1;

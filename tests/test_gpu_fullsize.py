@@ -62,8 +62,12 @@ def test_full_size_workload_vs_oracle(torch_dev, r2p, synth, oracle, kind):
     log(f"case full-size {kind} {s.img.width}x{s.img.height}")
     d_src = torch.from_numpy(np.ascontiguousarray(s.pixels).view(np.uint8).reshape(-1).copy()).cuda()
     exp = oracle_filtered(oracle, r2p, s.img, s.pixels)
-    for lz77 in (1, 0):
-        png, filt, adler = encode_plain(torch, r2p, s.img, d_src, r2p.options(lz77=lz77), want_filtered=True)
+    # automatic slices: the stream's last partial round of 1024 slices is
+    # cut into half-size slices (1024 + 257 for the radiograph); explicit
+    # 16 KiB slices keep every slice whole; dynamic-Huffman blocks over both
+    for opt in (r2p.options(lz77=1), r2p.options(lz77=0), r2p.options(lz77=1, slice_bytes=16384),
+                r2p.options(lz77=1, huffman=1)):
+        png, filt, adler = encode_plain(torch, r2p, s.img, d_src, opt, want_filtered=True)
         assert np.array_equal(filt, exp), "filtered scanlines differ from the oracle"
         hdr, z, raw = check_and_inflate(png)
         assert raw == exp.tobytes(), "inflated IDAT differs from the oracle's filtered bytes"
