@@ -149,6 +149,68 @@ def measured_traffic(args, kernel):
     return k["bytes"] if k else None
 
 
+def live_traffic(args):
+    """HBM bytes per launch of the plain encode's kernels, measured now: two
+    rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE) of a short plain encode
+    of this workload, each in a child process that ends with three copies of
+    256 MiB; the copies' counters give the bytes per counter unit (FETCH_SIZE
+    counts about half the bytes read).  Run before this process touches the
+    GPU; None if the profiler is missing or a run fails (bounded by a time
+    limit)."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    child = [sys.executable, "-u", os.path.abspath(__file__), "--mode", "plain", "--no-cpu", "--no-dynamic",
+             "--no-ratios", "--steps", "3", "--warmup", "1", "--calib-copy-mib", "256", "--checksum-mib", "0",
+             "--traffic", "0", "--workload", args.workload, "--seg", str(args.seg), "--slice", str(args.slice),
+             "--wg", str(args.wg), "--lz77", str(args.lz77)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="r2p_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--"] + child
+        log(f"bench: traffic pass {counter}")
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                             start_new_session=True)
+        try:
+            rc = p.wait(timeout=150)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            log(f"bench: traffic pass {counter} timed out")
+            return None
+        files = [os.path.join(d, f) for d, _, fs in os.walk(out) for f in fs if f.endswith("counter_collection.csv")]
+        if rc != 0 or not files:
+            log(f"bench: traffic pass {counter} failed (rc {rc})")
+            return None
+        vals = {}
+        for row in csv.DictReader(open(files[0])):
+            if row.get("Counter_Name") == counter:
+                vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        shutil.rmtree(out, ignore_errors=True)
+        copies = [max(v) for k, v in vals.items() if "copyBuffer" in k]
+        if not copies or max(copies) <= 0:
+            return None
+        unit = float(256 << 20) / (max(copies) * 1024)      # bytes per KiB the counter reports
+        for key in ("k_filter", "k_deflate", "k_assemble"):
+            v = sorted(x for k, xs in vals.items() if "r2p::" + key in k for x in xs)
+            if v:
+                got.setdefault(key, {})[counter] = (v[len(v) // 2], unit)
+    res = {}
+    for key, c in got.items():
+        if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+            (f, fu), (w, wu) = c["FETCH_SIZE"], c["WRITE_SIZE"]
+            res[key.replace("k_", "")] = {"fetch_kib": round(f), "write_kib": round(w),
+                                          "bytes": int(round(fu * 1024 * f + wu * 1024 * w)),
+                                          "fetch_factor": round(fu, 4), "write_factor": round(wu, 4)}
+    return res or None
+
+
 def kernels_sha16():
     import hashlib
     with open(os.path.join(PKG_DIR, "csrc", "r2p_kernels.hip"), "rb") as f:
@@ -573,6 +635,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=64, help="distinct request images of the mix")
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--traffic", type=int, default=1,
+                    help="measure the plain kernels' HBM bytes with two rocprofv3 --pmc child runs first (0: no)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times (profiler calibration)")
     ap.add_argument("--cpu-level", type=int, default=3)
@@ -583,6 +647,8 @@ def main():
               "(the path shards by image with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    # the byte counters first, while this process has not touched the GPU
+    traffic = live_traffic(args) if args.traffic else None
     import torch
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
@@ -613,8 +679,17 @@ def main():
         "dynamic_huffman": r["dynamic"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args, dom)},
+                     "traffic": None},
     }
+    if traffic and dom in traffic:
+        out["roofline"]["traffic"] = traffic[dom]["bytes"]
+        out["roofline"]["traffic_source"] = "measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated"
+        out["traffic_by_kernel"] = {k: dict(v, algorithmic=int(r["algo_bytes"][k])) for k, v in traffic.items()
+                                    if k in r["algo_bytes"]}
+    else:
+        out["roofline"]["traffic"] = measured_traffic(args, dom)
+        if out["roofline"]["traffic"] is not None:
+            out["roofline"]["traffic_source"] = "profiles/traffic_radiograph3072.json (counters of these kernel sources)"
     if r["dynamic"]:
         out["ratio"]["gpu_dynamic_huffman_png_over_filtered"] = r["dynamic"]["png_over_filtered"]
     log("bench: plain " + json.dumps({k: out[k] for k in ("value", "ms_per_step", "latency_ms", "kernels_ms", "roofline")}))
