@@ -408,7 +408,7 @@ int r2p_batch_set(r2p_batch* b, int n, const r2p_image* imgs, const void* const*
         if (sl <= 0 || o.segment_bytes % sl) sl = resolve_slice_bytes(o, g.filtered);
         b->job_slice[(size_t)i] = auto_sl ? 0 : sl;
         const int cap_dw = slice_cap_for(sl, o.huffman), cap_dw2 = slice_cap_for(sl / 2, o.huffman);
-        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, sl, auto_sl);
+        const SliceGeom sg = slice_geom((int64_t)g.filtered, o.segment_bytes, sl, auto_sl && !o.huffman);
         const int nseg = sg.nseg;
         const size_t dw_i = (size_t)lay_buf(sg.nslice, sg.spb, sg.half_from, cap_dw, cap_dw2);
         // token slots: the slices' positions, each slice's region rounded up to its size

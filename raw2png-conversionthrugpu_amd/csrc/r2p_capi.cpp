@@ -875,7 +875,10 @@ int r2p_encode_device(r2p_encoder* e, const r2p_image* img, const void* d_src, v
     w.out_len = e->d_misc; w.adler_out = (uint32_t*)(e->d_misc + 1);
     w.host_res = e->dh_misc;               // the results reach h_misc without a copy
     w.ev = e->timing ? e->ev : nullptr;
-    w.halve_tail = e->opt.slice_bytes == 0;
+    // (dynamic-Huffman streams keep whole slices: a highly compressible
+    // tail, such as a saturated border, becomes slices of a few bytes, and
+    // the assembly gathers pieces across many of them byte by byte)
+    w.halve_tail = e->opt.slice_bytes == 0 && !o.huffman;
     uint32_t nseg = 0;
     if ((rc = encode_into(o, w, img, g, wp, d_src, d_png, png_cap, d_filtered_out, s, &nseg))) return rc;
     e->last_nseg = nseg;

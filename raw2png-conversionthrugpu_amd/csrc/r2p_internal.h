@@ -271,7 +271,7 @@ __host__ __device__ inline int64_t lay_buf(int k, int spb, int half_from, int ca
 
 // Segment / slice counts of a stream of `len` bytes (an empty stream still
 // has one segment and one slice: the block header and its end).  With
-// halve_tail (the plain encoder with the slice size left to it), slices of
+// halve_tail (fixed-Huffman streams whose slice size is left to the encoder), slices of
 // 8 KiB or more fill whole rounds of kSliceRound workgroups (MI355X's
 // resident DEFLATE workgroups, 4 on each of 256 CUs) at full size, and the
 // segments past those rounds are cut into half-size slices: the last,
