@@ -232,7 +232,7 @@ constexpr int kHuffHdrWords = 96;        // a dynamic block header: at most 3 + 
 // the last data byte, so its 16-byte loads need no tail guard.
 constexpr int kTailSlack = 64;
 constexpr int kChecksumBlock = 16384;    // bytes per Adler-32 partial
-constexpr int kCrcBlock = 32768;         // bytes per CRC-32 partial
+constexpr int kCrcBlock = 131072;        // bytes per CRC-32 partial
 
 // Worst-case token dwords of one slice of `slice_bytes` input bytes (9 bits
 // per literal, the 3-bit block header), plus the slack k_assemble's 5-dword

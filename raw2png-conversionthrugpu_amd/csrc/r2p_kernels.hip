@@ -1290,66 +1290,80 @@ __device__ __forceinline__ uint32_t wave_xor(uint32_t v)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// CRC register of one block, run from 0 with no final inversion.  Thread t
-// takes the 64 bytes [e - 64, e) with e counted back from the block's end
-// (a short block's first pieces are partly or wholly empty: leading zeros
-// leave a register that starts at 0 unchanged), so the pieces combine in a
-// tree whose level-k shift is the fixed x^(8*64*2^k).
 constexpr int kFinalThreads = 1024;     // the combining workgroup
 constexpr int kCrcPartialGrid = 2048;   // 8 workgroups on each of 256 CUs
+constexpr int kCrcRow = 256 * 32;       // a row: 32 bytes for each of 256 threads
+constexpr int kCrcRows = kCrcBlock / kCrcRow;
+static_assert(kCrcRows % 4 == 0, "rows in groups of four");
 
-// CRC partials.  Blocks are counted back from the end: block j covers
-// [n - (j + 1) B, n - j B) (the data's first block may be short: leading
-// zeros leave a register that starts at 0 unchanged), so the register of
-// the whole input is XOR_j p_j * x^(8 j B) with every shift a table.  In a
-// block, thread t takes the 128 bytes [e - 128, e), e counted back from the
-// block's end, as two independent 64-byte chains joined by the 64-byte
-// shift; the threads' pieces then combine in a tree from the 128-byte shift
-// up.  One table lookup per byte.
-static_assert(kCrcBlock == 256 * 128, "two 64-byte chains per thread");
+// CRC partials: per block, its register run from 0 with no final inversion.
+// Blocks are counted back from the end: block j covers [n - (j + 1) B,
+// n - j B) (the data's first block may be short: leading zeros leave a
+// register that starts at 0 unchanged, so it is read as if padded at the
+// front).  A block is kCrcRows rows of 8 KiB; thread t takes the 32 bytes at
+// 32 t of every row, so each load of a wave covers 1 KiB of whole cache lines.
+// A thread runs four rows' pieces as four chains and folds them Horner-wise
+// (h = h x^(8*8 KiB) + c_q), so its register covers its pieces of every row;
+// the threads then combine in a tree from the 32-byte shift up, the tree and
+// the workgroup's one barrier once per block.  One table lookup per byte and
+// one more per 32.
 __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n, uint32_t* part)
 {
     __shared__ uint32_t tab[4][256];
+    __shared__ uint32_t srow[4][256];     // x^(8 kCrcRow): one row on
     __shared__ uint32_t red[2][4];
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) (&srow[0][0])[i] = (&c_crcshift[7][0][0])[i];
+    static_assert((64 << 7) == kCrcRow, "row shift table");
     load_crc_tables(tab);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = (int)((n + kCrcBlock - 1) / kCrcBlock);
     // persistent: the tables are loaded once, blocks are dealt round-robin
     int it = 0;
     for (int blk = blockIdx.x; blk < nb; blk += gridDim.x, it ^= 1) {
-        const int64_t b1 = n - (int64_t)blk * kCrcBlock;
-        const int64_t b0 = max64(0, b1 - kCrcBlock);
-        const int64_t e = b1 - (int64_t)(255 - tid) * 128;
-        const int64_t b = max64(b0, e - 128);
-        uint32_t r = 0;
-        if (e - b == 128) {
-            u32x4 v[8];
+        const int64_t pb = n - (int64_t)(blk + 1) * kCrcBlock;    // negative for a short first block
+        const uint8_t* s0 = p + pb + tid * 32;                    // row q's piece at s0 + kCrcRow q
+        uint32_t h = 0;
+        if (pb >= 0) {
+            for (int q0 = 0; q0 < kCrcRows; q0 += 4) {
+                u32x4 v[4][2];
 #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = g128u(p + b + 16 * q);
-            uint32_t r0 = 0, r1 = 0;
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    r0 = crc_step4(r0, v[q][k], tab);
-                    r1 = crc_step4(r1, v[q + 4][k], tab);
+                for (int q = 0; q < 4; q++) {
+                    v[q][0] = g128u(s0 + (int64_t)kCrcRow * (q0 + q));
+                    v[q][1] = g128u(s0 + (int64_t)kCrcRow * (q0 + q) + 16);
                 }
-            r = crc_shift(r0, c_crcshift[0]) ^ r1;
+                uint32_t c[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) c[q] = crc_step4(c[q], v[q][k >> 2][k & 3], tab);
+#pragma unroll
+                for (int q = 0; q < 4; q++) h = crc_shift(h, srow) ^ c[q];
+            }
         } else {
-            for (int64_t i = b; i < e; i++) r = tab[0][(r ^ p[i]) & 0xFF] ^ (r >> 8);
+            for (int q = 0; q < kCrcRows; q++) {
+                const int64_t e = pb + tid * 32 + (int64_t)kCrcRow * q + 32;
+                uint32_t c = 0;
+                for (int64_t i = max64(0, e - 32); i < e; i++) c = tab[0][(c ^ p[i]) & 0xFF] ^ (c >> 8);
+                h = crc_shift(h, srow) ^ c;
+            }
+        }
+        // thread t's register stands (255 - t) 32 bytes before the block's end
+        {
+            const uint32_t right = __shfl_down(h, 1, 64);
+            if ((lane & 1) == 0) h = crc_shift(h, c_crcshift32) ^ right;
         }
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const uint32_t right = __shfl_down(r, 1 << k, 64);
-            if ((lane & ((2 << k) - 1)) == 0) r = crc_shift(r, c_crcshift[k + 1]) ^ right;
+        for (int k = 1; k < 6; k++) {
+            const uint32_t right = __shfl_down(h, 1 << k, 64);
+            if ((lane & ((2 << k) - 1)) == 0) h = crc_shift(h, c_crcshift[k - 1]) ^ right;
         }
         // red is double-buffered by iteration, so one barrier per block
-        if (lane == 0) red[it][wave] = r;
+        if (lane == 0) red[it][wave] = h;
         __syncthreads();
         if (tid == 0) {
-            const uint32_t v01 = crc_shift(red[it][0], c_crcshift[7]) ^ red[it][1];
-            const uint32_t v23 = crc_shift(red[it][2], c_crcshift[7]) ^ red[it][3];
-            part[blk] = crc_shift(v01, c_crcshift[8]) ^ v23;
+            const uint32_t v01 = crc_shift(red[it][0], c_crcshift[5]) ^ red[it][1];
+            const uint32_t v23 = crc_shift(red[it][2], c_crcshift[5]) ^ red[it][3];
+            part[blk] = crc_shift(v01, c_crcshift[6]) ^ v23;
         }
     }
 }
@@ -1359,7 +1373,7 @@ __global__ __launch_bounds__(256) void k_crc_partial(const uint8_t* p, int64_t n
 // then the threads combine in a tree whose level-k shift is x^(8 m B 2^k);
 // m is a power of two, so every shift is one of the tables.  The result is
 // ~(init ^ total) with init = ~0 * x^(8n), the initial register's term.
-constexpr int kBlockLevel = 9;                         // x^(8 B): 64 * 2^9 = B
+constexpr int kBlockLevel = 11;                        // x^(8 B): 64 * 2^11 = B
 static_assert((64 << kBlockLevel) == kCrcBlock, "block shift table");
 __global__ __launch_bounds__(kFinalThreads) void k_crc_final(const uint32_t* part, int nb, int lm, uint32_t init,
                                                              uint32_t* result)
@@ -1375,7 +1389,14 @@ __global__ __launch_bounds__(kFinalThreads) void k_crc_final(const uint32_t* par
     const int m = 1 << lm;
     const int lo = min(nb, tid << lm), hi = min(nb, lo + m);
     uint32_t h = 0;
-    for (int j = hi - 1; j >= lo; j--) h = crc_shift(h, shb) ^ part[j];
+    for (int j0 = hi - 1; j0 >= lo; j0 -= 8) {
+        uint32_t pv[8];                                     // eight loads in flight
+#pragma unroll
+        for (int u = 0; u < 8; u++) pv[u] = j0 - u >= lo ? part[j0 - u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (j0 - u >= lo) h = crc_shift(h, shb) ^ pv[u];
+    }
     // thread t's register is worth x^(8 t m B): level k shifts by m B 2^k
 #pragma unroll
     for (int k = 0; k < 6; k++) {

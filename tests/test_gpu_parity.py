@@ -160,8 +160,8 @@ def test_window_exhaustive(torch_dev, r2p, oracle, case, wide):
     assert np.array_equal(out, oracle.filter_rows(ready, h, rb, bpp))
 
 
-@pytest.mark.parametrize("n", [0, 1, 3, 15, 16, 17, 4095, 65535, 65536, 65537, 1 << 20, (1 << 22) + 7,
-                               (40 << 20) + 3])
+@pytest.mark.parametrize("n", [0, 1, 3, 15, 16, 17, 4095, 65535, 65536, 65537, 131071, 131072, 131073,
+                               1 << 20, (1 << 22) + 7, (40 << 20) + 3, (160 << 20) + 11])
 def test_checksums_vs_zlib(torch_dev, r2p, n):
     log(f"case checksum n={n}")
     rng = np.random.default_rng(n)
