@@ -2943,13 +2943,16 @@ __device__ uint32_t seg_bits32(const AssembleJob& A, int s, uint32_t p, int& k)
 
 // NQ * 16 bytes of segment s from its bit p (a multiple of 8), into v: the
 // token bits of slice k (the slice holding p, or the segment's last) and,
-// past its end, of slice k + 1; zero past the segment's token bits (EOB, the
-// stored block's header and the pad are zero bits), then 0xFF for the sync
-// flush's last two bytes.  False (v untouched) when a third slice would be
-// needed.  Every load is issued up front: slice k + 1 is read from ceil(n1
-// / 32) dwords before its start (slice k's buffer), so that dword i of the
-// result takes its high part from loaded word i at one lane-uniform shift.
-template <int NQ>
+// past its end, of slices k + 1 and k + 2; zero past the segment's token bits
+// (EOB, the stored block's header and the pad are zero bits), then 0xFF for
+// the sync flush's last two bytes.  False (v untouched) when a fourth slice
+// would be needed.  Every load is issued up front: slice k + 1 is read from
+// ceil(n1 / 32) dwords before its start (slice k's buffer), so that dword i
+// of the result takes its high part from loaded word i at one lane-uniform
+// shift, and slice k + 2 likewise from ceil((n1 + n2) / 32) dwords before its
+// start (the short slices of a flat tail are often under 64 bytes; THREE
+// off: a third slice is not read, and false when it would be needed).
+template <int NQ, bool THREE = false>
 __device__ __forceinline__ bool seg_bytes(const AssembleJob& A, int s, uint32_t p, int k, u32x4* v)
 {
     constexpr int ND = 4 * NQ;
@@ -2958,7 +2961,10 @@ __device__ __forceinline__ bool seg_bytes(const AssembleJob& A, int s, uint32_t 
     const uint32_t n1 = p < ek ? ek - p : 32u * ND;          // bits from slice k
     const bool has2 = k + 1 < k1;
     const uint32_t n2 = has2 ? A.slice_bits[k + 1] : 0u;
-    if (n1 < 32u * ND && has2 && n1 + n2 < 32u * ND && k + 2 < k1) return false;
+    const bool need3 = n1 < 32u * ND && has2 && n1 + n2 < 32u * ND && k + 2 < k1;
+    if (!THREE && need3) return false;
+    const uint32_t n3 = THREE && need3 ? A.slice_bits[k + 2] : 0u;
+    if (THREE && need3 && n1 + n2 + n3 < 32u * ND && k + 3 < k1) return false;
     const uint32_t off = p - bk;
     const uint32_t* w = A.slice_out + lay_buf(k, A.spb, A.half_from, A.slice_cap_dw, A.slice_cap_dw2) + (off >> 5);
     uint32_t t[ND + 1];
@@ -2991,6 +2997,26 @@ __device__ __forceinline__ bool seg_bytes(const AssembleJob& A, int s, uint32_t 
         for (int i = 0; i < ND; i++) {
             const uint32_t vu = __builtin_amdgcn_alignbit(u[i + 1], u[i], sh2);
             const int r = (int)n1 - 32 * i;                    // bits of dword i from slice k
+            const uint32_t m = r >= 32 ? ~0u : r <= 0 ? 0u : (1u << r) - 1u;
+            d[i] = (d[i] & m) | (vu & ~m);
+        }
+    }
+    if (THREE && need3) {
+        const uint32_t n12 = n1 + n2;
+        const uint32_t c = (n12 + 31) >> 5;
+        const uint32_t* w3 = A.slice_out + lay_buf(k + 2, A.spb, A.half_from, A.slice_cap_dw, A.slice_cap_dw2) - c;
+        uint32_t u[ND + 1];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const u32x4 x = *(R2P_GLOBAL const u32x4_ua*)(w3 + 4 * q);
+            u[4 * q] = x[0]; u[4 * q + 1] = x[1]; u[4 * q + 2] = x[2]; u[4 * q + 3] = x[3];
+        }
+        u[ND] = w3[ND];
+        const uint32_t sh3 = (32u - (n12 & 31u)) & 31u;
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const uint32_t vu = __builtin_amdgcn_alignbit(u[i + 1], u[i], sh3);
+            const int r = (int)n12 - 32 * i;                   // bits of dword i from slices k, k + 1
             const uint32_t m = r >= 32 ? ~0u : r <= 0 ? 0u : (1u << r) - 1u;
             d[i] = (d[i] & m) | (vu & ~m);
         }
@@ -3073,6 +3099,23 @@ __device__ int wave_find_seg(const uint64_t* seg_off, int nseg, int64_t t)
     return lo;
 }
 
+
+// The 64 bytes in w moved up by o bytes (0 < o < 64), zeros below: a
+// barrel shift by whole dwords (per-lane selects), then a byte align.
+__device__ __forceinline__ void shift_up_bytes(uint32_t (&w)[16], int o)
+{
+    const int oq = o >> 2, orr = o & 3;
+#pragma unroll
+    for (int bit = 8; bit >= 1; bit >>= 1) {
+        const bool on = (oq & bit) != 0;
+#pragma unroll
+        for (int d = 15; d >= 0; d--) w[d] = on ? (d >= bit ? w[d - bit] : 0u) : w[d];
+    }
+    if (orr) {
+#pragma unroll
+        for (int d = 15; d >= 0; d--) w[d] = __builtin_amdgcn_alignbyte(w[d], d > 0 ? w[d - 1] : 0u, 4 - orr);
+    }
+}
 
 // Stage slot of 16-byte group jg of lane l's piece (slots are 16 bytes, four
 // per lane): the group index is XORed with bits 1..2 of the lane, so the
@@ -3227,11 +3270,37 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             whole = e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
                          (int64_t)A.seg_off[s + 1] >= a0 + kAsmBytes;
             u32x4 v[4];
-            if (whole) {
+            // a piece across exactly one segment boundary: segment s's last
+            // o bytes, then segment s + 1's first 64 - o moved up into place
+            const bool cross = SELF && e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
+                               (int64_t)A.seg_off[s + 1] < a0 + kAsmBytes &&
+                               (int64_t)A.seg_off[s + 2] >= a0 + kAsmBytes;
+            if (whole || cross) {
                 const uint32_t pw = 8 * (uint32_t)(a0 - (int64_t)A.seg_off[s]);
                 const int k1 = min(lay_first(s + 1, A.spb, A.half_from), A.nslice);
                 while (k + 1 < k1 && A.slice_boff[k + 1] <= pw) k++;
-                whole = seg_bytes<4>(A, s, pw, k, v);
+                whole = seg_bytes<4, SELF>(A, s, pw, k, v);
+            }
+            if (cross && whole) {
+                u32x4 v2[4];
+                whole = seg_bytes<4, SELF>(A, s + 1, 0u, lay_first(s + 1, A.spb, A.half_from), v2);
+                if (whole) {
+                    const int o = (int)((int64_t)A.seg_off[s + 1] - a0);
+                    uint32_t w1[16], w2[16];
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) { w1[4 * q + i] = v[q][i]; w2[4 * q + i] = v2[q][i]; }
+                    shift_up_bytes(w2, o);
+#pragma unroll
+                    for (int d = 0; d < 16; d++) {
+                        const int r = o - 4 * d;                 // bytes of dword d from segment s
+                        const uint32_t m = r >= 4 ? ~0u : r <= 0 ? 0u : (1u << (8 * r)) - 1u;
+                        w1[d] = (w1[d] & m) | (w2[d] & ~m);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = (u32x4){w1[4 * q], w1[4 * q + 1], w1[4 * q + 2], w1[4 * q + 3]};
+                }
             }
             if (whole) {
 #pragma unroll

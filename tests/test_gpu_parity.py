@@ -310,6 +310,25 @@ def test_assembly_boundaries(torch_dev, r2p, oracle, seg, chunk, w, h):
         assert adler == zlib.adler32(filt.tobytes())
 
 
+def test_adler_across_chunks(torch_dev, r2p, oracle):
+    """Chunk sizes that end the zlib stream's first IDAT chunk 1..5 bytes
+    before the stream's end, so that the Adler-32 trailer is split across two
+    chunks (at 1 byte the last chunk holds a single byte)."""
+    log("case adler across chunks")
+    rng = np.random.default_rng(11)
+    w, h = 300, 200
+    px = (np.cumsum(rng.integers(-2, 3, (h, w)), axis=1) % 256).astype(np.uint8)
+    img = r2p.Image.make(w, h, 1, 8, 0)
+    png, filt, _, _, _ = encode_device(torch_dev, r2p, px, img)
+    z = len(check_and_inflate(png)[1])                  # zlib stream bytes
+    assert z <= 65536 + 5
+    for chunk in [z - 1, z - 2, z - 3, z - 4, z - 5]:
+        opt = r2p.options(idat_chunk_bytes=chunk)
+        png2, filt2, adler, _, _ = encode_device(torch_dev, r2p, px, img, opt)
+        _check_png(oracle, r2p, png2, px, img, filt2)
+        assert adler == zlib.adler32(filt2.tobytes())
+
+
 @pytest.mark.parametrize("seg,slc,w,h", [
     (65536, 4096, 2340, 7),        # 16387 bytes: four full 4 KiB slices and one of 3 bytes
     (8192, 4096, 2340, 7),         # three segments, the last one a 3-byte slice
