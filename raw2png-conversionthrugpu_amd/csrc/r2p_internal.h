@@ -215,7 +215,7 @@ inline int filter_items_per_lane(int items, int band_rows)
     return (items + lanes - 1) / lanes;
 }
 constexpr int kDeflateThreads = 256;
-constexpr int kXcdRun = 8;              // consecutive DEFLATE slices dealt to one XCD
+constexpr int kXcdRun = 16;             // consecutive DEFLATE slices dealt to one XCD
 constexpr int kAssembleThreads = 1024;
 constexpr int kAsmScanSlices = 2048;  // a single job of up to this many slices and
 constexpr int kAsmScanSegs = 1024;    // segments is scanned by k_assemble itself

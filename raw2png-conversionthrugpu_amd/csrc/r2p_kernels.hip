@@ -435,6 +435,22 @@ __device__ __forceinline__ uint32_t window_byte(const FilterJob& J, const uint8_
     }
 }
 
+// Work item (DEFLATE slice, filter band) of workgroup b: workgroups are dealt
+// to the 8 XCDs in turn (b on XCD b % 8), so consecutive b run on different
+// L2s.  Mapping b to items in runs of kXcdRun consecutive items per XCD puts
+// most of what an item reads of its neighbours (a slice's 32 KiB window, a
+// band's row above) in the L2 of its own XCD, where the neighbouring
+// workgroups have read or written it, while the runs stay interleaved across
+// the XCDs so that their loads remain even.  A bijection on [0, total): the
+// items past the last whole 8 * kXcdRun keep their order.
+__device__ __forceinline__ uint32_t xcd_slice(uint32_t b, uint32_t total)
+{
+    constexpr uint32_t X = 8, R = kXcdRun;
+    if (R <= 1 || b >= total / (X * R) * (X * R)) return b;
+    const uint32_t x = b % X, i = b / X;
+    return ((i / R) * X + x) * R + i % R;
+}
+
 // Sum of v over the 64 lanes of the wave (DPP: quad swaps, half-row and row
 // mirrors, then the row broadcasts), returned uniform.
 __device__ __forceinline__ uint32_t wave_total(uint32_t v)
@@ -486,8 +502,11 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
     const uint32_t total = L.counts ? L.counts[1] : L.total_bands;
     const uint32_t first = L.counts ? L.counts[2] : 0u;
     const int njobs = L.counts ? (int)L.counts[0] : L.njobs;
-    // bands after a workgroup's first: every grid-th, or from the queue
-    for (uint32_t g = first + blockIdx.x; g < total;) {
+    // bands after a workgroup's first: every grid-th, or from the queue; a
+    // single job's launch of one band per workgroup deals them to the XCDs in
+    // runs (xcd_slice), so that a band's row above is mostly in its XCD's L2
+    const bool runs = !L.jobs && !L.queue && first == 0 && gridDim.x >= total;
+    for (uint32_t g = runs ? xcd_slice(blockIdx.x, total) : first + blockIdx.x; g < total;) {
         int j = 0;
         if (L.jobs) {
             int lo = 0, hi = njobs - 1;
@@ -1746,21 +1765,6 @@ __device__ __forceinline__ void load_own(const uint8_t* p, uint32_t* o)
 // not carry their registers.  DYN: the parse's tokens and their symbol
 // histogram go to memory (DeflateLaunch::tokens / tok_count / hist) for a
 // dynamic-Huffman block per segment; no bits are emitted here.
-// Slice of workgroup b: workgroups are dealt to the 8 XCDs in turn (b on XCD
-// b % 8), so consecutive b run on different L2s.  Mapping b to slices in
-// runs of kXcdRun consecutive slices per XCD puts most of a slice's 32 KiB
-// window (the slices just before it) in the L2 of its own XCD, where the
-// neighbouring workgroups have read it, while the runs stay interleaved
-// across the XCDs so that their loads remain even.  A bijection on
-// [0, total): the slices past the last whole 8 * kXcdRun keep their order.
-__device__ __forceinline__ uint32_t xcd_slice(uint32_t b, uint32_t total)
-{
-    constexpr uint32_t X = 8, R = kXcdRun;
-    if (R <= 1 || b >= total / (X * R) * (X * R)) return b;
-    const uint32_t x = b % X, i = b / X;
-    return ((i / R) * X + x) * R + i % R;
-}
-
 template <int NT, bool STATS, bool DYN, int SS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_deflate(DeflateLaunch DL)
 {

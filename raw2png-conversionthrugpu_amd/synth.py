@@ -282,10 +282,13 @@ def _ds(v) -> bytes:
 
 
 def write_dicom(path: str, pixels: np.ndarray, p: DicomParams, frames: int = 1,
-                slope_str=None, intercept_str=None, wc_str=None, ww_str=None, implicit: bool = False) -> None:
+                slope_str=None, intercept_str=None, wc_str=None, ww_str=None, implicit: bool = False,
+                high_bit: bool = True, window=None, rescale=None) -> None:
     """Uncompressed little-endian Part 10 file holding `pixels` (uint16 or
     uint8, (F,)H,W[,C]): Explicit VR, or Implicit VR with `implicit` (the file
-    meta group is Explicit VR either way)."""
+    meta group is Explicit VR either way).  `high_bit` False leaves out
+    HighBit (0028,0102); `window` / `rescale` write the window or rescale
+    attributes (with the *_str values) whatever p's flags say."""
     import struct
 
     def el_explicit(g, e, vr, val: bytes):
@@ -315,11 +318,13 @@ def write_dicom(path: str, pixels: np.ndarray, p: DicomParams, frames: int = 1,
         ds += el(0x0028, 0x0008, b"IS", str(frames).encode())
     ds += el(0x0028, 0x0010, b"US", us(h)) + el(0x0028, 0x0011, b"US", us(w))
     ds += el(0x0028, 0x0100, b"US", us(p.bits_allocated)) + el(0x0028, 0x0101, b"US", us(p.bits_stored))
-    ds += el(0x0028, 0x0102, b"US", us(p.high_bit)) + el(0x0028, 0x0103, b"US", us(p.pixel_rep))
-    if p.apply_window:
+    if high_bit:
+        ds += el(0x0028, 0x0102, b"US", us(p.high_bit))
+    ds += el(0x0028, 0x0103, b"US", us(p.pixel_rep))
+    if (p.apply_window if window is None else window):
         ds += el(0x0028, 0x1050, b"DS", wc_str or _ds(p.window_center))
         ds += el(0x0028, 0x1051, b"DS", ww_str or _ds(p.window_width))
-    if p.apply_rescale:
+    if (p.apply_rescale if rescale is None else rescale):
         ds += el(0x0028, 0x1052, b"DS", intercept_str or _ds(p.rescale_intercept))
         ds += el(0x0028, 0x1053, b"DS", slope_str or _ds(p.rescale_slope))
     data = px.astype("<u2" if p.bits_allocated == 16 else np.uint8).tobytes()
@@ -328,8 +333,10 @@ def write_dicom(path: str, pixels: np.ndarray, p: DicomParams, frames: int = 1,
         f.write(b"\0" * 128 + b"DICM" + meta + ds)
 
 
-def write_tiff(path: str, pixels: np.ndarray, rows_per_strip: int = 64) -> None:
-    """Uncompressed little-endian TIFF (chunky), uint8 or uint16, gray or RGB."""
+def write_tiff(path: str, pixels: np.ndarray, rows_per_strip: int = 64, photometric=None) -> None:
+    """Uncompressed little-endian TIFF (chunky), uint8 or uint16, gray or RGB;
+    PhotometricInterpretation RGB / BlackIsZero unless `photometric` is given
+    (0: WhiteIsZero)."""
     import struct
     px = np.ascontiguousarray(pixels)
     h, w = px.shape[:2]
@@ -364,7 +371,8 @@ def write_tiff(path: str, pixels: np.ndarray, rows_per_strip: int = 64) -> None:
             entries.append(struct.pack("<HHII", t, typ, len(vals), arr(vals, typ)))
 
     tag(256, 4, [w]); tag(257, 4, [h]); tag(258, 3, [bps] * spp); tag(259, 3, [1])
-    tag(262, 3, [2 if spp >= 3 else 1]); tag(273, 4, offs); tag(277, 3, [spp])
+    tag(262, 3, [photometric if photometric is not None else (2 if spp >= 3 else 1)])
+    tag(273, 4, offs); tag(277, 3, [spp])
     tag(278, 4, [rows_per_strip]); tag(279, 4, cnts); tag(284, 3, [1])
     ifd_off = extra_off + len(extra)
     if ifd_off % 2:

@@ -106,3 +106,53 @@ def test_file_readers_through_pipeline(torch_dev, r2p, synth, oracle, tmp_path, 
     exp = oracle.dicom_window(rgb.reshape(-1), op).reshape(61, 97, 3)
     got = np.array(PILImage.open(str(tmp_path / "rgb.png")))
     assert got.shape == (61, 97, 3) and np.array_equal(got, exp)
+
+
+def test_reader_attribute_edges(torch_dev, r2p, synth, oracle, tmp_path):
+    """Reader cases the loaders decide, through encode_*_to_png: a
+    WhiteIsZero TIFF is inverted (image_loader.cpp:204-210); DICOM without
+    HighBit takes BitsStored - 1 (dicom_loader.cpp:329-330); WindowWidth 0
+    turns the window off and RescaleSlope "1" / RescaleIntercept "0" turn the
+    rescale off (dicom_loader.cpp:380-397)."""
+    import numpy as np
+    from PIL import Image as PILImage
+    exe = os.path.join(REPO, "tests", "cpp", "facade_check")
+    print("case reader attribute edges", flush=True)
+
+    def encode(kind, src, name):
+        out = str(tmp_path / name)
+        r = subprocess.run([exe, "encode", kind, src, out, "1"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        return np.array(PILImage.open(out)).astype(np.int64)
+
+    rng = np.random.default_rng(5)
+    g8 = rng.integers(0, 256, (30, 41)).astype(np.uint8)
+    synth.write_tiff(str(tmp_path / "w8.tif"), g8, rows_per_strip=7, photometric=0)
+    assert np.array_equal(encode("tiff", str(tmp_path / "w8.tif"), "w8.png"), 255 - g8.astype(np.int64))
+    g16 = rng.integers(0, 65536, (20, 33)).astype(np.uint16)
+    synth.write_tiff(str(tmp_path / "w16.tif"), g16, rows_per_strip=6, photometric=0)
+    assert np.array_equal(encode("tiff", str(tmp_path / "w16.tif"), "w16.png"), 65535 - g16.astype(np.int64))
+
+    ct = synth.ct_slice(3, 96)
+    px = ct.pixels
+
+    def expect(p):
+        op = oracle.DicomParams(*[getattr(p, f) for f, _ in p._fields_])
+        return oracle.dicom_window(px.reshape(-1), op).view(">u2").reshape(px.shape).astype(np.int64)
+
+    # HighBit absent: BitsStored - 1 (here 11, where a default of 15 would shift the codes)
+    p = r2p.DicomParams.make(16, 12, 11, 0, 1, 1.0, -1024.0, 1, 40.0, 400.0)
+    synth.write_dicom(str(tmp_path / "nohb.dcm"), px, p, high_bit=False)
+    assert np.array_equal(encode("dicom", str(tmp_path / "nohb.dcm"), "nohb.png"), expect(p))
+    # WindowWidth 0: no window (the rescale still applies)
+    synth.write_dicom(str(tmp_path / "ww0.dcm"), px, p, ww_str=b"0")
+    p_nowin = r2p.DicomParams.make(16, 12, 11, 0, 1, 1.0, -1024.0, 0, 0.0, 0.0)
+    got = encode("dicom", str(tmp_path / "ww0.dcm"), "ww0.png")
+    assert np.array_equal(got, expect(p_nowin)) and not np.array_equal(got, expect(p))
+    # RescaleSlope "1" and RescaleIntercept "0" present: no rescale (a window
+    # wide enough that the rescaled and the plain codes give different rows)
+    pw = r2p.DicomParams.make(16, 12, 11, 0, 1, 1.0, -1024.0, 1, 500.0, 3000.0)
+    synth.write_dicom(str(tmp_path / "r10.dcm"), px, pw, rescale=True, slope_str=b"1", intercept_str=b"0")
+    p_norescale = r2p.DicomParams.make(16, 12, 11, 0, 0, 1.0, 0.0, 1, 500.0, 3000.0)
+    got = encode("dicom", str(tmp_path / "r10.dcm"), "r10.png")
+    assert np.array_equal(got, expect(p_norescale)) and not np.array_equal(got, expect(pw))
