@@ -2669,20 +2669,15 @@ static_assert(kEmitThreads * kEmitPer == 4096, "a round is one unit of k_deflate
 // after a match start the distance's with its extra bits), a block scan of the
 // bit counts, every code ored into LDS at its offset, flush of the whole
 // words.  Slot npos of the segment's last slice is the end-of-block code.
-// Rounds alternate between two LDS bit buffers: a round's flush (its whole
-// words out and cleared, its partial word handed to the other buffer) runs
-// while the next round computes its codes, so a round needs two barriers (the
-// scan's, and all codes ored in) rather than four.
 template <bool JOBS>
 __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
 {
     constexpr int NT = kEmitThreads, NW = NT / 64;
     __shared__ uint32_t T[kHuffTab + 1];                      // + kTokNone's zero entry
-    __shared__ __attribute__((aligned(8))) uint32_t obufs[2][kEmitWords];
+    __shared__ __attribute__((aligned(8))) uint32_t obuf[kEmitWords];
     __shared__ uint32_t red[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < 2 * kEmitWords; i += NT) (&obufs[0][0])[i] = 0;
-    uint32_t* obuf = obufs[0];
+    for (int i = tid; i < kEmitWords; i += NT) obuf[i] = 0;
     if (tid == 0) T[kTokNone] = 0;
     const uint32_t total = HL.counts ? HL.counts[2] : HL.total_slices;
     const int njobs = HL.counts ? (int)HL.counts[0] : HL.njobs;
@@ -2752,29 +2747,24 @@ __global__ __launch_bounds__(kEmitThreads) void k_huff_emit(HuffLaunch HL)
             uint32_t wbase = 0, tot = 0;
             for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) wbase += red[w2]; tot += red[w2]; }
             uint32_t off = carry_bits + wbase + incl - nbits;
-            uint32_t* ob = obufs[r & 1];                      // (its word 0 holds the carry)
 #pragma unroll
             for (int i = 0; i < kEmitPer; i++) {
-                if (n[i]) put_bits64(ob, a[i], n[i], off);
+                if (n[i]) put_bits64(obuf, a[i], n[i], off);
                 off += n[i];
             }
             __syncthreads();
-            // flush into the slice's buffer, clearing what it read; the partial
-            // word goes to the other buffer's word 0 (thread 0 writes it before
-            // the next round's first barrier, that round's codes go in after it;
-            // this buffer is next written two rounds on, after both barriers of
-            // the round between)
             const uint32_t tb = carry_bits + tot;
             const uint32_t full = tb >> 5;
-            for (uint32_t i = tid; i < full; i += NT) { out[out_dw + i] = ob[i]; ob[i] = 0; }
-            if (tid == 0) { obufs[(r + 1) & 1][0] = (tb & 31) ? ob[full] : 0u; ob[full] = 0; }
+            for (uint32_t i = tid; i < full; i += NT) { out[out_dw + i] = obuf[i]; obuf[i] = 0; }
+            __syncthreads();
+            if (tid == 0) { obuf[0] = (tb & 31) ? obuf[full] : 0u; if (full) obuf[full] = 0; }
+            __syncthreads();
             carry_bits = tb & 31;
             out_dw += full;
         }
         if (tid == 0) {
-            uint32_t* ob = obufs[rounds & 1];
-            if (carry_bits) out[out_dw] = ob[0];
-            ob[0] = 0;
+            if (carry_bits) out[out_dw] = obuf[0];
+            obuf[0] = 0;
             H.slice_bits[sl] = (uint32_t)(out_dw * 32 + carry_bits);
         }
     }
