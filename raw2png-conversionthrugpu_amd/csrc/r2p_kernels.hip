@@ -3279,15 +3279,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             const bool cross = SELF && e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes <= segbytes &&
                                (int64_t)A.seg_off[s + 1] < a0 + kAsmBytes &&
                                (int64_t)A.seg_off[s + 2] >= a0 + kAsmBytes;
-            // the stream's last piece: the last segment's final bytes (zeros
-            // past its end), then the Adler-32 in its last four
-            const bool tail = SELF && A.mode == 1 && e - b == kAsmBytes && a0 >= 0 && a0 + kAsmBytes == segbytes + 4 &&
-                              s == A.nseg - 1;
-            // a chunk's short first piece inside one segment: read like a
-            // whole one, its e - b bytes then staged for the byte-wise CRC
-            const bool part = SELF && e - b < kAsmBytes && a0 >= 0 && a0 + (e - b) <= segbytes &&
-                              (int64_t)A.seg_off[s + 1] >= a0 + (e - b);
-            if (whole || cross || tail || part) {
+            if (whole || cross) {
                 const uint32_t pw = 8 * (uint32_t)(a0 - (int64_t)A.seg_off[s]);
                 const int k1 = min(lay_first(s + 1, A.spb, A.half_from), A.nslice);
                 while (k + 1 < k1 && A.slice_boff[k + 1] <= pw) k++;
@@ -3314,13 +3306,6 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                     for (int q = 0; q < 4; q++) v[q] = (u32x4){w1[4 * q], w1[4 * q + 1], w1[4 * q + 2], w1[4 * q + 3]};
                 }
             }
-            if (tail && whole) v[3][3] |= __builtin_bswap32(*A.adler_out);
-            if (part && whole) {                          // staged, not gathered
-#pragma unroll
-                for (int jg = 0; jg < 4; jg++) stage[wave][stage_slot(lane, jg)] = v[jg];
-                whole = false;
-                s = -1;                                   // (marks it staged)
-            }
             if (whole) {
 #pragma unroll
                 for (int jg = 0; jg < 4; jg++) stage[wave][stage_slot(lane, jg)] = v[jg];
@@ -3330,7 +3315,7 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
             // bytes, short, or over three slices) its 16-byte groups are
             // gathered by the wave below, all loads of a round in flight
             // together
-            if (!whole && s >= 0) {
+            if (!whole) {
                 sc = s; kc = k;
                 slow = (1u << (int)((e - b + 15) / 16)) - 1u;
             }
