@@ -527,7 +527,12 @@ __global__ __launch_bounds__(kFilterThreads) __attribute__((amdgpu_waves_per_eu(
         if (j != cur_job) {
             if (J.lut_bytes > 0) {
                 const int n16 = (J.lut_bytes + 15) >> 4;
-                for (int i = tid; i < n16; i += blockDim.x)
+                // (at 6 items per lane the first index is made opaque here,
+                // so that the loop's 64-bit start address is not kept live
+                // across the kernel, where it took a spill slot)
+                int i0 = tid;
+                if (K == 6) asm volatile("" : "+v"(i0));
+                for (int i = i0; i < n16; i += blockDim.x)
                     ((u32x4*)lut_lds)[i] = ((const u32x4*)J.lut)[i];
             }
             cur_job = j;
@@ -3409,7 +3414,12 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
         if (lane == 0) red[wave] = r;
         __syncthreads();
         if (wave == 0) {
-            r = lane < kAssembleThreads / 64 ? red[lane] : 0u;
+            // (the lane index and the IHDR CRC below are made opaque at
+            // their use, so that the values derived from them are not kept
+            // live, and spilled, across the chunk loop)
+            int li = lane;
+            asm volatile("" : "+v"(li));
+            r = li < kAssembleThreads / 64 ? red[li] : 0u;
             // 16 waves of 4 KiB: the shifts by 4, 8, 16 and 32 KiB
             static_assert(kAssembleThreads == 1024, "four cross-wave levels");
 #pragma unroll
@@ -3427,7 +3437,9 @@ __global__ __launch_bounds__(kAssembleThreads) void k_assemble(AssembleLaunch AL
                 ih[12] = A.bit_depth; ih[13] = A.color_type; ih[14] = 0; ih[15] = 0; ih[16] = 0;
                 put_be32(A.out + 8, 13);
                 for (int i = 0; i < 17; i++) A.out[12 + i] = ih[i];
-                put_be32(A.out + 29, A.ihdr_crc);
+                uint32_t ihdr_crc = A.ihdr_crc;
+                asm volatile("" : "+v"(ihdr_crc));
+                put_be32(A.out + 29, ihdr_crc);
             }
             put_be32(o, (uint32_t)n);
             o[4] = 'I'; o[5] = 'D'; o[6] = 'A'; o[7] = 'T';
